@@ -1,0 +1,177 @@
+/*
+ * ocean_gpu.h -- C-ABI of the MI355X (gfx950) core-level BSE action and Krylov
+ * solvers.  Drop-in boundary for OCEAN2's hot path (OCEAN 3.2.1 tree):
+ *
+ *   OCEAN_xact            OCEAN2/OCEAN_action.f90:286   -> ocean_gpu_xact, ocean_gpu_xact_vec
+ *   OCEAN_haydock_do      OCEAN2/OCEAN_haydock.f90:167  -> ocean_gpu_haydock
+ *   OCEAN_gmres_do_recycle OCEAN2/OCEAN_gmres.f90:508   -> ocean_gpu_gmres
+ *   write_core            OCEAN2/OCEAN_haydock.f90:1801 -> ocean_gpu_spectrum
+ *   OCEAN_psi_* vector ops OCEAN2/OCEAN_psi.f90         -> ocean_gpu_vec_*
+ *   module state loaded by lr_init / OCEAN_create_central / OCEAN_energies_load
+ *                          (long_range.f90:108-260, OCEAN_multiplet.f90:248-570,
+ *                           OCEAN_energies.f90:303-368) -> ocean_gpu_load_*
+ *
+ * Conventions (those of the reference):
+ *   - every entry point returns an int error code; 0 is success (the reference's
+ *     ierr, OCEAN2/OCEAN.f90:92-93);
+ *   - host arrays are caller-owned, column-major, with separate real and
+ *     imaginary FP64 planes (OCEAN2/OCEAN_psi.f90:64-65); they are borrowed for
+ *     the duration of the call only;
+ *   - energies are in Hartree unless a name says eV;
+ *   - one handle per host thread; calls on one handle are serialised by the
+ *     caller (the reference runs MPI_THREAD_SERIALIZED, OCEAN2/OCEAN_mpi.f90:40).
+ *
+ * Callable from Fortran through iso_c_binding (ocean_amd/fortran/ocean_gpu_iface.f90)
+ * and from Python through ctypes (ocean_amd/core.py).
+ */
+#ifndef OCEAN_GPU_H
+#define OCEAN_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes mirror the reference where it has one. */
+#define OCEAN_GPU_OK              0
+#define OCEAN_GPU_ERR_NAN        -1     /* NaN in Haydock a: OCEAN2/OCEAN_haydock.f90:912-920 */
+#define OCEAN_GPU_ERR_ARG      1001     /* inconsistent sizes or a null pointer */
+#define OCEAN_GPU_ERR_STATE    1002     /* required module state not loaded */
+#define OCEAN_GPU_ERR_HIP      1003     /* HIP runtime failure */
+#define OCEAN_GPU_ERR_NOMEM    1004     /* device allocation failed */
+#define OCEAN_GPU_ERR_UNSUPPORTED 9963  /* as OCEAN_gmres.f90:538-542 for backf */
+/* GMRES least-squares failure returns the zhptrf-style pivot index (> 0),
+ * as OCEAN2/OCEAN_gmres.f90:312-329 returns LAPACK's info. */
+
+/* Term flags of OCEAN_xact (OCEAN2/OCEAN_system.f90:465-471): hflag[0] lr,
+ * hflag[4] fg (multiplet), hflag[5] spin-orbit.  Ladder/bubble/ct are valence
+ * or legacy terms and are not part of the core path. */
+
+/* Precision forms */
+#define OCEAN_GPU_FP64   0   /* the drop-in: everything in FP64 */
+#define OCEAN_GPU_MIXED  1   /* u.psi, FFT and W in FP32; recurrences in FP64 */
+
+/* System description: the o_system / o_run fields the core action reads
+ * (OCEAN2/OCEAN_system.f90:14-124). */
+typedef struct ocean_bse_desc {
+  int32_t num_bands;     /* sys%num_bands */
+  int32_t kmesh[3];      /* sys%kmesh */
+  int32_t xmesh[3];      /* sys%xmesh */
+  int32_t nspn;          /* sys%nspn (1 or 2) */
+  int32_t lc;            /* core l, sys%cur_run%ZNL(3); nalpha = 4 (2 lc + 1) */
+  int32_t lmin, lmax;    /* projector l range (prjfile) */
+  int32_t nproj[8];      /* nproj(lmin..lmax) */
+  int32_t lvl, lvh;      /* exchange l range (bse.in line 1); must cover lmin..lmax */
+  double  celvol;        /* sys%celvol (bohr^3) */
+  double  inter_scale;   /* sys%interactionScale */
+  int32_t e0;            /* sys%e0: include the band-energy term */
+  int32_t mult;          /* sys%mult: include the multiplet/spin-orbit terms */
+  int32_t long_range;    /* sys%long_range: include the screened direct term */
+  int32_t precision;     /* OCEAN_GPU_FP64 or OCEAN_GPU_MIXED */
+} ocean_bse_desc;
+
+typedef struct ocean_gpu_handle ocean_gpu_handle;
+
+int  ocean_gpu_create(const ocean_bse_desc* desc, ocean_gpu_handle** out);
+int  ocean_gpu_destroy(ocean_gpu_handle* h);
+const char* ocean_gpu_strerror(int code);
+int  ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision);
+
+/* Phased Bloch functions re/im_bloch_state(nb, nk, nx, nspn) after
+ * OCEAN_bloch_lrLOAD (OCEAN2/OCEAN_bloch.f90:54-219; long_range.f90:186-187). */
+int  ocean_gpu_load_bloch(ocean_gpu_handle* h, const double* u_re, const double* u_im);
+
+/* Raw Bloch functions (con.u2.dat order: OCEAN2/OCEAN_bloch.f90:998-1115) phased
+ * on the device as OCEAN_bloch_lrLOAD does; xoff(3, nx) are the wrapped integer x
+ * offsets (xph + ix - 1, ...) of OCEAN2/OCEAN_bloch.f90:133-203. */
+int  ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const double* u_im,
+                              const int32_t* xoff);
+
+/* W(nk, nx) as filled by lr_populate_W2 (OCEAN2/long_range.f90:1659-1853). */
+int  ocean_gpu_load_w(ocean_gpu_handle* h, const double* W);
+
+/* Build W on the device from the radial table (rtab, ptab: rpottrim,
+ * OCEAN2/long_range.f90:1688-1696) and the geometry: rdist(nk, nx) is r = |A(R+x-tau)|
+ * of long_range.f90:1814.  W = s * {intval cap/cap for r <= rtab(n); 1/(eps r) beyond}.
+ * Also keeps the hat weights that the differentiation form contracts with. */
+int  ocean_gpu_build_w(ocean_gpu_handle* h, const double* rdist, const double* rtab,
+                       const double* ptab, int32_t nptab, double epsilon0, double scale);
+
+/* Multiplet state of OCEAN_create_central (OCEAN2/OCEAN_multiplet.f90:248-570):
+ * mpcr/mpci(nb, nk, npmax, -lmax:lmax, lmin:lmax, nspn), mpm(npmax, npmax, lmin:lmax),
+ * mhr/mhi packed per l = lvl..lvh at jbeg with mham = 4(2lc+1)(2l+1)nproj(l)
+ * (OCEAN_multiplet.f90:318-327), somelr/someli(nalpha, nalpha) (:572-690).
+ * have_so = 0 reproduces have_spin_orbit = .false. */
+int  ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* mpci,
+                         const double* mpm, const double* mhr, const double* mhi,
+                         const double* somelr, const double* someli, int32_t have_so);
+
+/* p_energy%r/%i (nb, nk, nalpha) and allow%r (nb, nk, nalpha)
+ * (OCEAN2/OCEAN_energies.f90:829-906, :68-112). e_im may be NULL (zero). */
+int  ocean_gpu_load_energies(ocean_gpu_handle* h, const double* e_re, const double* e_im,
+                             const double* allow);
+
+/* OCEAN_xact (OCEAN2/OCEAN_action.f90:286-618), core branch, on host planes
+ * (nb, nk, nalpha).  hflag may be NULL (all terms on). */
+int  ocean_gpu_xact(ocean_gpu_handle* h, const double* psi_re, const double* psi_im,
+                    double* hpsi_re, double* hpsi_im, int32_t backwards,
+                    const int32_t* hflag);
+
+/* ---- device-resident vectors: the OCEAN_vector of OCEAN2/OCEAN_psi.f90:63-133 ---- */
+int  ocean_gpu_vec_new(ocean_gpu_handle* h, int32_t* id);
+int  ocean_gpu_vec_free(ocean_gpu_handle* h, int32_t id);
+int  ocean_gpu_vec_put(ocean_gpu_handle* h, int32_t id, const double* re, const double* im);
+int  ocean_gpu_vec_get(ocean_gpu_handle* h, int32_t id, double* re, double* im);
+/* <p|q> = sum p_r q_r + p_i q_i + i sum p_r q_i - p_i q_r (OCEAN_psi.f90:1930-2126) */
+int  ocean_gpu_vec_dot(ocean_gpu_handle* h, int32_t p, int32_t q, double* rval, double* ival);
+/* squared norm (OCEAN_psi.f90:1673-1760) */
+int  ocean_gpu_vec_nrm(ocean_gpu_handle* h, int32_t p, double* val);
+/* y += (rval + i ival) x (OCEAN_psi.f90:1386-1458) */
+int  ocean_gpu_vec_axpy(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y);
+/* x *= rval (OCEAN_psi.f90:1876-1917) */
+int  ocean_gpu_vec_scal(ocean_gpu_handle* h, double rval, int32_t x);
+int  ocean_gpu_vec_copy(ocean_gpu_handle* h, int32_t dst, int32_t src);
+/* xact between device vectors; the form bench.py times (inputs resident in HBM). */
+int  ocean_gpu_xact_vec(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t backwards,
+                        const int32_t* hflag);
+/* Repeat xact_vec nrep times on one stream with no host synchronisation. */
+int  ocean_gpu_xact_vec_repeat(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t nrep);
+
+/* ---- solvers ---- */
+/* Hermitian Haydock recursion (OCEAN2/OCEAN_haydock.f90:190-320, :824-938).
+ * psi0 is the normalised starting vector.  real_a, real_b, imag_a have niter+1
+ * entries (index 0..niter, as the reference's real_a(0:niter)).  niter_done
+ * returns the iterations completed (less than niter only on NaN). */
+int  ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im,
+                       int32_t niter, double* real_a, double* real_b, double* imag_a,
+                       int32_t* niter_done);
+
+/* Recycled preconditioned GMRES over energies (OCEAN2/OCEAN_gmres.f90:508-908).
+ * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
+ * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,
+ * rel_error, complete_iter, global_iter) as written to absspct (:864-869).
+ * x_re/x_im (nb, nk, nalpha, nE) may be NULL. */
+int  ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im,
+                     int32_t nE, const double* energies_eV, double gres_eV, double gprc_eV,
+                     double ffff, int32_t depth, double kpref, double* out,
+                     double* x_re, double* x_im);
+
+/* Continued fraction with the square-root terminator (write_core,
+ * OCEAN2/OCEAN_haydock.f90:1801-1894).  gam, el, eh in Hartree; ne+1 energies.
+ * out[5*(ne+1)], row-major per energy: E (eV), Spect, Spect(0), SPKK, SPKK(0).
+ * literal_cmplx = 1 rounds the kind-less cmplx() sites (:1846-1847, :1870-1871)
+ * to single precision as OCEAN2's build does; 0 keeps everything in FP64. */
+int  ocean_gpu_spectrum(const double* real_a, const double* real_b, const double* imag_a,
+                        int32_t niter, double kpref, double gam, double el, double eh,
+                        int32_t ne, int32_t literal_cmplx, double* out);
+
+/* Kernel-time probe used by the benchmark: average duration (ms) of the last
+ * ocean_gpu_xact_vec_repeat call per kernel family, measured with HIP events on
+ * the library's stream.  names: "proj", "fft", "adj", "mult", "total". */
+int  ocean_gpu_last_timing(ocean_gpu_handle* h, const char* name, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCEAN_GPU_H */

@@ -1,0 +1,334 @@
+"""ctypes binding of libocean_gpu.so and a Python mirror of OCEAN2's call surface.
+
+    OCEAN_xact(sys, inter_scale, psi, new_psi, ierr, backwards, nhflag)
+        OCEAN2/OCEAN_action.f90:286            -> OceanGPU.xact
+    OCEAN_haydock_do(sys, hay_vec, restartBSE, newEps, ierr)
+        OCEAN2/OCEAN_haydock.f90:167           -> OceanGPU.haydock
+    OCEAN_gmres_do_recycle(sys, hay_vec, ierr)
+        OCEAN2/OCEAN_gmres.f90:508             -> OceanGPU.gmres
+    write_core(fh, iter, kpref, ...)
+        OCEAN2/OCEAN_haydock.f90:1801          -> spectrum
+
+Errors raise ``OceanGPUError`` carrying the library's integer code (the
+reference's ierr).  There is no CPU fallback: without the built library every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libocean_gpu.so")
+EV2HARTREE = 0.036749322175666
+HARTREE2EV = 27.211386245980
+
+FP64 = 0
+MIXED = 1
+
+
+class OceanGPUError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = _lib().ocean_gpu_strerror(code).decode() if _LIB is not None else str(code)
+        super().__init__(f"{where}: ierr={code} ({msg})")
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("num_bands", C.c_int32), ("kmesh", C.c_int32 * 3), ("xmesh", C.c_int32 * 3),
+        ("nspn", C.c_int32), ("lc", C.c_int32), ("lmin", C.c_int32), ("lmax", C.c_int32),
+        ("nproj", C.c_int32 * 8), ("lvl", C.c_int32), ("lvh", C.c_int32),
+        ("celvol", C.c_double), ("inter_scale", C.c_double), ("e0", C.c_int32),
+        ("mult", C.c_int32), ("long_range", C.c_int32), ("precision", C.c_int32),
+    ]
+
+
+_LIB = None
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+_H = C.c_void_p
+
+_SIGS = {
+    "ocean_gpu_create": (C.c_int, [C.POINTER(Desc), C.POINTER(_H)]),
+    "ocean_gpu_destroy": (C.c_int, [_H]),
+    "ocean_gpu_strerror": (C.c_char_p, [C.c_int]),
+    "ocean_gpu_set_precision": (C.c_int, [_H, C.c_int32]),
+    "ocean_gpu_load_bloch": (C.c_int, [_H, _DP, _DP]),
+    "ocean_gpu_load_bloch_raw": (C.c_int, [_H, _DP, _DP, _IP]),
+    "ocean_gpu_load_w": (C.c_int, [_H, _DP]),
+    "ocean_gpu_build_w": (C.c_int, [_H, _DP, _DP, _DP, C.c_int32, C.c_double, C.c_double]),
+    "ocean_gpu_load_mult": (C.c_int, [_H, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int32]),
+    "ocean_gpu_load_energies": (C.c_int, [_H, _DP, _DP, _DP]),
+    "ocean_gpu_xact": (C.c_int, [_H, _DP, _DP, _DP, _DP, C.c_int32, _IP]),
+    "ocean_gpu_vec_new": (C.c_int, [_H, _IP]),
+    "ocean_gpu_vec_free": (C.c_int, [_H, C.c_int32]),
+    "ocean_gpu_vec_put": (C.c_int, [_H, C.c_int32, _DP, _DP]),
+    "ocean_gpu_vec_get": (C.c_int, [_H, C.c_int32, _DP, _DP]),
+    "ocean_gpu_vec_dot": (C.c_int, [_H, C.c_int32, C.c_int32, _DP, _DP]),
+    "ocean_gpu_vec_nrm": (C.c_int, [_H, C.c_int32, _DP]),
+    "ocean_gpu_vec_axpy": (C.c_int, [_H, C.c_double, C.c_double, C.c_int32, C.c_int32]),
+    "ocean_gpu_vec_scal": (C.c_int, [_H, C.c_double, C.c_int32]),
+    "ocean_gpu_vec_copy": (C.c_int, [_H, C.c_int32, C.c_int32]),
+    "ocean_gpu_xact_vec": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32, _IP]),
+    "ocean_gpu_xact_vec_repeat": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
+    "ocean_gpu_haydock": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, _DP, _DP, _IP]),
+    "ocean_gpu_gmres": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
+                                  C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
+    "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
+    "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
+}
+
+
+def _lib():
+    """Load libocean_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m ocean_amd.build` "
+                               "(the GPU path has no CPU fallback)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim <= 1), "need F-ordered float64"
+    return a.ctypes.data_as(_DP)
+
+
+def _planes(z):
+    z = np.asarray(z)
+    return (np.asfortranarray(z.real, dtype=np.float64), np.asfortranarray(z.imag, dtype=np.float64))
+
+
+def _chk(rc, where):
+    if rc != 0:
+        raise OceanGPUError(rc, where)
+
+
+def spectrum(real_a, real_b, niter, kpref, gam, el, eh, ne, imag_a=None, literal_cmplx=True):
+    """write_core continued fraction on the GPU; returns (ne+1, 5) columns."""
+    lib = _lib()
+    ra = np.ascontiguousarray(real_a, np.float64)
+    rb = np.ascontiguousarray(real_b, np.float64)
+    ia = None if imag_a is None else np.ascontiguousarray(imag_a, np.float64)
+    out = np.zeros((ne + 1, 5))
+    _chk(lib.ocean_gpu_spectrum(_p(ra), _p(rb), _p(ia), niter, kpref, gam, el, eh, ne,
+                                1 if literal_cmplx else 0, out.ctypes.data_as(_DP)), "spectrum")
+    return out
+
+
+class OceanGPU:
+    """One device-resident core-level BSE problem (the handle of the C-ABI)."""
+
+    def __init__(self, nb, kmesh, xmesh, lc, nspn, celvol, inter_scale=1.0, lmin=0, lmax=3,
+                 nproj=(1, 1, 1, 1), lvl=None, lvh=None, e0=True, mult=True, long_range=True,
+                 precision=FP64):
+        lib = _lib()
+        d = Desc()
+        d.num_bands = nb
+        d.kmesh[:] = list(kmesh)
+        d.xmesh[:] = list(xmesh)
+        d.nspn = nspn
+        d.lc = lc
+        d.lmin, d.lmax = lmin, lmax
+        npj = list(nproj) + [0] * (8 - len(nproj))
+        d.nproj[:] = npj[:8]
+        d.lvl = lmin if lvl is None else lvl
+        d.lvh = lmax if lvh is None else lvh
+        d.celvol = celvol
+        d.inter_scale = inter_scale
+        d.e0, d.mult, d.long_range = int(e0), int(mult), int(long_range)
+        d.precision = precision
+        self.desc = d
+        self.nb, self.nk, self.nx = nb, int(np.prod(kmesh)), int(np.prod(xmesh))
+        self.na = 4 * (2 * lc + 1)
+        self.shape = (nb, self.nk, self.na)
+        h = _H()
+        _chk(lib.ocean_gpu_create(C.byref(d), C.byref(h)), "ocean_gpu_create")
+        self.h = h
+        self.kpref = None
+
+    @classmethod
+    def from_inputs(cls, inp, precision=FP64, e0=True, mult=True, long_range=True):
+        """Create a handle and load every piece of state from synthetic inputs."""
+        m = inp.mult
+        g = cls(inp.nb, inp.kmesh, inp.xmesh, inp.lc, inp.nspn, inp.celvol, inp.inter_scale,
+                m.lmin, m.lmax, tuple(m.nproj), m.lvl, m.lvh, e0, mult, long_range, precision)
+        g.load_inputs(inp)
+        return g
+
+    def load_inputs(self, inp):
+        self.load_bloch(inp.u)
+        self.load_w(inp.W)
+        self.load_mult(inp.mult)
+        self.load_energies(inp.energy, inp.allow)
+        self.kpref = inp.kpref
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().ocean_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- loads (module state) ----
+    def set_precision(self, precision):
+        _chk(_lib().ocean_gpu_set_precision(self.h, precision), "set_precision")
+
+    def load_bloch(self, u):
+        re, im = _planes(u)
+        _chk(_lib().ocean_gpu_load_bloch(self.h, _p(re), _p(im)), "load_bloch")
+
+    def load_bloch_raw(self, u_raw, xoff):
+        re, im = _planes(u_raw)
+        xo = np.ascontiguousarray(xoff, np.int32)
+        _chk(_lib().ocean_gpu_load_bloch_raw(self.h, _p(re), _p(im), xo.ctypes.data_as(_IP)),
+             "load_bloch_raw")
+
+    def load_w(self, W):
+        Wf = np.asfortranarray(W, np.float64)
+        _chk(_lib().ocean_gpu_load_w(self.h, _p(Wf)), "load_w")
+
+    def build_w(self, rdist, rtab, ptab, epsilon0, scale=1.0):
+        rd = np.asfortranarray(rdist, np.float64)
+        rt = np.ascontiguousarray(rtab, np.float64)
+        pt = np.ascontiguousarray(ptab, np.float64)
+        _chk(_lib().ocean_gpu_build_w(self.h, _p(rd), _p(rt), _p(pt), len(rt), epsilon0, scale),
+             "build_w")
+
+    def load_mult(self, m):
+        npmax = max(m.nproj)
+        nl = m.lmax - m.lmin + 1
+        c = m.c
+        if c.shape[2] != npmax or c.shape[3] != 2 * m.lmax + 1 or c.shape[4] != nl:
+            raise ValueError("projector array has the wrong shape")
+        cr, ci = _planes(c)
+        mpm = np.asfortranarray(m.mpm, np.float64)
+        mhr, mhi, _, _ = m.packed()
+        sr = np.asfortranarray(m.somelr, np.float64)
+        si = np.asfortranarray(m.someli, np.float64)
+        _chk(_lib().ocean_gpu_load_mult(self.h, _p(cr), _p(ci), _p(mpm), _p(mhr), _p(mhi), _p(sr),
+                                        _p(si), int(bool(m.have_spin_orbit))), "load_mult")
+
+    def load_energies(self, energy, allow):
+        er, ei = _planes(energy)
+        al = np.asfortranarray(allow, np.float64)
+        _chk(_lib().ocean_gpu_load_energies(self.h, _p(er), _p(ei), _p(al)), "load_energies")
+
+    # ---- the action ----
+    def xact(self, psi, backwards=False, hflag=None):
+        pr, pi = _planes(psi)
+        hr = np.zeros(self.shape, order="F")
+        hi = np.zeros(self.shape, order="F")
+        hf = None if hflag is None else np.ascontiguousarray(hflag, np.int32)
+        _chk(_lib().ocean_gpu_xact(self.h, _p(pr), _p(pi), _p(hr), _p(hi), int(bool(backwards)),
+                                   None if hf is None else hf.ctypes.data_as(_IP)), "xact")
+        return hr + 1j * hi
+
+    # ---- vectors ----
+    def vec_new(self, z=None):
+        i = C.c_int32()
+        _chk(_lib().ocean_gpu_vec_new(self.h, C.byref(i)), "vec_new")
+        if z is not None:
+            self.vec_put(i.value, z)
+        return i.value
+
+    def vec_put(self, vid, z):
+        re, im = _planes(z)
+        _chk(_lib().ocean_gpu_vec_put(self.h, vid, _p(re), _p(im)), "vec_put")
+
+    def vec_get(self, vid):
+        re = np.zeros(self.shape, order="F")
+        im = np.zeros(self.shape, order="F")
+        _chk(_lib().ocean_gpu_vec_get(self.h, vid, _p(re), _p(im)), "vec_get")
+        return re + 1j * im
+
+    def vec_dot(self, p, q):
+        r, i = C.c_double(), C.c_double()
+        _chk(_lib().ocean_gpu_vec_dot(self.h, p, q, C.byref(r), C.byref(i)), "vec_dot")
+        return r.value, i.value
+
+    def vec_nrm(self, p):
+        v = C.c_double()
+        _chk(_lib().ocean_gpu_vec_nrm(self.h, p, C.byref(v)), "vec_nrm")
+        return v.value
+
+    def vec_axpy(self, a, x, y):
+        a = complex(a)
+        _chk(_lib().ocean_gpu_vec_axpy(self.h, a.real, a.imag, x, y), "vec_axpy")
+
+    def vec_scal(self, a, x):
+        _chk(_lib().ocean_gpu_vec_scal(self.h, float(a), x), "vec_scal")
+
+    def vec_copy(self, dst, src):
+        _chk(_lib().ocean_gpu_vec_copy(self.h, dst, src), "vec_copy")
+
+    def xact_vec(self, vin, vout, backwards=False, hflag=None):
+        hf = None if hflag is None else np.ascontiguousarray(hflag, np.int32)
+        _chk(_lib().ocean_gpu_xact_vec(self.h, vin, vout, int(bool(backwards)),
+                                       None if hf is None else hf.ctypes.data_as(_IP)), "xact_vec")
+
+    def xact_repeat(self, vin, vout, nrep):
+        _chk(_lib().ocean_gpu_xact_vec_repeat(self.h, vin, vout, nrep), "xact_vec_repeat")
+        out = {}
+        for k in ("proj", "fft", "adj", "mult", "total"):
+            v = C.c_double()
+            _chk(_lib().ocean_gpu_last_timing(self.h, k.encode(), C.byref(v)), "last_timing")
+            out[k] = v.value
+        return out
+
+    # ---- solvers ----
+    def haydock(self, psi0, niter):
+        """Returns (real_a, real_b, imag_a) with niter+1 entries each (index 0..niter)."""
+        pr, pi = _planes(psi0)
+        a = np.zeros(niter + 1)
+        b = np.zeros(niter + 1)
+        ai = np.zeros(niter + 1)
+        done = C.c_int32()
+        rc = _lib().ocean_gpu_haydock(self.h, _p(pr), _p(pi), niter, _p(a), _p(b), _p(ai),
+                                      C.byref(done))
+        _chk(rc, "haydock")
+        return a, b, ai
+
+    def gmres(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref=None,
+              return_x=False):
+        """Rows (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref, rel_err, iters, global_iters)."""
+        kpref = self.kpref if kpref is None else kpref
+        pr, pi = _planes(hay_vec)
+        E = np.ascontiguousarray(energies_eV, np.float64)
+        out = np.zeros((len(E), 6))
+        xr = xi = None
+        if return_x:
+            xr = np.zeros(self.shape + (len(E),), order="F")
+            xi = np.zeros(self.shape + (len(E),), order="F")
+        rc = _lib().ocean_gpu_gmres(self.h, _p(pr), _p(pi), len(E), _p(E), gres_eV, gprc_eV, ffff,
+                                    depth, kpref, out.ctypes.data_as(_DP), _p(xr), _p(xi))
+        _chk(rc, "gmres")
+        if return_x:
+            return out, xr + 1j * xi
+        return out

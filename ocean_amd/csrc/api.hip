@@ -1,0 +1,1069 @@
+// C-ABI of the gfx950 core-level BSE library (declared in include/ocean_gpu.h).
+#include <algorithm>
+#include <complex>
+
+#include "action.hpp"
+#include "common.hpp"
+#include "vec.hpp"
+
+using og::kBlock;
+using og::kRedBlocks;
+using H = ocean_gpu_handle;
+using cplx = std::complex<double>;
+
+namespace {
+
+constexpr double kEV2Hartree = 0.036749322175666;   // OCEAN2/OCEAN_constants.f90:15
+constexpr double kHartree2eV = 27.211386245980;     // OCEAN2/OCEAN_constants.f90:16
+
+int grid_for(size_t n) {
+  size_t g = (n + kBlock - 1) / kBlock;
+  return (int)std::min<size_t>(std::max<size_t>(g, 1), 4096);
+}
+
+template <typename T>
+int dmalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return OCEAN_GPU_ERR_NOMEM;
+  }
+  return OCEAN_GPU_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+int ensure_scal(H* h, size_t n) {
+  if (h->scal_n >= n) return OCEAN_GPU_OK;
+  dfree(h->scal);
+  OG_TRY(dmalloc(&h->scal, n));
+  h->scal_n = n;
+  return OCEAN_GPU_OK;
+}
+
+int ensure_pin(H* h, size_t ndouble) {
+  if (h->host_pin_n >= ndouble) return OCEAN_GPU_OK;
+  if (h->host_pin) (void)hipHostFree(h->host_pin);
+  h->host_pin = nullptr;
+  if (hipHostMalloc((void**)&h->host_pin, ndouble * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    h->host_pin_n = 0;
+    return OCEAN_GPU_ERR_NOMEM;
+  }
+  h->host_pin_n = ndouble;
+  return OCEAN_GPU_OK;
+}
+
+// host planes (n) -> device interleaved complex
+int put_planes(H* h, const double* re, const double* im, double2* dst, size_t n) {
+  OG_TRY(ensure_pin(h, 2 * n));
+  double* p = h->host_pin;
+  for (size_t i = 0; i < n; ++i) {
+    p[2 * i] = re[i];
+    p[2 * i + 1] = im ? im[i] : 0.0;
+  }
+  OG_CHECK(hipMemcpyAsync(dst, p, n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  return OCEAN_GPU_OK;
+}
+
+int get_planes(H* h, const double2* src, double* re, double* im, size_t n) {
+  OG_TRY(ensure_pin(h, 2 * n));
+  double* p = h->host_pin;
+  OG_CHECK(hipMemcpyAsync(p, src, n * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < n; ++i) {
+    re[i] = p[2 * i];
+    im[i] = p[2 * i + 1];
+  }
+  return OCEAN_GPU_OK;
+}
+
+int na_tmpl(int acnt) {
+  if (acnt <= 2) return 2;
+  if (acnt <= 4) return 4;
+  if (acnt <= 6) return 6;
+  return 12;
+}
+
+bool dim_ok(int L) {
+  return L == 1 || L == 2 || L == 3 || L == 4 || L == 5 || L == 6 || L == 7 || L == 8 || L == 9 ||
+         L == 10 || L == 12 || L == 16;
+}
+
+int flags_of(const H* h, int backwards, const int32_t* hflag) {
+  int hf[6] = {1, 0, 0, 0, 1, 1};
+  if (hflag)
+    for (int i = 0; i < 6; ++i) hf[i] = hflag[i];
+  int f = 0;
+  if (h->d.long_range && hf[0] == 1) f |= og::F_LR;
+  if (h->d.mult && hf[4] == 1) f |= og::F_FG;
+  if (h->d.mult && hf[5] == 1 && h->have_so) f |= og::F_SO;
+  if (h->d.e0) f |= og::F_E0;
+  if (backwards) f |= og::F_BACK;
+  return f;
+}
+
+int check_ready(const H* h, int flags) {
+  if (!h->has_e) return OCEAN_GPU_ERR_STATE;
+  if ((flags & og::F_LR) && !(h->has_u() && h->has_w())) return OCEAN_GPU_ERR_STATE;
+  if ((flags & (og::F_FG | og::F_SO)) && !h->has_mult) return OCEAN_GPU_ERR_STATE;
+  return OCEAN_GPU_OK;
+}
+
+// --------------------------------------------------------------------------
+// The action, device pointer to device pointer, on h->stream.
+// --------------------------------------------------------------------------
+template <typename T2, int NA>
+int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename og::Scalar<T2>::T* W,
+                       hipEvent_t* ev) {
+  const int ns = h->ns, acnt = (ns == 2) ? h->na / 2 : h->na;
+  dim3 g1((h->nx + kBlock - 1) / kBlock, h->nk, ns);
+  size_t sm1 = (size_t)acnt * h->nb * sizeof(T2);
+  og::k_proj<T2, NA><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
+                                                     h->na, ns);
+  OG_CHECK(hipGetLastError());
+  if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
+  const int ncol = h->nx * h->na;
+  const int cap = (int)(65536 / sizeof(T2));
+  int G = std::max(1, cap / h->nk);
+  G = std::min(G, ncol);
+  dim3 g2((ncol + G - 1) / G);
+  size_t sm2 = (size_t)h->nk * G * sizeof(T2);
+  og::k_fft_w<T2><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
+                                                  h->km[2]);
+  OG_CHECK(hipGetLastError());
+  if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
+  return OCEAN_GPU_OK;
+}
+
+template <typename T2, int NA>
+int launch_adj(H* h, const double2* in, double2* out, const T2* u, const T2* phi, int flags) {
+  const int ns = h->ns;
+  int XC = h->nx;
+  const size_t budget = 60 * 1024 - (size_t)h->na * h->nb * sizeof(double2);
+  const int cap = (int)(budget / (NA * sizeof(T2)));
+  if (XC > cap) XC = std::max(64, (cap / 64) * 64);
+  size_t sm = (size_t)h->na * h->nb * sizeof(double2) + (size_t)XC * NA * sizeof(T2);
+  og::k_adj<T2, NA><<<h->nk, kBlock, sm, h->stream>>>(u, phi, in, h->energy, h->allow, h->cproj,
+                                                      h->hmult, h->Smat, out, h->nb, h->nk, h->nx,
+                                                      h->na, ns, h->P, XC, flags);
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+template <typename T2, int NA>
+int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* phi,
+           const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
+  if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
+  if (flags & og::F_LR) {
+    OG_TRY((launch_lr_proj_fft<T2, NA>(h, in, u, phi, W, ev)));
+  } else if (ev) {
+    OG_CHECK(hipEventRecord(ev[1], h->stream));
+    OG_CHECK(hipEventRecord(ev[2], h->stream));
+  }
+  if (flags & og::F_FG) {
+    og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->stream>>>(h->cproj, in, h->allow, h->apart, h->nb,
+                                                             h->nk, h->na, h->ns, h->P);
+    OG_CHECK(hipGetLastError());
+    const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
+    og::k_mred<<<1, kBlock, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
+    OG_CHECK(hipGetLastError());
+  }
+  if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
+  OG_TRY((launch_adj<T2, NA>(h, in, out, u, phi, flags)));
+  if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
+  return OCEAN_GPU_OK;
+}
+
+int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr) {
+  const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
+  const int NA = na_tmpl(acnt);
+  const bool mixed = (h->precision == OCEAN_GPU_MIXED);
+#define OG_DISPATCH(NAV)                                                              \
+  if (NA == NAV)                                                                      \
+    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->phi32, h->W32, ev) \
+                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->phi64, h->W64, ev);
+  OG_DISPATCH(2)
+  OG_DISPATCH(4)
+  OG_DISPATCH(6)
+  OG_DISPATCH(12)
+#undef OG_DISPATCH
+  return OCEAN_GPU_ERR_ARG;
+}
+
+int ensure_precision_buffers(H* h) {
+  const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
+  const size_t nphi = (size_t)h->nk * h->nx * h->na;
+  const size_t nw = (size_t)h->nk * h->nx;
+  if (h->precision == OCEAN_GPU_MIXED) {
+    if (!h->u32) OG_TRY(dmalloc(&h->u32, nu));
+    if (!h->phi32) OG_TRY(dmalloc(&h->phi32, nphi));
+    if (!h->W32) OG_TRY(dmalloc(&h->W32, nw));
+  } else {
+    if (!h->u64) OG_TRY(dmalloc(&h->u64, nu));
+    if (!h->phi64) OG_TRY(dmalloc(&h->phi64, nphi));
+    if (!h->W64) OG_TRY(dmalloc(&h->W64, nw));
+  }
+  return OCEAN_GPU_OK;
+}
+
+double2* vec_ptr(H* h, int32_t id) {
+  if (id < 0 || (size_t)id >= h->vecs.size()) return nullptr;
+  return h->vecs[id];
+}
+
+int reduce_to_host(H* h, int nv, double* out) {
+  og::k_finish<<<1, 64, 0, h->stream>>>(h->red, kRedBlocks, nv, h->red + kRedBlocks * 8);
+  OG_CHECK(hipGetLastError());
+  OG_CHECK(hipMemcpyAsync(out, h->red + kRedBlocks * 8, nv * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  return OCEAN_GPU_OK;
+}
+
+int dev_dot(H* h, const double2* p, const double2* q, double* r, double* i) {
+  og::k_dot_part<<<kRedBlocks, kBlock, 0, h->stream>>>(p, q, h->nvec(), h->red);
+  OG_CHECK(hipGetLastError());
+  double o[2];
+  OG_TRY(reduce_to_host(h, 2, o));
+  *r = o[0];
+  if (i) *i = o[1];
+  return OCEAN_GPU_OK;
+}
+
+int dev_nrm(H* h, const double2* p, double* v) {
+  og::k_nrm_part<<<kRedBlocks, kBlock, 0, h->stream>>>(p, h->nvec(), h->red);
+  OG_CHECK(hipGetLastError());
+  return reduce_to_host(h, 1, v);
+}
+
+// ---------------------------------------------------------------------------
+// GMRES helpers
+// ---------------------------------------------------------------------------
+// <W_i|v1>, <W_i|v2> for i < m: partials [blk][i][4]
+__global__ __launch_bounds__(kBlock) void k_multidot2(const double2* const* __restrict__ Wl, int m,
+                                                      const double2* __restrict__ v1,
+                                                      const double2* __restrict__ v2, size_t n,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4 * 4];
+  for (int i = 0; i < m; ++i) {
+    const double2* w = Wl[i];
+    double v[4] = {0, 0, 0, 0};
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+      const double2 a = w[e], b = v1[e], c = v2[e];
+      v[0] = fma(a.x, b.x, v[0]); v[0] = fma(a.y, b.y, v[0]);
+      v[1] = fma(a.x, b.y, v[1]); v[1] = fma(-a.y, b.x, v[1]);
+      v[2] = fma(a.x, c.x, v[2]); v[2] = fma(a.y, c.y, v[2]);
+      v[3] = fma(a.x, c.y, v[3]); v[3] = fma(-a.y, c.x, v[3]);
+    }
+    og::block_sum<4>(v, lds);
+    if (threadIdx.x == 0) {
+      double* o = part + ((size_t)blockIdx.x * m + i) * 4;
+      o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_multifinish(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * nv + j];
+    out[j] = s;
+  }
+}
+
+// y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341)
+__global__ void k_multiaxpy(const double2* const* __restrict__ Wl, const double2* __restrict__ c, int m,
+                            double2* __restrict__ y, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double2 yv = y[e];
+    for (int i = 0; i < m; ++i) {
+      const double2 w = Wl[i][e];
+      const double2 ci = c[i];
+      yv.x += ci.x * w.x; yv.y += ci.x * w.y;
+      yv.x += -ci.y * w.y; yv.y += ci.y * w.x;
+    }
+    y[e] = yv;
+  }
+}
+
+// y = (er + i ei) x - y  (OCEAN_psi_axmy, OCEAN_psi.f90:1468-1562)
+__global__ void k_axmy(const double2* __restrict__ x, double2* __restrict__ y, double er, double ei, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 xv = x[e];
+    double2 yv = y[e];
+    double br = er * xv.x - yv.x;
+    double bi = er * xv.y - yv.y;
+    yv.x = br - ei * xv.y;
+    yv.y = bi + ei * xv.x;
+    y[e] = yv;
+  }
+}
+
+// z = x .* y (complex)  (OCEAN_psi_3element_mult)
+__global__ void k_cmul3(double2* __restrict__ z, const double2* __restrict__ x, const double2* __restrict__ y, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    z[e] = og::cmul(x[e], y[e]);
+}
+
+// g = ax - rhs ; partials of |g|^2 and <rhs|x>
+__global__ __launch_bounds__(kBlock) void k_resid(const double2* __restrict__ ax, const double2* __restrict__ rhs,
+                                                  const double2* __restrict__ x, double2* __restrict__ g, size_t n,
+                                                  double* __restrict__ part) {
+  __shared__ double lds[4 * 3];
+  double v[3] = {0, 0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = ax[e], b = rhs[e], xv = x[e];
+    const double2 gv = make_double2(a.x - b.x, a.y - b.y);
+    g[e] = gv;
+    v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
+    v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
+    v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
+  }
+  og::block_sum<3>(v, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = v[0]; part[3 * blockIdx.x + 1] = v[1]; part[3 * blockIdx.x + 2] = v[2];
+  }
+}
+
+// pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2)  (OCEAN_psi_min_set_prec, :2907-2994)
+__global__ void k_set_prec(const double2* __restrict__ h1, double2* __restrict__ pc, double E, double gprc, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 hv = h1[e];
+    const double dr = E - hv.x, di = -hv.y;
+    const double2 ct = make_double2(dr * dr - di * di + gprc * gprc, 2.0 * dr * di);
+    const double2 num = make_double2(E - hv.x, -(hv.y + gprc));
+    pc[e] = og::c_div(num, ct);
+  }
+}
+
+// a .* psi for the sfact copy (OCEAN_energies_sfact_copy, OCEAN_energies.f90:40-53)
+__global__ void k_mask(double2* __restrict__ y, const double2* __restrict__ x, const double* __restrict__ m, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double2 v = x[e];
+    v.x *= m[e]; v.y *= m[e];
+    y[e] = v;
+  }
+}
+
+// Packed Hermitian solve by LDL^H (no pivoting), returning the 1-based index of a
+// zero pivot as zhptrf's info would (OCEAN_gmres.f90:303-321).
+int hermitian_solve(const std::vector<cplx>& Cup, int m, std::vector<cplx>& rhs) {
+  // Cup(i,j) for i <= j at j*(j+1)/2 + i
+  auto C = [&](int i, int j) -> cplx {
+    if (i <= j) return Cup[(size_t)j * (j + 1) / 2 + i];
+    return std::conj(Cup[(size_t)i * (i + 1) / 2 + j]);
+  };
+  std::vector<cplx> L((size_t)m * m, cplx(0, 0));
+  std::vector<double> D(m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    cplx s = C(j, j);
+    for (int k = 0; k < j; ++k) s -= L[(size_t)j * m + k] * std::conj(L[(size_t)j * m + k]) * D[k];
+    D[j] = s.real();
+    if (D[j] == 0.0) return j + 1;
+    L[(size_t)j * m + j] = 1.0;
+    for (int i = j + 1; i < m; ++i) {
+      cplx t = C(i, j);
+      for (int k = 0; k < j; ++k) t -= L[(size_t)i * m + k] * std::conj(L[(size_t)j * m + k]) * D[k];
+      L[(size_t)i * m + j] = t / D[j];
+    }
+  }
+  for (int i = 0; i < m; ++i) {
+    for (int k = 0; k < i; ++k) rhs[i] -= L[(size_t)i * m + k] * rhs[k];
+  }
+  for (int i = 0; i < m; ++i) rhs[i] /= D[i];
+  for (int i = m - 1; i >= 0; --i) {
+    for (int k = i + 1; k < m; ++k) rhs[i] -= std::conj(L[(size_t)k * m + i]) * rhs[k];
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* ocean_gpu_strerror(int code) {
+  switch (code) {
+    case OCEAN_GPU_OK: return "ok";
+    case OCEAN_GPU_ERR_NAN: return "NaN detected in the Haydock recursion";
+    case OCEAN_GPU_ERR_ARG: return "invalid argument or inconsistent sizes";
+    case OCEAN_GPU_ERR_STATE: return "required state (bloch, W, multiplet or energies) not loaded";
+    case OCEAN_GPU_ERR_HIP: return "HIP runtime error";
+    case OCEAN_GPU_ERR_NOMEM: return "device allocation failed";
+    case OCEAN_GPU_ERR_UNSUPPORTED: return "unsupported option";
+    default: return code > 0 ? "GMRES least-squares pivot is zero (zhptrf info)" : "unknown error";
+  }
+}
+
+int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
+  if (!d || !out) return OCEAN_GPU_ERR_ARG;
+  *out = nullptr;
+  if (d->num_bands <= 0 || d->nspn < 1 || d->nspn > 2 || d->lc < 0 || d->lc > 1) return OCEAN_GPU_ERR_ARG;
+  for (int i = 0; i < 3; ++i)
+    if (d->kmesh[i] <= 0 || d->xmesh[i] <= 0 || !dim_ok(d->kmesh[i])) return OCEAN_GPU_ERR_ARG;
+  if (d->lmin < 0 || d->lmax < d->lmin || d->lmax - d->lmin >= 8) return OCEAN_GPU_ERR_ARG;
+  if (d->lvl > d->lmin || d->lvh < d->lmax) return OCEAN_GPU_ERR_ARG;
+  if (d->celvol <= 0.0) return OCEAN_GPU_ERR_ARG;
+  H* h = new H();
+  h->d = *d;
+  h->nb = d->num_bands;
+  h->km[0] = d->kmesh[0]; h->km[1] = d->kmesh[1]; h->km[2] = d->kmesh[2];
+  h->nk = d->kmesh[0] * d->kmesh[1] * d->kmesh[2];
+  h->nx = d->xmesh[0] * d->xmesh[1] * d->xmesh[2];
+  h->na = 4 * (2 * d->lc + 1);
+  h->ns = d->nspn;
+  h->precision = d->precision;
+  int P = 0;
+  for (int l = d->lmin; l <= d->lmax; ++l) {
+    if (d->nproj[l - d->lmin] < 0) { delete h; return OCEAN_GPU_ERR_ARG; }
+    P += (2 * l + 1) * d->nproj[l - d->lmin];
+  }
+  h->P = P;
+  if (h->na * std::max(P, 1) > og::kMaxAlpha * 64) { delete h; return OCEAN_GPU_ERR_ARG; }
+  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return OCEAN_GPU_ERR_HIP;
+  }
+  int rc = OCEAN_GPU_OK;
+  const size_t nv = h->nvec();
+  if (rc == 0) rc = dmalloc(&h->energy, nv);
+  if (rc == 0) rc = dmalloc(&h->allow, nv);
+  if (rc == 0) rc = dmalloc(&h->red, (size_t)kRedBlocks * 8 + 64);
+  if (rc == 0) rc = dmalloc(&h->tmp_in, nv);
+  if (rc == 0) rc = dmalloc(&h->tmp_out, nv);
+  if (rc == 0) rc = ensure_precision_buffers(h);
+  if (rc != 0) {
+    ocean_gpu_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_destroy(ocean_gpu_handle* h) {
+  if (!h) return OCEAN_GPU_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  dfree(h->u64); dfree(h->u32); dfree(h->W64); dfree(h->W32); dfree(h->phi64); dfree(h->phi32);
+  dfree(h->w_i0); dfree(h->w_w0); dfree(h->w_w1);
+  dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
+  dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
+  for (auto& v : h->vecs) dfree(v);
+  if (h->host_pin) (void)hipHostFree(h->host_pin);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision) {
+  if (!h || (precision != OCEAN_GPU_FP64 && precision != OCEAN_GPU_MIXED)) return OCEAN_GPU_ERR_ARG;
+  if (precision == h->precision) return OCEAN_GPU_OK;
+  h->precision = precision;
+  OG_TRY(ensure_precision_buffers(h));
+  const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
+  const size_t nw = (size_t)h->nk * h->nx;
+  // FP64 -> FP32 conversion of loaded state; the FP32 state cannot be promoted
+  // back (an FP64 reload is required), so the FP64 copy is kept when it exists.
+  if (precision == OCEAN_GPU_MIXED) {
+    if (h->u64_ok && !h->u32_ok) {
+      og::k_cast<float><<<4096, kBlock, 0, h->stream>>>((const double*)h->u64, (float*)h->u32, 2 * nu);
+      h->u32_ok = true;
+    }
+    if (h->w64_ok && !h->w32_ok) {
+      og::k_cast<float><<<4096, kBlock, 0, h->stream>>>(h->W64, h->W32, nw);
+      h->w32_ok = true;
+    }
+  }
+  OG_CHECK(hipGetLastError());
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  return OCEAN_GPU_OK;
+}
+
+static int load_u_common(H* h, const double* re, const double* im, const int32_t* xoff) {
+  if (!h || !re || !im) return OCEAN_GPU_ERR_ARG;
+  const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
+  double *dre = nullptr, *dim_ = nullptr;
+  int* dxo = nullptr;
+  OG_TRY(dmalloc(&dre, nu));
+  if (dmalloc(&dim_, nu) != 0) { dfree(dre); return OCEAN_GPU_ERR_NOMEM; }
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(dre, re, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dim_, im, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0 && xoff) {
+    rc = dmalloc(&dxo, (size_t)3 * h->nx);
+    if (rc == 0 && hipMemcpy(dxo, xoff, 3 * h->nx * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0) {
+    // the FP64 form keeps an FP64 copy; the mixed form keeps FP32 only
+    dim3 g((h->nb + 31) / 32, (h->nx + 31) / 32, h->ns * h->nk);
+    if (h->precision == OCEAN_GPU_MIXED) {
+      if (xoff)
+        og::k_load_u_phase<float2><<<g, 256, 0, h->stream>>>(dre, dim_, dxo, h->u32, h->nb, h->nk, h->nx,
+            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2]);
+      else
+        og::k_load_u<float2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u32, h->nb, h->nk, h->nx);
+    } else {
+      if (xoff)
+        og::k_load_u_phase<double2><<<g, 256, 0, h->stream>>>(dre, dim_, dxo, h->u64, h->nb, h->nk, h->nx,
+            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2]);
+      else
+        og::k_load_u<double2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u64, h->nb, h->nk, h->nx);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(dre); dfree(dim_); dfree(dxo);
+  if (rc == 0) {
+    if (h->precision == OCEAN_GPU_MIXED) { h->u32_ok = true; h->u64_ok = false; }
+    else { h->u64_ok = true; h->u32_ok = false; }
+  }
+  return rc;
+}
+
+int ocean_gpu_load_bloch(ocean_gpu_handle* h, const double* u_re, const double* u_im) {
+  return load_u_common(h, u_re, u_im, nullptr);
+}
+
+int ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const double* u_im,
+                             const int32_t* xoff) {
+  if (!xoff) return OCEAN_GPU_ERR_ARG;
+  return load_u_common(h, u_re, u_im, xoff);
+}
+
+static int finish_w(H* h, const double* dW64) {
+  const size_t nw = (size_t)h->nk * h->nx;
+  if (h->precision == OCEAN_GPU_MIXED) {
+    og::k_cast<float><<<grid_for(nw), kBlock, 0, h->stream>>>(dW64, h->W32, nw);
+    OG_CHECK(hipGetLastError());
+    h->w32_ok = true;
+    h->w64_ok = false;
+  } else {
+    OG_CHECK(hipMemcpyAsync(h->W64, dW64, nw * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    h->w64_ok = true;
+    h->w32_ok = false;
+  }
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_load_w(ocean_gpu_handle* h, const double* W) {
+  if (!h || !W) return OCEAN_GPU_ERR_ARG;
+  const size_t nw = (size_t)h->nk * h->nx;
+  double* dW = nullptr;
+  OG_TRY(dmalloc(&dW, nw));
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(dW, W, nw * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) rc = finish_w(h, dW);
+  dfree(dW);
+  return rc;
+}
+
+int ocean_gpu_build_w(ocean_gpu_handle* h, const double* rdist, const double* rtab, const double* ptab,
+                      int32_t nptab, double epsilon0, double scale) {
+  if (!h || !rdist || !rtab || !ptab || nptab < 2 || epsilon0 == 0.0) return OCEAN_GPU_ERR_ARG;
+  for (int i = 1; i < nptab; ++i)
+    if (!(rtab[i] > rtab[i - 1])) return OCEAN_GPU_ERR_ARG;
+  const size_t nw = (size_t)h->nk * h->nx;
+  if (!h->w_i0) { OG_TRY(dmalloc(&h->w_i0, nw)); OG_TRY(dmalloc(&h->w_w0, nw)); OG_TRY(dmalloc(&h->w_w1, nw)); }
+  double *dr = nullptr, *dt = nullptr, *dW = nullptr;
+  OG_TRY(dmalloc(&dr, nw));
+  int rc = dmalloc(&dt, 2 * (size_t)nptab);
+  if (rc == 0) rc = dmalloc(&dW, nw);
+  if (rc == 0 && (hipMemcpy(dr, rdist, nw * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(dt, rtab, nptab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(dt + nptab, ptab, nptab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_build_w<<<grid_for(nw), kBlock, 0, h->stream>>>(dr, dt, dt + nptab, nptab, 1.0 / epsilon0, scale, nw,
+                                                         dW, h->w_i0, h->w_w0, h->w_w1);
+    if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0) rc = finish_w(h, dW);
+  if (rc == 0) h->nptab = nptab;
+  dfree(dr); dfree(dt); dfree(dW);
+  return rc;
+}
+
+int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* mpci, const double* mpm,
+                        const double* mhr, const double* mhi, const double* somelr, const double* someli,
+                        int32_t have_so) {
+  if (!h || !mpcr || !mpci || !mpm || !mhr || !mhi || !somelr || !someli) return OCEAN_GPU_ERR_ARG;
+  const ocean_bse_desc& d = h->d;
+  const int nb = h->nb, nk = h->nk, ns = h->ns, na = h->na, P = h->P, lc = d.lc;
+  const int lmin = d.lmin, lmax = d.lmax, nl = lmax - lmin + 1;
+  int npmax = 0;
+  for (int l = lmin; l <= lmax; ++l) npmax = std::max(npmax, d.nproj[l - lmin]);
+  const int nm = 2 * lmax + 1;
+  // compact projector index p = (l, m, nu) in the reference's ip order (OCEAN_multiplet.f90:451-462)
+  std::vector<int> p_l, p_m, p_nu, p_off(nl + 1, 0);
+  for (int l = lmin; l <= lmax; ++l) {
+    p_off[l - lmin] = (int)p_l.size();
+    for (int m = -l; m <= l; ++m)
+      for (int nu = 0; nu < d.nproj[l - lmin]; ++nu) { p_l.push_back(l); p_m.push_back(m); p_nu.push_back(nu); }
+  }
+  // c[s][k][p][n]
+  std::vector<double2> c((size_t)ns * nk * P * nb);
+  for (int s = 0; s < ns; ++s)
+    for (int k = 0; k < nk; ++k)
+      for (int p = 0; p < P; ++p) {
+        const int li = p_l[p] - lmin, mi = p_m[p] + lmax, nu = p_nu[p];
+        const size_t base = ((((size_t)s * nl + li) * nm + mi) * npmax + nu) * nk + k;
+        for (int n = 0; n < nb; ++n) {
+          const size_t ref = base * nb + n;
+          c[(((size_t)s * nk + k) * P + p) * nb + n] = make_double2(mpcr[ref], mpci[ref]);
+        }
+      }
+  // K = -mpm (direct, diagonal in (l,m,alpha)) + X (exchange), row i = a*P + p
+  const int nj = na * P;
+  std::vector<cplx> K((size_t)nj * nj, cplx(0, 0));
+  for (int a = 0; a < na; ++a)
+    for (int p = 0; p < P; ++p)
+      for (int q = 0; q < P; ++q) {
+        if (p_l[p] != p_l[q] || p_m[p] != p_m[q]) continue;
+        const int li = p_l[p] - lmin;
+        const double v = mpm[((size_t)li * npmax + p_nu[q]) * npmax + p_nu[p]];  // mpm(mu=nu_p, nu=nu_q, l)
+        K[(size_t)(a * P + p) * nj + (a * P + q)] -= v;
+      }
+  size_t jb = 0;
+  for (int l = d.lvl; l <= d.lvh; ++l) {
+    const int npl = (l >= lmin && l <= lmax) ? d.nproj[l - lmin] : 0;
+    const int mham = 4 * (2 * lc + 1) * (2 * l + 1) * npl;
+    if (l >= lmin && l <= lmax) {
+      for (int ii = 0; ii < mham; ++ii) {
+        const int nu_i = ii % npl, m_i = (ii / npl) % (2 * l + 1) - l, a_i = ii / (npl * (2 * l + 1));
+        const int p_i = p_off[l - lmin] + (m_i + l) * npl + nu_i;
+        for (int jj = 0; jj < mham; ++jj) {
+          const int nu_j = jj % npl, m_j = (jj / npl) % (2 * l + 1) - l, a_j = jj / (npl * (2 * l + 1));
+          const int p_j = p_off[l - lmin] + (m_j + l) * npl + nu_j;
+          const size_t j1 = jb + (size_t)jj + (size_t)ii * mham;  // mh(jj, ii)
+          K[(size_t)(a_i * P + p_i) * nj + (a_j * P + p_j)] += cplx(mhr[j1], mhi[j1]);
+        }
+      }
+    }
+    jb += (size_t)mham * mham;
+  }
+  std::vector<double2> Kd((size_t)nj * nj);
+  for (size_t i = 0; i < Kd.size(); ++i) Kd[i] = make_double2(K[i].real(), K[i].imag());
+  std::vector<double2> S((size_t)na * na);
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < na; ++b) S[a * na + b] = make_double2(somelr[(size_t)b * na + a], someli[(size_t)b * na + a]);
+  dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
+  OG_TRY(dmalloc(&h->cproj, c.size()));
+  OG_TRY(dmalloc(&h->Kmat, Kd.size()));
+  OG_TRY(dmalloc(&h->apart, (size_t)nk * nj));
+  OG_TRY(dmalloc(&h->hmult, (size_t)nj));
+  OG_TRY(dmalloc(&h->Smat, S.size()));
+  OG_CHECK(hipMemcpy(h->cproj, c.data(), c.size() * sizeof(double2), hipMemcpyHostToDevice));
+  OG_CHECK(hipMemcpy(h->Kmat, Kd.data(), Kd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  OG_CHECK(hipMemcpy(h->Smat, S.data(), S.size() * sizeof(double2), hipMemcpyHostToDevice));
+  h->have_so = have_so ? 1 : 0;
+  h->has_mult = true;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_load_energies(ocean_gpu_handle* h, const double* e_re, const double* e_im, const double* allow) {
+  if (!h || !e_re || !allow) return OCEAN_GPU_ERR_ARG;
+  const size_t nv = h->nvec();
+  OG_TRY(put_planes(h, e_re, e_im, h->energy, nv));
+  OG_CHECK(hipMemcpy(h->allow, allow, nv * sizeof(double), hipMemcpyHostToDevice));
+  h->has_e = true;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_xact(ocean_gpu_handle* h, const double* psi_re, const double* psi_im, double* hpsi_re,
+                   double* hpsi_im, int32_t backwards, const int32_t* hflag) {
+  if (!h || !psi_re || !psi_im || !hpsi_re || !hpsi_im) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, backwards, hflag);
+  OG_TRY(check_ready(h, f));
+  const size_t nv = h->nvec();
+  OG_TRY(put_planes(h, psi_re, psi_im, h->tmp_in, nv));
+  OG_TRY(xact_dev(h, h->tmp_in, h->tmp_out, f));
+  return get_planes(h, h->tmp_out, hpsi_re, hpsi_im, nv);
+}
+
+// ---- vectors ----
+int ocean_gpu_vec_new(ocean_gpu_handle* h, int32_t* id) {
+  if (!h || !id) return OCEAN_GPU_ERR_ARG;
+  double2* p = nullptr;
+  OG_TRY(dmalloc(&p, h->nvec()));
+  og::k_fill<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(p, make_double2(0.0, 0.0), h->nvec());
+  OG_CHECK(hipGetLastError());
+  for (size_t i = 0; i < h->vecs.size(); ++i)
+    if (!h->vecs[i]) { h->vecs[i] = p; *id = (int32_t)i; return OCEAN_GPU_OK; }
+  h->vecs.push_back(p);
+  *id = (int32_t)(h->vecs.size() - 1);
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_free(ocean_gpu_handle* h, int32_t id) {
+  if (!h || !vec_ptr(h, id)) return OCEAN_GPU_ERR_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  dfree(h->vecs[id]);
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_put(ocean_gpu_handle* h, int32_t id, const double* re, const double* im) {
+  double2* p = h ? vec_ptr(h, id) : nullptr;
+  if (!p || !re || !im) return OCEAN_GPU_ERR_ARG;
+  return put_planes(h, re, im, p, h->nvec());
+}
+
+int ocean_gpu_vec_get(ocean_gpu_handle* h, int32_t id, double* re, double* im) {
+  double2* p = h ? vec_ptr(h, id) : nullptr;
+  if (!p || !re || !im) return OCEAN_GPU_ERR_ARG;
+  return get_planes(h, p, re, im, h->nvec());
+}
+
+int ocean_gpu_vec_dot(ocean_gpu_handle* h, int32_t p, int32_t q, double* rval, double* ival) {
+  if (!h || !vec_ptr(h, p) || !vec_ptr(h, q) || !rval) return OCEAN_GPU_ERR_ARG;
+  return dev_dot(h, vec_ptr(h, p), vec_ptr(h, q), rval, ival);
+}
+
+int ocean_gpu_vec_nrm(ocean_gpu_handle* h, int32_t p, double* val) {
+  if (!h || !vec_ptr(h, p) || !val) return OCEAN_GPU_ERR_ARG;
+  return dev_nrm(h, vec_ptr(h, p), val);
+}
+
+int ocean_gpu_vec_axpy(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y) {
+  if (!h || !vec_ptr(h, x) || !vec_ptr(h, y)) return OCEAN_GPU_ERR_ARG;
+  og::k_axpy<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(rval, ival, vec_ptr(h, x), vec_ptr(h, y), h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_scal(ocean_gpu_handle* h, double rval, int32_t x) {
+  if (!h || !vec_ptr(h, x)) return OCEAN_GPU_ERR_ARG;
+  og::k_scal<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(rval, vec_ptr(h, x), h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_copy(ocean_gpu_handle* h, int32_t dst, int32_t src) {
+  if (!h || !vec_ptr(h, dst) || !vec_ptr(h, src)) return OCEAN_GPU_ERR_ARG;
+  og::k_copy<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(vec_ptr(h, dst), vec_ptr(h, src), h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_xact_vec(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t backwards, const int32_t* hflag) {
+  if (!h || !vec_ptr(h, in) || !vec_ptr(h, out) || in == out) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, backwards, hflag);
+  OG_TRY(check_ready(h, f));
+  OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f));
+  OG_CHECK(hipStreamSynchronize(h->stream));
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_xact_vec_repeat(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t nrep) {
+  if (!h || !vec_ptr(h, in) || !vec_ptr(h, out) || in == out || nrep <= 0) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  // Two timing passes on the library stream: (1) nrep back-to-back actions
+  // bracketed by one event pair (throughput); (2) nrep actions with events
+  // between the kernel families (per-family average duration).
+  hipEvent_t e0, e1, ev[5];
+  OG_CHECK(hipEventCreate(&e0));
+  OG_CHECK(hipEventCreate(&e1));
+  for (auto& e : ev) OG_CHECK(hipEventCreate(&e));
+  OG_CHECK(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < nrep; ++r) OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f));
+  OG_CHECK(hipEventRecord(e1, h->stream));
+  OG_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  OG_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  h->t_total = ms / nrep;
+  double tp = 0, tf = 0, tm = 0, ta = 0;
+  for (int r = 0; r < nrep; ++r) {
+    OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f, ev));
+    OG_CHECK(hipEventSynchronize(ev[4]));
+    float a = 0, b = 0, c = 0, d = 0;
+    OG_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    OG_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    OG_CHECK(hipEventElapsedTime(&c, ev[2], ev[3]));
+    OG_CHECK(hipEventElapsedTime(&d, ev[3], ev[4]));
+    tp += a; tf += b; tm += c; ta += d;
+  }
+  h->t_proj = tp / nrep; h->t_fft = tf / nrep; h->t_mult = tm / nrep; h->t_adj = ta / nrep;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_last_timing(ocean_gpu_handle* h, const char* name, double* ms) {
+  if (!h || !name || !ms) return OCEAN_GPU_ERR_ARG;
+  const std::string n(name);
+  if (n == "proj") *ms = h->t_proj;
+  else if (n == "fft") *ms = h->t_fft;
+  else if (n == "adj") *ms = h->t_adj;
+  else if (n == "mult") *ms = h->t_mult;
+  else if (n == "total") *ms = h->t_total;
+  else return OCEAN_GPU_ERR_ARG;
+  return OCEAN_GPU_OK;
+}
+
+// ---- Haydock ----
+int ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
+                      double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
+  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  const size_t nv = h->nvec();
+  OG_TRY(ensure_scal(h, 3 * (size_t)(niter + 1) + 2 * kRedBlocks + kRedBlocks));
+  double* da = h->scal;
+  double* dai = da + (niter + 1);
+  double* db = dai + (niter + 1);
+  double* part1 = db + (niter + 1);
+  double* part2 = part1 + 2 * kRedBlocks;
+  double2 *v0 = nullptr, *v1 = nullptr, *v2 = nullptr;
+  OG_TRY(dmalloc(&v0, nv));
+  if (dmalloc(&v1, nv) || dmalloc(&v2, nv)) { dfree(v0); dfree(v1); return OCEAN_GPU_ERR_NOMEM; }
+  int rc = put_planes(h, psi0_re, psi0_im, v0, nv);
+  double2 *psi = v0, *old = v1, *hv = v2;
+  if (rc == 0) {
+    og::k_fill<<<grid_for(nv), kBlock, 0, h->stream>>>(old, make_double2(0.0, 0.0), nv);
+    og::k_fill_d<<<grid_for(3 * (size_t)(niter + 1)), kBlock, 0, h->stream>>>(da, 0.0, 3 * (size_t)(niter + 1));
+    for (int it = 1; it <= niter && rc == 0; ++it) {
+      rc = xact_dev(h, psi, hv, f);
+      if (rc) break;
+      og::k_hay1<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, old, psi, db + (it - 1), nv, part1);
+      og::k_hay2<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, psi, part1, kRedBlocks, nv, da + (it - 1),
+                                                      dai + (it - 1), part2);
+      og::k_hay3<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, part2, kRedBlocks, nv, db + it);
+      if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+      double2* t = old;
+      old = psi;
+      psi = hv;
+      hv = t;
+    }
+  }
+  std::vector<double> hb(3 * (size_t)(niter + 1));
+  if (rc == 0) {
+    if (hipMemcpyAsync(hb.data(), da, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(v0); dfree(v1); dfree(v2);
+  if (rc) return rc;
+  int done = niter;
+  for (int i = 0; i < niter; ++i)
+    if (hb[i] != hb[i]) { done = i + 1; rc = OCEAN_GPU_ERR_NAN; break; }
+  for (int i = 0; i <= niter; ++i) {
+    real_a[i] = hb[i];
+    real_b[i] = hb[2 * (size_t)(niter + 1) + i];
+    if (imag_a) imag_a[i] = hb[(size_t)(niter + 1) + i];
+  }
+  if (niter_done) *niter_done = done;
+  return rc;
+}
+
+// ---- spectrum ----
+int ocean_gpu_spectrum(const double* real_a, const double* real_b, const double* imag_a, int32_t niter,
+                       double kpref, double gam, double el, double eh, int32_t ne, int32_t literal_cmplx,
+                       double* out) {
+  if (!real_a || !real_b || !out || niter <= 0 || ne <= 0) return OCEAN_GPU_ERR_ARG;
+  double *d = nullptr;
+  const size_t n1 = (size_t)niter + 1;
+  OG_TRY(dmalloc(&d, 3 * n1 + 5 * ((size_t)ne + 1)));
+  std::vector<double> ia(n1, 0.0);
+  if (imag_a) std::copy(imag_a, imag_a + n1, ia.begin());
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(d, real_a, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + n1, real_b, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + 2 * n1, ia.data(), n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_cfrac<<<(ne + 1 + 127) / 128, 128>>>(d, d + n1, d + 2 * n1, niter, kpref, gam, el, eh, ne,
+                                               literal_cmplx, kHartree2eV, d + 3 * n1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(out, d + 3 * n1, 5 * ((size_t)ne + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(d);
+  return rc;
+}
+
+// ---- GMRES ----
+int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im, int32_t nE,
+                    const double* energies_eV, double gres_eV, double gprc_eV, double ffff, int32_t depth,
+                    double kpref, double* out, double* x_re, double* x_im) {
+  if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !out) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  const size_t nv = h->nvec();
+  const int grid = grid_for(nv);
+  const double gres = gres_eV * kEV2Hartree, gprc = gprc_eV * kEV2Hartree;
+  // vectors: rhs, h1, x, ax, g, pcdiv, u[1..depth], au[1..depth]
+  const int nfix = 6;
+  std::vector<double2*> V(nfix + 2 * depth, nullptr);
+  int rc = OCEAN_GPU_OK;
+  for (auto& p : V)
+    if (rc == 0) rc = dmalloc(&p, nv);
+  double2** dlist = nullptr;
+  double2* dcoef = nullptr;
+  double* part = nullptr;
+  if (rc == 0) rc = dmalloc(&dlist, 2 * (size_t)depth + 2);
+  if (rc == 0) rc = dmalloc(&dcoef, (size_t)depth);
+  if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 16);
+  auto cleanup = [&]() {
+    for (auto& p : V) dfree(p);
+    dfree(dlist); dfree(dcoef); dfree(part);
+  };
+  if (rc) { cleanup(); return rc; }
+  double2 *rhs = V[0], *h1 = V[1], *x = V[2], *ax = V[3], *g = V[4], *pc = V[5];
+  double2** U = &V[nfix];            // U[i], i = 0..depth-1  (reference u_matrix(i+1))
+  double2** AU = &V[nfix + depth];   // AU[i]
+  std::vector<double2*> hlist(2 * depth);
+  for (int i = 0; i < depth; ++i) { hlist[i] = AU[i]; hlist[depth + i] = U[i]; }
+  std::vector<cplx> Cup((size_t)depth * (depth + 1) / 2);
+  std::vector<double> pin(4 * (size_t)depth + 8);
+
+  auto sync_check = [&]() -> int {
+    if (hipGetLastError() != hipSuccess) return OCEAN_GPU_ERR_HIP;
+    return OCEAN_GPU_OK;
+  };
+  // rhs = allow .* hay_vec (sfact copy, allow%i = allow%r for core)
+  rc = put_planes(h, hay_re, hay_im, g, nv);
+  if (rc == 0) {
+    k_mask<<<grid, kBlock, 0, h->stream>>>(rhs, g, h->allow, nv);
+    og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(1.0, 0.0), nv);
+    rc = sync_check();
+  }
+  if (rc == 0) rc = xact_dev(h, x, h1, f);  // h1 = H . 1
+  if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
+                        ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
+
+  double gval = 0.0, rval = 0.0, ival = 0.0;
+  auto axmy_xact = [&](double2* in, double2* outv, double E) -> int {
+    // outv = (E + i gres) in - H in
+    OG_TRY(xact_dev(h, in, outv, f));
+    k_axmy<<<grid, kBlock, 0, h->stream>>>(in, outv, E, gres, nv);
+    return sync_check();
+  };
+  auto resid = [&]() -> int {  // g = ax - rhs ; gval ; (rval, ival) = <rhs|x>
+    k_resid<<<kRedBlocks, kBlock, 0, h->stream>>>(ax, rhs, x, g, nv, part);
+    og::k_finish<<<1, 64, 0, h->stream>>>(part, kRedBlocks, 3, part + kRedBlocks * 4 * depth);
+    OG_TRY(sync_check());
+    OG_CHECK(hipMemcpyAsync(pin.data(), part + kRedBlocks * 4 * depth, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+    OG_CHECK(hipStreamSynchronize(h->stream));
+    gval = pin[0]; rval = pin[1]; ival = pin[2];
+    return OCEAN_GPU_OK;
+  };
+  // update_gmres(cur): Gram column cur and rhs, LS solve, x/ax updates (OCEAN_gmres.f90:241-344)
+  auto update = [&](int cur) -> int {
+    k_multidot2<<<kRedBlocks, kBlock, 0, h->stream>>>((const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
+    double* fin = part + (size_t)kRedBlocks * 4 * depth;
+    k_multifinish<<<(4 * cur + 255) / 256, 256, 0, h->stream>>>(part, kRedBlocks, 4 * cur, fin);
+    OG_TRY(sync_check());
+    OG_CHECK(hipMemcpyAsync(pin.data(), fin, 4 * cur * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    OG_CHECK(hipStreamSynchronize(h->stream));
+    const size_t col = (size_t)(cur - 1) * cur / 2;
+    for (int i = 0; i < cur; ++i) {
+      // C(i, cur) = <au_i|au_cur> ; diagonal from the squared norm
+      Cup[col + i] = (i == cur - 1) ? cplx(pin[4 * i], 0.0) : cplx(pin[4 * i], pin[4 * i + 1]);
+    }
+    std::vector<cplx> coef(cur);
+    for (int i = 0; i < cur; ++i) coef[i] = -cplx(pin[4 * i + 2], pin[4 * i + 3]);
+    std::vector<cplx> Cs(Cup.begin(), Cup.begin() + (size_t)cur * (cur + 1) / 2);
+    const int info = hermitian_solve(Cs, cur, coef);
+    if (info != 0) return info;
+    std::vector<double2> cd(cur);
+    for (int i = 0; i < cur; ++i) cd[i] = make_double2(coef[i].real(), coef[i].imag());
+    OG_CHECK(hipMemcpyAsync(dcoef, cd.data(), cur * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)(dlist + depth), dcoef, cur, x, nv);
+    k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)dlist, dcoef, cur, ax, nv);
+    OG_TRY(sync_check());
+    OG_CHECK(hipStreamSynchronize(h->stream));  // cd must stay valid until the copy ran
+    return OCEAN_GPU_OK;
+  };
+
+  int global_iter = 0, prev_iter = 0;
+  double prev_energy = -1e300;
+  for (int step = 0; step < nE && rc == 0; ++step) {
+    int complete_iter = 0;
+    const double E = energies_eV[step] * kEV2Hartree;
+    if (prev_iter == 0) {
+      prev_energy = E;
+      k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
+      if ((rc = sync_check())) break;
+    }
+    bool converged = false;
+    if (prev_iter == 0) {
+      if (step > 0) {
+        if ((rc = axmy_xact(x, ax, E))) break;
+      } else {
+        og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(0.0, 0.0), nv);
+        og::k_fill<<<grid, kBlock, 0, h->stream>>>(ax, make_double2(0.0, 0.0), nv);
+      }
+      if ((rc = resid())) break;
+    } else {
+      const double delta = E - energies_eV[step - 1] * kEV2Hartree;
+      for (int i = 0; i < prev_iter; ++i) og::k_axpy<<<grid, kBlock, 0, h->stream>>>(delta, 0.0, U[i], AU[i], nv);
+      if ((rc = axmy_xact(x, ax, E))) break;
+      if ((rc = resid())) break;
+      for (int it = 1; it <= prev_iter && rc == 0; ++it) {
+        if ((rc = update(it))) break;
+        rc = resid();
+      }
+      if (rc) break;
+      if (gval < ffff) {
+        const double rel = -gval / ival;
+        if (rel < 0.001 || gval < ffff / 10.0) converged = true;
+      }
+      if (!converged && std::fabs(prev_energy - E) > gprc / 4.0) {
+        prev_iter = 0;
+        prev_energy = E;
+        k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
+        if ((rc = sync_check())) break;
+      }
+    }
+    while (!converged && rc == 0) {
+      for (int it = prev_iter + 1; it <= depth; ++it) {
+        ++complete_iter;
+        ++global_iter;
+        k_cmul3<<<grid, kBlock, 0, h->stream>>>(U[it - 1], g, pc, nv);
+        if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
+        if ((rc = update(it))) break;
+        if (it == depth) {
+          if ((rc = axmy_xact(x, ax, E))) break;
+          if (E != prev_energy) {
+            k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
+            prev_energy = E;
+          }
+          prev_iter = 0;
+        }
+        if ((rc = resid())) break;
+        if (gval < ffff) {
+          const double rel = -gval / ival;
+          if (rel < 0.001 || gval < ffff / 10.0) {
+            prev_iter = (it == depth) ? 0 : it;
+            converged = true;
+            break;
+          }
+        }
+      }
+      if (!converged) prev_iter = 0;
+    }
+    if (rc) break;
+    double* o = out + 6 * (size_t)step;
+    o[0] = energies_eV[step];
+    o[1] = 1.0 - rval * kpref;
+    o[2] = -ival * kpref;
+    o[3] = -gval / ival;
+    o[4] = complete_iter;
+    o[5] = global_iter;
+    if (x_re && x_im) rc = get_planes(h, x, x_re + step * nv, x_im + step * nv, nv);
+  }
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Shared definitions for the gfx950 BSE action library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ocean_gpu.h"
+
+#define OG_CHECK(expr)                                                         \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      std::fprintf(stderr, "ocean_gpu: %s failed: %s (%s:%d)\n", #expr,        \
+                   hipGetErrorString(e_), __FILE__, __LINE__);                 \
+      return OCEAN_GPU_ERR_HIP;                                                \
+    }                                                                          \
+  } while (0)
+
+#define OG_TRY(expr)              \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != OCEAN_GPU_OK) return r_; \
+  } while (0)
+
+namespace og {
+
+constexpr int kBlock = 256;      // threads per workgroup for streaming kernels
+constexpr int kRedBlocks = 256;  // fixed partial count: deterministic reductions
+constexpr int kMaxAlpha = 12;    // nalpha <= 4 (2 lc + 1) with lc <= 1
+constexpr int kMaxDim = 16;      // largest k-mesh edge handled by the LDS DFT
+
+// complex helpers (double2 / float2 as interleaved complex)
+template <typename T2> struct Scalar;
+template <> struct Scalar<double2> { using T = double; };
+template <> struct Scalar<float2> { using T = float; };
+
+template <typename T2>
+__device__ __forceinline__ T2 cmake(typename Scalar<T2>::T r, typename Scalar<T2>::T i) {
+  T2 z; z.x = r; z.y = i; return z;
+}
+// a * b
+template <typename T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return cmake<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// acc += a * b
+template <typename T2>
+__device__ __forceinline__ void cfma(T2& acc, T2 a, T2 b) {
+  acc.x = fma(a.x, b.x, acc.x); acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y); acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a) * b
+template <typename T2>
+__device__ __forceinline__ void cfmac(T2& acc, T2 a, T2 b) {
+  acc.x = fma(a.x, b.x, acc.x); acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y); acc.y = fma(-a.y, b.x, acc.y);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of nv doubles per thread into out[0..nv) (valid in thread 0).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= 4*NV */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) lds[wid * NV + j] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += lds[w * NV + j];
+      v[j] = s;
+    }
+  }
+}
+
+}  // namespace og
+
+// ---------------------------------------------------------------------------
+// The handle: module state of long_range / OCEAN_multiplet / OCEAN_energies,
+// device-resident, plus the vector pool and scratch.
+// ---------------------------------------------------------------------------
+struct ocean_gpu_handle {
+  ocean_bse_desc d{};
+  int nb = 0, nk = 0, nx = 0, na = 0, ns = 0;
+  int km[3] = {0, 0, 0};
+  int P = 0;        // compact projector count sum_l (2l+1) nproj(l)
+  int precision = OCEAN_GPU_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // long-range state
+  double2* u64 = nullptr;   // [s][k][n][x]
+  float2* u32 = nullptr;    // same layout, mixed form
+  double* W64 = nullptr;    // [x][k]
+  float* W32 = nullptr;
+  double2* phi64 = nullptr; // [k][x][a]
+  float2* phi32 = nullptr;
+  int32_t* w_i0 = nullptr;  // hat-weight bracket per (x,k) for dW/dptab
+  double* w_w0 = nullptr;
+  double* w_w1 = nullptr;
+  int nptab = 0;
+
+  // multiplet state
+  double2* cproj = nullptr; // [s][k][p][n]
+  double2* Kmat = nullptr;  // (P*na)^2, h_i = sum_j K[i*(P*na)+j] a_j, i=(a*P+p)
+  double2* apart = nullptr; // [k][a][p]
+  double2* hmult = nullptr; // [a][p]
+  double2* Smat = nullptr;  // [a][b]
+  int have_so = 0;
+
+  // energies
+  double2* energy = nullptr; // [a][k][n]
+  double* allow = nullptr;   // [a][k][n]
+
+  bool u64_ok = false, u32_ok = false, w64_ok = false, w32_ok = false;
+  bool has_mult = false, has_e = false;
+  bool has_u() const { return precision == OCEAN_GPU_MIXED ? u32_ok : u64_ok; }
+  bool has_w() const { return precision == OCEAN_GPU_MIXED ? w32_ok : w64_ok; }
+
+  // scratch
+  double* red = nullptr;     // reduction partials (kRedBlocks * 8 doubles)
+  double* scal = nullptr;    // device scalars (Haydock a/b etc.), sized on demand
+  size_t scal_n = 0;
+  double* host_pin = nullptr;  // pinned staging for host planes
+  size_t host_pin_n = 0;
+  double2* tmp_in = nullptr;   // host-plane xact staging
+  double2* tmp_out = nullptr;
+
+  std::vector<double2*> vecs;  // device vectors (OCEAN_vector analogue)
+
+  // timing of the last repeat call
+  double t_proj = 0, t_fft = 0, t_adj = 0, t_mult = 0, t_total = 0;
+
+  size_t nvec() const { return (size_t)nb * nk * na; }
+};

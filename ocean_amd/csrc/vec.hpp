@@ -1,0 +1,332 @@
+// Vector layer (OCEAN_vector ops of OCEAN2/OCEAN_psi.f90), Haydock recurrence
+// kernels, continued fraction, and the load-time layout transforms.
+//
+// Reductions are deterministic: a fixed grid of kRedBlocks workgroups writes
+// partials, and every consumer sums them in the same order.
+#pragma once
+#include "common.hpp"
+
+namespace og {
+
+// <p|q> partials: (sum p_r q_r + p_i q_i, sum p_r q_i - p_i q_r)   OCEAN_psi.f90:1930-2126
+__global__ __launch_bounds__(kBlock) void k_dot_part(const double2* __restrict__ p,
+                                                      const double2* __restrict__ q, size_t n,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[8 * 2];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = p[i], b = q[i];
+    v[0] = fma(a.x, b.x, v[0]); v[0] = fma(a.y, b.y, v[0]);
+    v[1] = fma(a.x, b.y, v[1]); v[1] = fma(-a.y, b.x, v[1]);
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// sum of nv-wide partials over nb blocks, fixed order
+__device__ __forceinline__ double sum_parts(const double* part, int nblk, int nv, int j) {
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[b * nv + j];
+  return s;
+}
+
+__global__ void k_finish(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
+  const int j = threadIdx.x;
+  if (j < nv) out[j] = sum_parts(part, nblk, nv, j);
+}
+
+// squared norm partials   OCEAN_psi.f90:1673-1760
+__global__ __launch_bounds__(kBlock) void k_nrm_part(const double2* __restrict__ p, size_t n,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[8];
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = p[i];
+    v[0] = fma(a.x, a.x, v[0]); v[0] = fma(a.y, a.y, v[0]);
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// y += (ar + i ai) x      OCEAN_psi.f90:1386-1458
+__global__ void k_axpy(double ar, double ai, const double2* __restrict__ x, double2* __restrict__ y, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 xv = x[i];
+    double2 yv = y[i];
+    yv.x += ar * xv.x; yv.y += ar * xv.y;
+    yv.x += -ai * xv.y; yv.y += ai * xv.x;
+    y[i] = yv;
+  }
+}
+
+__global__ void k_scal(double a, double2* __restrict__ x, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double2 v = x[i]; v.x *= a; v.y *= a; x[i] = v;
+  }
+}
+
+__global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_fill(double2* __restrict__ dst, double2 v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+__global__ void k_fill_d(double* __restrict__ dst, double v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Haydock two-term step (OCEAN_hay_ab_twoterm, OCEAN2/OCEAN_haydock.f90:824-938)
+// split into three streaming passes with device-resident scalars:
+//   hay1: h -= b(i-1) old ; partial <h|psi>
+//   hay2: a(i-1) = Re<h|psi> ; h -= a(i-1) psi ; partial <h|h>
+//   hay3: b(i) = sqrt(<h|h>) ; h *= 1/b(i)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_hay1(double2* __restrict__ h, const double2* __restrict__ old,
+                                                  const double2* __restrict__ psi,
+                                                  const double* __restrict__ bprev, size_t n,
+                                                  double* __restrict__ part) {
+  __shared__ double lds[8 * 2];
+  const double mb = -bprev[0];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double2 a = h[i];
+    const double2 o = old[i];
+    a.x += mb * o.x; a.y += mb * o.y;
+    h[i] = a;
+    const double2 b = psi[i];
+    v[0] = fma(a.x, b.x, v[0]); v[0] = fma(a.y, b.y, v[0]);
+    v[1] = fma(a.x, b.y, v[1]); v[1] = fma(-a.y, b.x, v[1]);
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hay2(double2* __restrict__ h, const double2* __restrict__ psi,
+                                                  const double* __restrict__ part1, int nblk, size_t n,
+                                                  double* __restrict__ a_out, double* __restrict__ ai_out,
+                                                  double* __restrict__ part2) {
+  __shared__ double lds[8];
+  __shared__ double s_a;
+  if (threadIdx.x == 0) {
+    const double ar = sum_parts(part1, nblk, 2, 0);
+    s_a = ar;
+    if (blockIdx.x == 0) { a_out[0] = ar; ai_out[0] = sum_parts(part1, nblk, 2, 1); }
+  }
+  __syncthreads();
+  const double ma = -s_a;
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double2 a = h[i];
+    const double2 p = psi[i];
+    a.x += ma * p.x; a.y += ma * p.y;
+    h[i] = a;
+    v[0] = fma(a.x, a.x, v[0]); v[0] = fma(a.y, a.y, v[0]);
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) part2[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_hay3(double2* __restrict__ h, const double* __restrict__ part2,
+                                                  int nblk, size_t n, double* __restrict__ b_out) {
+  __shared__ double s_inv;
+  if (threadIdx.x == 0) {
+    const double b = sqrt(sum_parts(part2, nblk, 1, 0));
+    s_inv = 1.0 / b;
+    if (blockIdx.x == 0) b_out[0] = b;
+  }
+  __syncthreads();
+  const double inv = s_inv;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double2 a = h[i];
+    a.x *= inv; a.y *= inv;
+    h[i] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Continued fraction with the square-root terminator (write_core,
+// OCEAN2/OCEAN_haydock.f90:1801-1894), one thread per energy.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 c_sqrt(double2 z) {
+  const double r = hypot(z.x, z.y);
+  if (r == 0.0) return make_double2(0.0, 0.0);
+  if (z.x >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.x));
+    return make_double2(t, z.y / (2.0 * t));
+  }
+  const double t = sqrt(0.5 * (r - z.x));
+  return make_double2(fabs(z.y) / (2.0 * t), copysign(t, z.y));
+}
+
+// a / b by Smith's method
+__device__ __forceinline__ double2 c_div(double2 a, double2 b) {
+  if (fabs(b.y) <= fabs(b.x)) {
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  }
+  const double r = b.x / b.y, den = b.y + b.x * r;
+  return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
+__device__ __forceinline__ double f32r(double v) { return (double)(float)v; }
+
+__global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict__ rb,
+                        const double* __restrict__ ia, int n, double kpref, double gam, double el,
+                        double eh, int ne, int literal, double hartree2ev, double* __restrict__ out) {
+  const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ie > ne) return;
+  const double e = el + (eh - el) * (double)(2 * ie) / (double)(2 * ne);
+  double2 ct = make_double2(e - ra[n - 1], gam + ia[n - 1]);
+  double fourbc;
+  if (literal) {
+    const float b32 = (float)rb[n];
+    fourbc = (double)((4.0f * b32) * b32);
+  } else {
+    fourbc = 4.0 * rb[n] * rb[n];
+  }
+  double2 z2 = make_double2(ct.x * ct.x - ct.y * ct.y - fourbc, 2.0 * ct.x * ct.y);
+  const double2 disc = c_sqrt(z2);
+  double2 delta = (disc.y > 0.0) ? make_double2(0.5 * (ct.x + disc.x), 0.5 * (ct.y + disc.y))
+                                 : make_double2(0.5 * (ct.x - disc.x), 0.5 * (ct.y - disc.y));
+  for (int jj = n - 1; jj >= 0; --jj) {
+    double bc;
+    double2 z;
+    if (literal) {
+      const float b32 = (float)rb[jj + 1];
+      bc = (double)(b32 * b32);
+      z = make_double2(f32r(e - ra[jj]), f32r(gam + ia[jj]));
+    } else {
+      bc = rb[jj + 1] * rb[jj + 1];
+      z = make_double2(e - ra[jj], gam + ia[jj]);
+    }
+    const double2 q = c_div(make_double2(bc, 0.0), delta);
+    delta = make_double2(z.x - q.x, z.y - q.y);
+  }
+  const double dr = delta.x, di = fabs(delta.y);
+  const double den = dr * dr + di * di;
+  const double sp = kpref * di / den, kk = kpref * dr / den;
+  double* o = out + (size_t)ie * 5;
+  o[0] = hartree2ev * e;
+  o[1] = sp; o[2] = sp; o[3] = kk; o[4] = kk;
+}
+
+// ---------------------------------------------------------------------------
+// Load-time transforms
+// ---------------------------------------------------------------------------
+// Reference u (nb, nk, nx, ns) column-major planes -> device [s][k][n][x] complex.
+// Tiled 32x32 transpose of the (x, n) plane of each (s, k).
+template <typename T2>
+__global__ void k_load_u(const double* __restrict__ re, const double* __restrict__ im,
+                         T2* __restrict__ u, int nb, int nk, int nx) {
+  using T = typename Scalar<T2>::T;
+  __shared__ double2 tile[32][33];
+  const int sk = blockIdx.z;  // s*nk + k
+  const int s = sk / nk, k = sk - s * nk;
+  const int n0 = blockIdx.x * 32, x0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: ty 0..7
+  for (int r = ty; r < 32; r += 8) {
+    const int x = x0 + r, n = n0 + tx;
+    if (x < nx && n < nb) {
+      const size_t off = (((size_t)s * nx + x) * nk + k) * nb + n;
+      tile[r][tx] = make_double2(re[off], im[off]);
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int n = n0 + r, x = x0 + tx;
+    if (x < nx && n < nb) {
+      const double2 v = tile[tx][r];
+      u[(((size_t)s * nk + k) * nb + n) * nx + x] = cmake<T2>((T)v.x, (T)v.y);
+    }
+  }
+}
+
+// Same, applying the supercell phase of OCEAN_bloch_lrLOAD (OCEAN2/OCEAN_bloch.f90:177-203)
+// from the wrapped x offsets xoff(3, nx).
+template <typename T2>
+__global__ void k_load_u_phase(const double* __restrict__ re, const double* __restrict__ im,
+                               const int* __restrict__ xoff, T2* __restrict__ u, int nb, int nk,
+                               int nx, int K1, int K2, int K3, int X1, int X2, int X3) {
+  using T = typename Scalar<T2>::T;
+  __shared__ double2 tile[32][33];
+  const int sk = blockIdx.z;
+  const int s = sk / nk, k = sk - s * nk;
+  const int q1 = k / (K2 * K3), q2 = (k / K3) % K2, q3 = k % K3;
+  const int n0 = blockIdx.x * 32, x0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const double twopi = 6.283185307179586476925286766559;
+  for (int r = ty; r < 32; r += 8) {
+    const int x = x0 + r, n = n0 + tx;
+    if (x < nx && n < nb) {
+      const size_t off = (((size_t)s * nx + x) * nk + k) * nb + n;
+      const double px = twopi * (double)(xoff[3 * x + 0] * q1) / (double)(X1 * K1);
+      const double py = twopi * (double)(xoff[3 * x + 1] * q2) / (double)(X2 * K2);
+      const double pz = twopi * (double)(xoff[3 * x + 2] * q3) / (double)(X3 * K3);
+      double sp, cp;
+      sincos(px + py + pz, &sp, &cp);
+      const double vr = re[off], vi = im[off];
+      tile[r][tx] = make_double2(cp * vr - sp * vi, cp * vi + sp * vr);
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int n = n0 + r, x = x0 + tx;
+    if (x < nx && n < nb) {
+      const double2 v = tile[tx][r];
+      u[(((size_t)s * nk + k) * nb + n) * nx + x] = cmake<T2>((T)v.x, (T)v.y);
+    }
+  }
+}
+
+// W(k,x) from the radial table (lr_populate_W2 + intval 'cap','cap'), keeping the
+// bracket and hat weights for dW/dptab.
+__global__ void k_build_w(const double* __restrict__ rdist, const double* __restrict__ rtab,
+                          const double* __restrict__ ptab, int nptab, double epsi, double scale,
+                          size_t nel, double* __restrict__ W, int* __restrict__ i0,
+                          double* __restrict__ w0, double* __restrict__ w1) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nel; e += (size_t)gridDim.x * blockDim.x) {
+    const double r = rdist[e];
+    double pot, a0 = 0.0, a1 = 0.0;
+    int j = 0;
+    if (r > rtab[nptab - 1]) {
+      pot = epsi / r;
+    } else if (r < rtab[0]) {
+      pot = ptab[0];
+      a0 = 1.0;
+    } else {
+      int lo = 0, hi = nptab - 1;  // first index with rtab >= r, minus one
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (rtab[mid] < r) lo = mid; else hi = mid;
+      }
+      j = (rtab[0] >= r) ? 0 : lo;
+      const double rat = (r - rtab[j]) / (rtab[j + 1] - rtab[j]);
+      pot = ptab[j] + rat * (ptab[j + 1] - ptab[j]);
+      a0 = 1.0 - rat;
+      a1 = rat;
+    }
+    W[e] = pot * scale;
+    i0[e] = j;
+    w0[e] = a0 * scale;
+    w1[e] = a1 * scale;
+  }
+}
+
+template <typename T>
+__global__ void k_cast(const double* __restrict__ src, T* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (T)src[i];
+}
+
+}  // namespace og

@@ -1,0 +1,222 @@
+"""Synthetic core-level BSE workloads (fixed seeds) for the five named configurations.
+
+All arrays are Fortran-ordered in the reference's index order, so their memory is
+the column-major layout OCEAN2's Fortran host holds (see ``ocean_amd.core``):
+
+    u[n, k, x, s]          phased Bloch functions      (re/im_bloch_state)
+    W[k, x]                screened core-hole W        (W in long_range.f90)
+    energy[n, k, a]        E + i Sigma''               (p_energy%r/%i)
+    allow[n, k, a]         occupation mask             (allow%r)
+    c[n, k, nu, m, l, s]   projector overlaps          (mpcr/mpci)
+    mpm[mu, nu, l]         direct core-hole matrix     (mpm)
+    X[l]                   exchange block, [i1, i2]     (mhr/mhi)
+    somelr/someli[ic, jc]  spin-orbit mixing
+
+Generators (shared by every configuration):
+  * E(n,k): uniform in [0, 30] eV, sorted per k.
+  * u(x,n,k): complex Gaussian, QR-orthonormalised over x for each k, then the
+    supercell phase that centres the core site.
+  * W(r) = exp(-r/3)/(4 r), r clamped at 0.2 bohr, 200 knots to 12 bohr; 1/(eps r)
+    beyond.  Cubic cell a = 6 bohr, eps = 4, core site tau = (0.13, 0.27, 0.41).
+  * Multiplet: one projector per l = 0..3 with Gaussian radial functions of width
+    0.8 bohr; F^k and G^k uniform in [0.5, 2] eV; projector overlaps complex
+    Gaussian with variance Omega/nb.
+  * psi0: complex Gaussian, normalised; kpref as the reference's core_pnorm.
+  * allow = 1 (Fermi level below every band).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import setup_host as sh
+
+EV2HARTREE = sh.EV2HARTREE
+
+
+@dataclass
+class MultipletInputs:
+    lmin: int
+    lmax: int
+    nproj: list
+    lvl: int
+    lvh: int
+    c: np.ndarray
+    mpm: np.ndarray
+    X: list
+    somelr: np.ndarray
+    someli: np.ndarray
+    have_spin_orbit: bool
+
+    @property
+    def npmax(self):
+        return max(self.nproj)
+
+    def packed(self):
+        """(mhr, mhi, jbeg, mham): the reference's packed exchange storage."""
+        mhr, mhi, jbeg, mham = [], [], [], []
+        j = 0
+        for Xl in self.X:
+            flat = np.asfortranarray(Xl).ravel(order="F")
+            jbeg.append(j)
+            mham.append(Xl.shape[0])
+            mhr.append(flat.real.copy())
+            mhi.append(flat.imag.copy())
+            j += flat.size
+        return (np.concatenate(mhr), np.concatenate(mhi), np.array(jbeg, np.int32),
+                np.array(mham, np.int32))
+
+
+@dataclass
+class BSEInputs:
+    nb: int
+    kmesh: tuple
+    xmesh: tuple
+    lc: int
+    nspn: int
+    celvol: float
+    avec: np.ndarray
+    tau: np.ndarray          # centred site (after the x-shift)
+    xshift: np.ndarray
+    inter_scale: float
+    epsilon0: float
+    u: np.ndarray
+    W: np.ndarray
+    rtab: np.ndarray
+    ptab: np.ndarray
+    energy: np.ndarray
+    allow: np.ndarray
+    mult: MultipletInputs
+    psi0: np.ndarray
+    kpref: float
+    params: dict = field(default_factory=dict)
+
+    @property
+    def nk(self):
+        return int(np.prod(self.kmesh))
+
+    @property
+    def nx(self):
+        return int(np.prod(self.xmesh))
+
+    @property
+    def nalpha(self):
+        return 4 * (2 * self.lc + 1)
+
+
+def w_table(n=200, rmax=12.0, rclamp=0.2):
+    rtab = rmax * np.arange(1, n + 1, dtype=np.float64) / n
+    ptab = np.exp(-rtab / 3.0) / (4.0 * np.maximum(rtab, rclamp))
+    return rtab, ptab
+
+
+def _gaussian_complex(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+
+
+def make_inputs(nb, kmesh, xmesh, lc=0, nspn=1, seed=0, xi_eV=0.0, lmin=0, lmax=3,
+                w_scale=1.0, a_bohr=6.0, epsilon0=4.0, tau=(0.13, 0.27, 0.41),
+                inter_scale=1.0, scissor_eV=0.0, w_bump=None, params=None):
+    """Build one synthetic core-level problem.  Deterministic in ``seed``."""
+    rng = np.random.default_rng(seed)
+    kmesh = tuple(int(v) for v in kmesh)
+    xmesh = tuple(int(v) for v in xmesh)
+    nk = int(np.prod(kmesh))
+    nx = int(np.prod(xmesh))
+    if nb > nx:
+        raise ValueError("nb must not exceed the number of x points")
+    na = 4 * (2 * lc + 1)
+    avec = np.eye(3) * a_bohr
+    celvol = abs(float(np.linalg.det(avec)))
+
+    # band energies, sorted per k (Hartree), plus the scissor shift
+    E = np.sort(rng.uniform(0.0, 30.0, size=(nspn, nk, nb)), axis=2) * EV2HARTREE
+    E = E + scissor_eV * EV2HARTREE
+    energy = np.zeros((nb, nk, na), np.complex128, order="F")
+    vs = np.zeros(na, np.int64) if nspn == 1 else np.arange(na) % 2
+    for a in range(na):
+        energy[:, :, a] = E[vs[a]].T
+    allow = np.ones((nb, nk, na), np.float64, order="F")
+
+    # Bloch functions: QR over x per k, then the supercell phase
+    u_raw = np.zeros((nb, nk, nx, nspn), np.complex128, order="F")
+    for s in range(nspn):
+        for k in range(nk):
+            q, _ = np.linalg.qr(_gaussian_complex(rng, (nx, nb)))
+            u_raw[:, k, :, s] = q.T
+    u, tau_c, xshift = sh.bloch_phase(u_raw, xmesh, kmesh, np.asarray(tau, np.float64))
+    del u_raw
+
+    rtab, ptab = w_table()
+    ptab = ptab * w_scale
+    if w_bump is not None:
+        amp, r0, wid = w_bump
+        ptab = ptab + amp * np.exp(-((rtab - r0) / wid) ** 2)
+    W = sh.build_w(avec, kmesh, xmesh, tau_c, xshift, rtab, ptab, epsilon0, inter_scale)
+
+    # multiplet
+    ls = list(range(lmin, lmax + 1))
+    nproj = [1] * len(ls)
+    rgrid = np.linspace(1e-4, 9.0, 4001)
+    rproj = 2.0
+    v = np.exp(-rgrid / 3.0) / (4.0 * np.maximum(rgrid, 0.2))
+    vdiff = np.where(rgrid < rproj, v - np.exp(-rproj / 3.0) / (4.0 * rproj), 0.0)
+    mpm = np.zeros((1, 1, len(ls)), order="F")
+    for li, l in enumerate(ls):
+        phi = (rgrid ** l) * np.exp(-0.5 * (rgrid / 0.8) ** 2)
+        phi = phi / np.sqrt(np.trapezoid(phi * phi * rgrid ** 2, rgrid))
+        mpm[:, :, li] = sh.radial_cmel(rgrid, phi[:, None], vdiff) * EV2HARTREE
+    X = []
+    for l in ls:
+        fk = {k: np.array([[rng.uniform(0.5, 2.0)]]) for k in range(2, min(2 * lc, 2 * l) + 1, 2)}
+        gk = {k: np.array([[rng.uniform(0.5, 2.0)]]) for k in range(abs(lc - l), lc + l + 1, 2)}
+        X.append(np.asfortranarray(sh.exchange_matrix(lc, l, 1, fk, gk) * EV2HARTREE))
+    c = np.zeros((nb, nk, 1, 2 * lmax + 1, len(ls), nspn), np.complex128, order="F")
+    sig = np.sqrt(celvol / nb)
+    for li, l in enumerate(ls):
+        for m in range(-l, l + 1):
+            c[:, :, 0, m + lmax, li, :] = _gaussian_complex(rng, (nb, nk, nspn)) * sig
+    sr, si, have_so = sh.spin_orbit(lc, xi_eV)
+    mult = MultipletInputs(lmin, lmax, nproj, lmin, lmax, c, mpm, X, sr, si, have_so)
+
+    psi_raw = np.asfortranarray(_gaussian_complex(rng, (nb, nk, na)))
+    psi0, kpref = sh.normalise_psi0(psi_raw, nk, celvol)
+    return BSEInputs(nb, kmesh, xmesh, lc, nspn, celvol, avec, tau_c, xshift, inter_scale,
+                     epsilon0, u, W, rtab, ptab, energy, allow, mult, psi0, kpref,
+                     dict(params or {}))
+
+
+# Solver settings per named configuration (energies in eV)
+CONFIGS = {
+    0: dict(nb=16, kmesh=(4, 4, 4), xmesh=(6, 6, 6), lc=0,
+            solver=dict(niter=100, gam=0.5, el=-5.0, eh=40.0, ne=2000)),
+    1: dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
+            solver=dict(niter=300, gam=0.5, el=-5.0, eh=40.0, ne=2000,
+                        gmres_n=64, gmres_tol=1e-8)),
+    2: dict(nb=128, kmesh=(8, 8, 8), xmesh=(10, 10, 10), lc=1, xi=10.0 / 1.5,
+            solver=dict(niter=500, gam=0.5, el=-5.0, eh=40.0, ne=4000)),
+    4: dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
+            solver=dict(gam=0.5, el=-5.0, eh=40.0, gmres_n=128, gmres_tol=1e-8)),
+}
+
+
+def config(i, seed=0):
+    """Inputs for named configuration ``i`` (0, 1, 2 or 4)."""
+    c = CONFIGS[i]
+    return make_inputs(c["nb"], c["kmesh"], c["xmesh"], lc=c["lc"], seed=seed,
+                       xi_eV=c.get("xi", 0.0), params=dict(c["solver"]))
+
+
+def batch_members(n=96, seed=0):
+    """Member descriptions of the uneven batch: half K edges, half L2,3 edges."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        lc = 0 if i < n // 2 else 1
+        nb = int(rng.choice(np.arange(32, 161, 16)))
+        km = int(rng.choice([4, 6, 8]))
+        out.append(dict(nb=nb, kmesh=(km, km, km), xmesh=(8, 8, 8), lc=lc,
+                        seed=1000 + i, w_scale=float(rng.uniform(0.7, 1.3)),
+                        pol=int(rng.integers(0, 3))))
+    return out

@@ -1,0 +1,1139 @@
+"""CPU restatement of OCEAN2's core-level BSE hot path (the oracle).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker.  The product (``ocean_amd``) never imports it; the GPU path fails loudly
+when its HIP library is missing instead of falling back here.
+
+Every function restates one reference routine and cites it (paths relative to
+the OCEAN 3.2.1 tree).  Arrays use the reference's index order (math order
+below) and are allocated Fortran-ordered so that their memory equals the
+reference's column-major layout:
+
+    u[n, k, x, s]         re/im_bloch_state(nb, nk, nx, nspn)   long_range.f90:186-187
+    W[k, x]               W(nk, nx)                            long_range.f90:188
+    psi[n, k, a]          OCEAN_vector %r/%i (nb, nk, nalpha)   OCEAN_psi.f90:64-65
+    c[n, k, nu, m, l, s]  mpcr/mpci(nb, nk, npmax, -lmax:lmax, lmin:lmax, nspn)
+                                                               OCEAN_multiplet.f90:401-402
+
+k index: k1 outer, k3 inner (OCEAN_bloch.f90:171-175); x index: ix fastest
+(OCEAN_bloch.f90:133-156); alpha = (core spin, core m, valence spin), valence
+spin fastest (OCEAN_multiplet.f90:343-355).  Energies are in Hartree.
+
+Pinning (what checks each function; "parity unpinned" where none does):
+  write_core           TEST/Fe/Results abslanc->absspct fixture, Lorentzian closed form
+  redtrid              TEST/Fe/Results abslanc eigenvalue block
+  haydock              dense Lanczos/eigendecomposition brute force (tests/test_oracle.py)
+  lr_act               dense operator brute force, loop-literal lr_kernel restatement
+  fg_combo/new_soact   dense operator brute force, Hermiticity
+  gaunt/threey         Wigner-3j closed form vs Gauss-Legendre quadrature restatement
+  soprep               closed-form <L.S> eigenvalues
+  populate_W/intval    closed forms of the geometry and the interpolant
+  bloch_phase          closed form (phase of a plane wave)
+  gmres_do_recycle     dense direct solve of (E + iG - H) x = b
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# OCEAN_constants.f90:13-20 (2022 CODATA)
+EV2HARTREE = 0.036749322175666
+HARTREE2EV = 27.211386245980
+# OCEAN_constants.f90:25 (commented older constant, used by the TEST/Fe outputs)
+HARTREE2EV_OLD = 27.21138386
+
+
+# ----------------------------------------------------------------------------
+# small helpers
+# ----------------------------------------------------------------------------
+def fzeros(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+def intval(xtab, ytab, x, lopt="cap", hopt="cap"):
+    """Linear interpolation with 'cap'/'ext' ends.
+
+    CNBSE/zcmsnmult/zf90/intval.f90:8-57.  The bracket search starts at ii=1 and
+    advances while xtab(ii+1) < x.
+    """
+    xtab = np.asarray(xtab, np.float64)
+    ytab = np.asarray(ytab, np.float64)
+    x = np.asarray(x, np.float64)
+    n = xtab.shape[0]
+    below = x < xtab[0]
+    above = x > xtab[n - 1]
+    idx = np.searchsorted(xtab, x, side="left")
+    ii = np.clip(idx - 1, 0, n - 2)
+    if lopt == "ext":
+        ii = np.where(below, 0, ii)
+    if hopt == "ext":
+        ii = np.where(above, n - 2, ii)
+    if (lopt == "err" and np.any(below)) or (hopt == "err" and np.any(above)):
+        raise ValueError("intval: out of table range")
+    rat = (x - xtab[ii]) / (xtab[ii + 1] - xtab[ii])
+    y = ytab[ii] + rat * (ytab[ii + 1] - ytab[ii])
+    if lopt == "cap":
+        y = np.where(below, ytab[0], y)
+    if hopt == "cap":
+        y = np.where(above, ytab[n - 1], y)
+    return y
+
+
+def getomega(avec):
+    """Cell volume |det avec|, avec(:, i) = lattice vector i.
+
+    CNBSE/zcmsnmult/zf90/getomega.f90:8-26.
+    """
+    a = np.asarray(avec, np.float64)
+    om = 0.0
+    for i in range(3):
+        j = (i + 1) % 3
+        k = (j + 1) % 3
+        om += a[i, 0] * a[j, 1] * a[k, 2]
+        om -= a[i, 0] * a[k, 1] * a[j, 2]
+    return abs(om)
+
+
+def alpha_labels(lc):
+    """(cms, cml, vms) per alpha, valence spin fastest.
+
+    OCEAN_multiplet.f90:343-355.
+    """
+    out = []
+    for icms in (-1, 1):
+        for icml in range(-lc, lc + 1):
+            for ivms in (-1, 1):
+                out.append((0.5 * icms, float(icml), 0.5 * ivms))
+    return out
+
+
+def val_spin_of_alpha(lc, nspn):
+    """0-based valence-spin index per alpha.  long_range.f90:793-805."""
+    na = 4 * (2 * lc + 1)
+    if nspn == 1:
+        return np.zeros(na, dtype=np.int64)
+    return np.array([a % 2 for a in range(na)], dtype=np.int64)
+
+
+# ----------------------------------------------------------------------------
+# 3-D FFT over the k-mesh with the reference's sign/normalisation conventions
+# ----------------------------------------------------------------------------
+def fft_backward(v, kmesh, axis=0):
+    """FFTW_BACKWARD (exp(+2 pi i ...)), unnormalised, over (k1,k2,k3), k3 fastest.
+
+    Modules/FFT_wrapper.f90:67-72 (plan over dims (k3,k2,k1) reversed),
+    :185-215 (FFT_wrapper_single with norm=.false.).
+    """
+    k1, k2, k3 = kmesh
+    v = np.moveaxis(v, axis, -1)
+    sh = v.shape[:-1]
+    t = v.reshape(sh + (k1, k2, k3))
+    t = np.fft.ifftn(t, axes=(-3, -2, -1)) * (k1 * k2 * k3)
+    return np.moveaxis(t.reshape(sh + (k1 * k2 * k3,)), -1, axis)
+
+
+def fft_forward(v, kmesh, axis=0):
+    """FFTW_FORWARD (exp(-2 pi i ...)), unnormalised.  Modules/FFT_wrapper.f90:185-215."""
+    k1, k2, k3 = kmesh
+    v = np.moveaxis(v, axis, -1)
+    sh = v.shape[:-1]
+    t = v.reshape(sh + (k1, k2, k3))
+    t = np.fft.fftn(t, axes=(-3, -2, -1))
+    return np.moveaxis(t.reshape(sh + (k1 * k2 * k3,)), -1, axis)
+
+
+def dft_matrix_backward(kmesh):
+    """Dense matrix of fft_backward (brute-force check of the convention)."""
+    k1, k2, k3 = kmesh
+    idx = np.array([(a, b, c) for a in range(k1) for b in range(k2) for c in range(k3)], float)
+    ph = (np.outer(idx[:, 0], idx[:, 0]) / k1 + np.outer(idx[:, 1], idx[:, 1]) / k2
+          + np.outer(idx[:, 2], idx[:, 2]) / k3)
+    return np.exp(2j * np.pi * ph)
+
+
+# ----------------------------------------------------------------------------
+# Setup restatements
+# ----------------------------------------------------------------------------
+def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
+    """Centre the core site and apply the supercell phase to u.
+
+    OCEAN_bloch.f90:54-219 (OCEAN_bloch_lrLOAD).  Returns (u, tau_new, xshift).
+    u_raw[n, k, x, s] is the content of con.u2.dat (OCEAN_bloch.f90:998-1115).
+    """
+    xmesh = tuple(int(v) for v in xmesh)
+    kmesh = tuple(int(v) for v in kmesh)
+    tau = np.array(tau, np.float64)
+    xshift = np.zeros(3, np.int64)
+    for d in range(3):
+        if kmesh[d] % 2 == 0:
+            xshift[d] = math.floor(float(xmesh[d]) * tau[d])
+        else:
+            xshift[d] = math.floor(float(xmesh[d]) * (tau[d] - 0.5))
+    xshift = xshift * np.array(xshift_override, np.int64)
+    tau_new = tau - xshift / np.array(xmesh, np.float64)
+
+    def wrap(i, d):  # i 1-based
+        targ = i - xshift[d]
+        if targ > xmesh[d]:
+            return -xmesh[d]
+        if targ < 1:
+            return xmesh[d]
+        return 0
+
+    nx = xmesh[0] * xmesh[1] * xmesh[2]
+    nk = kmesh[0] * kmesh[1] * kmesh[2]
+    xoff = np.zeros((nx, 3), np.float64)
+    it = 0
+    for iz in range(1, xmesh[2] + 1):
+        zph = wrap(iz, 2)
+        for iy in range(1, xmesh[1] + 1):
+            yph = wrap(iy, 1)
+            for ix in range(1, xmesh[0] + 1):
+                xph = wrap(ix, 0)
+                xoff[it] = (xph + ix - 1, yph + iy - 1, zph + iz - 1)
+                it += 1
+    kidx = np.array([(a, b, c) for a in range(kmesh[0]) for b in range(kmesh[1])
+                     for c in range(kmesh[2])], np.float64)
+    phsx = 2.0 * np.pi * np.outer(kidx[:, 0], xoff[:, 0]) / (xmesh[0] * kmesh[0])
+    phsy = 2.0 * np.pi * np.outer(kidx[:, 1], xoff[:, 1]) / (xmesh[1] * kmesh[1])
+    phsz = 2.0 * np.pi * np.outer(kidx[:, 2], xoff[:, 2]) / (xmesh[2] * kmesh[2])
+    ph = phsx + phsy + phsz  # [k, x]
+    rot = np.cos(ph) + 1j * np.sin(ph)
+    assert u_raw.shape[1] == nk and u_raw.shape[2] == nx
+    u = np.asfortranarray(u_raw * rot[None, :, :, None])
+    return u, tau_new, xshift
+
+
+def _w_geometry(avec, kmesh, xmesh, tau, xshift, pbc=None):
+    """r(k,x) = |A (R_k + f_x - tau)| and the pbc prefactor.  long_range.f90:1744-1817."""
+    avec = np.asarray(avec, np.float64)
+    amet = avec.T @ avec  # amet(i,j) = avec(:,i).avec(:,j)
+    kmesh = tuple(int(v) for v in kmesh)
+    xmesh = tuple(int(v) for v in xmesh)
+    if pbc is None:
+        pbc = kmesh
+    else:
+        pbc = tuple(kmesh[i] if pbc[i] != 0 else 0 for i in range(3))
+    fr = []
+    for iz in range(1, xmesh[2] + 1):
+        zt = iz - xshift[2]
+        zt = zt - xmesh[2] if zt > xmesh[2] else (zt + xmesh[2] if zt < 1 else zt)
+        for iy in range(1, xmesh[1] + 1):
+            yt = iy - xshift[1]
+            yt = yt - xmesh[1] if yt > xmesh[1] else (yt + xmesh[1] if yt < 1 else yt)
+            for ix in range(1, xmesh[0] + 1):
+                xt = ix - xshift[0]
+                xt = xt - xmesh[0] if xt > xmesh[0] else (xt + xmesh[0] if xt < 1 else xt)
+                fr.append(((xt - 1) / xmesh[0], (yt - 1) / xmesh[1], (zt - 1) / xmesh[2]))
+    fr = np.array(fr)  # [x, 3]
+
+    def kk_of(n):
+        kk = np.arange(n)
+        kk = np.where(kk >= (n + 1) // 2, kk - n, kk)
+        if n == 1:
+            kk[:] = 0
+        return kk
+
+    kk1, kk2, kk3 = (kk_of(kmesh[d]) for d in range(3))
+    xk = np.array([(a, b, c) for a in kk1 for b in kk2 for c in kk3], np.float64)  # [k,3]
+    p1 = np.array([a <= pbc[0] for a in kk1 for b in kk2 for c in kk3])
+    p2 = np.array([(b <= pbc[1]) for a in kk1 for b in kk2 for c in kk3]) & p1
+    p3 = np.array([(c <= pbc[2]) for a in kk1 for b in kk2 for c in kk3]) & p2
+    alf = xk[:, None, :] + fr[None, :, :] - np.asarray(tau, np.float64)[None, None, :]
+    r = np.sqrt(np.einsum("kxi,ij,kxj->kx", alf, amet, alf))
+    cart = np.einsum("ij,kxj->kxi", avec, alf)
+    return r, p3.astype(np.float64), cart
+
+
+def populate_W(avec, kmesh, xmesh, tau, xshift, rtab, ptab, epsilon0,
+               interaction_scale=1.0, isolated=False, iso_cut=0.0, pbc=None,
+               epsilon3d=None):
+    """W(k, x) for the screened core-hole potential.
+
+    long_range.f90:1659-1853 (lr_populate_W2); interpolation by intval 'cap','cap'
+    (:1834), 1/(eps r) tail beyond rtab(n) (:1819-1832), isolated cutoff (:1818),
+    x-shift wrap (:1749-1772), kk >= (N+1)/2 wrap (:1783, :1795, :1806).
+    """
+    epsi = 1.0 / epsilon0
+    rtab = np.asarray(rtab, np.float64)
+    ptab = np.asarray(ptab, np.float64)
+    r, pref, cart = _w_geometry(avec, kmesh, xmesh, tau, xshift, pbc)
+    potn = intval(rtab, ptab, r, "cap", "cap")
+    far = r > rtab[-1]
+    rr = np.where(r > 0, r, 1.0)
+    if epsilon3d is not None:
+        d = cart / rr[..., None]
+        e3 = np.asarray(epsilon3d, np.float64)
+        pt = (d[..., 0] ** 2 / e3[0] + d[..., 1] ** 2 / e3[1] + d[..., 2] ** 2 / e3[2]) / rr
+        blend = r < rtab[-1] + 5.0
+        pt = np.where(blend, pt * (r - rtab[-1]) / 5.0 + (rtab[-1] + 5.0 - r) * epsi / (5.0 * rr), pt)
+        potn = np.where(far, pt, potn)
+    else:
+        potn = np.where(far, epsi / rr, potn)
+    if isolated:
+        potn = np.where(r > iso_cut, 0.0, potn)
+    W = potn * pref[:, None] * interaction_scale
+    return np.asfortranarray(W)
+
+
+def populate_W_weights(avec, kmesh, xmesh, tau, xshift, rtab, interaction_scale=1.0,
+                       isolated=False, iso_cut=0.0, pbc=None):
+    """dW(k,x)/dptab as two hat weights per (k,x).
+
+    W is linear in ptab through intval (CNBSE/zcmsnmult/zf90/intval.f90:47-53) for
+    r <= rtab(n), capped at ptab(1) below rtab(1); the 1/(eps r) tail and the
+    isolated cutoff carry no ptab dependence (long_range.f90:1818-1840).
+    Returns (i0, w0, w1): dW = w0 dptab[i0] + w1 dptab[i0 + 1].
+    """
+    rtab = np.asarray(rtab, np.float64)
+    n = rtab.shape[0]
+    r, pref, _ = _w_geometry(avec, kmesh, xmesh, tau, xshift, pbc)
+    idx = np.searchsorted(rtab, r, side="left")
+    i0 = np.clip(idx - 1, 0, n - 2)
+    rat = (r - rtab[i0]) / (rtab[i0 + 1] - rtab[i0])
+    w0 = 1.0 - rat
+    w1 = rat
+    below = r < rtab[0]
+    w0 = np.where(below, 1.0, w0)
+    w1 = np.where(below, 0.0, w1)
+    dead = r > rtab[-1]
+    if isolated:
+        dead = dead | (r > iso_cut)
+    s = pref[:, None] * interaction_scale
+    w0 = np.where(dead, 0.0, w0) * s
+    w1 = np.where(dead, 0.0, w1) * s
+    return i0, w0, w1
+
+
+# ----------------------------------------------------------------------------
+# Multiplet setup restatements
+# ----------------------------------------------------------------------------
+def getprefs(lmax):
+    """Ylm normalisation prefactors.  CNBSE/zcmsnmult/zf90/newgetprefs.f90:8-40."""
+    prefs = np.zeros(1001)
+    for l in range(lmax + 1):
+        prefs[l] = (2 * l + 1) / (4.0 * np.pi)
+        lamold = l
+        for m in range(1, l + 1):
+            lam = 10 * m + l
+            prefs[lam] = prefs[lamold] / ((l - m + 1) * (l + m))
+            lamold = lam
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            lam = 10 * m + l
+            prefs[lam] = math.sqrt(prefs[lam])
+    return prefs
+
+
+_YLM_POLY = {  # newgetylm.f90 case table: f(u) for lam = 10*|m| + l
+    0: lambda u: 1.0 + 0 * u,
+    11: lambda u: -1.0 + 0 * u, 1: lambda u: u,
+    22: lambda u: 3.0 + 0 * u, 12: lambda u: -3 * u, 2: lambda u: (3 * u ** 2 - 1) / 2,
+    33: lambda u: -15.0 + 0 * u, 23: lambda u: 15 * u, 13: lambda u: -(15 * u ** 2 - 3) / 2,
+    3: lambda u: (5 * u ** 3 - 3 * u) / 2,
+    44: lambda u: 105.0 + 0 * u, 34: lambda u: -105 * u, 24: lambda u: (105 * u ** 2 - 15) / 2,
+    14: lambda u: -(35 * u ** 3 - 15 * u) / 2, 4: lambda u: (35 * u ** 4 - 30 * u ** 2 + 3) / 8,
+    55: lambda u: -945.0 + 0 * u, 45: lambda u: 945 * u, 35: lambda u: -(945 * u ** 2 - 105) / 2,
+    25: lambda u: (315 * u ** 3 - 105 * u) / 2, 15: lambda u: -(315 * u ** 4 - 210 * u ** 2 + 15) / 8,
+    5: lambda u: (63 * u ** 5 - 70 * u ** 3 + 15 * u) / 8,
+}
+
+
+def getylm(l, m, x, y, z, prefs):
+    """Complex Ylm, Condon-Shortley phase.  CNBSE/zcmsnmult/zf90/newgetylm.f90."""
+    if l > 5:
+        raise ValueError("l > 5 not yet allowed")
+    r = np.sqrt(x * x + y * y + z * z)
+    r = np.where(r == 0, 1.0, r)
+    xr, yr, zr = x / r, y / r, z / r
+    mm = abs(m)
+    lam = 10 * mm + l
+    f = _YLM_POLY.get(lam, lambda u: 0 * u)(zr)
+    ylm = prefs[lam] * f + 0j
+    if m > 0:
+        for _ in range(m):
+            ylm = ylm * (xr + 1j * yr)
+    if m < 0:
+        for _ in range(mm):
+            ylm = -ylm * (xr - 1j * yr)
+    return ylm
+
+
+def threey(ll, ml, lmid, mmid, lr, mr, conjug, x, w, prefs):
+    """Gaunt integral int conj(Y_ll,ml) Y_lmid,mmid(^*) Y_lr,mr by quadrature.
+
+    CNBSE/zcmsnmult/zf90/newthreey.f90:8-52: Gauss-Legendre in cos(theta), phi
+    integral done analytically (2 pi) once m conservation holds.
+    """
+    mdo = (ml == mr - mmid) if conjug else (ml == mr + mmid)
+    ldo = False
+    j = ll + lr
+    while j >= max(abs(mmid), abs(ll - lr)):
+        if j == lmid:
+            ldo = True
+        j -= 2
+    if not (ldo and mdo):
+        return 0j
+    ct = np.asarray(x, np.float64)
+    st = np.sqrt(1.0 - ct ** 2)
+    rx, ry, rz = st, 0.0 * st, ct
+    yl = getylm(ll, ml, rx, ry, rz, prefs)
+    ym = getylm(lmid, mmid, rx, ry, rz, prefs)
+    if conjug:
+        ym = np.conj(ym)
+    yr = getylm(lr, mr, rx, ry, rz, prefs)
+    return complex(np.sum(w * np.conj(yl) * ym * yr) * 2 * np.pi)
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    """Racah formula (closed form used to pin threey)."""
+    if m1 + m2 + m3 != 0:
+        return 0.0
+    if j3 < abs(j1 - j2) or j3 > j1 + j2 or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    f = math.factorial
+    pre = (-1) ** (j1 - j2 - m3) * math.sqrt(
+        f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) / f(j1 + j2 + j3 + 1)
+        * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3))
+    s = 0.0
+    for k in range(0, j1 + j2 + j3 + 2):
+        d = [k, j1 + j2 - j3 - k, j1 - m1 - k, j2 + m2 - k, j3 - j2 + m1 + k, j3 - j1 - m2 + k]
+        if min(d) < 0:
+            continue
+        s += (-1) ** k / (f(d[0]) * f(d[1]) * f(d[2]) * f(d[3]) * f(d[4]) * f(d[5]))
+    return pre * s
+
+
+def gaunt_closed(ll, ml, lmid, mmid, lr, mr, conjug):
+    """Closed form of threey via 3j symbols (Condon-Shortley Ylm)."""
+    # int conj(Y_l1m1) Y_l2m2 Y_l3m3 = (-1)^m1 sqrt((2l1+1)(2l2+1)(2l3+1)/4pi)
+    #                                  (l1 l2 l3;0 0 0)(l1 l2 l3;-m1 m2 m3)
+    if conjug:  # conj(Y_l,m) = (-1)^m Y_l,-m
+        sgn = (-1) ** mmid
+        mm = -mmid
+    else:
+        sgn = 1
+        mm = mmid
+    val = ((-1) ** ml * math.sqrt((2 * ll + 1) * (2 * lmid + 1) * (2 * lr + 1) / (4 * math.pi))
+           * wigner3j(ll, lmid, lr, 0, 0, 0) * wigner3j(ll, lmid, lr, -ml, mm, mr))
+    return sgn * val
+
+
+def gauss_legendre(npt=16):
+    """The Pquadrature rule (weights renormalised to sum 2, OCEAN_multiplet.f90:2055-2063)."""
+    x, w = np.polynomial.legendre.leggauss(npt)
+    return x, w * 2.0 / w.sum()
+
+
+def mham_size(lc, lv, np_l):
+    """mham(l) = 4 (2 lc + 1)(2 l + 1) nproj(l).  OCEAN_multiplet.f90:325."""
+    return 4 * (2 * lc + 1) * (2 * lv + 1) * np_l
+
+
+def hamiltonian_labels(lc, lv, np_l):
+    """(cms, cml, vms, vml, vnu) per exchange index.  OCEAN_multiplet.f90:358-373."""
+    out = []
+    for (cms, cml, vms) in alpha_labels(lc):
+        for ivml in range(-lv, lv + 1):
+            for nu in range(1, np_l + 1):
+                out.append((cms, cml, vms, float(ivml), nu))
+    return out
+
+
+def nbsemhsetup2(lc, lv, np_l, fk, scfk, gk, scgk, scfac=1.0, tdlda=False, quad=None,
+                 use_closed_form=False):
+    """Exchange/direct multiplet matrix mh_l(i1, i2) (eV) for one valence l.
+
+    OCEAN_multiplet.f90:2007-2181.  fk[k] / gk[k] are (np, np) Slater matrices and
+    scfk[k] / scgk[k] their scale factors, keyed by k.  Returns complex (mham, mham)
+    with element [i1, i2] as stored in mhr_l/mhi_l.
+    """
+    mham = mham_size(lc, lv, np_l)
+    labels = hamiltonian_labels(lc, lv, np_l)
+    if quad is None:
+        quad = gauss_legendre(16)
+    x, w = quad
+    maxll = min(max(2 * lc, 2 * lv), 5)
+    prefs = getprefs(maxll)
+    kfh = min(2 * lc, 2 * lv)
+    kfl = 2
+    if tdlda:
+        kfl = kfh = 0
+    kgh = lc + lv
+    kgl = abs(lc - lv)
+    ty = (lambda *a: gaunt_closed(*a)) if use_closed_form else \
+        (lambda l1, m1, k, mk, l3, m3, cj: threey(l1, m1, k, mk, l3, m3, cj, x, w, prefs))
+    mh = np.zeros((mham, mham), np.complex128, order="F")
+    for i1 in range(mham):
+        nu1 = labels[i1][4]
+        for i2 in range(mham):
+            nu2 = labels[i2][4]
+            if kfh >= kfl:
+                l1, m1, s1 = lc, int(round(labels[i2][1])), int(round(2 * labels[i2][0]))
+                l2, m2, s2 = lv, int(round(labels[i1][3])), int(round(2 * labels[i1][2]))
+                l3, m3, s3 = lc, int(round(labels[i1][1])), int(round(2 * labels[i1][0]))
+                l4, m4, s4 = lv, int(round(labels[i2][3])), int(round(2 * labels[i2][2]))
+                if s1 == s3 and s2 == s4:
+                    mk = m1 - m3
+                    if m1 + m2 == m3 + m4:
+                        for k in range(kfl, kfh + 1, 2):
+                            if abs(mk) <= k:
+                                ffk = scfk[k] * fk[k][nu1 - 1, nu2 - 1] * scfac
+                                f1 = ty(l1, m1, k, mk, l3, m3, False)
+                                f2 = ty(l2, m2, k, mk, l4, m4, True)
+                                mh[i1, i2] += -ffk * f1 * f2 * (4 * np.pi / (2 * k + 1))
+            if kgh >= kgl:
+                l1, m1, s1 = lc, int(round(labels[i2][1])), int(round(2 * labels[i2][0]))
+                l2, m2, s2 = lv, int(round(labels[i1][3])), int(round(2 * labels[i1][2]))
+                l3, m3, s3 = lv, int(round(labels[i2][3])), int(round(2 * labels[i2][2]))
+                l4, m4, s4 = lc, int(round(labels[i1][1])), int(round(2 * labels[i1][0]))
+                if s1 == s3 and s2 == s4:
+                    mk = m1 - m3
+                    if m1 + m2 == m3 + m4:
+                        for k in range(kgl, kgh + 1, 2):
+                            if abs(mk) <= k:
+                                ggk = scgk[k] * gk[k][nu1 - 1, nu2 - 1] * scfac
+                                f1 = ty(l1, m1, k, mk, l3, m3, False)
+                                f2 = ty(l2, m2, k, mk, l4, m4, True)
+                                mh[i1, i2] += ggk * f1 * f2 * (4 * np.pi / (2 * k + 1))
+    return mh
+
+
+def jimel(j, m, mp, i):
+    """<j m | J_i | j mp> (i = x, y, z).  CNBSE/zcmsnmult/zf90/jimel.f90:10-38."""
+    x = abs(j * (j + 1) - m * mp)
+    mdiff = int(round(m - mp))
+    c = 0j
+    if i == 1:
+        if mdiff in (1, -1):
+            c = 0.5 * math.sqrt(x)
+    elif i == 2:
+        if mdiff == 1:
+            c = 0.5 * math.sqrt(x) / 1j
+        elif mdiff == -1:
+            c = -0.5 * math.sqrt(x) / 1j
+    elif i == 3:
+        if mdiff == 0:
+            c = m
+    return complex(c)
+
+
+def soprep(lc, xi_eV, life_time_eV=None):
+    """Spin-orbit alpha-mixing matrix S = somelr + i someli (Hartree), [ic, jc].
+
+    OCEAN_multiplet.f90:572-690 (OCEAN_soprep).  xi is read in eV from bse.in
+    line 2 (:298-303).  life_time (core_broaden.ipt) enables the imaginary part.
+    Returns (somelr, someli, have_spin_orbit).
+    """
+    xi = xi_eV * EV2HARTREE
+    labels = alpha_labels(lc)
+    na = len(labels)
+    have_so = xi >= 1.0e-12
+    if life_time_eV is not None:
+        have_so = True
+        lt = np.asarray(life_time_eV, np.float64) * EV2HARTREE
+        l_alpha = 2.0 / (2 * lc + 1) * (lt[0] - lt[1])
+        l_beta = (lc + 1) / (2 * lc + 1) * lt[0] + lc / (2 * lc + 1) * lt[1]
+    else:
+        l_alpha = 0.0
+        l_beta = 0.0
+    sr = np.zeros((na, na), order="F")
+    si = np.zeros((na, na), order="F")
+    for ic in range(na):
+        for jc in range(na):
+            if labels[ic][2] == labels[jc][2]:
+                ct = 0j
+                for i in (1, 2, 3):
+                    ct += jimel(float(lc), labels[jc][1], labels[ic][1], i) * \
+                        jimel(0.5, labels[jc][0], labels[ic][0], i)
+                sr[ic, jc] = -xi * ct.real
+                si[ic, jc] = -ct.real * l_alpha
+                if ic == jc:
+                    si[ic, jc] -= l_beta
+    return sr, si, have_so
+
+
+def nbsemkcmel(v, rv, rmax, rad, phi):
+    """Trim the core-hole potential and integrate cmel for one l.
+
+    OCEAN2/nbsemkcmel.f90:8-209.  v, rv: rpot table; rad: log radial grid;
+    phi[i, nu]: projector radial functions on rad.  Returns (vtrim, cmel_eV, nmel).
+    """
+    v = np.asarray(v, np.float64)
+    rv = np.asarray(rv, np.float64)
+    nv = v.shape[0]
+    # binary search for the first rv >= rmax (nbsemkcmel.f90:98-110), 1-based j
+    j, k = 1, nv
+    i = (j + k) // 2
+    while j < k:
+        i = (j + k) // 2
+        if rv[i - 1] < rmax:
+            j = i + 1
+        elif rv[i - 1] > rmax:
+            k = i - 1
+        else:
+            break
+    if rv[j - 1] < rmax:
+        j += 1
+    vrun = v[j - 1]
+    vtrim = v.copy()
+    vdiff = np.zeros(nv)
+    vtrim[: j - 1] = vrun
+    vdiff[: j - 1] = v[: j - 1] - vrun
+    bwgt = np.array([28.0, 64.0, 24.0, 64.0]) / 45.0
+    nr = rad.shape[0]
+    dl = math.log(rad[nr - 1] / rad[0]) / (nr - 1)
+    xx = 1 + round(math.log(rmax / rad[0]) / dl)
+    ix = int(xx)
+    dr = np.zeros(nr)
+    dr[0] = 0.5 * dl * rad[0]
+    dr[1: ix - 1] = dl * rad[1: ix - 1]
+    dr[ix - 1] = 0.5 * dl * rad[ix - 1]
+    val = intval(rv, vdiff, rad[:ix], "err", "err")
+    nnu = phi.shape[1]
+    cmel = np.zeros((nnu, nnu))
+    nmel = np.zeros((nnu, nnu))
+    for nu in range(nnu):
+        for mu in range(nnu):
+            su = 0.0
+            su1 = 0.0
+            for ii0 in range(ix, 4, -4):
+                for jj in range(4):
+                    q = ii0 - jj - 1
+                    t = phi[q, mu] * phi[q, nu] * val[q]
+                    su += bwgt[jj] * dr[q] * rad[q] ** 2 * t
+                    su1 += bwgt[jj] * dr[q] * rad[q] ** 2 * phi[q, mu] * phi[q, nu]
+            cmel[mu, nu] = su * HARTREE2EV
+            nmel[mu, nu] = su1
+    return vtrim, cmel, nmel
+
+
+# ----------------------------------------------------------------------------
+# Problem container
+# ----------------------------------------------------------------------------
+@dataclass
+class Multiplet:
+    """Module state of OCEAN_multiplet after OCEAN_create_central (Hartree)."""
+    lmin: int
+    lmax: int
+    nproj: list            # nproj[l - lmin]
+    lvl: int
+    lvh: int
+    c: np.ndarray          # complex (nb, nk, npmax, 2lmax+1, nl, nspn), F-order
+    mpm: np.ndarray        # real (npmax, npmax, nl)
+    X: list                # complex (mham_l, mham_l) per l in lvl..lvh, [i1, i2] storage
+    somelr: np.ndarray     # (na, na) [ic, jc]
+    someli: np.ndarray
+    have_spin_orbit: bool = True
+
+    @property
+    def npmax(self):
+        return max(self.nproj)
+
+    def packed(self):
+        """mhr/mhi flat arrays and jbeg/mham (OCEAN_multiplet.f90:318-327, :2125-2175)."""
+        mhr, mhi, jbeg, mham = [], [], [], []
+        j = 0
+        for Xl in self.X:
+            jbeg.append(j)
+            mham.append(Xl.shape[0])
+            flat = np.asfortranarray(Xl).ravel(order="F")
+            mhr.append(flat.real)
+            mhi.append(flat.imag)
+            j += flat.size
+        return (np.concatenate(mhr), np.concatenate(mhi), np.array(jbeg), np.array(mham))
+
+
+@dataclass
+class Problem:
+    """Everything the core-level action reads (o_system fields + module state)."""
+    nb: int
+    kmesh: tuple
+    xmesh: tuple
+    lc: int
+    nspn: int
+    celvol: float
+    inter_scale: float
+    u: np.ndarray            # complex (nb, nk, nx, nspn), phased
+    W: np.ndarray            # real (nk, nx)
+    energy: np.ndarray       # complex (nb, nk, na): E + i Sigma''
+    allow: np.ndarray        # real (nb, nk, na)
+    mult: Multiplet | None
+    e0: bool = True
+    do_mult: bool = True
+    long_range: bool = True
+    extras: dict = field(default_factory=dict)
+
+    @property
+    def nk(self):
+        return int(np.prod(self.kmesh))
+
+    @property
+    def nx(self):
+        return int(np.prod(self.xmesh))
+
+    @property
+    def nalpha(self):
+        return 4 * (2 * self.lc + 1)
+
+
+# ----------------------------------------------------------------------------
+# The action
+# ----------------------------------------------------------------------------
+def lr_act(P: Problem, psi):
+    """Screened direct (long-range) term, vectorised over x.
+
+    long_range.f90:758-870 (lr_act_traditional_x) and :873-963 (lr_kernel):
+      phi_k = sum_n u(n,k,x) psi(n,k,a)            (:897-906, no conjugation)
+      phi'  = (1/N_k) F_- [ W(:,x) . F_+ phi ]     (:928-939)
+      hpsi(n,k,a) -= conj(u(n,k,x)) phi'_k         (:943-961)
+    """
+    vs = val_spin_of_alpha(P.lc, P.nspn)
+    hp = np.zeros(psi.shape, np.complex128, order="F")
+    nk = P.nk
+    for a in range(psi.shape[2]):
+        us = P.u[:, :, :, vs[a]]
+        phi = np.einsum("nkx,nk->kx", us, psi[:, :, a], optimize=True)
+        t = fft_backward(phi, P.kmesh, axis=0)
+        t = t * P.W
+        t = fft_forward(t, P.kmesh, axis=0)
+        t = t * (1.0 / nk)
+        hp[:, :, a] -= np.einsum("nkx,kx->nk", np.conj(us), t, optimize=True)
+    return hp
+
+
+def lr_kernel_loops(P: Problem, psi):
+    """Loop-literal restatement of lr_kernel for tiny sizes (pins lr_act)."""
+    vs = val_spin_of_alpha(P.lc, P.nspn)
+    nb, nk, na = psi.shape
+    hpr = np.zeros((nb, nk, na))
+    hpi = np.zeros((nb, nk, na))
+    for a in range(na):
+        for x in range(P.nx):
+            ur = P.u[:, :, x, vs[a]].real
+            ui = P.u[:, :, x, vs[a]].imag
+            xr = np.zeros(nk)
+            xi = np.zeros(nk)
+            for k in range(nk):
+                xr[k] = np.dot(psi[:, k, a].real, ur[:, k]) - np.dot(psi[:, k, a].imag, ui[:, k])
+                xi[k] = np.dot(psi[:, k, a].real, ui[:, k]) + np.dot(psi[:, k, a].imag, ur[:, k])
+            s = xr + 1j * xi
+            # explicit DFTs (no library FFT) with the FFTW sign conventions
+            F = dft_matrix_backward(P.kmesh)
+            s = F @ s
+            s = s * P.W[:, x]
+            s = np.conj(F) @ s
+            xr = s.real / nk
+            xi = s.imag / nk
+            for k in range(nk):
+                hpr[:, k, a] += -ur[:, k] * xr[k] - ui[:, k] * xi[k]
+                hpi[:, k, a] += -ur[:, k] * xi[k] + ui[:, k] * xr[k]
+    return hpr + 1j * hpi
+
+
+def fg_combo(P: Problem, psi, inter):
+    """Core-site multiplet: projection, direct -M_l a, exchange X_l a, back-projection.
+
+    OCEAN_multiplet.f90:1617-2005.  amp = sum_{n,k} c psi (no conjugation,
+    :1746-1753); direct h = -mpm a (:1819-1832); exchange over the (nu, m, alpha)
+    block with mh(jj, ii) (:1867-1936); both x mul = s/(N_k Omega) (:1641);
+    back-projection hpsi += conj(c) h (:1943-1998).
+    """
+    M = P.mult
+    mul = inter / (P.nk * P.celvol)
+    nb, nk, na = psi.shape
+    nspn = P.nspn
+    out = np.zeros(psi.shape, np.complex128, order="F")
+    lmax = M.lmax
+    # amp[(l, m, nu), a]
+    amp = {}
+    for li, l in enumerate(range(M.lmin, M.lmax + 1)):
+        for m in range(-l, l + 1):
+            for nu in range(M.nproj[li]):
+                for a in range(na):
+                    ispn = (0 if (a % 2 == 0) else 1) if nspn == 2 else 0
+                    cc = M.c[:, :, nu, m + lmax, li, ispn]
+                    amp[(l, m, nu, a)] = np.sum(psi[:, :, a] * cc)
+    hamp = {}
+    for li, l in enumerate(range(M.lmin, M.lmax + 1)):
+        npl = M.nproj[li]
+        for m in range(-l, l + 1):
+            for a in range(na):
+                for mu in range(npl):
+                    s = 0j
+                    for nu in range(npl):
+                        s -= M.mpm[mu, nu, li] * amp[(l, m, nu, a)]
+                    hamp[(l, m, mu, a)] = s * mul
+    # exchange (requires lvl <= lmin and lvh >= lmax as the reference loop does)
+    for li, l in enumerate(range(M.lmin, M.lmax + 1)):
+        npl = M.nproj[li]
+        Xl = M.X[l - M.lvl]
+        mh = Xl.shape[0]
+        pw = np.zeros(mh, np.complex128)
+        for a in range(na):
+            for m in range(-l, l + 1):
+                for nu in range(npl):
+                    ii = nu + (m + l) * npl + a * (2 * l + 1) * npl
+                    pw[ii] = amp[(l, m, nu, a)]
+        hpw = (Xl.T @ pw) * mul  # hpw(ii) = sum_jj mh(jj, ii) pw(jj)
+        for a in range(na):
+            for m in range(-l, l + 1):
+                for nu in range(npl):
+                    ii = nu + (m + l) * npl + a * (2 * l + 1) * npl
+                    hamp[(l, m, nu, a)] += hpw[ii]
+    for a in range(na):
+        ispn = (0 if (a % 2 == 0) else 1) if nspn == 2 else 0
+        acc = np.zeros((nb, nk), np.complex128)
+        for li, l in enumerate(range(M.lmin, M.lmax + 1)):
+            for m in range(-l, l + 1):
+                for nu in range(M.nproj[li]):
+                    cc = M.c[:, :, nu, m + lmax, li, ispn]
+                    acc += np.conj(cc) * hamp[(l, m, nu, a)]
+        out[:, :, a] += acc
+    return out
+
+
+def new_soact(P: Problem, psi, backwards=False):
+    """hpsi(:,:,ic) += sum_jc (Sr +/- i Si)(ic, jc) psi(:,:,jc).
+
+    OCEAN_multiplet.f90:1543-1612; backwards conjugates Si.
+    """
+    M = P.mult
+    out = np.zeros(psi.shape, np.complex128, order="F")
+    if not M.have_spin_orbit:
+        return out
+    S = M.somelr - 1j * M.someli if backwards else M.somelr + 1j * M.someli
+    out[...] = np.einsum("ij,nkj->nki", S, psi)
+    return out
+
+
+def energies_act(P: Problem, psi, backwards=False):
+    """hpsi = (E + i Sigma'') psi; conjugate if backwards.
+
+    OCEAN_energies.f90:890-906 via OCEAN_psi_f2m_3element_mult (OCEAN_psi.f90:3213-3266).
+    """
+    e = np.conj(P.energy) if backwards else P.energy
+    return e * psi
+
+
+def xact(P: Problem, psi, backwards=False, hflag=(1, 0, 0, 0, 1, 1), inter=None):
+    """One H.psi, core branch, in the reference's order.
+
+    OCEAN_action.f90:286-618: psi_o = A.psi (:337-340); fg with SO off (:344-351);
+    lr if hflag(1) (:353-359); allow on the result (:361); SO on the unmasked psi
+    (:368-372); energies into min storage (:596-598) plus the interaction part (:608).
+    """
+    if inter is None:
+        inter = P.inter_scale
+    psi = np.asarray(psi, np.complex128)
+    psi_o = P.allow * psi
+    h = np.zeros(psi.shape, np.complex128, order="F")
+    if P.do_mult and P.mult is not None and hflag[4] == 1:
+        h += fg_combo(P, psi_o, inter)
+    if P.long_range and hflag[0] == 1:
+        h += lr_act(P, psi_o)
+    h = P.allow * h
+    if hflag[5] == 1 and P.do_mult and P.mult is not None:
+        h += new_soact(P, psi, backwards)
+    if P.e0:
+        out = energies_act(P, psi, backwards)
+    else:
+        out = np.zeros_like(h)
+    return np.asfortranarray(out + h)
+
+
+def dense_H(P: Problem, **kw):
+    """Brute-force dense matrix of xact (the OCEAN_exact idea; tiny sizes only)."""
+    shape = (P.nb, P.nk, P.nalpha)
+    N = int(np.prod(shape))
+    H = np.zeros((N, N), np.complex128)
+    for j in range(N):
+        e = np.zeros(N, np.complex128)
+        e[j] = 1.0
+        H[:, j] = xact(P, e.reshape(shape, order="F"), **kw).ravel(order="F")
+    return H
+
+
+# ----------------------------------------------------------------------------
+# Vector layer (OCEAN_psi.f90)
+# ----------------------------------------------------------------------------
+def psi_dot(p, q):
+    """(rval, ival) = (sum p_r q_r + p_i q_i, sum p_r q_i - p_i q_r).  OCEAN_psi.f90:1930-2126."""
+    d = np.vdot(p.ravel(order="F"), q.ravel(order="F"))
+    return d.real, d.imag
+
+
+def psi_nrm(p):
+    """Squared norm.  OCEAN_psi.f90:1673-1760."""
+    return float(np.sum(p.real ** 2 + p.imag ** 2))
+
+
+def core_pnorm(psi, nk, celvol):
+    """Normalise psi0 and return kpref = 4 pi |psi|^2 / (N_k Omega^2).  OCEAN_psi.f90:5589-5638."""
+    val = 0.0
+    for a in range(psi.shape[2]):
+        val += np.sum(psi[:, :, a].real ** 2 + psi[:, :, a].imag ** 2)
+    val = math.sqrt(val)
+    kpref = 4.0 * math.pi * val ** 2 / (nk * celvol ** 2)
+    return np.asfortranarray(psi * (1.0 / val)), kpref
+
+
+def psi_dotter(mels, cks, lc, nb, nk, nspn, bands_in_file=None):
+    """psi0(n,k,a) from core->projector dipole elements and projector overlaps.
+
+    OCEAN_psi.f90:5887-6031.  mels: complex (nptot, 2lc+1) [p, m_c]; cks: complex
+    (nptot, ntot, nspn) with ntot = bands_in_file * nk.  Nonzero only where core
+    spin equals valence spin (:5985); value = conj-combination (:5990-5995).
+    """
+    if bands_in_file is None:
+        bands_in_file = nb
+    na = 4 * (2 * lc + 1)
+    psi = np.zeros((nb, nk, na), np.complex128, order="F")
+    mer, mei = mels.real, mels.imag
+    ia = 0
+    for icms in (-1, 1):
+        for icml in range(-lc, lc + 1):
+            for ivms in (-1, 1):
+                if icms == ivms:
+                    sp = 0 if nspn == 1 else (0 if ivms == -1 else 1)
+                    pcr = cks[:, :, sp].real
+                    pci = cks[:, :, sp].imag
+                    it = 0
+                    for k in range(nk):
+                        sl = slice(it, it + nb)
+                        rr = mer[:, icml + lc] @ pcr[:, sl]
+                        ri = mer[:, icml + lc] @ pci[:, sl]
+                        ir = mei[:, icml + lc] @ pcr[:, sl]
+                        ii = mei[:, icml + lc] @ pci[:, sl]
+                        psi[:, k, ia] = (rr - ii) + 1j * (-ri - ir)
+                        it += bands_in_file
+                ia += 1
+    return psi
+
+
+# ----------------------------------------------------------------------------
+# Haydock
+# ----------------------------------------------------------------------------
+def haydock(P: Problem, psi0, niter, act=None):
+    """Hermitian two-term Haydock recursion.
+
+    OCEAN_haydock.f90:190-320 (loop) and :824-938 (OCEAN_hay_ab_twoterm):
+    h -= b(i-1) psi_old; a(i-1) = Re<h|psi>; h -= a(i-1) psi; b(i) = sqrt(<h|h>);
+    psi_old <- psi; psi <- h / b(i).  b(0) = 0, psi_old = 0 (:2223-2239).
+    Returns (real_a, real_b, imag_a, ierr); ierr = -1 on NaN in a (:912-920).
+    """
+    if act is None:
+        act = lambda v: xact(P, v)
+    psi = np.asfortranarray(np.array(psi0, np.complex128))
+    old = np.zeros_like(psi)
+    ra = np.zeros(niter + 1)
+    rb = np.zeros(niter + 1)
+    ia = np.zeros(niter + 1)
+    for it in range(1, niter + 1):
+        h = act(psi)
+        h = h + (-rb[it - 1]) * old
+        ar, ai = psi_dot(h, psi)
+        ra[it - 1] = ar
+        h = h + (-ar) * psi
+        bb = psi_nrm(h)
+        old = psi
+        rb[it] = math.sqrt(bb)
+        psi = h * (1.0 / rb[it])
+        ia[it - 1] = ai
+        if ar != ar:
+            return ra, rb, ia, -1
+    return ra, rb, ia, 0
+
+
+def write_core(a, b, niter, kpref, gam0, el, eh, ne, imag_a=None, literal_cmplx=True,
+               hartree2ev=HARTREE2EV, ebase=0.0):
+    """Continued fraction with square-root terminator (Lorentzian, gamGauss off).
+
+    OCEAN_haydock.f90:1801-1894.  With literal_cmplx the three kind-less
+    ``cmplx(...)`` sites (:1846-1847, :1870-1871) are rounded to default
+    (single) precision, as OCEAN2/Makefile builds them without default-real
+    promotion.  Returns columns (E_eV, Spect, Spect(0), SPKK, SPKK(0)).
+    """
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    ia = np.zeros_like(a) if imag_a is None else np.asarray(imag_a, np.float64)
+    n = niter
+    ie = np.arange(0, 2 * ne + 1, 2, dtype=np.float64)
+    e = el + (eh - el) * ie / (2 * ne)
+    gam = gam0
+    f32 = (lambda v: np.float64(np.float32(v))) if literal_cmplx else (lambda v: v)
+
+    def bc(j):  # cmplx(b) * cmplx(c), c = b, imag parts 0
+        if literal_cmplx:
+            return np.float64(np.float32(np.float32(b[j]) * np.float32(b[j])))
+        return b[j] * b[j]
+
+    ctmp = (e - a[n - 1]) + 1j * (gam + ia[n - 1])
+    if literal_cmplx:
+        four_bc = np.float64(np.float32(np.float32(4.0 * np.float32(b[n])) * np.float32(b[n])))
+    else:
+        four_bc = 4.0 * b[n] * b[n]
+    disc = np.sqrt(ctmp ** 2 - four_bc)
+    delta = np.where(disc.imag > 0.0, (ctmp + disc) / 2.0, (ctmp - disc) / 2.0)
+    for jj in range(n - 1, -1, -1):
+        t = bc(jj + 1)
+        if literal_cmplx:
+            z = np.float32(e - a[jj]).astype(np.float64) + 1j * np.float64(np.float32(gam + ia[jj]))
+        else:
+            z = (e - a[jj]) + 1j * (gam + ia[jj])
+        delta = z - t / delta
+    dr = delta.real
+    di = np.abs(delta.imag)
+    spct = kpref * di / (dr ** 2 + di ** 2)
+    spkk = kpref * dr / (dr ** 2 + di ** 2)
+    ener = ebase + hartree2ev * e
+    del f32
+    return np.stack([ener, spct, spct, spkk, spkk], axis=1)
+
+
+def redtrid_eigs(a, b, n):
+    """Eigenvalues of the (n+1) x (n+1) tridiagonal Lanczos matrix.
+
+    OCEAN_haydock.f90:2273-2316 (redtrid, via elsch; Hermitian case).
+    """
+    T = np.zeros((n + 1, n + 1))
+    for i in range(n + 1):
+        T[i, i] = a[i]
+        if i < n:
+            T[i + 1, i] = b[i + 1]
+            T[i, i + 1] = b[i + 1]
+    return np.linalg.eigvalsh(T)
+
+
+# ----------------------------------------------------------------------------
+# GMRES (recycled, preconditioned)
+# ----------------------------------------------------------------------------
+def min_set_prec(energy, gprc, h1):
+    """pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2).  OCEAN_psi.f90:2907-2994."""
+    ct = (energy - h1) ** 2 + gprc * gprc
+    num = energy - (h1.real + 1j * (h1.imag + gprc))
+    return np.asfortranarray(num / ct)
+
+
+def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth,
+                     kpref, act=None, nhflag=(1, 0, 0, 0, 1, 1), return_x=False):
+    """Recycled, preconditioned GMRES over a list of energies.
+
+    OCEAN_gmres.f90:508-908 (OCEAN_gmres_do_recycle), :241-344 (update_gmres:
+    normal-equation least squares on the au basis, zhptrf/zhptrs), :207-239
+    (make_recycle), :1213-1283 (set_initial_vector; force_reuse_x = .true.),
+    OCEAN_psi.f90:2907-2994 (preconditioner).  Output per energy:
+    (E_eV, 1 - Re<b|x> f, -Im<b|x> f, rel_error, complete_iter, global_iter).
+    """
+    if act is None:
+        act = lambda v: xact(P, v, hflag=nhflag)
+    gres = gres_eV * EV2HARTREE
+    gprc = gprc_eV * EV2HARTREE
+    rhs = np.asfortranarray(P.allow * np.asarray(hay_vec, np.complex128))  # sfact_copy
+    one = np.ones(rhs.shape, np.complex128, order="F")
+    h1 = act(one)
+    x = np.zeros_like(rhs)
+    ax = np.zeros_like(rhs)
+    u_mat = [None] * (depth + 1)
+    au_mat = [None] * (depth + 1)
+    C = np.zeros((depth, depth), np.complex128)  # upper triangle of the packed Gram matrix
+    rows = []
+    xs = []
+    global_iter = 0
+    prev_iter = 0
+    prev_energy = -np.inf
+    pcdiv = None
+    gval = 0.0
+    rval = ival = 0.0
+
+    def update(cur, g):
+        # C(i, cur) = conj(<au_cur|au_i>) for i < cur; C(cur,cur) = |au_cur|^2
+        nonlocal x, ax
+        for i in range(cur):
+            r, im = psi_dot(au_mat[cur], au_mat[i + 1]) if i < cur - 1 else (psi_nrm(au_mat[cur]), 0.0)
+            C[i, cur - 1] = r - 1j * im
+        rvec = np.array([complex(*psi_dot(au_mat[i + 1], g)) for i in range(cur)])
+        A = np.triu(C[:cur, :cur])
+        A = A + np.conj(np.triu(A, 1)).T
+        coeff = np.linalg.solve(A, -rvec)
+        for i in range(cur):
+            x = x + coeff[i] * u_mat[i + 1]
+            ax = ax + coeff[i] * au_mat[i + 1]
+
+    energies = list(energies_eV)
+    for step, E_eV in enumerate(energies, start=1):
+        complete_iter = 0
+        ener = E_eV * EV2HARTREE
+        if prev_iter == 0:
+            prev_energy = ener
+            pcdiv = min_set_prec(ener, gprc, h1)
+        if prev_iter == 0:
+            # set_initial_vector, 'keep' for step > 1 (force_reuse_x)
+            if step > 1:
+                ax = ener * x + 1j * gres * x - act(x)
+            else:
+                x = np.zeros_like(rhs)
+                ax = np.zeros_like(rhs)
+            g = ax - rhs
+        else:
+            delta = ener - energies[step - 2] * EV2HARTREE
+            for i in range(1, prev_iter + 1):
+                au_mat[i] = au_mat[i] + delta * u_mat[i]
+            ax = ener * x + 1j * gres * x - act(x)
+            g = ax - rhs
+            for it in range(1, prev_iter + 1):
+                update(it, g)
+                g = ax - rhs
+            gval = psi_nrm(g)
+            rval, ival = psi_dot(rhs, x)
+            converged = False
+            if gval < ffff:
+                rel = -gval / ival
+                if rel < 0.001 or gval < ffff / 10.0:
+                    converged = True
+            if converged:
+                rows.append((E_eV, 1.0 - rval * kpref, -ival * kpref, -gval / ival, complete_iter,
+                             global_iter))
+                xs.append(x.copy())
+                continue
+            if abs(prev_energy - ener) > gprc / 4.0:
+                prev_iter = 0
+                prev_energy = ener
+                pcdiv = min_set_prec(ener, gprc, h1)
+        done = False
+        while not done:
+            for it in range(prev_iter + 1, depth + 1):
+                complete_iter += 1
+                global_iter += 1
+                pg = g * pcdiv
+                u_mat[it] = pg
+                apg = ener * pg + 1j * gres * pg - act(pg)
+                au_mat[it] = apg
+                update(it, g)
+                if it == depth:
+                    ax = ener * x + 1j * gres * x - act(x)
+                    if ener != prev_energy:
+                        pcdiv = min_set_prec(ener, gprc, h1)
+                        prev_energy = ener
+                    prev_iter = 0
+                g = ax - rhs
+                gval = psi_nrm(g)
+                rval, ival = psi_dot(rhs, x)
+                if gval < ffff:
+                    rel = -gval / ival
+                    if rel < 0.001 or gval < ffff / 10.0:
+                        prev_iter = it
+                        if it == depth:
+                            prev_iter = 0
+                        done = True
+                        break
+            else:
+                prev_iter = 0
+                continue
+        rows.append((E_eV, 1.0 - rval * kpref, -ival * kpref, -gval / ival, complete_iter, global_iter))
+        xs.append(x.copy())
+    out = np.array(rows)
+    return (out, xs) if return_x else out

@@ -1,0 +1,137 @@
+"""GPU (HIP C-ABI) vs the oracle on identical seeded inputs.
+
+Tolerances: the FP64 drop-in is held to 1e-12 relative on H.psi, Haydock a/b to
+1e-10 relative over the first 50 steps, spectra to 1e-8 of peak.
+"""
+import numpy as np
+import pytest
+
+import ocean_oracle as orc
+from helpers import relerr, to_problem
+
+pytestmark = pytest.mark.gpu
+
+from ocean_amd import synth  # noqa: E402
+
+CASES = {
+    "k_tiny": dict(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=3),
+    "k_odd": dict(nb=16, kmesh=(3, 4, 2), xmesh=(4, 3, 5), lc=0, seed=4),
+    "l23_so": dict(nb=16, kmesh=(4, 4, 4), xmesh=(4, 4, 4), lc=1, seed=5, xi_eV=6.0),
+    "k_6cube": dict(nb=32, kmesh=(6, 6, 6), xmesh=(6, 6, 6), lc=0, seed=6),
+    "nspn2": dict(nb=8, kmesh=(2, 2, 4), xmesh=(3, 3, 3), lc=1, seed=7, nspn=2, xi_eV=3.0),
+}
+
+
+def _gpu(inp, **kw):
+    from ocean_amd import core
+    return core.OceanGPU.from_inputs(inp, **kw)
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    return synth.config(0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_xact_matches_oracle(name):
+    inp = synth.make_inputs(**CASES[name])
+    print("case", name, flush=True)
+    P = to_problem(inp)
+    g = _gpu(inp)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape)
+    psi = np.asfortranarray(psi)
+    for back in (False, True):
+        ref = orc.xact(P, psi, backwards=back)
+        got = g.xact(psi, backwards=back)
+        assert relerr(got, ref) < 1e-12, (name, back, relerr(got, ref))
+    # individual terms through hflag
+    for hf in ((1, 0, 0, 0, 0, 0), (0, 0, 0, 0, 1, 0), (0, 0, 0, 0, 0, 1)):
+        ref = orc.xact(P, psi, hflag=hf)
+        got = g.xact(psi, hflag=hf)
+        assert relerr(got, ref) < 1e-12, (name, hf)
+    g.close()
+
+
+def test_xact_config0(cfg0):
+    P = to_problem(cfg0)
+    g = _gpu(cfg0)
+    ref = orc.xact(P, cfg0.psi0)
+    got = g.xact(cfg0.psi0)
+    assert relerr(got, ref) < 1e-12
+    g.close()
+
+
+def test_haydock_config0(cfg0):
+    P = to_problem(cfg0)
+    g = _gpu(cfg0)
+    niter = cfg0.params["niter"]
+    a_ref, b_ref, ia_ref, ierr = orc.haydock(P, cfg0.psi0, niter)
+    a, b, ai = g.haydock(cfg0.psi0, niter)
+    assert ierr == 0
+    ra = np.abs(a[:50] - a_ref[:50]) / np.abs(a_ref[:50])
+    rb = np.abs(b[1:51] - b_ref[1:51]) / np.abs(b_ref[1:51])
+    print("max rel a", ra.max(), "b", rb.max())
+    assert ra.max() < 1e-10 and rb.max() < 1e-10
+    # spectra (literal write_core) within 1e-8 of peak
+    p = cfg0.params
+    E2H = orc.EV2HARTREE
+    sp_ref = orc.write_core(a_ref, b_ref, niter, cfg0.kpref, p["gam"] * E2H, p["el"] * E2H,
+                            p["eh"] * E2H, p["ne"])
+    from ocean_amd import core
+    sp = core.spectrum(a, b, niter, cfg0.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])
+    peak = np.abs(sp_ref[:, 2]).max()
+    err = np.abs(sp[:, 2] - sp_ref[:, 2]).max() / peak
+    print("spectrum err/peak", err)
+    assert err < 1e-8
+    g.close()
+
+
+def test_spectrum_kernel_matches_oracle():
+    from ocean_amd import core
+    rng = np.random.default_rng(2)
+    n = 80
+    a = rng.uniform(0.2, 1.0, n + 1)
+    b = rng.uniform(0.1, 0.5, n + 1)
+    b[0] = 0
+    for lit in (True, False):
+        ref = orc.write_core(a, b, n, 1.3e-3, 0.02, -0.2, 1.5, 1500, literal_cmplx=lit)
+        got = core.spectrum(a, b, n, 1.3e-3, 0.02, -0.2, 1.5, 1500, literal_cmplx=lit)
+        pk = np.abs(ref[:, 2]).max()
+        assert np.abs(got[:, 2] - ref[:, 2]).max() / pk < 1e-12
+        assert np.abs(got[:, 3] - ref[:, 3]).max() / np.abs(ref[:, 3]).max() < 1e-12
+        assert np.allclose(got[:, 0], ref[:, 0], rtol=0, atol=1e-12)
+
+
+def test_vector_ops(cfg0):
+    g = _gpu(cfg0)
+    rng = np.random.default_rng(5)
+    p = np.asfortranarray(rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
+    q = np.asfortranarray(rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
+    vp, vq = g.vec_new(p), g.vec_new(q)
+    r, i = g.vec_dot(vp, vq)
+    rr, ii = orc.psi_dot(p, q)
+    assert abs(r - rr) < 1e-10 * abs(rr) and abs(i - ii) < 1e-10 * max(abs(ii), 1)
+    assert abs(g.vec_nrm(vp) - orc.psi_nrm(p)) < 1e-12 * orc.psi_nrm(p)
+    g.vec_axpy(0.3 - 0.7j, vp, vq)
+    assert relerr(g.vec_get(vq), q + (0.3 - 0.7j) * p) < 1e-14
+    g.vec_scal(2.5, vp)
+    assert relerr(g.vec_get(vp), 2.5 * p) < 1e-15
+    vo = g.vec_new()
+    g.xact_vec(vp, vo)
+    assert relerr(g.vec_get(vo), orc.xact(to_problem(cfg0), 2.5 * p)) < 1e-12
+    g.close()
+
+
+def test_gmres_small():
+    inp = synth.make_inputs(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=9)
+    P = to_problem(inp)
+    g = _gpu(inp)
+    E = np.linspace(-2.0, 30.0, 6)
+    ref = orc.gmres_do_recycle(P, inp.psi0, E, 0.5, 13.605, 1e-10, 20, inp.kpref)
+    got = g.gmres(inp.psi0, E, 0.5, 13.605, 1e-10, 20)
+    print(ref[:, :3], got[:, :3])
+    pk = np.abs(ref[:, 2]).max()
+    assert np.abs(got[:, 2] - ref[:, 2]).max() / pk < 1e-8
+    assert np.array_equal(got[:, 4], ref[:, 4])
+    g.close()
