@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark of the core-level BSE action on one MI355X.
+
+Workload (the drop-in timing configuration): K edge, one site, nb = 64 bands,
+k-mesh 8^3, x-mesh 8^3, nalpha = 4, synthetic seeded inputs resident in HBM.
+A step is one H.psi (OCEAN_xact) in FP64.  The JSON line also carries the mixed
+precision H.psi rate, Haydock spectra/s (300 iterations + continued fraction) in
+both forms, the roofline of the dominant kernel, and the host-core CPU baseline
+(the C/OpenMP restatement timed on a bounded sample of the same workload).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("BSE H·ψ/s & spectra/s (64 bands, 8³ k, 8³ x); "
+          "FP64 plain vs mixed+batched")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(inp):
+    """Algorithmic HBM bytes per launch for each kernel family (FP64 or FP32 u)."""
+    nb, nk, nx, na, ns = inp.nb, inp.nk, inp.nx, inp.nalpha, inp.nspn
+    N = nb * nk * na
+    P = sum((2 * l + 1) * npj for l, npj in zip(range(inp.mult.lmin, inp.mult.lmax + 1), inp.mult.nproj))
+    nphi = nk * nx * na
+
+    def fam(ub, cb):
+        u = ub * nb * nk * nx * ns
+        return {
+            "proj": u + 24 * N + cb * nphi,                 # u, psi+allow, phi out
+            "fft": 2 * cb * nphi + (ub // 2) * nk * nx,     # phi in/out, W
+            "adj": u + cb * nphi + 16 * nb * nk * P * ns + 16 * N * 3 + 8 * N,  # u, phi', c, psi+E+out, allow
+        }
+    return {"fp64": fam(16, 16), "mixed": fam(8, 8)}
+
+
+def cpu_baseline(inp, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import c_oracle  # the checker/baseline, never the measured GPU product
+
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    cs = c_oracle.CSystem(inp)
+    t0 = time.perf_counter()
+    cs.xact(inp.psi0, nthreads=nthr)
+    one = time.perf_counter() - t0
+    nrep = max(1, min(200, int(budget_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    cs.xact(inp.psi0, nthreads=nthr, nrep=nrep)
+    dt = time.perf_counter() - t0
+    return {"value": nrep / dt, "unit": "H·ψ/s", "cores": nthr, "kind": "port",
+            "sample": f"{nrep} FP64 H.psi applications of the same workload "
+                      f"(C/OpenMP restatement, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--hay-solves", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    from ocean_amd import core, synth
+
+    t0 = time.perf_counter()
+    inp = synth.config(1)
+    log(f"inputs ready in {time.perf_counter() - t0:.1f}s")
+    g = core.OceanGPU.from_inputs(inp, precision=core.FP64)
+    vin = g.vec_new(inp.psi0)
+    vout = g.vec_new()
+    K, W = args.steps, args.warmup
+
+    g.xact_repeat(vin, vout, max(W, 1))
+    t0 = time.perf_counter()
+    ev_ms = g.xact_repeat(vin, vout, K)
+    wall = time.perf_counter() - t0
+    hpsi_fp64 = K / wall
+    log(f"FP64 H.psi: {wall / K * 1e3:.4f} ms wall, {ev_ms:.4f} ms events")
+    prof = g.profile_xact(vin, vout, max(20, K // 5))
+    log("per-kernel ms:", prof)
+
+    bytes_ = kernel_bytes(inp)
+    fam = max(("proj", "fft", "adj"), key=lambda k: prof[k])
+    kname = {"proj": "k_proj", "fft": "k_fft_w", "adj": "k_adj"}[fam]
+    ach = bytes_["fp64"][fam] / (prof[fam] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": bytes_["fp64"][fam],
+            "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()}}
+
+    extras = {}
+    if not args.no_extras:
+        p = inp.params
+        E2H = core.EV2HARTREE
+        # spectra/s, FP64
+        def spectra(n):
+            t = time.perf_counter()
+            for _ in range(n):
+                a, b, _ = g.haydock(inp.psi0, p["niter"])
+                core.spectrum(a, b, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H,
+                              p["eh"] * E2H, p["ne"])
+            return n / (time.perf_counter() - t), a, b
+        spectra(1)
+        sps64, a64, b64 = spectra(args.hay_solves)
+        extras["spectra_per_s_fp64"] = round(sps64, 3)
+        # mixed precision
+        g.set_precision(core.MIXED)
+        g.xact_repeat(vin, vout, max(W, 1))
+        t1 = time.perf_counter()
+        g.xact_repeat(vin, vout, K)
+        wall_m = time.perf_counter() - t1
+        extras["hpsi_per_s_mixed"] = round(K / wall_m, 1)
+        profm = g.profile_xact(vin, vout, max(20, K // 5))
+        extras["per_kernel_ms_mixed"] = {k: round(v, 5) for k, v in profm.items()}
+        spsm, am, bm = spectra(args.hay_solves)
+        extras["spectra_per_s_mixed"] = round(spsm, 3)
+        s64 = core.spectrum(a64, b64, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H,
+                            p["eh"] * E2H, p["ne"])
+        smx = core.spectrum(am, bm, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H,
+                            p["eh"] * E2H, p["ne"])
+        pk = np.abs(s64[:, 2]).max()
+        extras["mixed_spectrum_err_of_peak"] = float(np.abs(smx[:, 2] - s64[:, 2]).max() / pk)
+        extras["mixed_speedup_hpsi"] = round(extras["hpsi_per_s_mixed"] / hpsi_fp64, 3)
+        g.set_precision(core.FP64)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(inp, args.cpu_seconds)
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": METRIC, "value": round(hpsi_fp64, 1), "unit": "H·ψ/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "K-edge core BSE H.psi, FP64 drop-in (nb=64, k-mesh 8^3, x-mesh 8^3, "
+                               "nalpha=4, one site)", "nb": 64, "kmesh": [8, 8, 8], "xmesh": [8, 8, 8],
+                   "nalpha": 4, "haydock_niter": inp.params["niter"]},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    out.update(extras)
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -135,8 +135,12 @@ int  ocean_gpu_vec_copy(ocean_gpu_handle* h, int32_t dst, int32_t src);
 /* xact between device vectors; the form bench.py times (inputs resident in HBM). */
 int  ocean_gpu_xact_vec(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t backwards,
                         const int32_t* hflag);
-/* Repeat xact_vec nrep times on one stream with no host synchronisation. */
+/* Repeat xact_vec nrep times on one stream with no host synchronisation in
+ * between (returns after the last one completes). */
 int  ocean_gpu_xact_vec_repeat(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t nrep);
+/* nrep actions with HIP events between the kernel families, for per-kernel
+ * average durations (read back with ocean_gpu_last_timing). */
+int  ocean_gpu_profile_xact(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t nrep);
 
 /* ---- solvers ---- */
 /* Hermitian Haydock recursion (OCEAN2/OCEAN_haydock.f90:190-320, :824-938).
@@ -166,9 +170,9 @@ int  ocean_gpu_spectrum(const double* real_a, const double* real_b, const double
                         int32_t niter, double kpref, double gam, double el, double eh,
                         int32_t ne, int32_t literal_cmplx, double* out);
 
-/* Kernel-time probe used by the benchmark: average duration (ms) of the last
- * ocean_gpu_xact_vec_repeat call per kernel family, measured with HIP events on
- * the library's stream.  names: "proj", "fft", "adj", "mult", "total". */
+/* Average durations (ms) measured with HIP events on the library's stream:
+ * "total" from the last ocean_gpu_xact_vec_repeat, "proj", "fft", "mult", "adj"
+ * from the last ocean_gpu_profile_xact. */
 int  ocean_gpu_last_timing(ocean_gpu_handle* h, const char* name, double* ms);
 
 #ifdef __cplusplus

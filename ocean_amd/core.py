@@ -74,6 +74,7 @@ _SIGS = {
     "ocean_gpu_vec_copy": (C.c_int, [_H, C.c_int32, C.c_int32]),
     "ocean_gpu_xact_vec": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32, _IP]),
     "ocean_gpu_xact_vec_repeat": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
+    "ocean_gpu_profile_xact": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
     "ocean_gpu_haydock": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, _DP, _DP, _IP]),
     "ocean_gpu_gmres": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
                                   C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
@@ -294,9 +295,20 @@ class OceanGPU:
                                        None if hf is None else hf.ctypes.data_as(_IP)), "xact_vec")
 
     def xact_repeat(self, vin, vout, nrep):
+        """nrep back-to-back actions; returns the event-timed ms per action."""
         _chk(_lib().ocean_gpu_xact_vec_repeat(self.h, vin, vout, nrep), "xact_vec_repeat")
+        return self.timing("total")
+
+    def timing(self, name):
+        v = C.c_double()
+        _chk(_lib().ocean_gpu_last_timing(self.h, name.encode(), C.byref(v)), "last_timing")
+        return v.value
+
+    def profile_xact(self, vin, vout, nrep):
+        """Per-kernel-family average durations (ms) with events between the families."""
+        _chk(_lib().ocean_gpu_profile_xact(self.h, vin, vout, nrep), "profile_xact")
         out = {}
-        for k in ("proj", "fft", "adj", "mult", "total"):
+        for k in ("proj", "fft", "adj", "mult"):
             v = C.c_double()
             _chk(_lib().ocean_gpu_last_timing(self.h, k.encode(), C.byref(v)), "last_timing")
             out[k] = v.value

@@ -129,13 +129,22 @@ int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename o
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
   const int ncol = h->nx * h->na;
-  const int cap = (int)(65536 / sizeof(T2));
-  int G = std::max(1, cap / h->nk);
+  const int cap = (int)(57344 / sizeof(T2));
+  // G columns per workgroup: one x (all alphas) for 8^3 meshes, more for small meshes
+  int G = std::max(1, std::min(cap / h->nk, (256 * 8) / h->nk));
   G = std::min(G, ncol);
   dim3 g2((ncol + G - 1) / G);
-  size_t sm2 = (size_t)h->nk * G * sizeof(T2);
-  og::k_fft_w<T2><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
-                                                  h->km[2]);
+  const size_t tile = (size_t)h->nk * G;
+  const int lmax = std::max(h->km[0], std::max(h->km[1], h->km[2]));
+  if (lmax <= 8) {
+    size_t sm2 = (6 * 8 + tile + tile / 8 + 1) * sizeof(T2);
+    og::k_fft_w<T2, 8><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
+                                                       h->km[2]);
+  } else {
+    size_t sm2 = (6 * 16 + tile + tile / 8 + 1) * sizeof(T2);
+    og::k_fft_w<T2, 16><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
+                                                        h->km[2]);
+  }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
   return OCEAN_GPU_OK;
@@ -144,14 +153,20 @@ int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename o
 template <typename T2, int NA>
 int launch_adj(H* h, const double2* in, double2* out, const T2* u, const T2* phi, int flags) {
   const int ns = h->ns;
+  // split the band rows of each k over several workgroups so that the grid
+  // fills the 256 CUs several times over (>= 2048 workgroups when possible)
+  int nsplit = std::max(1, std::min(h->nb / 4, 2048 / std::max(h->nk, 1)));
+  const int rpb = (h->nb + nsplit - 1) / nsplit;
+  nsplit = (h->nb + rpb - 1) / rpb;
+  const size_t resb = (size_t)h->na * rpb * sizeof(double2);
   int XC = h->nx;
-  const size_t budget = 60 * 1024 - (size_t)h->na * h->nb * sizeof(double2);
+  const size_t budget = 60 * 1024 - resb;
   const int cap = (int)(budget / (NA * sizeof(T2)));
   if (XC > cap) XC = std::max(64, (cap / 64) * 64);
-  size_t sm = (size_t)h->na * h->nb * sizeof(double2) + (size_t)XC * NA * sizeof(T2);
-  og::k_adj<T2, NA><<<h->nk, kBlock, sm, h->stream>>>(u, phi, in, h->energy, h->allow, h->cproj,
-                                                      h->hmult, h->Smat, out, h->nb, h->nk, h->nx,
-                                                      h->na, ns, h->P, XC, flags);
+  size_t sm = resb + (size_t)XC * NA * sizeof(T2);
+  og::k_adj<T2, NA><<<dim3(h->nk * nsplit), kBlock, sm, h->stream>>>(
+      u, phi, in, h->energy, h->allow, h->cproj, h->hmult, h->Smat, out, h->nb, h->nk, h->nx, h->na, ns,
+      h->P, XC, rpb, flags);
   OG_CHECK(hipGetLastError());
   return OCEAN_GPU_OK;
 }
@@ -160,20 +175,36 @@ template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
+  // The multiplet projection and reduction depend only on psi: run them on the
+  // side stream, overlapped with the projection and FFT, and join before the
+  // adjoint (which consumes hmult).  Timed runs (ev != null) keep one stream.
+  const bool fork = (flags & og::F_FG) && !ev;
+  if (fork) {
+    OG_CHECK(hipEventRecord(h->ev_fork, h->stream));
+    OG_CHECK(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
+    og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->side>>>(h->cproj, in, h->allow, h->apart, h->nb,
+                                                           h->nk, h->na, h->ns, h->P);
+    OG_CHECK(hipGetLastError());
+    og::k_mred<<<1, 1024, 0, h->side>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
+    OG_CHECK(hipGetLastError());
+    OG_CHECK(hipEventRecord(h->ev_join, h->side));
+  }
   if (flags & og::F_LR) {
     OG_TRY((launch_lr_proj_fft<T2, NA>(h, in, u, phi, W, ev)));
   } else if (ev) {
     OG_CHECK(hipEventRecord(ev[1], h->stream));
     OG_CHECK(hipEventRecord(ev[2], h->stream));
   }
-  if (flags & og::F_FG) {
+  if ((flags & og::F_FG) && !fork) {
     og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->stream>>>(h->cproj, in, h->allow, h->apart, h->nb,
                                                              h->nk, h->na, h->ns, h->P);
     OG_CHECK(hipGetLastError());
     const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
-    og::k_mred<<<1, kBlock, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
+    og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
     OG_CHECK(hipGetLastError());
   }
+  if (fork) OG_CHECK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
   OG_TRY((launch_adj<T2, NA>(h, in, out, u, phi, flags)));
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
@@ -427,8 +458,11 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   }
   h->P = P;
   if (h->na * std::max(P, 1) > og::kMaxAlpha * 64) { delete h; return OCEAN_GPU_ERR_ARG; }
-  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
-    delete h;
+  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
+      hipStreamCreate(&h->side) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    ocean_gpu_destroy(h);
     return OCEAN_GPU_ERR_HIP;
   }
   int rc = OCEAN_GPU_OK;
@@ -456,6 +490,10 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
   for (auto& v : h->vecs) dfree(v);
   if (h->host_pin) (void)hipHostFree(h->host_pin);
+  if (h->side) (void)hipStreamSynchronize(h->side);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return OCEAN_GPU_OK;
@@ -765,20 +803,28 @@ int ocean_gpu_xact_vec_repeat(ocean_gpu_handle* h, int32_t in, int32_t out, int3
   if (!h || !vec_ptr(h, in) || !vec_ptr(h, out) || in == out || nrep <= 0) return OCEAN_GPU_ERR_ARG;
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
-  // Two timing passes on the library stream: (1) nrep back-to-back actions
-  // bracketed by one event pair (throughput); (2) nrep actions with events
-  // between the kernel families (per-family average duration).
-  hipEvent_t e0, e1, ev[5];
+  hipEvent_t e0, e1;
   OG_CHECK(hipEventCreate(&e0));
   OG_CHECK(hipEventCreate(&e1));
-  for (auto& e : ev) OG_CHECK(hipEventCreate(&e));
   OG_CHECK(hipEventRecord(e0, h->stream));
-  for (int r = 0; r < nrep; ++r) OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f));
+  int rc = OCEAN_GPU_OK;
+  for (int r = 0; r < nrep && rc == 0; ++r) rc = xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f);
   OG_CHECK(hipEventRecord(e1, h->stream));
   OG_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   OG_CHECK(hipEventElapsedTime(&ms, e0, e1));
   h->t_total = ms / nrep;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int ocean_gpu_profile_xact(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t nrep) {
+  if (!h || !vec_ptr(h, in) || !vec_ptr(h, out) || in == out || nrep <= 0) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  hipEvent_t ev[5];
+  for (auto& e : ev) OG_CHECK(hipEventCreate(&e));
   double tp = 0, tf = 0, tm = 0, ta = 0;
   for (int r = 0; r < nrep; ++r) {
     OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f, ev));
@@ -791,8 +837,6 @@ int ocean_gpu_xact_vec_repeat(ocean_gpu_handle* h, int32_t in, int32_t out, int3
     tp += a; tf += b; tm += c; ta += d;
   }
   h->t_proj = tp / nrep; h->t_fft = tf / nrep; h->t_mult = tm / nrep; h->t_adj = ta / nrep;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   for (auto& e : ev) (void)hipEventDestroy(e);
   return OCEAN_GPU_OK;
 }

@@ -72,6 +72,49 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Sum of V values held by every lane over the 64 lanes of a wave, by halving:
+// at each of the first log2(V) butterfly steps a lane keeps half of its values
+// and receives the partner's copy of that half, so V values cost about V + 5
+// shuffles instead of 6V.  On return lane l owns value rev3(l): the value index
+// is read from the lane's high bits (bit 5 selects the upper half first).
+// V must be a power of two <= 64; lanes hold the total in v[0].
+template <typename T, int W, int BIT>
+__device__ __forceinline__ void multi_sum_step(T* v, int lane, int& idx) {
+  if constexpr (BIT >= 1) {
+    if constexpr (W > 1) {
+      constexpr int H = W / 2;
+      const bool upper = (lane & BIT) != 0;
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const T send = upper ? v[j] : v[j + H];
+        const T keep = upper ? v[j + H] : v[j];
+        v[j] = keep + __shfl_xor(send, BIT, 64);
+      }
+      idx = idx * 2 + (upper ? 1 : 0);
+      multi_sum_step<T, H, BIT / 2>(v, lane, idx);
+    } else {
+      v[0] += __shfl_xor(v[0], BIT, 64);
+      multi_sum_step<T, 1, BIT / 2>(v, lane, idx);
+    }
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ T wave_multi_sum(T (&v)[V], int lane, int* owner) {
+  static_assert((V & (V - 1)) == 0 && V <= 64, "V must be a power of two <= 64");
+  int idx = 0;
+  multi_sum_step<T, V, 32>(v, lane, idx);
+  *owner = idx;
+  return v[0];
+}
+
+// Deterministic sum of nblk partials part[b*nv + j] by one wave (fixed order).
+__device__ __forceinline__ double wave_sum_parts(const double* part, int nblk, int nv, int j, int lane) {
+  double s = 0.0;
+  for (int b = lane; b < nblk; b += 64) s += part[(size_t)b * nv + j];
+  return wave_sum(s);
+}
+
 // Block-wide sum of nv doubles per thread into out[0..nv) (valid in thread 0).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= 4*NV */) {
@@ -108,6 +151,8 @@ struct ocean_gpu_handle {
   int precision = OCEAN_GPU_FP64;
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;      // multiplet kernels overlap projection + FFT
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // long-range state
   double2* u64 = nullptr;   // [s][k][n][x]

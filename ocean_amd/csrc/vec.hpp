@@ -24,16 +24,13 @@ __global__ __launch_bounds__(kBlock) void k_dot_part(const double2* __restrict__
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
 }
 
-// sum of nv-wide partials over nb blocks, fixed order
-__device__ __forceinline__ double sum_parts(const double* part, int nblk, int nv, int j) {
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[b * nv + j];
-  return s;
-}
-
+// one wave per value: out[j] = sum_b part[b*nv + j]
 __global__ void k_finish(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
-  const int j = threadIdx.x;
-  if (j < nv) out[j] = sum_parts(part, nblk, nv, j);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int j = wid; j < nv; j += nw) {
+    const double s = wave_sum_parts(part, nblk, nv, j, lane);
+    if (lane == 0) out[j] = s;
+  }
 }
 
 // squared norm partials   OCEAN_psi.f90:1673-1760
@@ -116,10 +113,13 @@ __global__ __launch_bounds__(kBlock) void k_hay2(double2* __restrict__ h, const 
                                                   double* __restrict__ part2) {
   __shared__ double lds[8];
   __shared__ double s_a;
-  if (threadIdx.x == 0) {
-    const double ar = sum_parts(part1, nblk, 2, 0);
-    s_a = ar;
-    if (blockIdx.x == 0) { a_out[0] = ar; ai_out[0] = sum_parts(part1, nblk, 2, 1); }
+  if (threadIdx.x < 64) {
+    const double ar = wave_sum_parts(part1, nblk, 2, 0, threadIdx.x);
+    const double ai = wave_sum_parts(part1, nblk, 2, 1, threadIdx.x);
+    if (threadIdx.x == 0) {
+      s_a = ar;
+      if (blockIdx.x == 0) { a_out[0] = ar; ai_out[0] = ai; }
+    }
   }
   __syncthreads();
   const double ma = -s_a;
@@ -139,10 +139,12 @@ __global__ __launch_bounds__(kBlock) void k_hay2(double2* __restrict__ h, const 
 __global__ __launch_bounds__(kBlock) void k_hay3(double2* __restrict__ h, const double* __restrict__ part2,
                                                   int nblk, size_t n, double* __restrict__ b_out) {
   __shared__ double s_inv;
-  if (threadIdx.x == 0) {
-    const double b = sqrt(sum_parts(part2, nblk, 1, 0));
-    s_inv = 1.0 / b;
-    if (blockIdx.x == 0) b_out[0] = b;
+  if (threadIdx.x < 64) {
+    const double b = sqrt(wave_sum_parts(part2, nblk, 1, 0, threadIdx.x));
+    if (threadIdx.x == 0) {
+      s_inv = 1.0 / b;
+      if (blockIdx.x == 0) b_out[0] = b;
+    }
   }
   __syncthreads();
   const double inv = s_inv;
