@@ -1,0 +1,124 @@
+! iso_c_binding interface to libocean_gpu.so (include/ocean_gpu.h).
+!
+! The C-ABI takes the reference's own arrays: column-major, separate real and
+! imaginary FP64 planes (OCEAN2/OCEAN_psi.f90:64-65), so OCEAN2's module state
+! (re/im_bloch_state, W, mpcr/mpci, mpm, mhr/mhi, somelr/someli, p_energy,
+! allow) and its OCEAN_vector %r/%i planes are passed without copies.
+module ocean_gpu_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: OCEAN_GPU_OK = 0
+  integer(c_int), parameter :: OCEAN_GPU_ERR_NAN = -1
+  integer(c_int32_t), parameter :: OCEAN_GPU_FP64 = 0
+  integer(c_int32_t), parameter :: OCEAN_GPU_MIXED = 1
+
+  ! o_system / o_run fields the core action reads (OCEAN2/OCEAN_system.f90:14-124)
+  type, bind(C) :: ocean_bse_desc
+    integer(c_int32_t) :: num_bands
+    integer(c_int32_t) :: kmesh(3)
+    integer(c_int32_t) :: xmesh(3)
+    integer(c_int32_t) :: nspn
+    integer(c_int32_t) :: lc
+    integer(c_int32_t) :: lmin
+    integer(c_int32_t) :: lmax
+    integer(c_int32_t) :: nproj(8)
+    integer(c_int32_t) :: lvl
+    integer(c_int32_t) :: lvh
+    real(c_double) :: celvol
+    real(c_double) :: inter_scale
+    integer(c_int32_t) :: e0
+    integer(c_int32_t) :: mult
+    integer(c_int32_t) :: long_range
+    integer(c_int32_t) :: precision
+  end type ocean_bse_desc
+
+  interface
+    integer(c_int) function ocean_gpu_create(desc, h) bind(C, name='ocean_gpu_create')
+      import :: c_int, c_ptr, ocean_bse_desc
+      type(ocean_bse_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function ocean_gpu_destroy(h) bind(C, name='ocean_gpu_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function ocean_gpu_set_precision(h, precision) bind(C, name='ocean_gpu_set_precision')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: precision
+    end function
+
+    integer(c_int) function ocean_gpu_load_bloch(h, u_re, u_im) bind(C, name='ocean_gpu_load_bloch')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u_re(*), u_im(*)
+    end function
+
+    integer(c_int) function ocean_gpu_load_w(h, w) bind(C, name='ocean_gpu_load_w')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: w(*)
+    end function
+
+    integer(c_int) function ocean_gpu_load_mult(h, mpcr, mpci, mpm, mhr, mhi, somelr, someli, have_so) &
+        bind(C, name='ocean_gpu_load_mult')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: mpcr(*), mpci(*), mpm(*), mhr(*), mhi(*), somelr(*), someli(*)
+      integer(c_int32_t), value :: have_so
+    end function
+
+    integer(c_int) function ocean_gpu_load_energies(h, e_re, e_im, allow) bind(C, name='ocean_gpu_load_energies')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: e_re(*), e_im(*), allow(*)
+    end function
+
+    integer(c_int) function ocean_gpu_xact(h, psi_re, psi_im, hpsi_re, hpsi_im, backwards, hflag) &
+        bind(C, name='ocean_gpu_xact')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: psi_re(*), psi_im(*)
+      real(c_double), intent(out) :: hpsi_re(*), hpsi_im(*)
+      integer(c_int32_t), value :: backwards
+      integer(c_int32_t), intent(in) :: hflag(6)
+    end function
+
+    integer(c_int) function ocean_gpu_haydock(h, psi_re, psi_im, niter, real_a, real_b, imag_a, niter_done) &
+        bind(C, name='ocean_gpu_haydock')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: psi_re(*), psi_im(*)
+      integer(c_int32_t), value :: niter
+      real(c_double), intent(out) :: real_a(*), real_b(*), imag_a(*)
+      integer(c_int32_t), intent(out) :: niter_done
+    end function
+
+    integer(c_int) function ocean_gpu_gmres(h, hay_re, hay_im, ne, energies_ev, gres_ev, gprc_ev, ffff, &
+        depth, kpref, out, x_re, x_im) bind(C, name='ocean_gpu_gmres')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: hay_re(*), hay_im(*)
+      integer(c_int32_t), value :: ne
+      real(c_double), intent(in) :: energies_ev(*)
+      real(c_double), value :: gres_ev, gprc_ev, ffff
+      integer(c_int32_t), value :: depth
+      real(c_double), value :: kpref
+      real(c_double), intent(out) :: out(*)
+      type(c_ptr), value :: x_re, x_im
+    end function
+
+    integer(c_int) function ocean_gpu_spectrum(real_a, real_b, imag_a, niter, kpref, gam, el, eh, ne, &
+        literal_cmplx, out) bind(C, name='ocean_gpu_spectrum')
+      import :: c_int, c_double, c_int32_t
+      real(c_double), intent(in) :: real_a(*), real_b(*), imag_a(*)
+      integer(c_int32_t), value :: niter
+      real(c_double), value :: kpref, gam, el, eh
+      integer(c_int32_t), value :: ne, literal_cmplx
+      real(c_double), intent(out) :: out(*)
+    end function
+  end interface
+end module ocean_gpu_iface

@@ -1,0 +1,46 @@
+"""The Fortran host (iso_c_binding module + driver) through the C-ABI on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import ocean_oracle as orc
+from fortran_io import read_driver_output, write_driver_input
+from helpers import relerr, to_problem
+from ocean_amd import synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "ocean_amd", "fortran", "ocean_gpu_driver.x")
+
+
+def test_fortran_host_xact_haydock_gmres(tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    inp = synth.make_inputs(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=1, seed=12, xi_eV=5.0)
+    P = to_problem(inp)
+    E2H = orc.EV2HARTREE
+    niter, ne = 40, 300
+    gam, el, eh = 0.5 * E2H, -5 * E2H, 40 * E2H
+    energies = np.array([0.0, 8.0, 16.0])
+    fin = tmp_path / "bse_gpu.in"
+    write_driver_input(fin, inp, niter, ne, gam, el, eh, energies, ffff=1e-12, depth=30)
+    print("fortran driver case: nb=8 k=2^3 x=3^3 lc=1", flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(tmp_path / "out")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ierr, done, h, a, b, rows = read_driver_output(tmp_path / "out.bin", (inp.nb, inp.nk, inp.nalpha),
+                                                   niter, len(energies))
+    assert ierr == 0 and done == niter
+    assert relerr(h, orc.xact(P, inp.psi0)) < 1e-12
+    ra, rb, ia, _ = orc.haydock(P, inp.psi0, niter)
+    assert np.max(np.abs(a[:niter] - ra[:niter]) / np.abs(ra[:niter])) < 1e-10
+    sp = orc.write_core(ra, rb, niter, inp.kpref, gam, el, eh, ne)
+    txt = np.loadtxt(tmp_path / "out.absspct", comments="#")
+    assert txt.shape == (ne + 1, 5)
+    assert np.abs(txt[:, 2] - sp[:, 2]).max() / np.abs(sp[:, 2]).max() < 1e-7  # 8 printed digits
+    lanc = open(tmp_path / "out.abslanc").read().split()
+    assert int(lanc[0]) == niter
+    ref_rows = orc.gmres_do_recycle(P, inp.psi0, energies, 0.5, 13.605, 1e-12, 30, inp.kpref)
+    assert np.abs(rows[:, 2] - ref_rows[:, 2]).max() / np.abs(ref_rows[:, 2]).max() < 1e-6

@@ -1,0 +1,456 @@
+"""Pin the oracle: reference fixtures, closed forms and brute force (CPU only).
+
+Each test names what it pins.  Retyping the oracle's formula would pin nothing,
+so every check compares against an independent quantity: the reference's own
+stored outputs (TEST/Fe/Results), a closed form, a dense matrix built another
+way, or a second implementation (the C restatement, setup_host's closed forms).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import ocean_oracle as orc
+from helpers import relerr, to_problem
+from ocean_amd import setup_host as sh
+from ocean_amd import synth
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- fixtures
+def load_lanc(fn):
+    lines = open(fn).read().split("\n")
+    n, kpref = lines[0].split()
+    n = int(n)
+    a = np.zeros(n + 1)
+    b = np.zeros(n + 1)
+    a[0] = float(lines[1])
+    for i in range(1, n + 1):
+        x = lines[1 + i].split()
+        a[i], b[i] = float(x[0]), float(x[1])
+    eig = np.array([float(l.split()[2]) for l in lines[n + 2:] if l.strip()])
+    return n, float(kpref), a, b, eig
+
+
+def load_spct(fn):
+    lines = open(fn).read().split("\n")
+    gam = float(lines[0].split("gam=")[1].split()[0])
+    d = np.array([[float(t) for t in l.split()] for l in lines[2:] if l.strip()])
+    return gam, d
+
+
+@pytest.mark.parametrize("site", ["01", "02"])
+@pytest.mark.parametrize("literal", [True, False])
+def test_write_core_reproduces_fe_fixture(site, literal):
+    """write_core (OCEAN_haydock.f90:1801-1894) against TEST/Fe/Results abslanc -> absspct."""
+    n, kpref, a, b, _ = load_lanc(os.path.join(GOLD, f"abslanc_Fe.0001_1s_{site}"))
+    gam, d = load_spct(os.path.join(GOLD, f"absspct_Fe.0001_1s_{site}"))
+    h2e = orc.HARTREE2EV_OLD  # the fixture predates the 2022 CODATA constant
+    ne = d.shape[0] - 1
+    el, eh = d[0, 0] / h2e, d[-1, 0] / h2e
+    out = orc.write_core(a, b, n, kpref, gam, el, eh, ne, literal_cmplx=literal, hartree2ev=h2e)
+    pk = np.abs(d[:, 2]).max()
+    # limited by the fixture's printed precision (10-decimal a/b, 8-digit columns)
+    assert np.abs(out[:, 2] - d[:, 2]).max() / pk < 1.5e-6
+    # columns 2 and 4 (Spect, SPKK) of this old fixture carry the gamfcn
+    # energy-dependent broadening (jdamp = 1); the Gamma-only columns are 3 and 5
+    assert np.abs(out[:, 4] - d[:, 4]).max() / np.abs(d[:, 4]).max() < 1.5e-6
+    assert np.allclose(out[:, 0], d[:, 0], rtol=0, atol=2e-6 * np.abs(d[:, 0]).max())
+
+
+@pytest.mark.parametrize("site", ["01", "02"])
+def test_redtrid_matches_fixture_eigenvalues(site):
+    """redtrid (OCEAN_haydock.f90:2273-2316) against the eigenvalue block of abslanc."""
+    n, _, a, b, eig = load_lanc(os.path.join(GOLD, f"abslanc_Fe.0001_1s_{site}"))
+    ours = orc.redtrid_eigs(a, b, n)
+    assert eig.shape == ours.shape == (n + 1,)
+    assert np.abs(np.sort(eig) - ours).max() < 1e-8
+
+
+def test_continued_fraction_one_level_is_lorentzian():
+    """One Lanczos level with b(1) = 0 reduces to kpref * G / ((e - a0)^2 + G^2)."""
+    a = np.array([0.3, 0.0])
+    b = np.array([0.0, 0.0])
+    out = orc.write_core(a, b, 1, 2.5, 0.01, -0.2, 0.8, 500, literal_cmplx=False)
+    e = out[:, 0] / orc.HARTREE2EV
+    ref = 2.5 * 0.01 / ((e - 0.3) ** 2 + 0.01 ** 2)
+    assert relerr(out[:, 2], ref) < 1e-12
+
+
+def test_continued_fraction_constant_chain_is_semicircle():
+    """Constant a, b: the terminator makes the fraction exact, G = (z-a-sqrt((z-a)^2-4b^2))/(2b^2)."""
+    n, a0, b0, gam, kp = 37, 0.25, 0.1, 0.004, 1.0
+    a = np.full(n + 1, a0)
+    b = np.full(n + 1, b0)
+    b[0] = 0
+    out = orc.write_core(a, b, n, kp, gam, -0.1, 0.6, 700, literal_cmplx=False)
+    e = out[:, 0] / orc.HARTREE2EV
+    z = e + 1j * gam
+    s = np.sqrt((z - a0) ** 2 - 4 * b0 ** 2)
+    s = np.where(s.imag > 0, s, -s)
+    G = (z - a0 - s) / (2 * b0 ** 2)  # retarded branch
+    delta = 1.0 / G
+    ref = kp * np.abs(delta.imag) / np.abs(delta) ** 2
+    assert relerr(out[:, 2], ref) < 1e-10
+
+
+def test_literal_cmplx_rounding_is_single_precision_sized():
+    """The kind-less cmplx() sites change the spectrum at the 1e-7 level, not more."""
+    rng = np.random.default_rng(0)
+    n = 100
+    a = rng.uniform(0.5, 1.2, n + 1)
+    b = rng.uniform(0.2, 0.6, n + 1)
+    s1 = orc.write_core(a, b, n, 1.0, 0.02, 0.0, 2.0, 800, literal_cmplx=True)
+    s2 = orc.write_core(a, b, n, 1.0, 0.02, 0.0, 2.0, 800, literal_cmplx=False)
+    d = np.abs(s1[:, 2] - s2[:, 2]).max() / np.abs(s2[:, 2]).max()
+    assert 1e-10 < d < 1e-5
+
+
+# ---------------------------------------------------------------- FFT / lr
+def test_fft_convention_matches_dense_dft():
+    """FFTW_BACKWARD over (k1,k2,k3), k3 fastest (Modules/FFT_wrapper.f90:67-72)."""
+    km = (3, 4, 2)
+    F = orc.dft_matrix_backward(km)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(24) + 1j * rng.standard_normal(24)
+    assert relerr(orc.fft_backward(v, km), F @ v) < 1e-13
+    assert relerr(orc.fft_forward(v, km), np.conj(F) @ v) < 1e-13
+
+
+def _tiny(**kw):
+    base = dict(nb=4, kmesh=(2, 2, 2), xmesh=(2, 2, 2), lc=0, seed=1)
+    base.update(kw)
+    return synth.make_inputs(**base)
+
+
+def test_lr_act_equals_dense_operator():
+    """H_lr = -sum_x U_x^H (1/N) F_- diag(W_x) F_+ U_x built from explicit matrices."""
+    inp = _tiny(kmesh=(2, 3, 2))
+    P = to_problem(inp)
+    nb, nk, na, nx = inp.nb, inp.nk, inp.nalpha, inp.nx
+    F = orc.dft_matrix_backward(inp.kmesh)
+    Hk = np.zeros((nb * nk, nb * nk), complex)
+    for x in range(nx):
+        U = np.zeros((nk, nb * nk), complex)
+        for k in range(nk):
+            U[k, k * nb:(k + 1) * nb] = inp.u[:, k, x, 0]
+        C = np.conj(F) @ np.diag(inp.W[:, x]) @ F / nk
+        Hk -= np.conj(U).T @ C @ U
+    rng = np.random.default_rng(2)
+    psi = np.asfortranarray(rng.standard_normal((nb, nk, na)) + 1j * rng.standard_normal((nb, nk, na)))
+    got = orc.lr_act(P, psi)
+    for a in range(na):
+        ref = (Hk @ psi[:, :, a].ravel(order="F")).reshape((nb, nk), order="F")
+        assert relerr(got[:, :, a], ref) < 1e-13
+    assert relerr(orc.lr_kernel_loops(P, psi), got) < 1e-13
+
+
+def test_fg_combo_equals_dense_operator():
+    """H_fg = conj(C) (mul K) C^T with K = -mpm (direct) + X^T block (exchange)."""
+    inp = _tiny(lc=1, seed=4)
+    P = to_problem(inp)
+    m = inp.mult
+    nb, nk, na = inp.nb, inp.nk, inp.nalpha
+    mul = inp.inter_scale / (nk * inp.celvol)
+    plist = [(l, mm, 0) for l in range(m.lmin, m.lmax + 1) for mm in range(-l, l + 1)]
+    Pn = len(plist)
+    N = nb * nk * na
+    Cm = np.zeros((N, Pn * na), complex)
+    for a in range(na):
+        for j, (l, mm, nu) in enumerate(plist):
+            col = np.zeros((nb, nk, na), complex)
+            col[:, :, a] = m.c[:, :, nu, mm + m.lmax, l - m.lmin, 0]
+            Cm[:, a * Pn + j] = col.ravel(order="F")
+    K = np.zeros((Pn * na, Pn * na), complex)
+    for a in range(na):
+        for j, (l, mm, nu) in enumerate(plist):
+            K[a * Pn + j, a * Pn + j] -= m.mpm[0, 0, l - m.lmin]
+    for l in range(m.lmin, m.lmax + 1):
+        X = m.X[l - m.lvl]
+        idx = [(a, (l, mm, 0)) for a in range(na) for mm in range(-l, l + 1)]
+        for ii, (a, p) in enumerate(idx):
+            for jj, (b, q) in enumerate(idx):
+                K[a * Pn + plist.index(p), b * Pn + plist.index(q)] += X[jj, ii]
+    H = np.conj(Cm) @ (mul * K) @ Cm.T
+    rng = np.random.default_rng(3)
+    psi = np.asfortranarray(rng.standard_normal((nb, nk, na)) + 1j * rng.standard_normal((nb, nk, na)))
+    got = orc.fg_combo(P, psi, inp.inter_scale).ravel(order="F")
+    assert relerr(got, H @ psi.ravel(order="F")) < 1e-12
+    assert np.abs(H - np.conj(H.T)).max() < 1e-14 * np.abs(H).max()
+
+
+@pytest.mark.parametrize("lc,nspn,xi", [(0, 1, 0.0), (1, 1, 6.0), (1, 2, 3.0)])
+def test_full_action_is_hermitian(lc, nspn, xi):
+    """H = E + A(H_lr + H_fg)A + S is Hermitian for real W, Hermitian K and S."""
+    inp = _tiny(lc=lc, nspn=nspn, xi_eV=xi, kmesh=(2, 2, 2), xmesh=(2, 2, 3), nb=3)
+    H = orc.dense_H(to_problem(inp))
+    assert np.abs(H - np.conj(H.T)).max() < 1e-13 * np.abs(H).max()
+
+
+def test_backwards_flag_conjugates_e_and_s():
+    inp = _tiny(lc=1, xi_eV=6.0, nb=3)
+    inp.energy = inp.energy + 1j * 0.01  # a lifetime term makes the conjugation visible
+    P = to_problem(inp)
+    Hf = orc.dense_H(P)
+    Hb = orc.dense_H(P, backwards=True)
+    Pn = to_problem(inp, e0=True)
+    Pn.mult.someli = Pn.mult.someli  # zero here; E's imaginary part carries the check
+    d = Hf - Hb
+    diag = np.diag(d)
+    assert np.allclose(diag.imag, 2 * 0.01, atol=1e-14)
+
+
+# ---------------------------------------------------------------- Haydock / GMRES
+def test_haydock_matches_dense_lanczos():
+    """a/b from the recursion equal Lanczos on the dense matrix (first 12 steps)."""
+    inp = _tiny(lc=1, xi_eV=4.0, nb=4, kmesh=(2, 2, 2), xmesh=(2, 2, 2))
+    P = to_problem(inp)
+    H = orc.dense_H(P)
+    v = inp.psi0.ravel(order="F").copy()
+    n = 12
+    ra, rb, _, ierr = orc.haydock(P, inp.psi0, n)
+    assert ierr == 0
+    # textbook Lanczos with full reorthogonalisation on the dense matrix
+    Q = [v / np.linalg.norm(v)]
+    aa, bb = [], [0.0]
+    for j in range(n):
+        w = H @ Q[-1]
+        aa.append(np.vdot(Q[-1], w).real)
+        w = w - aa[-1] * Q[-1] - (bb[-1] * Q[-2] if j > 0 else 0)
+        for q in Q:
+            w = w - np.vdot(q, w) * q
+        bb.append(np.linalg.norm(w))
+        Q.append(w / bb[-1])
+    assert np.allclose(ra[:n], aa, rtol=1e-11, atol=1e-13)
+    assert np.allclose(rb[1:n + 1], bb[1:], rtol=1e-10)
+
+
+def test_haydock_spectrum_equals_eigen_sum():
+    """Full-length recursion: the continued fraction equals sum_l |<v_l|psi0>|^2 Lorentzians."""
+    inp = _tiny(nb=2, kmesh=(2, 1, 1), xmesh=(2, 1, 1))  # N = 2*2*4 = 16
+    P = to_problem(inp)
+    H = orc.dense_H(P)
+    N = H.shape[0]
+    ra, rb, _, _ = orc.haydock(P, inp.psi0, N)
+    lam, V = np.linalg.eigh(H)
+    w = np.abs(V.conj().T @ inp.psi0.ravel(order="F")) ** 2
+    gam = 0.01
+    out = orc.write_core(ra, rb, N, 1.0, gam, lam.min() - 0.1, lam.max() + 0.1, 400, literal_cmplx=False)
+    e = out[:, 0] / orc.HARTREE2EV
+    ref = np.sum(w[None, :] * gam / ((e[:, None] - lam[None, :]) ** 2 + gam ** 2), axis=1)
+    assert relerr(out[:, 2], ref) < 1e-6
+
+
+def test_gmres_matches_dense_solve():
+    """-Im<b|x> from GMRES at each energy equals the direct solve of (E+iG-H)x=b."""
+    inp = _tiny(nb=3, seed=9, xmesh=(3, 1, 1))
+    P = to_problem(inp)
+    E = np.array([-1.0, 5.0, 12.0, 12.5, 25.0])
+    rows = orc.gmres_do_recycle(P, inp.psi0, E, 0.5, 13.605, 1e-14, 12, inp.kpref)
+    H = orc.dense_H(P)
+    b = (P.allow * inp.psi0).ravel(order="F")
+    for r in rows:
+        z = r[0] * orc.EV2HARTREE + 1j * 0.5 * orc.EV2HARTREE
+        x = np.linalg.solve(z * np.eye(H.shape[0]) - H, b)
+        d = np.vdot(b, x)
+        assert abs(-d.imag * inp.kpref - r[2]) < 1e-6 * abs(r[2])
+        assert abs((1 - d.real * inp.kpref) - r[1]) < 1e-6
+
+
+# ---------------------------------------------------------------- setup
+def test_gaunt_quadrature_equals_3j_closed_form():
+    """threey (newthreey.f90) by Gauss-Legendre quadrature vs Wigner-3j closed forms."""
+    x, w = orc.gauss_legendre(16)
+    prefs = orc.getprefs(5)
+    worst = 0.0
+    for l1 in range(0, 4):
+        for l3 in range(0, 4):
+            for k in range(abs(l1 - l3), min(l1 + l3, 5) + 1):
+                for m1 in range(-l1, l1 + 1):
+                    for m3 in range(-l3, l3 + 1):
+                        for cj in (False, True):
+                            q = (m3 - m1) if cj else (m1 - m3)
+                            if abs(q) > k:
+                                continue
+                            a = orc.threey(l1, m1, k, q, l3, m3, cj, x, w, prefs)
+                            b1 = orc.gaunt_closed(l1, m1, k, q, l3, m3, cj)
+                            b2 = sh.gaunt(l1, m1, k, q, l3, m3, cj)
+                            worst = max(worst, abs(a - b1), abs(a - b2))
+    assert worst < 1e-13
+
+
+@pytest.mark.parametrize("lc,lv", [(0, 0), (0, 1), (0, 3), (1, 0), (1, 1), (1, 2), (1, 3)])
+def test_exchange_matrix_quadrature_vs_closed_form(lc, lv):
+    """nbsemhsetup2 (OCEAN_multiplet.f90:2007-2181) restated with quadrature vs setup_host."""
+    rng = np.random.default_rng(lc * 10 + lv)
+    fk = {k: np.array([[rng.uniform(0.5, 2)]]) for k in range(2, min(2 * lc, 2 * lv) + 1, 2)}
+    gk = {k: np.array([[rng.uniform(0.5, 2)]]) for k in range(abs(lc - lv), lc + lv + 1, 2)}
+    ones = {k: 1.0 for k in range(0, 8)}
+    a = orc.nbsemhsetup2(lc, lv, 1, fk, ones, gk, ones)
+    b = sh.exchange_matrix(lc, lv, 1, fk, gk)
+    assert np.abs(a - b).max() < 1e-12
+    assert np.abs(a - np.conj(a.T)).max() < 1e-12
+
+
+def test_spin_orbit_closed_form_splitting():
+    """-xi <L.S>: j = 3/2 at -xi/2, j = 1/2 at +xi; splitting 1.5 xi (OCEAN_soprep :572-690)."""
+    xi = 6.0
+    sr, si, have = orc.soprep(1, xi)
+    sr2, si2, have2 = sh.spin_orbit(1, xi)
+    assert have and have2
+    assert np.abs(sr - sr2).max() < 1e-15 and np.abs(si).max() == 0
+    # restrict to one valence spin: the 6 core states (spin x m)
+    idx = [a for a, lab in enumerate(orc.alpha_labels(1)) if lab[2] == -0.5]
+    ev = np.sort(np.linalg.eigvalsh(sr[np.ix_(idx, idx)])) / orc.EV2HARTREE
+    assert np.allclose(ev, [-3.0] * 4 + [6.0] * 2, atol=1e-12)
+    assert np.abs(orc.soprep(0, xi)[0]).max() == 0.0  # K edge: no orbital moment
+
+
+def test_jimel_matches_ladder_operators():
+    for j in (0.5, 1.0, 2.0):
+        ms = np.arange(-j, j + 1)
+        for m in ms:
+            for mp in ms:
+                for i, M in enumerate(sh._lm_matrix(int(j)) if j >= 1 else sh._lm_matrix_half()):
+                    ref = M[int(round(m + j)), int(round(mp + j))]
+                    assert abs(orc.jimel(j, m, mp, i + 1) - ref) < 1e-14
+
+
+def test_intval_matches_interp_and_caps():
+    xt = np.array([0.5, 1.0, 2.0, 4.0])
+    yt = np.array([3.0, 2.0, 1.5, 0.5])
+    x = np.linspace(0.0, 5.0, 51)
+    y = orc.intval(xt, yt, x)
+    assert np.allclose(y, np.interp(x, xt, yt), atol=1e-15)
+    ext = orc.intval(xt, yt, np.array([0.0, 5.0]), "ext", "ext")
+    assert np.allclose(ext, [3.0 + (0.0 - 0.5) / 0.5 * (2.0 - 3.0), 1.5 + (5.0 - 2.0) / 2.0 * (0.5 - 1.5)])
+
+
+def test_populate_w_closed_forms_and_host_agreement():
+    """lr_populate_W2: r = 0 gives ptab(1); beyond the table 1/(eps r); host builder agrees."""
+    rtab, ptab = synth.w_table()
+    W = orc.populate_W(np.eye(3) * 6.0, (1, 1, 1), (1, 1, 1), (0, 0, 0), (0, 0, 0), rtab, ptab, 4.0)
+    assert W.shape == (1, 1) and W[0, 0] == ptab[0]
+    W = orc.populate_W(np.eye(3) * 40.0, (2, 1, 1), (1, 1, 1), (0, 0, 0), (0, 0, 0), rtab, ptab, 4.0)
+    assert abs(W[1, 0] - 1.0 / (4.0 * 40.0)) < 1e-15  # R = -1 cell, r = 40 bohr
+    tau = np.array([0.13, 0.27, 0.41])
+    u = np.zeros((1, 27, 64, 1), complex)
+    _, tau_c, xs = orc.bloch_phase(u, (4, 4, 4), (3, 3, 3), tau)
+    a = orc.populate_W(np.eye(3) * 6.0, (3, 3, 3), (4, 4, 4), tau_c, xs, rtab, ptab, 4.0)
+    b = sh.build_w(np.eye(3) * 6.0, (3, 3, 3), (4, 4, 4), tau_c, xs, rtab, ptab, 4.0)
+    assert np.abs(a - b).max() < 1e-15
+
+
+def test_w_hat_weights_reconstruct_w():
+    """dW/dptab weights: W = sum of the two hat weights times ptab inside the table."""
+    rtab, ptab = synth.w_table()
+    tau = np.array([0.1, 0.2, 0.3])
+    u = np.zeros((1, 8, 27, 1), complex)
+    _, tau_c, xs = orc.bloch_phase(u, (3, 3, 3), (2, 2, 2), tau)
+    W = orc.populate_W(np.eye(3) * 3.0, (2, 2, 2), (3, 3, 3), tau_c, xs, rtab, ptab, 1e30)
+    i0, w0, w1 = orc.populate_W_weights(np.eye(3) * 3.0, (2, 2, 2), (3, 3, 3), tau_c, xs, rtab)
+    assert np.abs(w0 * ptab[i0] + w1 * ptab[i0 + 1] - W).max() < 1e-14
+
+
+def test_bloch_phase_closed_form_and_host_agreement():
+    """lrLOAD phase: with no shift, u(x,k) -> exp(2 pi i sum_d x_d k_d / (X_d K_d)) u."""
+    xm, km = (2, 3, 2), (2, 2, 3)
+    nk, nx = 12, 12
+    u = np.ones((1, nk, nx, 1), complex)
+    up, tau_c, xs = orc.bloch_phase(u, xm, km, (0.0, 0.0, 0.5))  # odd K3 centres at tau - 1/2
+    assert np.all(xs == 0)
+    kidx = [(a, b, c) for a in range(2) for b in range(2) for c in range(3)]
+    xidx = [(i, j, l) for l in range(2) for j in range(3) for i in range(2)]
+    for k, kv in enumerate(kidx):
+        for x, xv in enumerate(xidx):
+            ph = 2 * np.pi * sum(xv[d] * kv[d] / (xm[d] * km[d]) for d in range(3))
+            assert abs(up[0, k, x, 0] - np.exp(1j * ph)) < 1e-14
+    rng = np.random.default_rng(0)
+    ur = rng.standard_normal((3, nk, nx, 1)) + 1j * rng.standard_normal((3, nk, nx, 1))
+    a = orc.bloch_phase(ur, xm, km, (0.6, 0.1, 0.8))
+    b = sh.bloch_phase(ur, xm, km, (0.6, 0.1, 0.8))
+    assert np.abs(a[0] - b[0]).max() < 1e-14 and np.allclose(a[1], b[1]) and np.all(a[2] == b[2])
+
+
+def test_pnorm_kpref():
+    """core_pnorm (OCEAN_psi.f90:5589-5638): kpref = 4 pi |psi|^2 / (N_k Omega^2)."""
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal((3, 4, 4)) + 1j * rng.standard_normal((3, 4, 4))
+    q, kp = orc.core_pnorm(p, 4, 216.0)
+    assert abs(np.linalg.norm(q) - 1) < 1e-14
+    ref = 4 * np.pi * np.linalg.norm(p) ** 2 / (4 * 216.0 ** 2)
+    assert abs(kp - ref) < 1e-13 * ref
+    q2, kp2 = sh.normalise_psi0(p, 4, 216.0)
+    assert abs(kp - kp2) < 1e-13 * ref and np.abs(q - q2).max() < 1e-15
+
+
+def test_psi_dotter_selects_matching_spins():
+    """psi_dotter (OCEAN_psi.f90:5887-6031): only core spin == valence spin channels are set."""
+    rng = np.random.default_rng(1)
+    nb, nk, lc, nptot = 3, 2, 1, 4
+    mels = rng.standard_normal((nptot, 2 * lc + 1)) + 1j * rng.standard_normal((nptot, 2 * lc + 1))
+    cks = rng.standard_normal((nptot, nb * nk, 1)) + 1j * rng.standard_normal((nptot, nb * nk, 1))
+    psi = orc.psi_dotter(mels, cks, lc, nb, nk, 1)
+    for a, (cs, cm, vs) in enumerate(orc.alpha_labels(lc)):
+        if cs != vs:
+            assert np.all(psi[:, :, a] == 0)
+        else:
+            k, n = 1, 2
+            ref = np.conj(np.sum(mels[:, int(cm) + lc] * cks[:, k * nb + n, 0]))
+            assert abs(psi[n, k, a] - ref) < 1e-14
+
+
+def test_nbsemkcmel_bode_rule_converges():
+    """nbsemkcmel (OCEAN2/nbsemkcmel.f90:8-209): trim at rmax, Bode-rule cmel on the log grid."""
+    rmax = 1.5
+    nr = 1201
+    r0 = 1e-3
+    dl = math.log(rmax / r0) / 800.0
+    rad = r0 * np.exp(dl * np.arange(nr))
+    rv = np.linspace(1e-4, 10.0, 4000)
+    v = 1.0 / (1.0 + rv)
+    phi = np.exp(-rad)[:, None]
+    vtrim, cmel, nmel = orc.nbsemkcmel(v, rv, rmax, rad, phi)
+    vrun = v[np.searchsorted(rv, rmax)]
+    mask = rad <= rmax
+    f = np.exp(-2 * rad) * (np.interp(rad, rv, v - np.where(rv < rmax, vrun, v)) ) * rad ** 2
+    ref = np.trapezoid(np.where(mask, f, 0.0), rad) * orc.HARTREE2EV
+    assert abs(cmel[0, 0] - ref) < 2e-3 * abs(ref)
+    assert np.all(vtrim[rv < rmax] == vrun) and np.all(vtrim[rv >= rmax] == v[rv >= rmax])
+
+
+# ---------------------------------------------------------------- the C restatement
+@pytest.fixture(scope="module")
+def c_oracle_mod():
+    import c_oracle
+    c_oracle.build()
+    return c_oracle
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=3),
+    dict(nb=6, kmesh=(3, 4, 2), xmesh=(4, 3, 2), lc=0, seed=4),
+    dict(nb=6, kmesh=(2, 2, 2), xmesh=(2, 2, 2), lc=1, seed=5, xi_eV=6.0),
+    dict(nb=4, kmesh=(2, 2, 4), xmesh=(3, 1, 3), lc=1, seed=7, nspn=2, xi_eV=3.0),
+    dict(nb=4, kmesh=(5, 3, 6), xmesh=(2, 1, 2), lc=0, seed=8),
+])
+def test_c_restatement_equals_numpy_oracle(c_oracle_mod, kw):
+    inp = synth.make_inputs(**kw)
+    P = to_problem(inp)
+    cs = c_oracle_mod.CSystem(inp)
+    rng = np.random.default_rng(1)
+    psi = np.asfortranarray(rng.standard_normal(cs.shape) + 1j * rng.standard_normal(cs.shape))
+    for back in (False, True):
+        assert relerr(cs.xact(psi, backwards=back, nthreads=2), orc.xact(P, psi, backwards=back)) < 1e-13
+
+
+def test_c_restatement_haydock(c_oracle_mod):
+    inp = synth.make_inputs(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=1, seed=2, xi_eV=5.0)
+    P = to_problem(inp)
+    ra, rb, _, rc = c_oracle_mod.CSystem(inp).haydock(inp.psi0, 30, nthreads=2)
+    a, b, _, _ = orc.haydock(P, inp.psi0, 30)
+    assert rc == 0
+    assert np.abs(ra[:30] - a[:30]).max() < 1e-12 * np.abs(a).max()
+    assert np.abs(rb[1:31] - b[1:31]).max() < 1e-12 * np.abs(b).max()
