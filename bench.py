@@ -76,7 +76,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calib_copy:
+        log("calibration copy: importing torch")
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("calibration copy: 5 x 1 GiB read + 1 GiB write done")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding is a design note, not a measured path.", file=sys.stderr)

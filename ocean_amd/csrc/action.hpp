@@ -22,10 +22,10 @@ enum ActFlags : int {
 // Projection: phi(k,x,a) = sum_n u(n,k,x) (A psi)(n,k,a)
 //   long_range.f90:897-906 (no conjugation of u).  Grid (ceil(nx/256), nk, ns).
 // ---------------------------------------------------------------------------
-template <typename T2, int NA>
+template <typename T2, int NA, int XV>
 __global__ __launch_bounds__(kBlock) void k_proj(
     const T2* __restrict__ u, const double2* __restrict__ psi, const double* __restrict__ allow,
-    T2* __restrict__ phi, int nb, int nk, int nx, int na, int ns) {
+    T2* __restrict__ phi, int nb, int nk, int nx, int na, int ns, int ldu, int ldphi) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T2* ps = reinterpret_cast<T2*>(smem);  // [acnt][nb]
@@ -41,23 +41,49 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     ps[i] = cmake<T2>((T)(p.x * m), (T)(p.y * m));
   }
   __syncthreads();
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  // XV consecutive x per thread: one 16-byte load per band row in both precisions
+  const int x = (blockIdx.x * blockDim.x + threadIdx.x) * XV;
   if (x >= nx) return;
-  T2 acc[NA];
+  T2 acc[XV][NA];
 #pragma unroll
-  for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
-  const T2* ub = u + ((size_t)(s * nk + k) * nb) * nx + x;
-#pragma unroll 8
-  for (int n = 0; n < nb; ++n) {
-    const T2 uv = ub[(size_t)n * nx];
+  for (int v = 0; v < XV; ++v)
 #pragma unroll
-    for (int j = 0; j < NA; ++j)
-      if (j < acnt) cfma(acc[j], uv, ps[j * nb + n]);
+    for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
+  const T2* ub = u + ((size_t)(s * nk + k) * nb) * ldu + x;
+  // Batches of kB band rows: all kB loads are issued before the first use, so
+  // each wave keeps kB 1-KB row segments in flight.
+  constexpr int kB = 8;
+  for (int n0 = 0; n0 < nb; n0 += kB) {
+    T2 uv[kB][XV];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int n = min(n0 + i, nb - 1);
+      if constexpr (XV == 2) {
+        const float4 q = *reinterpret_cast<const float4*>(ub + (size_t)n * ldu);
+        uv[i][0] = cmake<T2>(q.x, q.y);
+        uv[i][1] = cmake<T2>(q.z, q.w);
+      } else {
+        uv[i][0] = ub[(size_t)n * ldu];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      if (n0 + i < nb) {
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+          const T2 pj = ps[j * nb + n0 + i];
+#pragma unroll
+          for (int v = 0; v < XV; ++v) cfma(acc[v][j], uv[i][v], pj);
+        }
+      }
+    }
   }
-  T2* out = phi + ((size_t)k * nx + x) * na;
 #pragma unroll
-  for (int j = 0; j < NA; ++j)
-    if (j < acnt) out[s + j * astep] = acc[j];
+  for (int v = 0; v < XV; ++v) {
+    T2* out = phi + (size_t)k * ldphi + (size_t)(x + v) * na;
+#pragma unroll
+    for (int j = 0; j < NA; ++j) out[s + j * astep] = acc[v][j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +177,7 @@ __device__ void dft_run(T2* lds, const T2* tw, int G, int nk, int L, int S) {
 template <typename T2, int LMAX>
 __global__ __launch_bounds__(256) void k_fft_w(
     T2* __restrict__ phi, const typename Scalar<T2>::T* __restrict__ W, int nk, int ncol, int na,
-    int G, int K1, int K2, int K3) {
+    int G, int K1, int K2, int K3, int ldphi) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   // twiddles: [dim][sign][m], dims ordered (K3, K2, K1), 2 * LMAX per dim
@@ -176,9 +202,26 @@ __global__ __launch_bounds__(256) void k_fft_w(
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
   const int c0 = b * G;
   const int gcnt = min(G, ncol - c0);
-  for (int e = threadIdx.x; e < nk * G; e += blockDim.x) {
-    const int k = e / G, g = e - k * G;
-    lds[lidx(e)] = (g < gcnt) ? phi[(size_t)k * ncol + c0 + g] : cmake<T2>(0, 0);
+  {
+    // batched tile load: 8 independent loads per thread in flight
+    const int tot = nk * G;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * blockDim.x) {
+      T2 v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int e = min(e0 + i * (int)blockDim.x, tot - 1);
+        const int k = e / G, g = e - k * G;
+        v[i] = phi[(size_t)k * ldphi + c0 + min(g, gcnt - 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int e = e0 + i * (int)blockDim.x;
+        if (e < tot) {
+          const int g = e % G;
+          lds[lidx(e)] = (g < gcnt) ? v[i] : cmake<T2>(0, 0);
+        }
+      }
+    }
   }
   __syncthreads();
   dft_run<T2, LMAX>(lds, twl + 0 * LMAX, G, nk, K3, 1);
@@ -210,7 +253,7 @@ __global__ __launch_bounds__(256) void k_fft_w(
     if (g < gcnt) {
       T2 v = lds[lidx(e)];
       v.x *= inv; v.y *= inv;
-      phi[(size_t)k * ncol + c0 + g] = v;
+      phi[(size_t)k * ldphi + c0 + g] = v;
     }
   }
 }
@@ -297,110 +340,95 @@ __global__ __launch_bounds__(1024) void k_mred(
 //   lr(n,k,a)  = - sum_x conj(u(n,k,x)) phi'(k,x,a)         long_range.f90:943-961
 //   fg(n,k,a)  =   sum_p conj(c(n,k,p)) h(p,a)               OCEAN_multiplet.f90:1943-1998
 //   out        =   E.psi + A.(fg + lr) + S psi                OCEAN_action.f90:337-372, :596-608
-// One workgroup per k; one wave per band row n, lanes over x, wave reduction.
+// Reads the transposed copy uT[s][k][x][n] (band index fastest): lanes run over
+// bands, so every wave load is a contiguous band row, phi'(k,x,:) is the same
+// for all lanes of a row (an LDS broadcast), and no cross-lane reduction is
+// needed until the XS x-splits of the workgroup are summed once at the end.
+// One workgroup per k; threads (n, xs) with n fastest.
 // ---------------------------------------------------------------------------
 template <typename T2, int NA>
-__global__ __launch_bounds__(kBlock) void k_adj(
-    const T2* __restrict__ u, const T2* __restrict__ phi, const double2* __restrict__ psi,
+__global__ __launch_bounds__(512) void k_adj(
+    const T2* __restrict__ uT, const T2* __restrict__ phi, const double2* __restrict__ psi,
     const double2* __restrict__ energy, const double* __restrict__ allow,
     const double2* __restrict__ cproj, const double2* __restrict__ hmult,
     const double2* __restrict__ S, double2* __restrict__ out, int nb, int nk, int nx, int na,
-    int ns, int P, int XC, int rpb, int flags) {
+    int ns, int P, int XC, int flags, int ldT, int ldphi) {
   using T = typename Scalar<T2>::T;
-  constexpr int RPW = 1;
-  constexpr int VV = 2 * NA * RPW;
-  constexpr int V = (VV <= 4) ? 4 : (VV <= 8) ? 8 : (VV <= 16) ? 16 : (VV <= 32) ? 32 : 64;
   extern __shared__ __align__(16) unsigned char smem[];
-  double* res = reinterpret_cast<double*>(smem);   // [na][rpb][2]
-  T2* ph = reinterpret_cast<T2*>(res + 2 * na * rpb);  // [XC][NA]
-  // 1-D grid of nk * nsplit workgroups walking k downwards: the projection
-  // streamed u with k ascending, so the adjoint's first rows are the ones most
-  // recently pulled through the Infinity Cache (and the next projection, going
-  // up again, starts where this pass ends).  The splits of one k are adjacent.
-  const int nsplit = (nb + rpb - 1) / rpb;
-  const int k = nk - 1 - (int)(blockIdx.x / nsplit);
-  const int n0 = (int)(blockIdx.x % nsplit) * rpb;
-  const int nrow = min(rpb, nb - n0);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int e = threadIdx.x; e < 2 * na * rpb; e += blockDim.x) res[e] = 0.0;
+  const int XS = blockDim.x / nb;  // x-splits (host guarantees nb <= blockDim.x)
+  double2* res = reinterpret_cast<double2*>(smem);           // [na][nb]
+  double2* red = res + na * nb;                                // [XS][nb]
+  T2* ph = reinterpret_cast<T2*>(red + XS * nb);               // [XC][NA]
+  // XCD-major workgroup order so that consecutive k share an XCD (phi' rows of
+  // neighbouring k are adjacent in memory)
+  int kk = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) kk = (kk & 7) * (nblk >> 3) + (kk >> 3);
+  const int k = kk;
+  const int t = threadIdx.x;
+  const int n = t % nb, xs = t / nb;
+  const bool active = xs < XS;
+  for (int e = t; e < na * nb; e += blockDim.x) res[e] = make_double2(0.0, 0.0);
   if (flags & F_LR) {
     for (int s = 0; s < ns; ++s) {
       const int astep = (ns == 2) ? 2 : 1;
       const int acnt = (ns == 2) ? na / 2 : na;
+      T2 acc[NA];
+#pragma unroll
+      for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
+      const T2* ub = uT + ((size_t)s * nk + k) * ldT;
       for (int x0 = 0; x0 < nx; x0 += XC) {
         const int xc = min(XC, nx - x0);
         __syncthreads();
-        for (int e = threadIdx.x; e < xc * acnt; e += blockDim.x) {
+        for (int e = t; e < xc * acnt; e += blockDim.x) {
           const int xl = e / acnt, j = e - xl * acnt;
-          ph[xl * NA + j] = phi[((size_t)k * nx + x0 + xl) * na + s + j * astep];
+          ph[xl * NA + j] = phi[(size_t)k * ldphi + (size_t)(x0 + xl) * na + s + j * astep];
         }
         __syncthreads();
-        // RPW band rows per wave pass: RPW*8 loads in flight per lane and one
-        // butterfly reduction over all RPW*2*NA values.
-        for (int rl0 = wid * RPW; rl0 < nrow; rl0 += nw * RPW) {
-          T2 acc[RPW][NA];
+        if (active) {
+          // kB independent row loads in flight per wave (addresses clamped, use predicated)
+          constexpr int kB = 8;
+          for (int xb = xs; xb < xc; xb += kB * XS) {
+            T2 uv[kB];
 #pragma unroll
-          for (int r = 0; r < RPW; ++r)
-#pragma unroll
-            for (int j = 0; j < NA; ++j) acc[r][j] = cmake<T2>(0, 0);
-          for (int xb = lane; xb < xc; xb += 8 * 64) {
-            T2 uv[RPW][8];
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-              const int rr = min(rl0 + r, nrow - 1);
-              const T2* urow = u + (((size_t)s * nk + k) * nb + n0 + rr) * nx + x0;
-#pragma unroll
-              for (int i = 0; i < 8; ++i) {
-                const int xl = xb + 64 * i;
-                uv[r][i] = (xl < xc) ? urow[xl] : cmake<T2>(0, 0);
-              }
+            for (int i = 0; i < kB; ++i) {
+              const int xl = min(xb + i * XS, xc - 1);
+              uv[i] = ub[(size_t)(x0 + xl) * nb + n];
             }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-              const int xl = xb + 64 * i;
+            for (int i = 0; i < kB; ++i) {
+              const int xl = xb + i * XS;
               if (xl < xc) {
 #pragma unroll
-                for (int j = 0; j < NA; ++j) {
-                  if (j < acnt) {
-                    const T2 pj = ph[xl * NA + j];
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) cfmac(acc[r][j], uv[r][i], pj);
-                  }
-                }
+                for (int j = 0; j < NA; ++j) cfmac(acc[j], uv[i], ph[xl * NA + j]);
               }
             }
           }
-          T v[V];
+        }
+      }
+      // sum the x-splits, one alpha at a time through LDS
 #pragma unroll
-          for (int j = 0; j < V; ++j) v[j] = 0;
-#pragma unroll
-          for (int r = 0; r < RPW; ++r)
-#pragma unroll
-            for (int j = 0; j < NA; ++j) {
-              v[(r * NA + j) * 2] = acc[r][j].x;
-              v[(r * NA + j) * 2 + 1] = acc[r][j].y;
-            }
-          int owner = 0;
-          const T tot = wave_multi_sum<T, V>(v, lane, &owner);
-          if ((lane & (64 / V - 1)) == 0) {
-            const int r = (owner >> 1) / NA, j = (owner >> 1) - r * NA, comp = owner & 1;
-            if (r < RPW && j < acnt && rl0 + r < nrow)
-              res[(((s + j * astep) * rpb) + rl0 + r) * 2 + comp] += (double)tot;
-          }
+      for (int j = 0; j < NA; ++j) {
+        __syncthreads();
+        if (active) red[xs * nb + n] = make_double2((double)acc[j].x, (double)acc[j].y);
+        __syncthreads();
+        if (t < nb) {
+          double2 sum = make_double2(0.0, 0.0);
+          for (int q = 0; q < XS; ++q) { sum.x += red[q * nb + t].x; sum.y += red[q * nb + t].y; }
+          res[(s + j * astep) * nb + t] = sum;
         }
       }
     }
   }
   __syncthreads();
   const bool back = (flags & F_BACK) != 0;
-  for (int e = threadIdx.x; e < na * nrow; e += blockDim.x) {
-    const int a = e / nrow, rl = e - a * nrow;
-    const int n = n0 + rl;
-    const size_t off = ((size_t)a * nk + k) * nb + n;
-    double2 inter = make_double2(-res[(a * rpb + rl) * 2], -res[(a * rpb + rl) * 2 + 1]);
+  for (int e = t; e < na * nb; e += blockDim.x) {
+    const int a = e / nb, nn = e - a * nb;
+    const size_t off = ((size_t)a * nk + k) * nb + nn;
+    double2 inter = make_double2(-res[e].x, -res[e].y);
     if (flags & F_FG) {
       const int s = (ns == 2) ? (a & 1) : 0;
-      const double2* cb = cproj + ((size_t)s * nk + k) * P * nb + n;
+      const double2* cb = cproj + ((size_t)s * nk + k) * P * nb + nn;
       for (int p = 0; p < P; ++p) cfmac(inter, cb[(size_t)p * nb], hmult[a * P + p]);
     }
     const double am = allow[off];
@@ -410,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_adj(
       for (int b = 0; b < na; ++b) {
         double2 sab = S[a * na + b];
         if (back) sab.y = -sab.y;
-        cfma(inter, sab, psi[((size_t)b * nk + k) * nb + n]);
+        cfma(inter, sab, psi[((size_t)b * nk + k) * nb + nn]);
       }
     }
     double2 r = inter;

@@ -122,10 +122,16 @@ template <typename T2, int NA>
 int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename og::Scalar<T2>::T* W,
                        hipEvent_t* ev) {
   const int ns = h->ns, acnt = (ns == 2) ? h->na / 2 : h->na;
-  dim3 g1((h->nx + kBlock - 1) / kBlock, h->nk, ns);
   size_t sm1 = (size_t)acnt * h->nb * sizeof(T2);
-  og::k_proj<T2, NA><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
-                                                     h->na, ns);
+  if (false) {
+    dim3 g1((h->nx / 2 + kBlock - 1) / kBlock, h->nk, ns);
+    og::k_proj<T2, NA, 2><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
+                                                          h->na, ns, h->ldu, h->ldphi);
+  } else {
+    dim3 g1((h->nx + kBlock - 1) / kBlock, h->nk, ns);
+    og::k_proj<T2, NA, 1><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
+                                                          h->na, ns, h->ldu, h->ldphi);
+  }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
   const int ncol = h->nx * h->na;
@@ -139,11 +145,11 @@ int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename o
   if (lmax <= 8) {
     size_t sm2 = (6 * 8 + tile + tile / 8 + 1) * sizeof(T2);
     og::k_fft_w<T2, 8><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
-                                                       h->km[2]);
+                                                       h->km[2], h->ldphi);
   } else {
     size_t sm2 = (6 * 16 + tile + tile / 8 + 1) * sizeof(T2);
     og::k_fft_w<T2, 16><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
-                                                        h->km[2]);
+                                                        h->km[2], h->ldphi);
   }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
@@ -151,28 +157,25 @@ int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename o
 }
 
 template <typename T2, int NA>
-int launch_adj(H* h, const double2* in, double2* out, const T2* u, const T2* phi, int flags) {
-  const int ns = h->ns;
-  // split the band rows of each k over several workgroups so that the grid
-  // fills the 256 CUs several times over (>= 2048 workgroups when possible)
-  int nsplit = std::max(1, std::min(h->nb / 4, 2048 / std::max(h->nk, 1)));
-  const int rpb = (h->nb + nsplit - 1) / nsplit;
-  nsplit = (h->nb + rpb - 1) / rpb;
-  const size_t resb = (size_t)h->na * rpb * sizeof(double2);
-  int XC = h->nx;
+int launch_adj(H* h, const double2* in, double2* out, const T2* uT, const T2* phi, int flags) {
+  // one workgroup per k; threads (band n, x-split) with n fastest, up to 512
+  const int XS = std::max(1, 512 / h->nb);
+  const int threads = h->nb * XS;
+  const size_t resb = ((size_t)h->na * h->nb + (size_t)XS * h->nb) * sizeof(double2);
   const size_t budget = 60 * 1024 - resb;
+  int XC = h->nx;
   const int cap = (int)(budget / (NA * sizeof(T2)));
-  if (XC > cap) XC = std::max(64, (cap / 64) * 64);
+  if (XC > cap) XC = std::max(XS, (cap / XS) * XS);
   size_t sm = resb + (size_t)XC * NA * sizeof(T2);
-  og::k_adj<T2, NA><<<dim3(h->nk * nsplit), kBlock, sm, h->stream>>>(
-      u, phi, in, h->energy, h->allow, h->cproj, h->hmult, h->Smat, out, h->nb, h->nk, h->nx, h->na, ns,
-      h->P, XC, rpb, flags);
+  og::k_adj<T2, NA><<<dim3(h->nk), threads, sm, h->stream>>>(
+      uT, phi, in, h->energy, h->allow, h->cproj, h->hmult, h->Smat, out, h->nb, h->nk, h->nx, h->na,
+      h->ns, h->P, XC, flags, h->ldT, h->ldphi);
   OG_CHECK(hipGetLastError());
   return OCEAN_GPU_OK;
 }
 
 template <typename T2, int NA>
-int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* phi,
+int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   // The multiplet projection and reduction depend only on psi: run them on the
@@ -206,7 +209,7 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* phi,
   }
   if (fork) OG_CHECK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
-  OG_TRY((launch_adj<T2, NA>(h, in, out, u, phi, flags)));
+  OG_TRY((launch_adj<T2, NA>(h, in, out, uT, phi, flags)));
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
   return OCEAN_GPU_OK;
 }
@@ -217,8 +220,8 @@ int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = 
   const bool mixed = (h->precision == OCEAN_GPU_MIXED);
 #define OG_DISPATCH(NAV)                                                              \
   if (NA == NAV)                                                                      \
-    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->phi32, h->W32, ev) \
-                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->phi64, h->W64, ev);
+    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev) \
+                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev);
   OG_DISPATCH(2)
   OG_DISPATCH(4)
   OG_DISPATCH(6)
@@ -228,14 +231,17 @@ int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = 
 }
 
 int ensure_precision_buffers(H* h) {
-  const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
-  const size_t nphi = (size_t)h->nk * h->nx * h->na;
+  const size_t nu = h->nu();
+  const size_t nphi = h->nphi();
   const size_t nw = (size_t)h->nk * h->nx;
+  const size_t nuT = h->nuT();
   if (h->precision == OCEAN_GPU_MIXED) {
+    if (!h->uT32) OG_TRY(dmalloc(&h->uT32, nuT));
     if (!h->u32) OG_TRY(dmalloc(&h->u32, nu));
     if (!h->phi32) OG_TRY(dmalloc(&h->phi32, nphi));
     if (!h->W32) OG_TRY(dmalloc(&h->W32, nw));
   } else {
+    if (!h->uT64) OG_TRY(dmalloc(&h->uT64, nuT));
     if (!h->u64) OG_TRY(dmalloc(&h->u64, nu));
     if (!h->phi64) OG_TRY(dmalloc(&h->phi64, nphi));
     if (!h->W64) OG_TRY(dmalloc(&h->W64, nw));
@@ -450,6 +456,10 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   h->nx = d->xmesh[0] * d->xmesh[1] * d->xmesh[2];
   h->na = 4 * (2 * d->lc + 1);
   h->ns = d->nspn;
+  h->ldu = h->nx + 32;
+  h->ldphi = h->nx * h->na + 32;
+  h->ldT = h->nx * h->nb + 32;
+  if (h->nb > 512) { delete h; return OCEAN_GPU_ERR_ARG; }
   h->precision = d->precision;
   int P = 0;
   for (int l = d->lmin; l <= d->lmax; ++l) {
@@ -484,7 +494,7 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
 int ocean_gpu_destroy(ocean_gpu_handle* h) {
   if (!h) return OCEAN_GPU_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  dfree(h->u64); dfree(h->u32); dfree(h->W64); dfree(h->W32); dfree(h->phi64); dfree(h->phi32);
+  dfree(h->u64); dfree(h->u32); dfree(h->uT64); dfree(h->uT32); dfree(h->W64); dfree(h->W32); dfree(h->phi64); dfree(h->phi32);
   dfree(h->w_i0); dfree(h->w_w0); dfree(h->w_w1);
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
@@ -504,13 +514,15 @@ int ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision) {
   if (precision == h->precision) return OCEAN_GPU_OK;
   h->precision = precision;
   OG_TRY(ensure_precision_buffers(h));
-  const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
+  const size_t nu = h->nu();
   const size_t nw = (size_t)h->nk * h->nx;
   // FP64 -> FP32 conversion of loaded state; the FP32 state cannot be promoted
   // back (an FP64 reload is required), so the FP64 copy is kept when it exists.
   if (precision == OCEAN_GPU_MIXED) {
     if (h->u64_ok && !h->u32_ok) {
       og::k_cast<float><<<4096, kBlock, 0, h->stream>>>((const double*)h->u64, (float*)h->u32, 2 * nu);
+      og::k_cast<float><<<4096, kBlock, 0, h->stream>>>((const double*)h->uT64, (float*)h->uT32,
+                                                       2 * h->nuT());
       h->u32_ok = true;
     }
     if (h->w64_ok && !h->w32_ok) {
@@ -545,16 +557,24 @@ static int load_u_common(H* h, const double* re, const double* im, const int32_t
     if (h->precision == OCEAN_GPU_MIXED) {
       if (xoff)
         og::k_load_u_phase<float2><<<g, 256, 0, h->stream>>>(dre, dim_, dxo, h->u32, h->nb, h->nk, h->nx,
-            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2]);
+            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2], h->ldu);
       else
-        og::k_load_u<float2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u32, h->nb, h->nk, h->nx);
+        og::k_load_u<float2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u32, h->nb, h->nk, h->nx, h->ldu);
     } else {
       if (xoff)
         og::k_load_u_phase<double2><<<g, 256, 0, h->stream>>>(dre, dim_, dxo, h->u64, h->nb, h->nk, h->nx,
-            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2]);
+            h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2], h->ldu);
       else
-        og::k_load_u<double2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u64, h->nb, h->nk, h->nx);
+        og::k_load_u<double2><<<g, 256, 0, h->stream>>>(dre, dim_, h->u64, h->nb, h->nk, h->nx, h->ldu);
     }
+    const size_t tot = (size_t)h->ns * h->nk * h->nx * h->nb;
+    const int gT = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    if (h->precision == OCEAN_GPU_MIXED)
+      og::k_load_uT<float2><<<gT, 256, 0, h->stream>>>(dre, dim_, xoff ? dxo : nullptr, h->uT32, h->nb, h->nk,
+          h->nx, h->ns, h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2], h->ldT);
+    else
+      og::k_load_uT<double2><<<gT, 256, 0, h->stream>>>(dre, dim_, xoff ? dxo : nullptr, h->uT64, h->nb, h->nk,
+          h->nx, h->ns, h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2], h->ldT);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }
   dfree(dre); dfree(dim_); dfree(dxo);
@@ -823,22 +843,25 @@ int ocean_gpu_profile_xact(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t
   if (!h || !vec_ptr(h, in) || !vec_ptr(h, out) || in == out || nrep <= 0) return OCEAN_GPU_ERR_ARG;
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
-  hipEvent_t ev[5];
+  // All repetitions are enqueued back to back with events between the kernel
+  // families and one synchronisation at the end, so the stream never idles and
+  // no family absorbs host launch latency; repetition 0 is discarded.
+  const int R = nrep + 1;
+  std::vector<hipEvent_t> ev((size_t)R * 5);
   for (auto& e : ev) OG_CHECK(hipEventCreate(&e));
-  double tp = 0, tf = 0, tm = 0, ta = 0;
-  for (int r = 0; r < nrep; ++r) {
-    OG_TRY(xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f, ev));
-    OG_CHECK(hipEventSynchronize(ev[4]));
-    float a = 0, b = 0, c = 0, d = 0;
-    OG_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
-    OG_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
-    OG_CHECK(hipEventElapsedTime(&c, ev[2], ev[3]));
-    OG_CHECK(hipEventElapsedTime(&d, ev[3], ev[4]));
-    tp += a; tf += b; tm += c; ta += d;
-  }
-  h->t_proj = tp / nrep; h->t_fft = tf / nrep; h->t_mult = tm / nrep; h->t_adj = ta / nrep;
+  int rc = OCEAN_GPU_OK;
+  for (int r = 0; r < R && rc == 0; ++r) rc = xact_dev(h, vec_ptr(h, in), vec_ptr(h, out), f, &ev[(size_t)r * 5]);
+  if (rc == 0) OG_CHECK(hipEventSynchronize(ev.back()));
+  double t[4] = {0, 0, 0, 0};
+  for (int r = 1; r < R && rc == 0; ++r)
+    for (int i = 0; i < 4; ++i) {
+      float ms = 0.f;
+      OG_CHECK(hipEventElapsedTime(&ms, ev[(size_t)r * 5 + i], ev[(size_t)r * 5 + i + 1]));
+      t[i] += ms;
+    }
+  h->t_proj = t[0] / nrep; h->t_fft = t[1] / nrep; h->t_mult = t[2] / nrep; h->t_adj = t[3] / nrep;
   for (auto& e : ev) (void)hipEventDestroy(e);
-  return OCEAN_GPU_OK;
+  return rc;
 }
 
 int ocean_gpu_last_timing(ocean_gpu_handle* h, const char* name, double* ms) {

@@ -148,6 +148,12 @@ struct ocean_gpu_handle {
   int nb = 0, nk = 0, nx = 0, na = 0, ns = 0;
   int km[3] = {0, 0, 0};
   int P = 0;        // compact projector count sum_l (2l+1) nproj(l)
+  // Padded leading dimensions: row strides that are large powers of two put the
+  // concurrent row reads of all workgroups on the same HBM channels (measured
+  // ~20% loss on MI355X), so u rows and phi k-rows carry 32 extra elements.
+  int ldu = 0;      // u row stride (elements), >= nx
+  int ldphi = 0;    // phi k-row stride (elements), >= nx * na
+  int ldT = 0;      // uT k-slab stride (elements), >= nx * nb
   int precision = OCEAN_GPU_FP64;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -155,8 +161,10 @@ struct ocean_gpu_handle {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // long-range state
-  double2* u64 = nullptr;   // [s][k][n][x]
+  double2* u64 = nullptr;   // [s][k][n][x]  (projection)
   float2* u32 = nullptr;    // same layout, mixed form
+  double2* uT64 = nullptr;  // [s][k][x][n]  (adjoint), k-slab stride ldT
+  float2* uT32 = nullptr;
   double* W64 = nullptr;    // [x][k]
   float* W32 = nullptr;
   double2* phi64 = nullptr; // [k][x][a]
@@ -198,4 +206,7 @@ struct ocean_gpu_handle {
   double t_proj = 0, t_fft = 0, t_adj = 0, t_mult = 0, t_total = 0;
 
   size_t nvec() const { return (size_t)nb * nk * na; }
+  size_t nu() const { return (size_t)ns * nk * nb * ldu; }
+  size_t nphi() const { return (size_t)nk * ldphi; }
+  size_t nuT() const { return (size_t)ns * nk * ldT; }
 };

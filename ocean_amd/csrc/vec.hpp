@@ -230,7 +230,7 @@ __global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict_
 // Tiled 32x32 transpose of the (x, n) plane of each (s, k).
 template <typename T2>
 __global__ void k_load_u(const double* __restrict__ re, const double* __restrict__ im,
-                         T2* __restrict__ u, int nb, int nk, int nx) {
+                         T2* __restrict__ u, int nb, int nk, int nx, int ldu) {
   using T = typename Scalar<T2>::T;
   __shared__ double2 tile[32][33];
   const int sk = blockIdx.z;  // s*nk + k
@@ -249,7 +249,7 @@ __global__ void k_load_u(const double* __restrict__ re, const double* __restrict
     const int n = n0 + r, x = x0 + tx;
     if (x < nx && n < nb) {
       const double2 v = tile[tx][r];
-      u[(((size_t)s * nk + k) * nb + n) * nx + x] = cmake<T2>((T)v.x, (T)v.y);
+      u[(((size_t)s * nk + k) * nb + n) * ldu + x] = cmake<T2>((T)v.x, (T)v.y);
     }
   }
 }
@@ -259,7 +259,7 @@ __global__ void k_load_u(const double* __restrict__ re, const double* __restrict
 template <typename T2>
 __global__ void k_load_u_phase(const double* __restrict__ re, const double* __restrict__ im,
                                const int* __restrict__ xoff, T2* __restrict__ u, int nb, int nk,
-                               int nx, int K1, int K2, int K3, int X1, int X2, int X3) {
+                               int nx, int K1, int K2, int K3, int X1, int X2, int X3, int ldu) {
   using T = typename Scalar<T2>::T;
   __shared__ double2 tile[32][33];
   const int sk = blockIdx.z;
@@ -286,8 +286,42 @@ __global__ void k_load_u_phase(const double* __restrict__ re, const double* __re
     const int n = n0 + r, x = x0 + tx;
     if (x < nx && n < nb) {
       const double2 v = tile[tx][r];
-      u[(((size_t)s * nk + k) * nb + n) * nx + x] = cmake<T2>((T)v.x, (T)v.y);
+      u[(((size_t)s * nk + k) * nb + n) * ldu + x] = cmake<T2>((T)v.x, (T)v.y);
     }
+  }
+}
+
+// Transposed copy uT[s][k][x][n] (band index fastest) from the reference planes,
+// optionally phased: the reference's (n, k, x, s) order already has bands
+// fastest, so this is a permutation of whole band rows (k, x) -> (x, k).
+template <typename T2>
+__global__ void k_load_uT(const double* __restrict__ re, const double* __restrict__ im,
+                          const int* __restrict__ xoff, T2* __restrict__ uT, int nb, int nk, int nx,
+                          int ns, int K1, int K2, int K3, int X1, int X2, int X3, int ldT) {
+  using T = typename Scalar<T2>::T;
+  const size_t tot = (size_t)ns * nk * nx * nb;
+  const double twopi = 6.283185307179586476925286766559;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int n = (int)(e % nb);
+    size_t r = e / nb;
+    const int x = (int)(r % nx);
+    r /= nx;
+    const int k = (int)(r % nk);
+    const int s = (int)(r / nk);
+    const size_t off = (((size_t)s * nx + x) * nk + k) * nb + n;
+    double vr = re[off], vi = im[off];
+    if (xoff) {
+      const int q1 = k / (K2 * K3), q2 = (k / K3) % K2, q3 = k % K3;
+      const double px = twopi * (double)(xoff[3 * x + 0] * q1) / (double)(X1 * K1);
+      const double py = twopi * (double)(xoff[3 * x + 1] * q2) / (double)(X2 * K2);
+      const double pz = twopi * (double)(xoff[3 * x + 2] * q3) / (double)(X3 * K3);
+      double sp, cp;
+      sincos(px + py + pz, &sp, &cp);
+      const double tr = cp * vr - sp * vi, ti = cp * vi + sp * vr;
+      vr = tr;
+      vi = ti;
+    }
+    uT[((size_t)s * nk + k) * ldT + (size_t)x * nb + n] = cmake<T2>((T)vr, (T)vi);
   }
 }
 
