@@ -18,14 +18,22 @@ enum ActFlags : int {
   F_LR = 1, F_FG = 2, F_SO = 4, F_E0 = 8, F_BACK = 16,
 };
 
+__device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const double2* __restrict__ psi,
+                                        const double* __restrict__ allow, double2* __restrict__ apart,
+                                        int nb, int nk, int na, int ns, int P, int k, int s);
+
 // ---------------------------------------------------------------------------
 // Projection: phi(k,x,a) = sum_n u(n,k,x) (A psi)(n,k,a)
 //   long_range.f90:897-906 (no conjugation of u).  Grid (ceil(nx/256), nk, ns).
+// With fg set, the last x-tile workgroup of each (k, s) also forms the multiplet
+// projection partials of that k (a 16-KB read beside the 512-KB u slab), so the
+// multiplet term needs no launch or stream of its own.
 // ---------------------------------------------------------------------------
 template <typename T2, int NA, int XV>
 __global__ __launch_bounds__(kBlock) void k_proj(
     const T2* __restrict__ u, const double2* __restrict__ psi, const double* __restrict__ allow,
-    T2* __restrict__ phi, int nb, int nk, int nx, int na, int ns, int ldu, int ldphi) {
+    T2* __restrict__ phi, int nb, int nk, int nx, int na, int ns, int ldu, int ldphi,
+    const double2* __restrict__ cproj, double2* __restrict__ apart, int P, int fg) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T2* ps = reinterpret_cast<T2*>(smem);  // [acnt][nb]
@@ -41,6 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     ps[i] = cmake<T2>((T)(p.x * m), (T)(p.y * m));
   }
   __syncthreads();
+  if (fg && blockIdx.x == gridDim.x - 1) mproj_k(cproj, psi, allow, apart, nb, nk, na, ns, P, k, s);
   // XV consecutive x per thread: one 16-byte load per band row in both precisions
   const int x = (blockIdx.x * blockDim.x + threadIdx.x) * XV;
   if (x >= nx) return;
@@ -63,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(
         uv[i][0] = cmake<T2>(q.x, q.y);
         uv[i][1] = cmake<T2>(q.z, q.w);
       } else {
-        uv[i][0] = ub[(size_t)n * ldu];
+        uv[i][0] = ld_stream(ub + (size_t)n * ldu);
       }
     }
 #pragma unroll
@@ -104,11 +113,14 @@ __device__ __forceinline__ constexpr int bitrev(int i) {
 // land on the same LDS banks.
 __device__ __forceinline__ int lidx(int e) { return e + (e >> 3); }
 
-// One pencil of length L: out[m] = sum_j v[j] w^(j m), w = exp(sign 2 pi i / L);
-// tw[m] = w^m comes from LDS.  Power-of-two L: radix-2 decimation in time on data
-// loaded in bit-reversed order; otherwise the direct sum.
-template <typename T2, int L>
-__device__ __forceinline__ void dft_dim(T2* lds, const T2* __restrict__ tw, int G, int nk, int S) {
+// One pass of length-L pencils along the k dimension of stride S, over the G
+// columns of the strip: out[m] = sum_j v[j] w^(j m), w = exp(sign 2 pi i / L),
+// tw[m] = w^m from LDS.  Power-of-two L: radix-2 decimation in time on data
+// loaded in bit-reversed order; otherwise the direct sum.  ld(k, g) / st(k, g, v)
+// move one element, so the first pass reads HBM straight into the pencil
+// registers and the last writes HBM from them.
+template <typename T2, int L, typename Ld, typename St>
+__device__ __forceinline__ void dft_pass(const T2* __restrict__ tw, int G, int nk, int S, Ld ld, St st) {
   const int npen = (G * nk) / L;
   for (int q = threadIdx.x; q < npen; q += blockDim.x) {
     const int g = q % G, r = q / G;
@@ -116,11 +128,9 @@ __device__ __forceinline__ void dft_dim(T2* lds, const T2* __restrict__ tw, int 
     T2 v[L];
     if constexpr ((L & (L - 1)) == 0) {
 #pragma unroll
-      for (int j = 0; j < L; ++j) v[bitrev<L>(j)] = lds[lidx((base + j * S) * G + g)];
+      for (int j = 0; j < L; ++j) v[bitrev<L>(j)] = ld(base + j * S, g);
 #pragma unroll
       for (int len = 2; len <= L; len <<= 1) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int half = len / 2, step = L / len;
 #pragma unroll
         for (int i = 0; i < L; i += len) {
@@ -134,39 +144,39 @@ __device__ __forceinline__ void dft_dim(T2* lds, const T2* __restrict__ tw, int 
         }
       }
 #pragma unroll
-      for (int m = 0; m < L; ++m) lds[lidx((base + m * S) * G + g)] = v[m];
+      for (int m = 0; m < L; ++m) st(base + m * S, g, v[m]);
     } else {
 #pragma unroll
-      for (int j = 0; j < L; ++j) v[j] = lds[lidx((base + j * S) * G + g)];
+      for (int j = 0; j < L; ++j) v[j] = ld(base + j * S, g);
 #pragma unroll
       for (int m = 0; m < L; ++m) {
         T2 acc = cmake<T2>(0, 0);
 #pragma unroll
         for (int j = 0; j < L; ++j) cfma(acc, v[j], tw[(j * m) % L]);
-        lds[lidx((base + m * S) * G + g)] = acc;
+        st(base + m * S, g, acc);
       }
     }
   }
 }
 
-template <typename T2, int LMAX>
-__device__ void dft_run(T2* lds, const T2* tw, int G, int nk, int L, int S) {
+template <typename T2, int LMAX, typename Ld, typename St>
+__device__ __forceinline__ void dft_run(const T2* tw, int G, int nk, int L, int S, Ld ld, St st) {
   switch (L) {
-    case 1: break;
-    case 2: dft_dim<T2, 2>(lds, tw, G, nk, S); break;
-    case 3: dft_dim<T2, 3>(lds, tw, G, nk, S); break;
-    case 4: dft_dim<T2, 4>(lds, tw, G, nk, S); break;
-    case 5: dft_dim<T2, 5>(lds, tw, G, nk, S); break;
-    case 6: dft_dim<T2, 6>(lds, tw, G, nk, S); break;
-    case 7: dft_dim<T2, 7>(lds, tw, G, nk, S); break;
-    case 8: dft_dim<T2, 8>(lds, tw, G, nk, S); break;
+    case 1: dft_pass<T2, 1>(tw, G, nk, S, ld, st); break;
+    case 2: dft_pass<T2, 2>(tw, G, nk, S, ld, st); break;
+    case 3: dft_pass<T2, 3>(tw, G, nk, S, ld, st); break;
+    case 4: dft_pass<T2, 4>(tw, G, nk, S, ld, st); break;
+    case 5: dft_pass<T2, 5>(tw, G, nk, S, ld, st); break;
+    case 6: dft_pass<T2, 6>(tw, G, nk, S, ld, st); break;
+    case 7: dft_pass<T2, 7>(tw, G, nk, S, ld, st); break;
+    case 8: dft_pass<T2, 8>(tw, G, nk, S, ld, st); break;
     default:
       if constexpr (LMAX > 8) {
         switch (L) {
-          case 9: dft_dim<T2, 9>(lds, tw, G, nk, S); break;
-          case 10: dft_dim<T2, 10>(lds, tw, G, nk, S); break;
-          case 12: dft_dim<T2, 12>(lds, tw, G, nk, S); break;
-          case 16: dft_dim<T2, 16>(lds, tw, G, nk, S); break;
+          case 9: dft_pass<T2, 9>(tw, G, nk, S, ld, st); break;
+          case 10: dft_pass<T2, 10>(tw, G, nk, S, ld, st); break;
+          case 12: dft_pass<T2, 12>(tw, G, nk, S, ld, st); break;
+          case 16: dft_pass<T2, 16>(tw, G, nk, S, ld, st); break;
           default: break;  // rejected on the host
         }
       }
@@ -174,15 +184,34 @@ __device__ void dft_run(T2* lds, const T2* tw, int G, int nk, int L, int S) {
   }
 }
 
+__device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict__ amp,
+                          const double2* __restrict__ K, double2* __restrict__ hmult, int* counter,
+                          int nk, int nj, double mul, int r, int nred, double2* lds);
+
+// Strip of G consecutive phi columns (c = x*na + a) per workgroup, all nk rows:
+//   pass 1 (K3, F_+): HBM -> registers -> LDS, with W(x, :) staged beside it
+//   passes 2, 3 (K2, K1, F_+), the last one scaling by W on its way back to LDS
+//   passes 4-6 (K3, K2, K1, F_-), the last one scaling by 1/N_k into HBM.
+// nred > 0 prepends that many workgroups running the multiplet reduction, which
+// is independent of phi and overlaps the transform.
 template <typename T2, int LMAX>
 __global__ __launch_bounds__(256) void k_fft_w(
     T2* __restrict__ phi, const typename Scalar<T2>::T* __restrict__ W, int nk, int ncol, int na,
-    int G, int K1, int K2, int K3, int ldphi) {
+    int G, int K1, int K2, int K3, int ldphi,
+    const double2* __restrict__ apart, double2* __restrict__ amp, const double2* __restrict__ Kmat,
+    double2* __restrict__ hmult, int* __restrict__ mcount, int nj, double mul, int nred) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
+  // the reduction workgroups come first so that they are dispatched first
+  const int nfft = gridDim.x - nred;
+  if ((int)blockIdx.x < nred) {
+    mred_tail(apart, amp, Kmat, hmult, mcount, nk, nj, mul, blockIdx.x, nred, reinterpret_cast<double2*>(smem));
+    return;
+  }
   // twiddles: [dim][sign][m], dims ordered (K3, K2, K1), 2 * LMAX per dim
   T2* twl = reinterpret_cast<T2*>(smem);
-  T2* lds = twl + 6 * LMAX;  // [k][g], padded
+  T2* lds = twl + 6 * LMAX;                               // [k][g], padded
+  T* wl = reinterpret_cast<T*>(lds + lidx(nk * G - 1) + 1);  // W(x, :) of the strip
   const int Ks[3] = {K3, K2, K1};
   for (int t = threadIdx.x; t < 6 * LMAX; t += blockDim.x) {
     const int d = t / (2 * LMAX), rem = t - d * 2 * LMAX, sg = rem / LMAX, m = rem - sg * LMAX;
@@ -197,78 +226,68 @@ __global__ __launch_bounds__(256) void k_fft_w(
   // Workgroups are dispatched round-robin over the 8 XCDs; remap so that
   // consecutive column strips (which share 128-B lines) run on the same XCD and
   // therefore hit the same L2.
-  int b = blockIdx.x;
-  const int nblk = gridDim.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  int b = blockIdx.x - nred;
+  if ((nfft & 7) == 0) b = (b & 7) * (nfft >> 3) + (b >> 3);
   const int c0 = b * G;
   const int gcnt = min(G, ncol - c0);
-  {
-    // batched tile load: 8 independent loads per thread in flight
-    const int tot = nk * G;
-    for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * blockDim.x) {
-      T2 v[8];
+  const int xlo = c0 / na, xhi = (c0 + gcnt - 1) / na;
+  const int nwl = (xhi - xlo + 1) * nk;
+  const T* wsrc = W + (size_t)xlo * nk;
+  // W loads issued ahead of the first pass, written to LDS after it
+  constexpr int kWR = 4;
+  T wr[kWR];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int e = min(e0 + i * (int)blockDim.x, tot - 1);
-        const int k = e / G, g = e - k * G;
-        v[i] = phi[(size_t)k * ldphi + c0 + min(g, gcnt - 1)];
-      }
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int e = e0 + i * (int)blockDim.x;
-        if (e < tot) {
-          const int g = e % G;
-          lds[lidx(e)] = (g < gcnt) ? v[i] : cmake<T2>(0, 0);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 0 * LMAX, G, nk, K3, 1);
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 2 * LMAX, G, nk, K2, K3);
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 4 * LMAX, G, nk, K1, K2 * K3);
-  __syncthreads();
-  for (int e = threadIdx.x; e < nk * G; e += blockDim.x) {
-    const int k = e / G, g = e - k * G;
-    if (g < gcnt) {
-      const int x = (c0 + g) / na;
-      const T w = W[(size_t)x * nk + k];
-      T2 v = lds[lidx(e)];
-      v.x *= w; v.y *= w;
-      lds[lidx(e)] = v;
-    }
-  }
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 1 * LMAX, G, nk, K3, 1);
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 3 * LMAX, G, nk, K2, K3);
-  __syncthreads();
-  dft_run<T2, LMAX>(lds, twl + 5 * LMAX, G, nk, K1, K2 * K3);
-  __syncthreads();
+  for (int i = 0; i < kWR; ++i) wr[i] = wsrc[min((int)threadIdx.x + i * (int)blockDim.x, nwl - 1)];
+  __syncthreads();  // twiddles
+  T2* __restrict__ gphi = phi + c0;
+  auto ld_hbm = [&](int k, int g) -> T2 {
+    const T2 v = gphi[(size_t)k * ldphi + min(g, gcnt - 1)];
+    return g < gcnt ? v : cmake<T2>(0, 0);
+  };
+  auto ld_lds = [&](int k, int g) -> T2 { return lds[lidx(k * G + g)]; };
+  auto st_lds = [&](int k, int g, T2 v) { lds[lidx(k * G + g)] = v; };
+  auto st_w = [&](int k, int g, T2 v) {
+    const T w = wl[((c0 + min(g, gcnt - 1)) / na - xlo) * nk + k];
+    v.x *= w; v.y *= w;
+    lds[lidx(k * G + g)] = v;
+  };
   const T inv = (T)1 / (T)nk;
-  for (int e = threadIdx.x; e < nk * G; e += blockDim.x) {
-    const int k = e / G, g = e - k * G;
+  auto st_hbm = [&](int k, int g, T2 v) {
     if (g < gcnt) {
-      T2 v = lds[lidx(e)];
       v.x *= inv; v.y *= inv;
-      phi[(size_t)k * ldphi + c0 + g] = v;
+      gphi[(size_t)k * ldphi + g] = v;
     }
+  };
+  dft_run<T2, LMAX>(twl + 0 * LMAX, G, nk, K3, 1, ld_hbm, st_lds);
+#pragma unroll
+  for (int i = 0; i < kWR; ++i) {
+    const int idx = threadIdx.x + i * blockDim.x;
+    if (idx < nwl) wl[idx] = wr[i];
   }
+  for (int idx = threadIdx.x + kWR * blockDim.x; idx < nwl; idx += blockDim.x) wl[idx] = wsrc[idx];
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 2 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 4 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_w);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 1 * LMAX, G, nk, K3, 1, ld_lds, st_lds);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 3 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
 }
 
 // ---------------------------------------------------------------------------
 // Multiplet projection partials: apart(j = a*P+p, k) = sum_n c(n,k,p) (A psi)(n,k,a)
-//   OCEAN_multiplet.f90:1730-1783 (no conjugation).  Grid (nk, ns); one wave per
-//   alpha, lanes over bands, 16 projectors per butterfly reduction.
+//   OCEAN_multiplet.f90:1730-1783 (no conjugation).  One workgroup per (k, s):
+//   one wave per alpha, lanes over bands, 16 projectors per butterfly reduction.
+//   Wave-level only (no barriers), so a workgroup of another kernel can run it.
 // ---------------------------------------------------------------------------
-constexpr int kProjChunk = 16;
+constexpr int kProjChunk = 8;
 
-__global__ __launch_bounds__(kBlock) void k_mproj(
-    const double2* __restrict__ c, const double2* __restrict__ psi, const double* __restrict__ allow,
-    double2* __restrict__ apart, int nb, int nk, int na, int ns, int P) {
-  const int k = blockIdx.x, s = blockIdx.y;
+__device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const double2* __restrict__ psi,
+                                        const double* __restrict__ allow, double2* __restrict__ apart,
+                                        int nb, int nk, int na, int ns, int P, int k, int s) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
@@ -306,26 +325,38 @@ __global__ __launch_bounds__(kBlock) void k_mproj(
   }
 }
 
-// Sum the partials over k (one wave per amplitude) and apply h = mul * K a:
-// direct -mpm a plus exchange X a (OCEAN_multiplet.f90:1812-1936).  One workgroup;
-// the matrix-vector product runs one wave per row with coalesced rows of K.
-__global__ __launch_bounds__(1024) void k_mred(
-    const double2* __restrict__ apart, const double2* __restrict__ K, double2* __restrict__ hmult,
-    int nk, int nj, double mul) {
-  __shared__ double2 amp[kMaxAlpha * 64];
+__global__ __launch_bounds__(kBlock) void k_mproj(
+    const double2* __restrict__ c, const double2* __restrict__ psi, const double* __restrict__ allow,
+    double2* __restrict__ apart, int nb, int nk, int na, int ns, int P) {
+  mproj_k(c, psi, allow, apart, nb, nk, na, ns, P, blockIdx.x, blockIdx.y);
+}
+
+// amp(j) = sum_k apart(j, k) for j in [j0, j1): one wave per amplitude, fixed order.
+__device__ __forceinline__ void mred_sum(const double2* __restrict__ apart, double2* __restrict__ amp,
+                                         int nk, int j0, int j1) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int j = wid; j < nj; j += nw) {
+  for (int j = j0 + wid; j < j1; j += nw) {
+    const double2* row = apart + (size_t)j * nk;
     double sx = 0.0, sy = 0.0;
-    for (int k = lane; k < nk; k += 64) {
-      const double2 v = apart[(size_t)j * nk + k];
-      sx += v.x;
-      sy += v.y;
+    for (int k0 = lane; k0 < nk; k0 += 8 * 64) {
+      double2 v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = row[min(k0 + i * 64, nk - 1)];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (k0 + i * 64 < nk) { sx += v[i].x; sy += v[i].y; }
     }
     sx = wave_sum(sx);
     sy = wave_sum(sy);
     if (lane == 0) amp[j] = make_double2(sx, sy);
   }
-  __syncthreads();
+}
+
+// h(i) = mul * sum_j K(i, j) amp(j): direct -mpm a plus exchange X a
+// (OCEAN_multiplet.f90:1812-1936); one wave per row, coalesced rows of K.
+__device__ __forceinline__ void mred_matvec(const double2* __restrict__ K, const double2* amp,
+                                            double2* __restrict__ hmult, int nj, double mul) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int i = wid; i < nj; i += nw) {
     double2 acc = make_double2(0.0, 0.0);
     for (int j = lane; j < nj; j += 64) cfma(acc, K[(size_t)i * nj + j], amp[j]);
@@ -333,6 +364,73 @@ __global__ __launch_bounds__(1024) void k_mred(
     acc.y = wave_sum(acc.y);
     if (lane == 0) hmult[i] = make_double2(acc.x * mul, acc.y * mul);
   }
+}
+
+// Stand-alone reduction (used when the long-range term is off): one workgroup.
+__global__ __launch_bounds__(1024) void k_mred(
+    const double2* __restrict__ apart, const double2* __restrict__ K, double2* __restrict__ hmult,
+    int nk, int nj, double mul) {
+  __shared__ double2 amp[kMaxAlpha * 64];
+  mred_sum(apart, amp, nk, 0, nj);
+  __syncthreads();
+  mred_matvec(K, amp, hmult, nj, mul);
+}
+
+// Reduction run by nred extra workgroups of another kernel's grid, one wave per
+// amplitude: workgroup r sums amplitudes [r*kRedPerBlock, ...) over k into amp
+// (global); the last workgroup to finish (device-scope counter) applies K, eight
+// rows per wave at a time with all their loads in flight.  The counter returns
+// to zero for the next launch.  Loads are batched throughout: the whole tail is
+// a few memory latencies, so it hides under the kernel it rides on.
+constexpr int kRedPerBlock = 4;
+
+__device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict__ amp,
+                          const double2* __restrict__ K, double2* __restrict__ hmult, int* counter,
+                          int nk, int nj, double mul, int r, int nred, double2* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  mred_sum(apart, amp, nk, r * kRedPerBlock, min(nj, (r + 1) * kRedPerBlock));
+  __threadfence();
+  __syncthreads();
+  __shared__ int is_last;
+  if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1) == nred - 1);
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  for (int j = threadIdx.x; j < nj; j += blockDim.x) {
+    // other workgroups (possibly on other XCDs) wrote amp: device-scope loads
+    const double ax = __hip_atomic_load(&amp[j].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double ay = __hip_atomic_load(&amp[j].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    lds[j] = make_double2(ax, ay);
+  }
+  __syncthreads();
+  constexpr int R = 8;
+  for (int i0 = wid * R; i0 < nj; i0 += nw * R) {
+    double v[2 * R];
+#pragma unroll
+    for (int q = 0; q < 2 * R; ++q) v[q] = 0.0;
+    for (int c0 = 0; c0 < nj; c0 += 64) {
+      const int j = c0 + lane;
+      const int jc = min(j, nj - 1);
+      double2 kv[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) kv[q] = K[(size_t)min(i0 + q, nj - 1) * nj + jc];
+      const double2 a = lds[jc];
+      if (j < nj) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          v[2 * q] = fma(kv[q].x, a.x, v[2 * q]); v[2 * q] = fma(-kv[q].y, a.y, v[2 * q]);
+          v[2 * q + 1] = fma(kv[q].x, a.y, v[2 * q + 1]); v[2 * q + 1] = fma(kv[q].y, a.x, v[2 * q + 1]);
+        }
+      }
+    }
+    int owner = 0;
+    const double tot = wave_multi_sum<double, 2 * R>(v, lane, &owner);
+    if ((lane & (64 / (2 * R) - 1)) == 0) {
+      const int q = owner >> 1;
+      if (i0 + q < nj) reinterpret_cast<double*>(hmult + i0 + q)[owner & 1] = tot * mul;
+    }
+  }
+  if (threadIdx.x == 0) atomicExch(counter, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,7 +442,13 @@ __global__ __launch_bounds__(1024) void k_mred(
 // bands, so every wave load is a contiguous band row, phi'(k,x,:) is the same
 // for all lanes of a row (an LDS broadcast), and no cross-lane reduction is
 // needed until the XS x-splits of the workgroup are summed once at the end.
-// One workgroup per k; threads (n, xs) with n fastest.
+// The projector rows c(:,k,p) have the same band-fastest shape and are streamed
+// by the same threads (p split like x).  Every operand of the epilogue and the
+// projector rows are loaded before the u stream starts, so their latency hides
+// under it: with one k per workgroup the whole grid is resident at once and any
+// load left to the end would be paid by every workgroup together.
+// One workgroup per k; threads (n, xs) with n fastest.  JR alphas per reduction
+// round through LDS (the reduction buffer reuses the phi' staging area).
 // ---------------------------------------------------------------------------
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) void k_adj(
@@ -352,37 +456,74 @@ __global__ __launch_bounds__(512) void k_adj(
     const double2* __restrict__ energy, const double* __restrict__ allow,
     const double2* __restrict__ cproj, const double2* __restrict__ hmult,
     const double2* __restrict__ S, double2* __restrict__ out, int nb, int nk, int nx, int na,
-    int ns, int P, int XC, int flags, int ldT, int ldphi) {
-  using T = typename Scalar<T2>::T;
+    int ns, int P, int XC, int JR, int flags, int ldT, int ldphi) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int XS = blockDim.x / nb;  // x-splits (host guarantees nb <= blockDim.x)
   double2* res = reinterpret_cast<double2*>(smem);           // [na][nb]
-  double2* red = res + na * nb;                                // [XS][nb]
-  T2* ph = reinterpret_cast<T2*>(red + XS * nb);               // [XC][NA]
+  T2* ph = reinterpret_cast<T2*>(res + na * nb);              // [XC][NA] during the stream
+  double2* red = res + na * nb;                               // [XS][JR][nb] afterwards
   // XCD-major workgroup order so that consecutive k share an XCD (phi' rows of
   // neighbouring k are adjacent in memory)
   int kk = blockIdx.x;
   const int nblk = gridDim.x;
   if ((nblk & 7) == 0) kk = (kk & 7) * (nblk >> 3) + (kk >> 3);
   const int k = kk;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, bd = blockDim.x;
   const int n = t % nb, xs = t / nb;
   const bool active = xs < XS;
-  for (int e = t; e < na * nb; e += blockDim.x) res[e] = make_double2(0.0, 0.0);
-  if (flags & F_LR) {
-    for (int s = 0; s < ns; ++s) {
-      const int astep = (ns == 2) ? 2 : 1;
-      const int acnt = (ns == 2) ? na / 2 : na;
-      T2 acc[NA];
+  const bool lr = (flags & F_LR) != 0, fg = (flags & F_FG) != 0;
+  const int nel = na * nb;
+
+  // epilogue operands of the first kEP elements owned by this thread
+  constexpr int kEP = 2;
+  double2 e_psi[kEP], e_en[kEP];
+  double e_m[kEP];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
+  for (int i = 0; i < kEP; ++i) {
+    const int e = min(t + i * bd, nel - 1);
+    const int a = e / nb, nn = e - a * nb;
+    const size_t off = ((size_t)a * nk + k) * nb + nn;
+    e_psi[i] = psi[off];
+    e_en[i] = energy[off];
+    e_m[i] = allow[off];
+  }
+
+  const int astep = (ns == 2) ? 2 : 1;
+  const int acnt = (ns == 2) ? na / 2 : na;
+  for (int s = 0; s < ns; ++s) {
+    // projector rows p = xs + i XS of this spin, issued first
+    constexpr int kCP = 2;
+    const double2* cb = cproj + ((size_t)s * nk + k) * P * nb + n;
+    double2 cpre[kCP];
+    if (fg && P > 0) {
+#pragma unroll
+      for (int i = 0; i < kCP; ++i) cpre[i] = cb[(size_t)min(xs + i * XS, P - 1) * nb];
+    }
+    T2 acc[NA];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
+    if (lr) {
       const T2* ub = uT + ((size_t)s * nk + k) * ldT;
       for (int x0 = 0; x0 < nx; x0 += XC) {
         const int xc = min(XC, nx - x0);
+        const int tot = xc * acnt;
         __syncthreads();
-        for (int e = t; e < xc * acnt; e += blockDim.x) {
-          const int xl = e / acnt, j = e - xl * acnt;
-          ph[xl * NA + j] = phi[(size_t)k * ldphi + (size_t)(x0 + xl) * na + s + j * astep];
+        for (int e0 = t; e0 < tot; e0 += 4 * bd) {
+          T2 v[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int e = min(e0 + i * bd, tot - 1);
+            const int xl = e / acnt, j = e - xl * acnt;
+            v[i] = phi[(size_t)k * ldphi + (size_t)(x0 + xl) * na + s + j * astep];
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int e = e0 + i * bd;
+            if (e < tot) {
+              const int xl = e / acnt, j = e - xl * acnt;
+              ph[xl * NA + j] = v[i];
+            }
+          }
         }
         __syncthreads();
         if (active) {
@@ -393,7 +534,7 @@ __global__ __launch_bounds__(512) void k_adj(
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
               const int xl = min(xb + i * XS, xc - 1);
-              uv[i] = ub[(size_t)(x0 + xl) * nb + n];
+              uv[i] = ld_stream(ub + (size_t)(x0 + xl) * nb + n);
             }
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
@@ -406,32 +547,56 @@ __global__ __launch_bounds__(512) void k_adj(
           }
         }
       }
-      // sum the x-splits, one alpha at a time through LDS
+    }
+    // this thread's share: fg - lr per alpha, in FP64
+    double2 part[NA];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) {
-        __syncthreads();
-        if (active) red[xs * nb + n] = make_double2((double)acc[j].x, (double)acc[j].y);
-        __syncthreads();
-        if (t < nb) {
-          double2 sum = make_double2(0.0, 0.0);
-          for (int q = 0; q < XS; ++q) { sum.x += red[q * nb + t].x; sum.y += red[q * nb + t].y; }
-          res[(s + j * astep) * nb + t] = sum;
+    for (int j = 0; j < NA; ++j) part[j] = make_double2(-(double)acc[j].x, -(double)acc[j].y);
+    if (fg && active) {
+#pragma unroll
+      for (int i = 0; i < kCP; ++i) {
+        const int p = xs + i * XS;
+        if (p < P) {
+#pragma unroll
+          for (int j = 0; j < NA; ++j)
+            if (j < acnt) cfmac(part[j], cpre[i], hmult[(s + j * astep) * P + p]);
         }
+      }
+      for (int p = xs + kCP * XS; p < P; p += XS) {
+        const double2 cv = cb[(size_t)p * nb];
+#pragma unroll
+        for (int j = 0; j < NA; ++j)
+          if (j < acnt) cfmac(part[j], cv, hmult[(s + j * astep) * P + p]);
+      }
+    }
+    // sum the x-splits, JR alphas per round
+    for (int j0 = 0; j0 < acnt; j0 += JR) {
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < NA; ++j)
+          if (j >= j0 && j < j0 + JR && j < acnt) red[(xs * JR + (j - j0)) * nb + n] = part[j];
+      }
+      __syncthreads();
+      const int jn = min(JR, acnt - j0);
+      for (int e = t; e < jn * nb; e += bd) {
+        const int jj = e / nb, nn = e - jj * nb;
+        double2 sum = make_double2(0.0, 0.0);
+        for (int q = 0; q < XS; ++q) {
+          const double2 r = red[(q * JR + jj) * nb + nn];
+          sum.x += r.x;
+          sum.y += r.y;
+        }
+        res[(s + (j0 + jj) * astep) * nb + nn] = sum;
       }
     }
   }
   __syncthreads();
   const bool back = (flags & F_BACK) != 0;
-  for (int e = t; e < na * nb; e += blockDim.x) {
+  auto finish = [&](int e, double2 pv, double2 ev, double am) {
     const int a = e / nb, nn = e - a * nb;
     const size_t off = ((size_t)a * nk + k) * nb + nn;
-    double2 inter = make_double2(-res[e].x, -res[e].y);
-    if (flags & F_FG) {
-      const int s = (ns == 2) ? (a & 1) : 0;
-      const double2* cb = cproj + ((size_t)s * nk + k) * P * nb + nn;
-      for (int p = 0; p < P; ++p) cfmac(inter, cb[(size_t)p * nb], hmult[a * P + p]);
-    }
-    const double am = allow[off];
+    double2 inter = res[e];
     inter.x *= am;
     inter.y *= am;
     if (flags & F_SO) {
@@ -443,13 +608,19 @@ __global__ __launch_bounds__(512) void k_adj(
     }
     double2 r = inter;
     if (flags & F_E0) {
-      double2 ev = energy[off];
       if (back) ev.y = -ev.y;
-      const double2 pv = psi[off];
       r.x = ev.x * pv.x - ev.y * pv.y + inter.x;
       r.y = ev.x * pv.y + ev.y * pv.x + inter.y;
     }
     out[off] = r;
+  };
+#pragma unroll
+  for (int i = 0; i < kEP; ++i)
+    if (t + i * bd < nel) finish(t + i * bd, e_psi[i], e_en[i], e_m[i]);
+  for (int e = t + kEP * bd; e < nel; e += bd) {
+    const int a = e / nb, nn = e - a * nb;
+    const size_t off = ((size_t)a * nk + k) * nb + nn;
+    finish(e, psi[off], energy[off], allow[off]);
   }
 }
 

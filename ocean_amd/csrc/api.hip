@@ -118,19 +118,23 @@ int check_ready(const H* h, int flags) {
 // --------------------------------------------------------------------------
 // The action, device pointer to device pointer, on h->stream.
 // --------------------------------------------------------------------------
+// The multiplet term rides on the long-range kernels: the projection partials
+// in the last x-tile workgroup of k_proj, the k-sum and K apply in extra
+// workgroups of k_fft_w.  Both only read psi, so one stream carries the action.
 template <typename T2, int NA>
 int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename og::Scalar<T2>::T* W,
-                       hipEvent_t* ev) {
+                       int flags, hipEvent_t* ev) {
   const int ns = h->ns, acnt = (ns == 2) ? h->na / 2 : h->na;
+  const int fg = (flags & og::F_FG) ? 1 : 0;
   size_t sm1 = (size_t)acnt * h->nb * sizeof(T2);
   if (false) {
     dim3 g1((h->nx / 2 + kBlock - 1) / kBlock, h->nk, ns);
-    og::k_proj<T2, NA, 2><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
-                                                          h->na, ns, h->ldu, h->ldphi);
+    og::k_proj<T2, NA, 2><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx, h->na,
+                                                          ns, h->ldu, h->ldphi, h->cproj, h->apart, h->P, fg);
   } else {
     dim3 g1((h->nx + kBlock - 1) / kBlock, h->nk, ns);
-    og::k_proj<T2, NA, 1><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx,
-                                                          h->na, ns, h->ldu, h->ldphi);
+    og::k_proj<T2, NA, 1><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx, h->na,
+                                                          ns, h->ldu, h->ldphi, h->cproj, h->apart, h->P, fg);
   }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
@@ -139,17 +143,25 @@ int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename o
   // G columns per workgroup: one x (all alphas) for 8^3 meshes, more for small meshes
   int G = std::max(1, std::min(cap / h->nk, (256 * 8) / h->nk));
   G = std::min(G, ncol);
-  dim3 g2((ncol + G - 1) / G);
+  const int nfft = (ncol + G - 1) / G;
+  const int nj = h->na * h->P;
+  const int nred = fg ? (nj + og::kRedPerBlock - 1) / og::kRedPerBlock : 0;
+  const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
   const size_t tile = (size_t)h->nk * G;
+  const size_t wl = (size_t)((G - 1) / h->na + 2) * h->nk * sizeof(typename og::Scalar<T2>::T);
   const int lmax = std::max(h->km[0], std::max(h->km[1], h->km[2]));
   if (lmax <= 8) {
-    size_t sm2 = (6 * 8 + tile + tile / 8 + 1) * sizeof(T2);
-    og::k_fft_w<T2, 8><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
-                                                       h->km[2], h->ldphi);
+    size_t sm2 = (6 * 8 + tile + tile / 8 + 1) * sizeof(T2) + wl;
+    sm2 = std::max(sm2, (size_t)nj * sizeof(double2));
+    og::k_fft_w<T2, 8><<<nfft + nred, kBlock, sm2, h->stream>>>(
+        phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1], h->km[2], h->ldphi, h->apart, h->amp, h->Kmat,
+        h->hmult, h->mcount, nj, mul, nred);
   } else {
-    size_t sm2 = (6 * 16 + tile + tile / 8 + 1) * sizeof(T2);
-    og::k_fft_w<T2, 16><<<g2, kBlock, sm2, h->stream>>>(phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1],
-                                                        h->km[2], h->ldphi);
+    size_t sm2 = (6 * 16 + tile + tile / 8 + 1) * sizeof(T2) + wl;
+    sm2 = std::max(sm2, (size_t)nj * sizeof(double2));
+    og::k_fft_w<T2, 16><<<nfft + nred, kBlock, sm2, h->stream>>>(
+        phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1], h->km[2], h->ldphi, h->apart, h->amp, h->Kmat,
+        h->hmult, h->mcount, nj, mul, nred);
   }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
@@ -161,15 +173,19 @@ int launch_adj(H* h, const double2* in, double2* out, const T2* uT, const T2* ph
   // one workgroup per k; threads (band n, x-split) with n fastest, up to 512
   const int XS = std::max(1, 512 / h->nb);
   const int threads = h->nb * XS;
-  const size_t resb = ((size_t)h->na * h->nb + (size_t)XS * h->nb) * sizeof(double2);
-  const size_t budget = 60 * 1024 - resb;
+  const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
+  const size_t resb = (size_t)h->na * h->nb * sizeof(double2);
+  const size_t budget = 64 * 1024 - resb;  // two workgroups per CU
+  // alphas per x-split reduction round, and the phi' chunk, sharing one area
+  const size_t per_j = (size_t)XS * h->nb * sizeof(double2);
+  const int JR = (int)std::max<size_t>(1, std::min<size_t>(acnt, budget / per_j));
   int XC = h->nx;
   const int cap = (int)(budget / (NA * sizeof(T2)));
   if (XC > cap) XC = std::max(XS, (cap / XS) * XS);
-  size_t sm = resb + (size_t)XC * NA * sizeof(T2);
-  og::k_adj<T2, NA><<<dim3(h->nk), threads, sm, h->stream>>>(
+  const size_t area = std::max((size_t)XC * NA * sizeof(T2), (size_t)JR * per_j);
+  og::k_adj<T2, NA><<<dim3(h->nk), threads, resb + area, h->stream>>>(
       uT, phi, in, h->energy, h->allow, h->cproj, h->hmult, h->Smat, out, h->nb, h->nk, h->nx, h->na,
-      h->ns, h->P, XC, flags, h->ldT, h->ldphi);
+      h->ns, h->P, XC, JR, flags, h->ldT, h->ldphi);
   OG_CHECK(hipGetLastError());
   return OCEAN_GPU_OK;
 }
@@ -178,28 +194,13 @@ template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
-  // The multiplet projection and reduction depend only on psi: run them on the
-  // side stream, overlapped with the projection and FFT, and join before the
-  // adjoint (which consumes hmult).  Timed runs (ev != null) keep one stream.
-  const bool fork = (flags & og::F_FG) && !ev;
-  if (fork) {
-    OG_CHECK(hipEventRecord(h->ev_fork, h->stream));
-    OG_CHECK(hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
-    og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->side>>>(h->cproj, in, h->allow, h->apart, h->nb,
-                                                           h->nk, h->na, h->ns, h->P);
-    OG_CHECK(hipGetLastError());
-    og::k_mred<<<1, 1024, 0, h->side>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
-    OG_CHECK(hipGetLastError());
-    OG_CHECK(hipEventRecord(h->ev_join, h->side));
-  }
   if (flags & og::F_LR) {
-    OG_TRY((launch_lr_proj_fft<T2, NA>(h, in, u, phi, W, ev)));
+    OG_TRY((launch_lr_proj_fft<T2, NA>(h, in, u, phi, W, flags, ev)));
   } else if (ev) {
     OG_CHECK(hipEventRecord(ev[1], h->stream));
     OG_CHECK(hipEventRecord(ev[2], h->stream));
   }
-  if ((flags & og::F_FG) && !fork) {
+  if ((flags & og::F_FG) && !(flags & og::F_LR)) {
     og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->stream>>>(h->cproj, in, h->allow, h->apart, h->nb,
                                                              h->nk, h->na, h->ns, h->P);
     OG_CHECK(hipGetLastError());
@@ -207,7 +208,6 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
     OG_CHECK(hipGetLastError());
   }
-  if (fork) OG_CHECK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
   OG_TRY((launch_adj<T2, NA>(h, in, out, uT, phi, flags)));
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
@@ -468,10 +468,7 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   }
   h->P = P;
   if (h->na * std::max(P, 1) > og::kMaxAlpha * 64) { delete h; return OCEAN_GPU_ERR_ARG; }
-  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
-      hipStreamCreate(&h->side) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
     ocean_gpu_destroy(h);
     return OCEAN_GPU_ERR_HIP;
   }
@@ -497,13 +494,10 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
   dfree(h->u64); dfree(h->u32); dfree(h->uT64); dfree(h->uT32); dfree(h->W64); dfree(h->W32); dfree(h->phi64); dfree(h->phi32);
   dfree(h->w_i0); dfree(h->w_w0); dfree(h->w_w1);
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
+  dfree(h->amp); dfree(h->mcount);
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
   for (auto& v : h->vecs) dfree(v);
   if (h->host_pin) (void)hipHostFree(h->host_pin);
-  if (h->side) (void)hipStreamSynchronize(h->side);
-  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return OCEAN_GPU_OK;
@@ -713,7 +707,11 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
   for (int a = 0; a < na; ++a)
     for (int b = 0; b < na; ++b) S[a * na + b] = make_double2(somelr[(size_t)b * na + a], someli[(size_t)b * na + a]);
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
+  dfree(h->amp); dfree(h->mcount);
   OG_TRY(dmalloc(&h->cproj, c.size()));
+  OG_TRY(dmalloc(&h->amp, (size_t)nj));
+  OG_TRY(dmalloc(&h->mcount, (size_t)1));
+  OG_CHECK(hipMemset(h->mcount, 0, sizeof(int)));
   OG_TRY(dmalloc(&h->Kmat, Kd.size()));
   OG_TRY(dmalloc(&h->apart, (size_t)nk * nj));
   OG_TRY(dmalloc(&h->hmult, (size_t)nj));

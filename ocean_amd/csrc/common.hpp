@@ -61,6 +61,19 @@ __device__ __forceinline__ void cfmac(T2& acc, T2 a, T2 b) {
   acc.y = fma(a.x, b.y, acc.y); acc.y = fma(-a.y, b.x, acc.y);
 }
 
+// Streaming load of data read once per action (the Bloch-state slabs): the
+// nontemporal hint keeps it from displacing reusable lines in L2 / MALL.
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  const v2d t = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  const v2f t = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(p));
+  return make_float2(t.x, t.y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -157,8 +170,6 @@ struct ocean_gpu_handle {
   int precision = OCEAN_GPU_FP64;
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;      // multiplet kernels overlap projection + FFT
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // long-range state
   double2* u64 = nullptr;   // [s][k][n][x]  (projection)
@@ -177,8 +188,10 @@ struct ocean_gpu_handle {
   // multiplet state
   double2* cproj = nullptr; // [s][k][p][n]
   double2* Kmat = nullptr;  // (P*na)^2, h_i = sum_j K[i*(P*na)+j] a_j, i=(a*P+p)
-  double2* apart = nullptr; // [k][a][p]
-  double2* hmult = nullptr; // [a][p]
+  double2* apart = nullptr; // [a*P+p][k] projection partials
+  double2* amp = nullptr;   // [a*P+p]   summed over k
+  double2* hmult = nullptr; // [a*P+p]   = mul K amp
+  int* mcount = nullptr;    // arrival counter of the reduction workgroups (rests at 0)
   double2* Smat = nullptr;  // [a][b]
   int have_so = 0;
 
