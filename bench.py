@@ -112,6 +112,13 @@ def main():
     log(f"FP64 H.psi: {wall / K * 1e3:.4f} ms wall, {ev_ms:.4f} ms events")
     prof = g.profile_xact(vin, vout, max(20, K // 5))
     log("per-kernel ms:", prof)
+    # the host hand-over (ocean_gpu_xact on host planes: PCIe both ways), never `value`
+    g.xact(inp.psi0)
+    nh = max(5, K // 10)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        g.xact(inp.psi0)
+    hpsi_host = nh / (time.perf_counter() - t0)
 
     bytes_ = kernel_bytes(inp)
     fam = max(("proj", "fft", "adj"), key=lambda k: prof[k])
@@ -171,6 +178,7 @@ def main():
                    "nalpha": 4, "haydock_niter": inp.params["niter"]},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    out["hpsi_per_s_host_planes"] = round(hpsi_host, 1)
     out.update(extras)
     print(json.dumps(out), flush=True)
     g.close()
