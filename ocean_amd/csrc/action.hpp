@@ -18,6 +18,47 @@ enum ActFlags : int {
   F_LR = 1, F_FG = 2, F_SO = 4, F_E0 = 8, F_BACK = 16,
 };
 
+// Everything one problem's action kernels read: pointers into its device state,
+// its sizes and its launch geometry.  Passed by value to the single-problem
+// kernels; the batched kernels read an array of them, one per problem.
+template <typename T2>
+struct ActArgs {
+  using T = typename Scalar<T2>::T;
+  const T2* u;                // [s][k][n][x], row stride ldu
+  const T2* uT;               // [s][k][x][n], k-slab stride ldT
+  T2* phi;                    // [k][x][a], row stride ldphi
+  const T* W;                 // [x][k]
+  const double2* psi;         // in  [a][k][n]
+  double2* out;               // out [a][k][n]
+  const double2* energy;
+  const double* allow;
+  const double2* cproj;       // [s][k][p][n]
+  double2* apart;             // [a*P+p][k]
+  double2* amp;               // [a*P+p]
+  const double2* Kmat;        // (na P)^2
+  double2* hmult;             // [a*P+p]
+  int* mcount;                // reduction arrival counter (rests at 0)
+  const double2* S;           // [a][b]
+  double mul;                 // inter_scale / (N_k celvol)
+  int nb, nk, nx, na, ns, P, nj;
+  int ldu, ldT, ldphi;
+  int flags;
+  int px;                     // projection x-tiles (blockDim.x x-points each)
+  int G, ncol, nfft, nred;    // transform: columns per strip, strips, reduction workgroups
+  int K1, K2, K3;
+  int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
+};
+
+// Problem p of a batched launch owns workgroups [off[p], off[p+1]).
+__device__ __forceinline__ int find_problem(const int* __restrict__ off, int np, int b) {
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const double2* __restrict__ psi,
                                         const double* __restrict__ allow, double2* __restrict__ apart,
                                         int nb, int nk, int na, int ns, int P, int k, int s);
@@ -29,70 +70,67 @@ __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const dou
 // projection partials of that k (a 16-KB read beside the 512-KB u slab), so the
 // multiplet term needs no launch or stream of its own.
 // ---------------------------------------------------------------------------
-template <typename T2, int NA, int XV>
-__global__ __launch_bounds__(kBlock) void k_proj(
-    const T2* __restrict__ u, const double2* __restrict__ psi, const double* __restrict__ allow,
-    T2* __restrict__ phi, int nb, int nk, int nx, int na, int ns, int ldu, int ldphi,
-    const double2* __restrict__ cproj, double2* __restrict__ apart, int P, int fg) {
+template <typename T2, int NA>
+__device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, int s) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T2* ps = reinterpret_cast<T2*>(smem);  // [acnt][nb]
-  const int k = blockIdx.y, s = blockIdx.z;
+  const int nb = A.nb, nk = A.nk, na = A.na, ns = A.ns;
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
   for (int i = threadIdx.x; i < acnt * nb; i += blockDim.x) {
     const int j = i / nb, n = i - j * nb;
     const int a = s + j * astep;
     const size_t off = ((size_t)a * nk + k) * nb + n;
-    const double2 p = psi[off];
-    const double m = allow[off];
+    const double2 p = A.psi[off];
+    const double m = A.allow[off];
     ps[i] = cmake<T2>((T)(p.x * m), (T)(p.y * m));
   }
   __syncthreads();
-  if (fg && blockIdx.x == gridDim.x - 1) mproj_k(cproj, psi, allow, apart, nb, nk, na, ns, P, k, s);
-  // XV consecutive x per thread: one 16-byte load per band row in both precisions
-  const int x = (blockIdx.x * blockDim.x + threadIdx.x) * XV;
-  if (x >= nx) return;
-  T2 acc[XV][NA];
+  if ((A.flags & F_FG) && bx == A.px - 1)
+    mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s);
+  const int x = bx * blockDim.x + threadIdx.x;
+  if (x >= A.nx) return;
+  T2 acc[NA];
 #pragma unroll
-  for (int v = 0; v < XV; ++v)
-#pragma unroll
-    for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
-  const T2* ub = u + ((size_t)(s * nk + k) * nb) * ldu + x;
+  for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
+  const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
   // Batches of kB band rows: all kB loads are issued before the first use, so
   // each wave keeps kB 1-KB row segments in flight.
   constexpr int kB = 8;
   for (int n0 = 0; n0 < nb; n0 += kB) {
-    T2 uv[kB][XV];
+    T2 uv[kB];
 #pragma unroll
-    for (int i = 0; i < kB; ++i) {
-      const int n = min(n0 + i, nb - 1);
-      if constexpr (XV == 2) {
-        const float4 q = *reinterpret_cast<const float4*>(ub + (size_t)n * ldu);
-        uv[i][0] = cmake<T2>(q.x, q.y);
-        uv[i][1] = cmake<T2>(q.z, q.w);
-      } else {
-        uv[i][0] = ld_stream(ub + (size_t)n * ldu);
-      }
-    }
+    for (int i = 0; i < kB; ++i) uv[i] = ld_stream(ub + (size_t)min(n0 + i, nb - 1) * A.ldu);
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       if (n0 + i < nb) {
 #pragma unroll
-        for (int j = 0; j < NA; ++j) {
-          const T2 pj = ps[j * nb + n0 + i];
-#pragma unroll
-          for (int v = 0; v < XV; ++v) cfma(acc[v][j], uv[i][v], pj);
-        }
+        for (int j = 0; j < NA; ++j) cfma(acc[j], uv[i], ps[j * nb + n0 + i]);
       }
     }
   }
+  T2* out = A.phi + (size_t)k * A.ldphi + (size_t)x * na;
 #pragma unroll
-  for (int v = 0; v < XV; ++v) {
-    T2* out = phi + (size_t)k * ldphi + (size_t)(x + v) * na;
-#pragma unroll
-    for (int j = 0; j < NA; ++j) out[s + j * astep] = acc[v][j];
-  }
+  for (int j = 0; j < NA; ++j)
+    if (j < acnt) out[s + j * astep] = acc[j];
+}
+
+// One problem: grid (px, nk, ns).
+template <typename T2, int NA>
+__global__ __launch_bounds__(512) void k_proj(const ActArgs<T2> A) {
+  proj_body<T2, NA>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Many problems: workgroup b belongs to the problem whose range holds it.
+template <typename T2, int NA>
+__global__ __launch_bounds__(512) void k_proj_batch(const ActArgs<T2>* __restrict__ As,
+                                                    const int* __restrict__ off, int np) {
+  const int p = find_problem(off, np, blockIdx.x);
+  const ActArgs<T2> A = As[p];
+  const int local = blockIdx.x - off[p];
+  const int bx = local % A.px, r = local / A.px;
+  proj_body<T2, NA>(A, bx, r % A.nk, r / A.nk);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,19 +233,20 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
 // nred > 0 prepends that many workgroups running the multiplet reduction, which
 // is independent of phi and overlaps the transform.
 template <typename T2, int LMAX>
-__global__ __launch_bounds__(256) void k_fft_w(
-    T2* __restrict__ phi, const typename Scalar<T2>::T* __restrict__ W, int nk, int ncol, int na,
-    int G, int K1, int K2, int K3, int ldphi,
-    const double2* __restrict__ apart, double2* __restrict__ amp, const double2* __restrict__ Kmat,
-    double2* __restrict__ hmult, int* __restrict__ mcount, int nj, double mul, int nred) {
+__device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   // the reduction workgroups come first so that they are dispatched first
-  const int nfft = gridDim.x - nred;
-  if ((int)blockIdx.x < nred) {
-    mred_tail(apart, amp, Kmat, hmult, mcount, nk, nj, mul, blockIdx.x, nred, reinterpret_cast<double2*>(smem));
+  const int nred = A.nred, nfft = A.nfft;
+  if (blk < nred) {
+    mred_tail(A.apart, A.amp, A.Kmat, A.hmult, A.mcount, A.nk, A.nj, A.mul, blk, nred,
+              reinterpret_cast<double2*>(smem));
     return;
   }
+  const int nk = A.nk, ncol = A.ncol, na = A.na, G = A.G, ldphi = A.ldphi;
+  const int K1 = A.K1, K2 = A.K2, K3 = A.K3;
+  T2* phi = A.phi;
+  const T* W = A.W;
   // twiddles: [dim][sign][m], dims ordered (K3, K2, K1), 2 * LMAX per dim
   T2* twl = reinterpret_cast<T2*>(smem);
   T2* lds = twl + 6 * LMAX;                               // [k][g], padded
@@ -226,7 +265,7 @@ __global__ __launch_bounds__(256) void k_fft_w(
   // Workgroups are dispatched round-robin over the 8 XCDs; remap so that
   // consecutive column strips (which share 128-B lines) run on the same XCD and
   // therefore hit the same L2.
-  int b = blockIdx.x - nred;
+  int b = blk - nred;
   if ((nfft & 7) == 0) b = (b & 7) * (nfft >> 3) + (b >> 3);
   const int c0 = b * G;
   const int gcnt = min(G, ncol - c0);
@@ -275,6 +314,19 @@ __global__ __launch_bounds__(256) void k_fft_w(
   dft_run<T2, LMAX>(twl + 3 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
   __syncthreads();
   dft_run<T2, LMAX>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
+}
+
+template <typename T2, int LMAX>
+__global__ __launch_bounds__(256) void k_fft_w(const ActArgs<T2> A) {
+  fft_body<T2, LMAX>(A, blockIdx.x);
+}
+
+template <typename T2, int LMAX>
+__global__ __launch_bounds__(256) void k_fft_w_batch(const ActArgs<T2>* __restrict__ As,
+                                                     const int* __restrict__ off, int np) {
+  const int p = find_problem(off, np, blockIdx.x);
+  const ActArgs<T2> A = As[p];
+  fft_body<T2, LMAX>(A, blockIdx.x - off[p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,21 +503,27 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
 // round through LDS (the reduction buffer reuses the phi' staging area).
 // ---------------------------------------------------------------------------
 template <typename T2, int NA>
-__global__ __launch_bounds__(512) void k_adj(
-    const T2* __restrict__ uT, const T2* __restrict__ phi, const double2* __restrict__ psi,
-    const double2* __restrict__ energy, const double* __restrict__ allow,
-    const double2* __restrict__ cproj, const double2* __restrict__ hmult,
-    const double2* __restrict__ S, double2* __restrict__ out, int nb, int nk, int nx, int na,
-    int ns, int P, int XC, int JR, int flags, int ldT, int ldphi) {
+__device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const int nb = A.nb, nk = A.nk, nx = A.nx, na = A.na, ns = A.ns, P = A.P;
+  const int XC = A.XC, JR = A.JR, flags = A.flags, ldT = A.ldT, ldphi = A.ldphi;
+  const T2* __restrict__ uT = A.uT;
+  const T2* __restrict__ phi = A.phi;
+  const double2* __restrict__ psi = A.psi;
+  const double2* __restrict__ energy = A.energy;
+  const double* __restrict__ allow = A.allow;
+  const double2* __restrict__ cproj = A.cproj;
+  const double2* __restrict__ hmult = A.hmult;
+  const double2* __restrict__ S = A.S;
+  double2* __restrict__ out = A.out;
   const int XS = blockDim.x / nb;  // x-splits (host guarantees nb <= blockDim.x)
   double2* res = reinterpret_cast<double2*>(smem);           // [na][nb]
   T2* ph = reinterpret_cast<T2*>(res + na * nb);              // [XC][NA] during the stream
   double2* red = res + na * nb;                               // [XS][JR][nb] afterwards
   // XCD-major workgroup order so that consecutive k share an XCD (phi' rows of
   // neighbouring k are adjacent in memory)
-  int kk = blockIdx.x;
-  const int nblk = gridDim.x;
+  int kk = blk;
+  const int nblk = nk;
   if ((nblk & 7) == 0) kk = (kk & 7) * (nblk >> 3) + (kk >> 3);
   const int k = kk;
   const int t = threadIdx.x, bd = blockDim.x;
@@ -622,6 +680,20 @@ __global__ __launch_bounds__(512) void k_adj(
     const size_t off = ((size_t)a * nk + k) * nb + nn;
     finish(e, psi[off], energy[off], allow[off]);
   }
+}
+
+// One problem: one workgroup per k.
+template <typename T2, int NA>
+__global__ __launch_bounds__(512) void k_adj(const ActArgs<T2> A) {
+  adj_body<T2, NA>(A, blockIdx.x);
+}
+
+template <typename T2, int NA>
+__global__ __launch_bounds__(512) void k_adj_batch(const ActArgs<T2>* __restrict__ As,
+                                                   const int* __restrict__ off, int np) {
+  const int p = find_problem(off, np, blockIdx.x);
+  const ActArgs<T2> A = As[p];
+  adj_body<T2, NA>(A, blockIdx.x - off[p]);
 }
 
 }  // namespace og

@@ -118,84 +118,93 @@ int check_ready(const H* h, int flags) {
 // --------------------------------------------------------------------------
 // The action, device pointer to device pointer, on h->stream.
 // --------------------------------------------------------------------------
-// The multiplet term rides on the long-range kernels: the projection partials
-// in the last x-tile workgroup of k_proj, the k-sum and K apply in extra
-// workgroups of k_fft_w.  Both only read psi, so one stream carries the action.
-template <typename T2, int NA>
-int launch_lr_proj_fft(H* h, const double2* in, T2* u, T2* phi, const typename og::Scalar<T2>::T* W,
-                       int flags, hipEvent_t* ev) {
-  const int ns = h->ns, acnt = (ns == 2) ? h->na / 2 : h->na;
-  const int fg = (flags & og::F_FG) ? 1 : 0;
-  size_t sm1 = (size_t)acnt * h->nb * sizeof(T2);
-  if (false) {
-    dim3 g1((h->nx / 2 + kBlock - 1) / kBlock, h->nk, ns);
-    og::k_proj<T2, NA, 2><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx, h->na,
-                                                          ns, h->ldu, h->ldphi, h->cproj, h->apart, h->P, fg);
-  } else {
-    dim3 g1((h->nx + kBlock - 1) / kBlock, h->nk, ns);
-    og::k_proj<T2, NA, 1><<<g1, kBlock, sm1, h->stream>>>(u, in, h->allow, phi, h->nb, h->nk, h->nx, h->na,
-                                                          ns, h->ldu, h->ldphi, h->cproj, h->apart, h->P, fg);
-  }
-  OG_CHECK(hipGetLastError());
-  if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
-  const int ncol = h->nx * h->na;
-  const int cap = (int)(57344 / sizeof(T2));
-  // G columns per workgroup: one x (all alphas) for 8^3 meshes, more for small meshes
-  int G = std::max(1, std::min(cap / h->nk, (256 * 8) / h->nk));
-  G = std::min(G, ncol);
-  const int nfft = (ncol + G - 1) / G;
-  const int nj = h->na * h->P;
-  const int nred = fg ? (nj + og::kRedPerBlock - 1) / og::kRedPerBlock : 0;
-  const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
-  const size_t tile = (size_t)h->nk * G;
-  const size_t wl = (size_t)((G - 1) / h->na + 2) * h->nk * sizeof(typename og::Scalar<T2>::T);
-  const int lmax = std::max(h->km[0], std::max(h->km[1], h->km[2]));
-  if (lmax <= 8) {
-    size_t sm2 = (6 * 8 + tile + tile / 8 + 1) * sizeof(T2) + wl;
-    sm2 = std::max(sm2, (size_t)nj * sizeof(double2));
-    og::k_fft_w<T2, 8><<<nfft + nred, kBlock, sm2, h->stream>>>(
-        phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1], h->km[2], h->ldphi, h->apart, h->amp, h->Kmat,
-        h->hmult, h->mcount, nj, mul, nred);
-  } else {
-    size_t sm2 = (6 * 16 + tile + tile / 8 + 1) * sizeof(T2) + wl;
-    sm2 = std::max(sm2, (size_t)nj * sizeof(double2));
-    og::k_fft_w<T2, 16><<<nfft + nred, kBlock, sm2, h->stream>>>(
-        phi, W, h->nk, ncol, h->na, G, h->km[0], h->km[1], h->km[2], h->ldphi, h->apart, h->amp, h->Kmat,
-        h->hmult, h->mcount, nj, mul, nred);
-  }
-  OG_CHECK(hipGetLastError());
-  if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
-  return OCEAN_GPU_OK;
-}
+using og::Tiles;
 
+constexpr size_t kLdsBudget = 64 * 1024;  // per workgroup: two adjoint workgroups per CU
+
+template <typename T2>
+struct Geom {
+  size_t sm_proj = 0, sm_fft = 0, sm_adj = 0;
+  int adj_threads = 0, nblk_proj = 0, nblk_fft = 0, nblk_adj = 0, lmax = 8;
+};
+
+// Fill the kernel arguments and launch geometry of one problem.  The
+// multiplet term rides on the long-range kernels when both are on: its partials
+// in the last x-tile workgroup of k_proj, its k-sum and K apply in workgroups
+// prepended to k_fft_w; both only read psi, so one stream carries the action.
 template <typename T2, int NA>
-int launch_adj(H* h, const double2* in, double2* out, const T2* uT, const T2* phi, int flags) {
-  // one workgroup per k; threads (band n, x-split) with n fastest, up to 512
-  const int XS = std::max(1, 512 / h->nb);
-  const int threads = h->nb * XS;
-  const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
-  const size_t resb = (size_t)h->na * h->nb * sizeof(double2);
-  const size_t budget = 64 * 1024 - resb;  // two workgroups per CU
-  // alphas per x-split reduction round, and the phi' chunk, sharing one area
-  const size_t per_j = (size_t)XS * h->nb * sizeof(double2);
-  const int JR = (int)std::max<size_t>(1, std::min<size_t>(acnt, budget / per_j));
-  int XC = h->nx;
-  const int cap = (int)(budget / (NA * sizeof(T2)));
-  if (XC > cap) XC = std::max(XS, (cap / XS) * XS);
-  const size_t area = std::max((size_t)XC * NA * sizeof(T2), (size_t)JR * per_j);
-  og::k_adj<T2, NA><<<dim3(h->nk), threads, resb + area, h->stream>>>(
-      uT, phi, in, h->energy, h->allow, h->cproj, h->hmult, h->Smat, out, h->nb, h->nk, h->nx, h->na,
-      h->ns, h->P, XC, JR, flags, h->ldT, h->ldphi);
-  OG_CHECK(hipGetLastError());
+int make_args(const H* h, const double2* in, double2* out, int flags, const Tiles& t, const T2* u,
+              const T2* uT, T2* phi, const typename og::Scalar<T2>::T* W, og::ActArgs<T2>& a,
+              Geom<T2>& g) {
+  using T = typename og::Scalar<T2>::T;
+  const int nb = h->nb, nk = h->nk, nx = h->nx, na = h->na, ns = h->ns;
+  const int acnt = (ns == 2) ? na / 2 : na;
+  a = og::ActArgs<T2>{};
+  a.u = u; a.uT = uT; a.phi = phi; a.W = W;
+  a.psi = in; a.out = out;
+  a.energy = h->energy; a.allow = h->allow;
+  a.cproj = h->cproj; a.apart = h->apart; a.amp = h->amp; a.Kmat = h->Kmat; a.hmult = h->hmult;
+  a.mcount = h->mcount; a.S = h->Smat;
+  a.mul = h->d.inter_scale / ((double)nk * h->d.celvol);
+  a.nb = nb; a.nk = nk; a.nx = nx; a.na = na; a.ns = ns; a.P = h->P; a.nj = na * h->P;
+  a.ldu = h->ldu; a.ldT = h->ldT; a.ldphi = h->ldphi;
+  a.flags = flags;
+  const bool lr = (flags & og::F_LR) != 0, fg = (flags & og::F_FG) != 0;
+  // projection
+  if (t.proj < 64 || t.proj > 512 || (t.proj & 63)) return OCEAN_GPU_ERR_ARG;
+  a.px = (nx + t.proj - 1) / t.proj;
+  g.sm_proj = (size_t)acnt * nb * sizeof(T2);
+  g.nblk_proj = a.px * nk * ns;
+  // transform
+  a.ncol = nx * na;
+  const int cap = (int)(57344 / sizeof(T2)) / nk;
+  int G = t.G > 0 ? t.G : std::min((256 * 8) / nk, cap);
+  G = std::max(1, std::min(std::min(G, std::max(cap, 1)), a.ncol));
+  a.G = G;
+  a.nfft = (a.ncol + G - 1) / G;
+  a.nred = (lr && fg) ? (a.nj + og::kRedPerBlock - 1) / og::kRedPerBlock : 0;
+  a.K1 = h->km[0]; a.K2 = h->km[1]; a.K3 = h->km[2];
+  g.lmax = std::max(h->km[0], std::max(h->km[1], h->km[2])) <= 8 ? 8 : 16;
+  const size_t tile = (size_t)nk * G;
+  g.sm_fft = (6 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2) +
+             (size_t)((G - 1) / na + 2) * nk * sizeof(T);
+  g.sm_fft = std::max(g.sm_fft, (size_t)a.nj * sizeof(double2));
+  g.nblk_fft = a.nred + a.nfft;
+  // adjoint: x-splits, phi' chunk and reduction rounds sharing one LDS area
+  if (nb > t.adj) return OCEAN_GPU_ERR_ARG;
+  const int XS = std::max(1, t.adj / nb);
+  g.adj_threads = nb * XS;
+  const size_t resb = (size_t)na * nb * sizeof(double2);
+  if (resb + (size_t)XS * nb * sizeof(double2) > kLdsBudget) return OCEAN_GPU_ERR_ARG;
+  const size_t budget = kLdsBudget - resb;
+  const size_t per_j = (size_t)XS * nb * sizeof(double2);
+  a.JR = (int)std::max<size_t>(1, std::min<size_t>(acnt, budget / per_j));
+  int XC = nx;
+  const int xcap = (int)(budget / (NA * sizeof(T2)));
+  if (XC > xcap) XC = std::max(XS, (xcap / XS) * XS);
+  a.XC = XC;
+  g.sm_adj = resb + std::max((size_t)XC * NA * sizeof(T2), (size_t)a.JR * per_j);
+  g.nblk_adj = nk;
   return OCEAN_GPU_OK;
 }
 
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
+  og::ActArgs<T2> a;
+  Geom<T2> g;
+  OG_TRY((make_args<T2, NA>(h, in, out, flags, h->tiles, u, uT, phi, W, a, g)));
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
-    OG_TRY((launch_lr_proj_fft<T2, NA>(h, in, u, phi, W, flags, ev)));
+    og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), h->tiles.proj, g.sm_proj, h->stream>>>(a);
+    OG_CHECK(hipGetLastError());
+    if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
+    if (g.lmax == 8)
+      og::k_fft_w<T2, 8><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+    else
+      og::k_fft_w<T2, 16><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+    OG_CHECK(hipGetLastError());
+    if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
   } else if (ev) {
     OG_CHECK(hipEventRecord(ev[1], h->stream));
     OG_CHECK(hipEventRecord(ev[2], h->stream));
@@ -204,12 +213,12 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->stream>>>(h->cproj, in, h->allow, h->apart, h->nb,
                                                              h->nk, h->na, h->ns, h->P);
     OG_CHECK(hipGetLastError());
-    const double mul = h->d.inter_scale / ((double)h->nk * h->d.celvol);
-    og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, h->na * h->P, mul);
+    og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, a.nj, a.mul);
     OG_CHECK(hipGetLastError());
   }
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
-  OG_TRY((launch_adj<T2, NA>(h, in, out, uT, phi, flags)));
+  og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
+  OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
   return OCEAN_GPU_OK;
 }
@@ -459,7 +468,8 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   h->ldu = h->nx + 32;
   h->ldphi = h->nx * h->na + 32;
   h->ldT = h->nx * h->nb + 32;
-  if (h->nb > 512) { delete h; return OCEAN_GPU_ERR_ARG; }
+  // the adjoint keeps one k's result (nalpha x nb complex) in LDS
+  if (h->nb > 512 || (size_t)h->na * h->nb * sizeof(double2) > 48 * 1024) { delete h; return OCEAN_GPU_ERR_ARG; }
   h->precision = d->precision;
   int P = 0;
   for (int l = d->lmin; l <= d->lmax; ++l) {

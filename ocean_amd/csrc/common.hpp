@@ -34,6 +34,13 @@ constexpr int kRedBlocks = 256;  // fixed partial count: deterministic reduction
 constexpr int kMaxAlpha = 12;    // nalpha <= 4 (2 lc + 1) with lc <= 1
 constexpr int kMaxDim = 16;      // largest k-mesh edge handled by the LDS DFT
 
+// Tile shapes of the action kernels (defaults; the autotuner picks per batch group).
+struct Tiles {
+  int proj = 256;  // projection workgroup size (x-points per workgroup)
+  int adj = 512;   // adjoint thread budget (x-splits = adj / nb)
+  int G = 0;       // transform columns per strip, 0 = the largest that fits
+};
+
 // complex helpers (double2 / float2 as interleaved complex)
 template <typename T2> struct Scalar;
 template <> struct Scalar<double2> { using T = double; };
@@ -214,6 +221,8 @@ struct ocean_gpu_handle {
   double2* tmp_out = nullptr;
 
   std::vector<double2*> vecs;  // device vectors (OCEAN_vector analogue)
+
+  og::Tiles tiles;  // tile shapes of the action kernels
 
   // timing of the last repeat call
   double t_proj = 0, t_fft = 0, t_adj = 0, t_mult = 0, t_total = 0;
