@@ -67,6 +67,77 @@ def cpu_baseline(inp, budget_s):
                       f"(C/OpenMP restatement, {dt:.1f} s)"}
 
 
+def batch_workload(n, niter, reps, log_every=8):
+    """Config 3: n uneven solves (K and L2,3 edges, nb 32..160, k-mesh 4^3..8^3,
+    x-mesh 8^3).  u is a complex Gaussian / sqrt(2 N_x) drawn in FP32 and widened
+    (not QR'd: same shapes and bytes as the QR'd generator, which would take
+    minutes at 20 GB); the other inputs come from the config-0 generators.
+    Returns the batched-form figures for the JSON line."""
+    from ocean_amd import core, synth
+
+    members = synth.batch_members(n)
+    gs, inps = [], []
+    t0 = time.perf_counter()
+    for i, m in enumerate(members):
+        kw = {k: v for k, v in m.items() if k != "pol"}
+        inp = synth.make_inputs(**kw, with_u=False)
+        g = core.OceanGPU.from_inputs(inp)
+        shape = (inp.nb, int(np.prod(inp.kmesh)), int(np.prod(inp.xmesh)), inp.nspn)
+        rng = np.random.default_rng(7000 + i)
+        scale = 1.0 / np.sqrt(2.0 * shape[2])
+        ure = np.asfortranarray(rng.standard_normal(shape, dtype=np.float32).astype(np.float64) * scale)
+        uim = np.asfortranarray(rng.standard_normal(shape, dtype=np.float32).astype(np.float64) * scale)
+        g.load_bloch_planes(ure, uim)
+        del ure, uim
+        gs.append(g)
+        inps.append(inp)
+        if (i + 1) % log_every == 0:
+            log(f"batch: {i + 1}/{n} problems loaded ({time.perf_counter() - t0:.0f}s)")
+    ubytes = sum(16.0 * i.nb * np.prod(i.kmesh) * np.prod(i.xmesh) for i in inps)
+    out = {"batch_problems": n, "batch_u_GB_fp64": round(ubytes / 1e9, 2)}
+    vin = [g.vec_new(i.psi0) for g, i in zip(gs, inps)]
+    vout = [g.vec_new() for g in gs]
+    # plain FP64 path: one problem after another
+    tot = 0.0
+    for g, a, b in zip(gs, vin, vout):
+        g.xact_repeat(a, b, 1)
+        tot += g.xact_repeat(a, b, reps)
+    out["batch_plain_fp64_hpsi_per_s"] = round(n / (tot * 1e-3), 1)
+    log(f"batch: plain FP64 {n} H.psi in {tot:.2f} ms")
+    res = {}
+    for prec, name in ((core.FP64, "fp64"), (core.MIXED, "mixed")):
+        for g in gs:
+            g.set_precision(prec)
+        bt = core.OceanBatch(gs)
+        groups = bt.autotune(reps=3)
+        bt.xact(vin, vout, 1)
+        ms = bt.xact(vin, vout, reps)
+        out[f"batch_{name}_hpsi_per_s"] = round(n / (ms * 1e-3), 1)
+        out[f"batch_{name}_tiles"] = [list(gr[:6]) for gr in groups]
+        log(f"batch {name}: {ms:.3f} ms per batched H.psi; groups {groups}")
+        coeffs, ms_h = bt.haydock([i.psi0 for i in inps], niter)
+        out[f"batch_{name}_spectra_per_s"] = round(n / (ms_h * 1e-3), 2)
+        res[name] = coeffs
+        log(f"batch {name}: {niter}-iteration recursions in {ms_h:.1f} ms")
+        bt.close()
+    # mixed vs FP64 spectra, per problem relative to its own peak
+    E2H = core.EV2HARTREE
+    worst = 0.0
+    for q, inp in enumerate(inps):
+        a64, b64, _ = res["fp64"][q]
+        am, bm, _ = res["mixed"][q]
+        s64 = core.spectrum(a64, b64, niter, inp.kpref, 0.5 * E2H, -5 * E2H, 40 * E2H, 2000)
+        sm = core.spectrum(am, bm, niter, inp.kpref, 0.5 * E2H, -5 * E2H, 40 * E2H, 2000)
+        pk = np.abs(s64[:, 2]).max()
+        worst = max(worst, float(np.abs(sm[:, 2] - s64[:, 2]).max() / pk))
+    out["batch_mixed_spectrum_err_of_peak"] = worst
+    out["batch_mixed_speedup_vs_plain_fp64"] = round(out["batch_mixed_hpsi_per_s"] /
+                                                     out["batch_plain_fp64_hpsi_per_s"], 2)
+    for g in gs:
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +147,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batched form")
+    ap.add_argument("--batch-n", type=int, default=96)
+    ap.add_argument("--batch-niter", type=int, default=200)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -164,6 +238,11 @@ def main():
         extras["mixed_speedup_hpsi"] = round(extras["hpsi_per_s_mixed"] / hpsi_fp64, 3)
         g.set_precision(core.FP64)
 
+    if not args.no_extras and not args.no_batch:
+        g.close()
+        g = None
+        extras.update(batch_workload(args.batch_n, args.batch_niter, reps=5))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(inp, args.cpu_seconds)
@@ -181,7 +260,8 @@ def main():
     out["hpsi_per_s_host_planes"] = round(hpsi_host, 1)
     out.update(extras)
     print(json.dumps(out), flush=True)
-    g.close()
+    if g is not None:
+        g.close()
 
 
 if __name__ == "__main__":

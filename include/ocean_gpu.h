@@ -85,6 +85,9 @@ int  ocean_gpu_load_bloch(ocean_gpu_handle* h, const double* u_re, const double*
 /* Raw Bloch functions (con.u2.dat order: OCEAN2/OCEAN_bloch.f90:998-1115) phased
  * on the device as OCEAN_bloch_lrLOAD does; xoff(3, nx) are the wrapped integer x
  * offsets (xph + ix - 1, ...) of OCEAN2/OCEAN_bloch.f90:133-203. */
+/* As ocean_gpu_load_bloch with the two planes already in device memory (same
+ * layout); for callers that build or keep u on the GPU. */
+int  ocean_gpu_load_bloch_dev(ocean_gpu_handle* h, const double* u_re_dev, const double* u_im_dev);
 int  ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const double* u_im,
                               const int32_t* xoff);
 
@@ -169,6 +172,36 @@ int  ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* ha
 int  ocean_gpu_spectrum(const double* real_a, const double* real_b, const double* imag_a,
                         int32_t niter, double kpref, double gam, double el, double eh,
                         int32_t ne, int32_t literal_cmplx, double* out);
+
+/* ---- batches (no reference counterpart: OCEAN2 runs one site per process) ----
+ * A batch runs many independent problems -- each a handle loaded as above, of
+ * any size (nb, k-mesh, x-mesh, edge, precision) -- together: one launch per
+ * kernel stage for every group of problems that shares a kernel instantiation
+ * (precision, alpha count, k-mesh edge class).  The handles stay owned by the
+ * caller and must outlive the batch; only the long-range-on action is batched. */
+typedef struct ocean_gpu_batch ocean_gpu_batch;
+int  ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_batch** out);
+int  ocean_gpu_batch_destroy(ocean_gpu_batch* b);
+int  ocean_gpu_batch_ngroups(ocean_gpu_batch* b, int32_t* ngroups);
+/* info[6] = (problems, precision, alpha template, projection workgroup, adjoint
+ * thread budget, transform strip width (0 = widest)); tune_ms[3] (nullable) = the
+ * autotuner's best time of each stage (ms per launch). */
+int  ocean_gpu_batch_group_info(ocean_gpu_batch* b, int32_t group, int32_t* info, double* tune_ms);
+/* Times every candidate tile shape of each stage on the batch's own problems
+ * (reps launches each) and keeps the fastest per group. */
+int  ocean_gpu_batch_autotune(ocean_gpu_batch* b, int32_t reps);
+/* nrep batched actions: problem q maps its vector in_ids[q] to out_ids[q] (ids in
+ * problem q's handle); the time per batched action is read back with
+ * ocean_gpu_batch_last_ms. */
+int  ocean_gpu_batch_xact(ocean_gpu_batch* b, const int32_t* in_ids, const int32_t* out_ids, int32_t nrep);
+int  ocean_gpu_batch_last_ms(ocean_gpu_batch* b, double* ms);
+/* Haydock recursions of all problems at once (as ocean_gpu_haydock per problem:
+ * pointer arrays of n host planes / coefficient arrays of niter+1 entries).
+ * Iterations replay a captured hipGraph of one three-iteration cycle.  Returns
+ * -1 if any problem hit a NaN (niter_done[q] says where). */
+int  ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, const double* const* psi_im,
+                             int32_t niter, double* const* real_a, double* const* real_b,
+                             double* const* imag_a, int32_t* niter_done);
 
 /* Average durations (ms) measured with HIP events on the library's stream:
  * "total" from the last ocean_gpu_xact_vec_repeat, "proj", "fft", "mult", "adj"

@@ -81,6 +81,16 @@ _SIGS = {
     "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
+    "ocean_gpu_load_bloch_dev": (C.c_int, [_H, C.c_void_p, C.c_void_p]),
+    "ocean_gpu_batch_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(_H)]),
+    "ocean_gpu_batch_destroy": (C.c_int, [_H]),
+    "ocean_gpu_batch_ngroups": (C.c_int, [_H, _IP]),
+    "ocean_gpu_batch_group_info": (C.c_int, [_H, C.c_int32, _IP, _DP]),
+    "ocean_gpu_batch_autotune": (C.c_int, [_H, C.c_int32]),
+    "ocean_gpu_batch_xact": (C.c_int, [_H, _IP, _IP, C.c_int32]),
+    "ocean_gpu_batch_last_ms": (C.c_int, [_H, _DP]),
+    "ocean_gpu_batch_haydock": (C.c_int, [_H, C.POINTER(_DP), C.POINTER(_DP), C.c_int32,
+                                          C.POINTER(_DP), C.POINTER(_DP), C.POINTER(_DP), _IP]),
 }
 
 
@@ -174,7 +184,8 @@ class OceanGPU:
         return g
 
     def load_inputs(self, inp):
-        self.load_bloch(inp.u)
+        if inp.u is not None:  # else the caller loads u itself (load_bloch_dev)
+            self.load_bloch(inp.u)
         self.load_w(inp.W)
         self.load_mult(inp.mult)
         self.load_energies(inp.energy, inp.allow)
@@ -204,6 +215,15 @@ class OceanGPU:
     def load_bloch(self, u):
         re, im = _planes(u)
         _chk(_lib().ocean_gpu_load_bloch(self.h, _p(re), _p(im)), "load_bloch")
+
+    def load_bloch_planes(self, re, im):
+        """u as two F-ordered float64 planes (nb, nk, nx, nspn)."""
+        _chk(_lib().ocean_gpu_load_bloch(self.h, _p(re), _p(im)), "load_bloch")
+
+    def load_bloch_dev(self, re_ptr, im_ptr):
+        """u from two device planes (integer device addresses, same layout as load_bloch)."""
+        _chk(_lib().ocean_gpu_load_bloch_dev(self.h, C.c_void_p(re_ptr), C.c_void_p(im_ptr)),
+             "load_bloch_dev")
 
     def load_bloch_raw(self, u_raw, xoff):
         re, im = _planes(u_raw)
@@ -344,3 +364,73 @@ class OceanGPU:
         if return_x:
             return out, xr + 1j * xi
         return out
+
+
+class OceanBatch:
+    """Many OceanGPU problems whose actions and Haydock recursions run together
+    (one launch per kernel stage per group; see ocean_gpu_batch_* in the header).
+    The problems must stay open while the batch is used."""
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        n = len(self.problems)
+        arr = (_H * n)(*[p.h for p in self.problems])
+        self.b = _H()
+        _chk(_lib().ocean_gpu_batch_create(arr, n, C.byref(self.b)), "batch_create")
+
+    def close(self):
+        if self.b:
+            _lib().ocean_gpu_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def groups(self):
+        """[(problems, precision, alpha template, proj tile, adj tile, G, tune_ms[3]), ...]"""
+        ng = C.c_int32()
+        _chk(_lib().ocean_gpu_batch_ngroups(self.b, C.byref(ng)), "batch_ngroups")
+        out = []
+        for g in range(ng.value):
+            info = np.zeros(6, np.int32)
+            ms = np.zeros(3)
+            _chk(_lib().ocean_gpu_batch_group_info(self.b, g, info.ctypes.data_as(_IP), _p(ms)),
+                 "batch_group_info")
+            out.append(tuple(int(v) for v in info) + (ms.tolist(),))
+        return out
+
+    def autotune(self, reps=5):
+        _chk(_lib().ocean_gpu_batch_autotune(self.b, reps), "batch_autotune")
+        return self.groups()
+
+    def xact(self, in_ids, out_ids, nrep=1):
+        """nrep batched actions; returns ms per batched action (all problems)."""
+        ii = np.ascontiguousarray(in_ids, np.int32)
+        oo = np.ascontiguousarray(out_ids, np.int32)
+        _chk(_lib().ocean_gpu_batch_xact(self.b, ii.ctypes.data_as(_IP), oo.ctypes.data_as(_IP), nrep),
+             "batch_xact")
+        ms = C.c_double()
+        _chk(_lib().ocean_gpu_batch_last_ms(self.b, C.byref(ms)), "batch_last_ms")
+        return ms.value
+
+    def haydock(self, psi0s, niter):
+        """Per problem (real_a, real_b, imag_a); also returns the recursion's ms."""
+        n = len(self.problems)
+        planes = [_planes(p) for p in psi0s]
+        pr = (_DP * n)(*[_p(x[0]) for x in planes])
+        pi = (_DP * n)(*[_p(x[1]) for x in planes])
+        A = [np.zeros(niter + 1) for _ in range(n)]
+        Bv = [np.zeros(niter + 1) for _ in range(n)]
+        AI = [np.zeros(niter + 1) for _ in range(n)]
+        pa = (_DP * n)(*[_p(x) for x in A])
+        pb = (_DP * n)(*[_p(x) for x in Bv])
+        pai = (_DP * n)(*[_p(x) for x in AI])
+        done = np.zeros(n, np.int32)
+        rc = _lib().ocean_gpu_batch_haydock(self.b, pr, pi, niter, pa, pb, pai, done.ctypes.data_as(_IP))
+        _chk(rc, "batch_haydock")
+        ms = C.c_double()
+        _chk(_lib().ocean_gpu_batch_last_ms(self.b, C.byref(ms)), "batch_last_ms")
+        return [(A[q], Bv[q], AI[q]) for q in range(n)], ms.value

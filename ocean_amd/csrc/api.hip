@@ -1,6 +1,7 @@
 // C-ABI of the gfx950 core-level BSE library (declared in include/ocean_gpu.h).
 #include <algorithm>
 #include <complex>
+#include <type_traits>
 
 #include "action.hpp"
 #include "common.hpp"
@@ -539,17 +540,23 @@ int ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision) {
   return OCEAN_GPU_OK;
 }
 
-static int load_u_common(H* h, const double* re, const double* im, const int32_t* xoff) {
+static int load_u_common(H* h, const double* re, const double* im, const int32_t* xoff, bool dev = false) {
   if (!h || !re || !im) return OCEAN_GPU_ERR_ARG;
   const size_t nu = (size_t)h->ns * h->nk * h->nb * h->nx;
   double *dre = nullptr, *dim_ = nullptr;
   int* dxo = nullptr;
-  OG_TRY(dmalloc(&dre, nu));
-  if (dmalloc(&dim_, nu) != 0) { dfree(dre); return OCEAN_GPU_ERR_NOMEM; }
   int rc = OCEAN_GPU_OK;
-  if (hipMemcpy(dre, re, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dim_, im, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-    rc = OCEAN_GPU_ERR_HIP;
+  if (dev) {
+    // caller's device planes are read in place
+    dre = const_cast<double*>(re);
+    dim_ = const_cast<double*>(im);
+  } else {
+    OG_TRY(dmalloc(&dre, nu));
+    if (dmalloc(&dim_, nu) != 0) { dfree(dre); return OCEAN_GPU_ERR_NOMEM; }
+    if (hipMemcpy(dre, re, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dim_, im, nu * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
   if (rc == 0 && xoff) {
     rc = dmalloc(&dxo, (size_t)3 * h->nx);
     if (rc == 0 && hipMemcpy(dxo, xoff, 3 * h->nx * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
@@ -581,7 +588,8 @@ static int load_u_common(H* h, const double* re, const double* im, const int32_t
           h->nx, h->ns, h->km[0], h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2], h->ldT);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }
-  dfree(dre); dfree(dim_); dfree(dxo);
+  if (!dev) { dfree(dre); dfree(dim_); }
+  dfree(dxo);
   if (rc == 0) {
     if (h->precision == OCEAN_GPU_MIXED) { h->u32_ok = true; h->u64_ok = false; }
     else { h->u64_ok = true; h->u32_ok = false; }
@@ -591,6 +599,10 @@ static int load_u_common(H* h, const double* re, const double* im, const int32_t
 
 int ocean_gpu_load_bloch(ocean_gpu_handle* h, const double* u_re, const double* u_im) {
   return load_u_common(h, u_re, u_im, nullptr);
+}
+
+int ocean_gpu_load_bloch_dev(ocean_gpu_handle* h, const double* u_re, const double* u_im) {
+  return load_u_common(h, u_re, u_im, nullptr, true);
 }
 
 int ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const double* u_im,
@@ -1142,3 +1154,464 @@ int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batches: many independent problems (one loaded handle each) whose actions and
+// Haydock recursions run together, one launch per stage for all problems of a
+// group.  A group shares the kernel instantiation: precision, alpha count and
+// the transform's edge class.
+// ===========================================================================
+struct ocean_gpu_batch {
+  struct Group {
+    int prec = OCEAN_GPU_FP64, NA = 4, lmax = 8;
+    std::vector<int> idx;  // problems of the group, in batch order
+    Tiles tiles;
+    float tune_ms[3] = {0.f, 0.f, 0.f};  // best stage times found by the autotuner
+  };
+  std::vector<H*> hs;
+  std::vector<Group> groups;
+  hipStream_t stream = nullptr;
+  int* dcount = nullptr;  // [0] Haydock iteration number, [1] arrival counter
+  double t_last = 0.0;    // ms per batched action (all problems) of the last timed call
+};
+
+namespace {
+
+using B = ocean_gpu_batch;
+
+template <typename T2> struct StateOf;
+template <> struct StateOf<double2> {
+  static const double2* u(const H* h) { return h->u64; }
+  static const double2* uT(const H* h) { return h->uT64; }
+  static double2* phi(const H* h) { return h->phi64; }
+  static const double* W(const H* h) { return h->W64; }
+};
+template <> struct StateOf<float2> {
+  static const float2* u(const H* h) { return h->u32; }
+  static const float2* uT(const H* h) { return h->uT32; }
+  static float2* phi(const H* h) { return h->phi32; }
+  static const float* W(const H* h) { return h->W32; }
+};
+
+// Device tables of one group for one set of (in, out) vectors.
+struct Tables {
+  void* args = nullptr;  // og::ActArgs<T2>[np]
+  int* off = nullptr;    // [3][np + 1] first workgroup of each problem: proj, fft, adj
+  int np = 0;
+  int nblk[3] = {0, 0, 0};
+  size_t sm[3] = {0, 0, 0};
+  int threads[3] = {256, kBlock, 512};
+  int lmax = 8;
+  void release() {
+    if (args) (void)hipFree(args);
+    if (off) (void)hipFree(off);
+    args = nullptr;
+    off = nullptr;
+  }
+};
+
+template <typename F>
+int group_call(const B::Group& g, F&& f) {
+  const bool mixed = g.prec == OCEAN_GPU_MIXED;
+  switch (g.NA) {
+    case 2: return mixed ? f(std::integral_constant<int, 2>{}, float2{}) : f(std::integral_constant<int, 2>{}, double2{});
+    case 4: return mixed ? f(std::integral_constant<int, 4>{}, float2{}) : f(std::integral_constant<int, 4>{}, double2{});
+    case 6: return mixed ? f(std::integral_constant<int, 6>{}, float2{}) : f(std::integral_constant<int, 6>{}, double2{});
+    case 12: return mixed ? f(std::integral_constant<int, 12>{}, float2{}) : f(std::integral_constant<int, 12>{}, double2{});
+    default: return OCEAN_GPU_ERR_ARG;
+  }
+}
+
+template <typename T2, int NA>
+int build_tables(const B* b, const B::Group& g, const std::vector<const double2*>& in,
+                 const std::vector<double2*>& out, const Tiles& t, Tables& tb) {
+  const int np = (int)g.idx.size();
+  std::vector<og::ActArgs<T2>> host((size_t)np);
+  std::vector<int> off((size_t)3 * (np + 1), 0);
+  tb = Tables{};
+  tb.np = np;
+  tb.lmax = g.lmax;
+  for (int q = 0; q < np; ++q) {
+    const H* h = b->hs[g.idx[q]];
+    const int f = flags_of(h, 0, nullptr);
+    if (!(f & og::F_LR)) return OCEAN_GPU_ERR_ARG;
+    OG_TRY(check_ready(h, f));
+    Geom<T2> ge;
+    OG_TRY((make_args<T2, NA>(h, in[q], out[q], f, t, StateOf<T2>::u(h), StateOf<T2>::uT(h),
+                              StateOf<T2>::phi(h), StateOf<T2>::W(h), host[q], ge)));
+    off[q + 1] = off[q] + ge.nblk_proj;
+    off[(np + 1) + q + 1] = off[(np + 1) + q] + ge.nblk_fft;
+    off[2 * (np + 1) + q + 1] = off[2 * (np + 1) + q] + ge.nblk_adj;
+    tb.sm[0] = std::max(tb.sm[0], ge.sm_proj);
+    tb.sm[1] = std::max(tb.sm[1], ge.sm_fft);
+    tb.sm[2] = std::max(tb.sm[2], ge.sm_adj);
+  }
+  for (int i = 0; i < 3; ++i) tb.nblk[i] = off[(size_t)i * (np + 1) + np];
+  tb.threads[0] = t.proj;
+  tb.threads[2] = t.adj;
+  OG_CHECK(hipMalloc(&tb.args, sizeof(og::ActArgs<T2>) * (size_t)np));
+  if (hipMalloc((void**)&tb.off, sizeof(int) * off.size()) != hipSuccess) { tb.release(); return OCEAN_GPU_ERR_NOMEM; }
+  if (hipMemcpy(tb.args, host.data(), sizeof(og::ActArgs<T2>) * (size_t)np, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(tb.off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    tb.release();
+    return OCEAN_GPU_ERR_HIP;
+  }
+  return OCEAN_GPU_OK;
+}
+
+// stage bits: 1 projection, 2 transform (+ multiplet reduction), 4 adjoint
+template <typename T2, int NA>
+int launch_tables(const Tables& tb, hipStream_t st, int stages) {
+  const auto* A = static_cast<const og::ActArgs<T2>*>(tb.args);
+  const int np = tb.np;
+  if (stages & 1)
+    og::k_proj_batch<T2, NA><<<tb.nblk[0], tb.threads[0], tb.sm[0], st>>>(A, tb.off, np);
+  if (stages & 2) {
+    if (tb.lmax == 8)
+      og::k_fft_w_batch<T2, 8><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
+    else
+      og::k_fft_w_batch<T2, 16><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
+  }
+  if (stages & 4)
+    og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int group_build(const B* b, const B::Group& g, const std::vector<const double2*>& in,
+                const std::vector<double2*>& out, const Tiles& t, Tables& tb) {
+  return group_call(g, [&](auto na, auto t2) {
+    return build_tables<decltype(t2), decltype(na)::value>(b, g, in, out, t, tb);
+  });
+}
+
+int group_launch(const B::Group& g, const Tables& tb, hipStream_t st, int stages) {
+  return group_call(g, [&](auto na, auto t2) {
+    return launch_tables<decltype(t2), decltype(na)::value>(tb, st, stages);
+  });
+}
+
+// the (in, out) vectors of group g: per problem, in[q] and out[q] of the batch order
+void group_vectors(const B::Group& g, const std::vector<const double2*>& in_all,
+                   const std::vector<double2*>& out_all, std::vector<const double2*>& in,
+                   std::vector<double2*>& out) {
+  in.clear();
+  out.clear();
+  for (int q : g.idx) {
+    in.push_back(in_all[q]);
+    out.push_back(out_all[q]);
+  }
+}
+
+// Time `stages` of group g over reps launches (ms per launch), after one warm-up.
+int time_group(B* b, const B::Group& g, const Tables& tb, int stages, int reps, float* ms) {
+  hipEvent_t e0, e1;
+  OG_CHECK(hipEventCreate(&e0));
+  OG_CHECK(hipEventCreate(&e1));
+  int rc = group_launch(g, tb, b->stream, stages);
+  if (rc == 0) rc = hipEventRecord(e0, b->stream) == hipSuccess ? 0 : OCEAN_GPU_ERR_HIP;
+  for (int r = 0; r < reps && rc == 0; ++r) rc = group_launch(g, tb, b->stream, stages);
+  if (rc == 0 && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) *ms /= (float)reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+}  // namespace
+
+int ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_batch** out) {
+  if (!hs || n <= 0 || !out) return OCEAN_GPU_ERR_ARG;
+  *out = nullptr;
+  B* b = new B();
+  for (int i = 0; i < n; ++i) {
+    // each problem owns its multiplet scratch and counter: no handle twice
+    if (!hs[i] || hs[i]->device != hs[0]->device ||
+        std::find(b->hs.begin(), b->hs.end(), hs[i]) != b->hs.end()) {
+      delete b;
+      return OCEAN_GPU_ERR_ARG;
+    }
+    b->hs.push_back(hs[i]);
+  }
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipMalloc((void**)&b->dcount, 2 * sizeof(int)) != hipSuccess) {
+    ocean_gpu_batch_destroy(b);
+    return OCEAN_GPU_ERR_HIP;
+  }
+  const int zero[2] = {1, 0};
+  if (hipMemcpy(b->dcount, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+    ocean_gpu_batch_destroy(b);
+    return OCEAN_GPU_ERR_HIP;
+  }
+  for (int i = 0; i < n; ++i) {
+    const H* h = hs[i];
+    const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
+    B::Group key;
+    key.prec = h->precision;
+    key.NA = na_tmpl(acnt);
+    key.lmax = std::max(h->km[0], std::max(h->km[1], h->km[2])) <= 8 ? 8 : 16;
+    auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const B::Group& g) {
+      return g.prec == key.prec && g.NA == key.NA && g.lmax == key.lmax;
+    });
+    if (it == b->groups.end()) {
+      b->groups.push_back(key);
+      it = b->groups.end() - 1;
+    }
+    it->idx.push_back(i);
+    // the largest band count of the group bounds the adjoint thread budget
+    int nbmax = 0;
+    for (int q : it->idx) nbmax = std::max(nbmax, b->hs[q]->nb);
+    it->tiles.adj = nbmax > 256 ? 512 : it->tiles.adj;
+  }
+  *out = b;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_destroy(ocean_gpu_batch* b) {
+  if (!b) return OCEAN_GPU_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->dcount) (void)hipFree(b->dcount);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_ngroups(ocean_gpu_batch* b, int32_t* ngroups) {
+  if (!b || !ngroups) return OCEAN_GPU_ERR_ARG;
+  *ngroups = (int32_t)b->groups.size();
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_group_info(ocean_gpu_batch* b, int32_t group, int32_t* info, double* tune_ms) {
+  if (!b || !info || group < 0 || group >= (int)b->groups.size()) return OCEAN_GPU_ERR_ARG;
+  const B::Group& g = b->groups[group];
+  info[0] = (int32_t)g.idx.size();
+  info[1] = g.prec;
+  info[2] = g.NA;
+  info[3] = g.tiles.proj;
+  info[4] = g.tiles.adj;
+  info[5] = g.tiles.G;
+  if (tune_ms)
+    for (int i = 0; i < 3; ++i) tune_ms[i] = g.tune_ms[i];
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_autotune(ocean_gpu_batch* b, int32_t reps) {
+  if (!b || reps <= 0) return OCEAN_GPU_ERR_ARG;
+  const int n = (int)b->hs.size();
+  std::vector<const double2*> in_all(n);
+  std::vector<double2*> out_all(n);
+  for (int q = 0; q < n; ++q) {
+    H* h = b->hs[q];
+    OG_CHECK(hipStreamSynchronize(h->stream));
+    og::k_fill<<<grid_for(h->nvec()), kBlock, 0, b->stream>>>(h->tmp_in, make_double2(1e-3, -2e-3), h->nvec());
+    OG_CHECK(hipGetLastError());
+    in_all[q] = h->tmp_in;
+    out_all[q] = h->tmp_out;
+  }
+  for (auto& g : b->groups) {
+    std::vector<const double2*> in;
+    std::vector<double2*> out;
+    group_vectors(g, in_all, out_all, in, out);
+    int nbmax = 0;
+    for (int q : g.idx) nbmax = std::max(nbmax, b->hs[q]->nb);
+    // each stage's tile only changes that stage, so they are tuned one at a time
+    const int proj_c[3] = {128, 256, 512};
+    const int adj_c[2] = {256, 512};
+    const int G_c[5] = {0, 1, 2, 4, 8};
+    Tiles best = g.tiles;
+    float bms[3] = {1e30f, 1e30f, 1e30f};
+    for (int stage = 0; stage < 3; ++stage) {
+      const int nc = stage == 0 ? 3 : (stage == 1 ? 5 : 2);
+      for (int c = 0; c < nc; ++c) {
+        Tiles t = best;
+        if (stage == 0) t.proj = proj_c[c];
+        if (stage == 1) t.G = G_c[c];
+        if (stage == 2) {
+          if (adj_c[c] < nbmax) continue;
+          t.adj = adj_c[c];
+        }
+        Tables tb;
+        if (group_build(b, g, in, out, t, tb) != OCEAN_GPU_OK) continue;  // shape not possible
+        float ms = 0.f;
+        const int rc = time_group(b, g, tb, 1 << stage, reps, &ms);
+        tb.release();
+        if (rc) return rc;
+        if (ms < bms[stage]) {
+          bms[stage] = ms;
+          if (stage == 0) best.proj = t.proj;
+          if (stage == 1) best.G = t.G;
+          if (stage == 2) best.adj = t.adj;
+        }
+      }
+    }
+    g.tiles = best;
+    for (int i = 0; i < 3; ++i) g.tune_ms[i] = bms[i];
+  }
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_xact(ocean_gpu_batch* b, const int32_t* in_ids, const int32_t* out_ids, int32_t nrep) {
+  if (!b || !in_ids || !out_ids || nrep <= 0) return OCEAN_GPU_ERR_ARG;
+  const int n = (int)b->hs.size();
+  std::vector<const double2*> in_all(n);
+  std::vector<double2*> out_all(n);
+  for (int q = 0; q < n; ++q) {
+    H* h = b->hs[q];
+    in_all[q] = vec_ptr(h, in_ids[q]);
+    out_all[q] = vec_ptr(h, out_ids[q]);
+    if (!in_all[q] || !out_all[q] || in_ids[q] == out_ids[q]) return OCEAN_GPU_ERR_ARG;
+    OG_CHECK(hipStreamSynchronize(h->stream));
+  }
+  std::vector<Tables> tabs(b->groups.size());
+  int rc = OCEAN_GPU_OK;
+  for (size_t gi = 0; gi < b->groups.size() && rc == 0; ++gi) {
+    std::vector<const double2*> in;
+    std::vector<double2*> out;
+    group_vectors(b->groups[gi], in_all, out_all, in, out);
+    rc = group_build(b, b->groups[gi], in, out, b->groups[gi].tiles, tabs[gi]);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0 && hipEventRecord(e0, b->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  for (int r = 0; r < nrep && rc == 0; ++r)
+    for (size_t gi = 0; gi < b->groups.size() && rc == 0; ++gi) rc = group_launch(b->groups[gi], tabs[gi], b->stream, 7);
+  if (rc == 0 && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+    b->t_last = ms / nrep;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (auto& t : tabs) t.release();
+  return rc;
+}
+
+int ocean_gpu_batch_last_ms(ocean_gpu_batch* b, double* ms) {
+  if (!b || !ms) return OCEAN_GPU_ERR_ARG;
+  *ms = b->t_last;
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, const double* const* psi_im,
+                            int32_t niter, double* const* real_a, double* const* real_b,
+                            double* const* imag_a, int32_t* niter_done) {
+  if (!b || !psi_re || !psi_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
+  const int n = (int)b->hs.size();
+  const size_t n1 = (size_t)niter + 1;
+  const size_t nscal = 3 * n1 + 3 * (size_t)kRedBlocks;
+  std::vector<double2*> V((size_t)3 * n, nullptr);
+  std::vector<double*> Sd((size_t)n, nullptr);
+  int rc = OCEAN_GPU_OK;
+  for (int q = 0; q < n && rc == 0; ++q) {
+    H* h = b->hs[q];
+    if (!psi_re[q] || !psi_im[q] || !real_a[q] || !real_b[q]) rc = OCEAN_GPU_ERR_ARG;
+    for (int i = 0; i < 3 && rc == 0; ++i) rc = dmalloc(&V[(size_t)3 * q + i], h->nvec());
+    if (rc == 0) rc = dmalloc(&Sd[q], nscal);
+    if (rc == 0) rc = put_planes(h, psi_re[q], psi_im[q], V[(size_t)3 * q], h->nvec());
+  }
+  // rotation r of an iteration: (psi, old, hv) = (V0, V1, V2), (V2, V0, V1), (V1, V2, V0)
+  const int rot[3][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}};
+  std::vector<std::vector<Tables>> tabs(3, std::vector<Tables>(b->groups.size()));
+  og::HayArgs* dhay[3] = {nullptr, nullptr, nullptr};
+  for (int r = 0; r < 3 && rc == 0; ++r) {
+    std::vector<const double2*> in_all(n);
+    std::vector<double2*> out_all(n);
+    std::vector<og::HayArgs> ha((size_t)n);
+    for (int q = 0; q < n; ++q) {
+      double2* psi = V[(size_t)3 * q + rot[r][0]];
+      double2* old = V[(size_t)3 * q + rot[r][1]];
+      double2* hv = V[(size_t)3 * q + rot[r][2]];
+      in_all[q] = psi;
+      out_all[q] = hv;
+      double* s = Sd[q];
+      ha[q] = og::HayArgs{hv, old, psi, s + 3 * n1, s + 3 * n1 + 2 * kRedBlocks, s, s + n1, s + 2 * n1,
+                          b->hs[q]->nvec()};
+    }
+    for (size_t gi = 0; gi < b->groups.size() && rc == 0; ++gi) {
+      std::vector<const double2*> in;
+      std::vector<double2*> out;
+      group_vectors(b->groups[gi], in_all, out_all, in, out);
+      rc = group_build(b, b->groups[gi], in, out, b->groups[gi].tiles, tabs[r][gi]);
+    }
+    if (rc == 0) rc = dmalloc(&dhay[r], (size_t)n);
+    if (rc == 0 && hipMemcpy(dhay[r], ha.data(), sizeof(og::HayArgs) * n, hipMemcpyHostToDevice) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  // initial state: psi_old = 0, coefficients 0 (b_0 = 0), iteration 1
+  if (rc == 0) {
+    for (int q = 0; q < n; ++q) {
+      og::k_fill<<<grid_for(b->hs[q]->nvec()), kBlock, 0, b->stream>>>(V[(size_t)3 * q + 1], make_double2(0.0, 0.0),
+                                                                       b->hs[q]->nvec());
+      og::k_fill_d<<<grid_for(3 * n1), kBlock, 0, b->stream>>>(Sd[q], 0.0, 3 * n1);
+    }
+    const int init[2] = {1, 0};
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(b->dcount, init, sizeof(init), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  auto enqueue_iter = [&](int r) -> int {
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) OG_TRY(group_launch(b->groups[gi], tabs[r][gi], b->stream, 7));
+    og::k_hay1_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount);
+    og::k_hay2_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount);
+    og::k_hay3_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount, b->dcount + 1);
+    OG_CHECK(hipGetLastError());
+    return OCEAN_GPU_OK;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = OCEAN_GPU_ERR_HIP;
+  // one captured graph holds a full rotation cycle (three iterations)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const int cycles = niter / 3, rem = niter % 3;
+  if (rc == 0 && cycles > 0) {
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+    int rc_cap = OCEAN_GPU_OK;
+    for (int r = 0; r < 3 && rc == 0 && rc_cap == 0; ++r) rc_cap = enqueue_iter(r);
+    if (rc == 0) {
+      if (hipStreamEndCapture(b->stream, &graph) != hipSuccess || rc_cap != 0) rc = rc_cap ? rc_cap : OCEAN_GPU_ERR_HIP;
+    }
+    if (rc == 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0 && hipEventRecord(e0, b->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  for (int c = 0; c < cycles && rc == 0; ++c)
+    if (hipGraphLaunch(exec, b->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  for (int r = 0; r < rem && rc == 0; ++r) rc = enqueue_iter(r);
+  if (rc == 0 && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) b->t_last = ms;
+  }
+  // coefficients back to the host, NaN check per problem as the single recursion
+  int rc_nan = OCEAN_GPU_OK;
+  std::vector<double> hb(3 * n1);
+  for (int q = 0; q < n && rc == 0; ++q) {
+    if (hipMemcpy(hb.data(), Sd[q], hb.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = OCEAN_GPU_ERR_HIP;
+      break;
+    }
+    int done = niter;
+    for (int i = 0; i < niter; ++i)
+      if (hb[i] != hb[i]) { done = i + 1; rc_nan = OCEAN_GPU_ERR_NAN; break; }
+    for (size_t i = 0; i < n1; ++i) {
+      real_a[q][i] = hb[i];
+      real_b[q][i] = hb[2 * n1 + i];
+      if (imag_a && imag_a[q]) imag_a[q][i] = hb[n1 + i];
+    }
+    if (niter_done) niter_done[q] = done;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (auto& tr : tabs)
+    for (auto& t : tr) t.release();
+  for (int r = 0; r < 3; ++r) dfree(dhay[r]);
+  for (auto& v : V) dfree(v);
+  for (auto& s : Sd) dfree(s);
+  return rc ? rc : rc_nan;
+}

@@ -157,6 +157,104 @@ __global__ __launch_bounds__(kBlock) void k_hay3(double2* __restrict__ h, const 
 }
 
 // ---------------------------------------------------------------------------
+// The same three passes for many independent recursions in one launch each.
+// Every problem gets kRedBlocks workgroups and the same partition as the
+// single-problem kernels, so a batched recursion reproduces each problem's
+// stand-alone coefficients bit for bit.  The iteration number lives on the
+// device (*itp) so that one captured iteration replays unchanged: k_hay3_batch's
+// last workgroup to finish advances it, after every workgroup has read it.
+// ---------------------------------------------------------------------------
+struct HayArgs {
+  double2* h;            // H psi on entry, psi_{i+1} on exit
+  const double2* old;    // psi_{i-1}
+  const double2* psi;    // psi_i
+  double* part1;         // [2 kRedBlocks]
+  double* part2;         // [kRedBlocks]
+  double* a;             // real_a(0:niter)
+  double* ai;            // imag_a(0:niter)
+  double* b;             // real_b(0:niter)
+  size_t n;
+};
+
+__global__ __launch_bounds__(kBlock) void k_hay1_batch(const HayArgs* __restrict__ As, const int* __restrict__ itp) {
+  __shared__ double lds[8 * 2];
+  const HayArgs A = As[blockIdx.x / kRedBlocks];
+  const int lb = blockIdx.x % kRedBlocks;
+  const int it = *itp;
+  const double mb = -A.b[it - 1];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < A.n; i += (size_t)kRedBlocks * blockDim.x) {
+    double2 a = A.h[i];
+    const double2 o = A.old[i];
+    a.x += mb * o.x; a.y += mb * o.y;
+    A.h[i] = a;
+    const double2 b = A.psi[i];
+    v[0] = fma(a.x, b.x, v[0]); v[0] = fma(a.y, b.y, v[0]);
+    v[1] = fma(a.x, b.y, v[1]); v[1] = fma(-a.y, b.x, v[1]);
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { A.part1[2 * lb] = v[0]; A.part1[2 * lb + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hay2_batch(const HayArgs* __restrict__ As, const int* __restrict__ itp) {
+  __shared__ double lds[8];
+  __shared__ double s_a;
+  const HayArgs A = As[blockIdx.x / kRedBlocks];
+  const int lb = blockIdx.x % kRedBlocks;
+  const int it = *itp;
+  if (threadIdx.x < 64) {
+    const double ar = wave_sum_parts(A.part1, kRedBlocks, 2, 0, threadIdx.x);
+    const double ai = wave_sum_parts(A.part1, kRedBlocks, 2, 1, threadIdx.x);
+    if (threadIdx.x == 0) {
+      s_a = ar;
+      if (lb == 0) { A.a[it - 1] = ar; A.ai[it - 1] = ai; }
+    }
+  }
+  __syncthreads();
+  const double ma = -s_a;
+  double v[1] = {0.0};
+  for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < A.n; i += (size_t)kRedBlocks * blockDim.x) {
+    double2 a = A.h[i];
+    const double2 p = A.psi[i];
+    a.x += ma * p.x; a.y += ma * p.y;
+    A.h[i] = a;
+    v[0] = fma(a.x, a.x, v[0]); v[0] = fma(a.y, a.y, v[0]);
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) A.part2[lb] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_hay3_batch(const HayArgs* __restrict__ As, int* __restrict__ itp,
+                                                       int* __restrict__ counter) {
+  __shared__ double s_inv;
+  const HayArgs A = As[blockIdx.x / kRedBlocks];
+  const int lb = blockIdx.x % kRedBlocks;
+  const int it = *itp;
+  if (threadIdx.x < 64) {
+    const double b = sqrt(wave_sum_parts(A.part2, kRedBlocks, 1, 0, threadIdx.x));
+    if (threadIdx.x == 0) {
+      s_inv = 1.0 / b;
+      if (lb == 0) A.b[it] = b;
+    }
+  }
+  __syncthreads();
+  const double inv = s_inv;
+  for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < A.n; i += (size_t)kRedBlocks * blockDim.x) {
+    double2 a = A.h[i];
+    a.x *= inv; a.y *= inv;
+    A.h[i] = a;
+  }
+  // every workgroup has read *itp above; the last one to arrive advances it
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (atomicAdd(counter, 1) == (int)gridDim.x - 1) {
+      *itp = it + 1;
+      atomicExch(counter, 0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Continued fraction with the square-root terminator (write_core,
 // OCEAN2/OCEAN_haydock.f90:1801-1894), one thread per energy.
 // ---------------------------------------------------------------------------

@@ -117,8 +117,11 @@ def _gaussian_complex(rng, shape):
 
 def make_inputs(nb, kmesh, xmesh, lc=0, nspn=1, seed=0, xi_eV=0.0, lmin=0, lmax=3,
                 w_scale=1.0, a_bohr=6.0, epsilon0=4.0, tau=(0.13, 0.27, 0.41),
-                inter_scale=1.0, scissor_eV=0.0, w_bump=None, params=None):
-    """Build one synthetic core-level problem.  Deterministic in ``seed``."""
+                inter_scale=1.0, scissor_eV=0.0, w_bump=None, params=None, with_u=True):
+    """Build one synthetic core-level problem.  Deterministic in ``seed``.
+
+    with_u=False leaves ``u`` as None (the Bloch-function draw and QR are skipped;
+    the caller generates u on the device), everything else is unchanged."""
     rng = np.random.default_rng(seed)
     kmesh = tuple(int(v) for v in kmesh)
     xmesh = tuple(int(v) for v in xmesh)
@@ -140,13 +143,18 @@ def make_inputs(nb, kmesh, xmesh, lc=0, nspn=1, seed=0, xi_eV=0.0, lmin=0, lmax=
     allow = np.ones((nb, nk, na), np.float64, order="F")
 
     # Bloch functions: QR over x per k, then the supercell phase
-    u_raw = np.zeros((nb, nk, nx, nspn), np.complex128, order="F")
-    for s in range(nspn):
-        for k in range(nk):
-            q, _ = np.linalg.qr(_gaussian_complex(rng, (nx, nb)))
-            u_raw[:, k, :, s] = q.T
-    u, tau_c, xshift = sh.bloch_phase(u_raw, xmesh, kmesh, np.asarray(tau, np.float64))
-    del u_raw
+    if with_u:
+        u_raw = np.zeros((nb, nk, nx, nspn), np.complex128, order="F")
+        for s in range(nspn):
+            for k in range(nk):
+                q, _ = np.linalg.qr(_gaussian_complex(rng, (nx, nb)))
+                u_raw[:, k, :, s] = q.T
+        u, tau_c, xshift = sh.bloch_phase(u_raw, xmesh, kmesh, np.asarray(tau, np.float64))
+        del u_raw
+    else:
+        u = None
+        _, tau_c, xshift = sh.bloch_phase(np.zeros((1, nk, nx, 1), np.complex128, order="F"),
+                                          xmesh, kmesh, np.asarray(tau, np.float64))
 
     rtab, ptab = w_table()
     ptab = ptab * w_scale
