@@ -164,6 +164,29 @@ int  ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* ha
                      double ffff, int32_t depth, double kpref, double* out,
                      double* x_re, double* x_im);
 
+/* Gradient of a spectral misfit (no reference counterpart: OCEAN2 computes no
+ * gradients).  Over the energy list, sigma(E) = -kpref Im<b|x_E> with
+ * (E + i gres - H) x_E = b, b = allow . hay (the spectrum ocean_gpu_gmres
+ * reports), and L = sum_E (sigma(E) - target[E])^2.  The adjoint vectors y_E
+ * solve the conjugate-energy systems (E - i gres - H) y_E = b with the same
+ * recycled GMRES; with r_E = sigma(E) - target[E]:
+ *   dL/dgres      =  sum_E 2 r_E kpref Re<y_E|x_E>
+ *   dL/dscissor   = -sum_E 2 r_E kpref Im<y_E|x_E>       (every band energy + Delta)
+ *   dL/dscale     = -sum_E 2 r_E kpref Im<y_E|(H_lr + H_mult) x_E> / scale
+ *                   (W built with scale = inter_scale, mul = inter_scale / (N_k Omega))
+ *   dL/dptab[i]   =  sum_(k,x) Q(k,x) dW(k,x)/dptab[i],
+ *                   Q = sum_E 2 r_E kpref / N_k Im sum_a conj(P_y) P_x,
+ *                   P_v = F_+ U A v (the projection and forward transform of v),
+ *                   dW/dptab the hat weights kept by ocean_gpu_build_w.
+ * gres, gprc and the gamma / scissor derivatives are per eV.  sigma[nE],
+ * g_ptab[nptab] and the scalar outputs may be NULL (not computed); g_ptab needs
+ * W from ocean_gpu_build_w. */
+int  ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const double* hay_im,
+                             int32_t nE, const double* energies_eV, double gres_eV, double gprc_eV,
+                             double ffff, int32_t depth, double kpref, const double* target,
+                             double* loss, double* sigma, double* g_ptab, double* g_scale,
+                             double* g_gamma_eV, double* g_scissor_eV);
+
 /* Continued fraction with the square-root terminator (write_core,
  * OCEAN2/OCEAN_haydock.f90:1801-1894).  gam, el, eh in Hartree; ne+1 energies.
  * out[5*(ne+1)], row-major per energy: E (eV), Spect, Spect(0), SPKK, SPKK(0).

@@ -82,6 +82,9 @@ _SIGS = {
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
     "ocean_gpu_load_bloch_dev": (C.c_int, [_H, C.c_void_p, C.c_void_p]),
+    "ocean_gpu_spectral_grad": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
+                                          C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP, _DP,
+                                          _DP, _DP, _DP]),
     "ocean_gpu_batch_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(_H)]),
     "ocean_gpu_batch_destroy": (C.c_int, [_H]),
     "ocean_gpu_batch_ngroups": (C.c_int, [_H, _IP]),
@@ -239,6 +242,7 @@ class OceanGPU:
         rd = np.asfortranarray(rdist, np.float64)
         rt = np.ascontiguousarray(rtab, np.float64)
         pt = np.ascontiguousarray(ptab, np.float64)
+        self.nptab = len(rt)
         _chk(_lib().ocean_gpu_build_w(self.h, _p(rd), _p(rt), _p(pt), len(rt), epsilon0, scale),
              "build_w")
 
@@ -346,6 +350,26 @@ class OceanGPU:
                                       C.byref(done))
         _chk(rc, "haydock")
         return a, b, ai
+
+    def spectral_grad(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, target, kpref=None,
+                      ptab=True, scale=True, gamma=True, scissor=True):
+        """Misfit L = sum_E (sigma(E) - target)^2 of the GMRES spectrum and its gradient
+        (ocean_gpu_spectral_grad): dict(loss, sigma, g_ptab, g_scale, g_gamma_eV, g_scissor_eV)."""
+        kpref = self.kpref if kpref is None else kpref
+        pr, pi = _planes(hay_vec)
+        E = np.ascontiguousarray(energies_eV, np.float64)
+        tg = np.ascontiguousarray(target, np.float64)
+        sig = np.zeros(len(E))
+        loss = C.c_double()
+        gp = np.zeros(getattr(self, "nptab", 0)) if ptab else None
+        gs, gg, gd = C.c_double(), C.c_double(), C.c_double()
+        rc = _lib().ocean_gpu_spectral_grad(
+            self.h, _p(pr), _p(pi), len(E), _p(E), gres_eV, gprc_eV, ffff, depth, kpref, _p(tg),
+            C.byref(loss), _p(sig), _p(gp), C.byref(gs) if scale else None,
+            C.byref(gg) if gamma else None, C.byref(gd) if scissor else None)
+        _chk(rc, "spectral_grad")
+        return dict(loss=loss.value, sigma=sig, g_ptab=gp, g_scale=gs.value if scale else None,
+                    g_gamma_eV=gg.value if gamma else None, g_scissor_eV=gd.value if scissor else None)
 
     def gmres(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref=None,
               return_x=False):

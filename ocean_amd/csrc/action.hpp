@@ -330,6 +330,73 @@ __global__ __launch_bounds__(256) void k_fft_w_batch(const ActArgs<T2>* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Differentiation: the forward half of the transform, P = F_+ (U A v), with no W
+// and no scaling (in place on phi), and the contraction of two such fields into
+// the W-derivative of <y|H_lr|x>:
+//   d<y|H_lr|x>/dW(k,x) = -(1/N_k) sum_a conj(P_y(k,x,a)) P_x(k,x,a)
+// (H_lr = -sum_x A U_x^+ (N_k^-1 F_- W_x F_+) U_x A with F_- = F_+^+).
+// ---------------------------------------------------------------------------
+template <typename T2, int LMAX>
+__global__ __launch_bounds__(256) void k_fft_fwd(const ActArgs<T2> A) {
+  using T = typename Scalar<T2>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T2* twl = reinterpret_cast<T2*>(smem);  // [dim][m], exp(+2 pi i m / L), dims (K3, K2, K1)
+  T2* lds = twl + 3 * LMAX;
+  const int nk = A.nk, ncol = A.ncol, G = A.G, ldphi = A.ldphi;
+  const int Ks[3] = {A.K3, A.K2, A.K1};
+  for (int t = threadIdx.x; t < 3 * LMAX; t += blockDim.x) {
+    const int d = t / LMAX, m = t - d * LMAX;
+    const int L = Ks[d];
+    T sn = 0, cs = 1;
+    if (m < L) {
+      if constexpr (sizeof(T) == 8) sincospi((double)(2 * m) / (double)L, &sn, &cs);
+      else sincospif((float)(2 * m) / (float)L, &sn, &cs);
+    }
+    twl[t] = cmake<T2>(cs, sn);
+  }
+  __syncthreads();
+  int b = blockIdx.x;
+  const int nfft = gridDim.x;
+  if ((nfft & 7) == 0) b = (b & 7) * (nfft >> 3) + (b >> 3);
+  const int c0 = b * G;
+  const int gcnt = min(G, ncol - c0);
+  T2* __restrict__ gphi = A.phi + c0;
+  auto ld_hbm = [&](int k, int g) -> T2 {
+    const T2 v = gphi[(size_t)k * ldphi + min(g, gcnt - 1)];
+    return g < gcnt ? v : cmake<T2>(0, 0);
+  };
+  auto ld_lds = [&](int k, int g) -> T2 { return lds[lidx(k * G + g)]; };
+  auto st_lds = [&](int k, int g, T2 v) { lds[lidx(k * G + g)] = v; };
+  auto st_hbm = [&](int k, int g, T2 v) {
+    if (g < gcnt) gphi[(size_t)k * ldphi + g] = v;
+  };
+  dft_run<T2, LMAX>(twl + 0 * LMAX, G, nk, A.K3, 1, ld_hbm, st_lds);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 1 * LMAX, G, nk, A.K2, A.K3, ld_lds, st_lds);
+  __syncthreads();
+  dft_run<T2, LMAX>(twl + 2 * LMAX, G, nk, A.K1, A.K2 * A.K3, ld_lds, st_hbm);
+}
+
+// Q[k][x] += coef * Im sum_a conj(PY(k,x,a)) PX(k,x,a); one thread per (k, x), each
+// element updated by one thread per launch (deterministic over energies).
+template <typename T2>
+__global__ void k_wgrad(const T2* __restrict__ PX, const T2* __restrict__ PY, int nk, int nx, int na,
+                        int ldphi, double coef, double* __restrict__ Q) {
+  const size_t tot = (size_t)nk * nx;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e / nx), x = (int)(e - (size_t)k * nx);
+    const T2* px = PX + (size_t)k * ldphi + (size_t)x * na;
+    const T2* py = PY + (size_t)k * ldphi + (size_t)x * na;
+    double im = 0.0;
+    for (int a = 0; a < na; ++a) {
+      const T2 p = px[a], q = py[a];
+      im += (double)q.x * (double)p.y - (double)q.y * (double)p.x;
+    }
+    Q[e] += coef * im;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Multiplet projection partials: apart(j = a*P+p, k) = sum_n c(n,k,p) (A psi)(n,k,a)
 //   OCEAN_multiplet.f90:1730-1783 (no conjugation).  One workgroup per (k, s):
 //   one wave per alpha, lanes over bands, 16 projectors per butterfly reduction.

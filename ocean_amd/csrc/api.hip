@@ -189,6 +189,21 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   return OCEAN_GPU_OK;
 }
 
+// The device state of one precision form.
+template <typename T2> struct StateOf;
+template <> struct StateOf<double2> {
+  static const double2* u(const H* h) { return h->u64; }
+  static const double2* uT(const H* h) { return h->uT64; }
+  static double2* phi(const H* h) { return h->phi64; }
+  static const double* W(const H* h) { return h->W64; }
+};
+template <> struct StateOf<float2> {
+  static const float2* u(const H* h) { return h->u32; }
+  static const float2* uT(const H* h) { return h->uT32; }
+  static float2* phi(const H* h) { return h->phi32; }
+  static const float* W(const H* h) { return h->W32; }
+};
+
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
@@ -978,10 +993,15 @@ int ocean_gpu_spectrum(const double* real_a, const double* real_b, const double*
 }
 
 // ---- GMRES ----
-int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im, int32_t nE,
-                    const double* energies_eV, double gres_eV, double gprc_eV, double ffff, int32_t depth,
-                    double kpref, double* out, double* x_re, double* x_im) {
-  if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !out) return OCEAN_GPU_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Recycled GMRES over the energy list for the device right-hand side rhs (already
+// masked).  Solutions go to xs_dev (nE vectors, nullable) and/or the host planes.
+int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, double gres_eV,
+              double gprc_eV, double ffff, int depth, double kpref, double* out, double2* xs_dev,
+              double* x_re, double* x_im) {
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
   const size_t nv = h->nvec();
@@ -1016,13 +1036,9 @@ int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay
     if (hipGetLastError() != hipSuccess) return OCEAN_GPU_ERR_HIP;
     return OCEAN_GPU_OK;
   };
-  // rhs = allow .* hay_vec (sfact copy, allow%i = allow%r for core)
-  rc = put_planes(h, hay_re, hay_im, g, nv);
-  if (rc == 0) {
-    k_mask<<<grid, kBlock, 0, h->stream>>>(rhs, g, h->allow, nv);
-    og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(1.0, 0.0), nv);
-    rc = sync_check();
-  }
+  OG_CHECK(hipMemcpyAsync(rhs, rhs_dev, nv * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+  og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(1.0, 0.0), nv);
+  rc = sync_check();
   if (rc == 0) rc = xact_dev(h, x, h1, f);  // h1 = H . 1
   if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
                         ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
@@ -1148,8 +1164,192 @@ int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay
     o[4] = complete_iter;
     o[5] = global_iter;
     if (x_re && x_im) rc = get_planes(h, x, x_re + step * nv, x_im + step * nv, nv);
+    if (xs_dev && rc == 0)
+      rc = hipMemcpyAsync(xs_dev + step * nv, x, nv * sizeof(double2), hipMemcpyDeviceToDevice, h->stream) == hipSuccess
+               ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
   }
+  if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   cleanup();
+  return rc;
+}
+
+// rhs = allow .* hay_vec (the reference's sfact copy, allow%i = allow%r for core)
+int gmres_rhs(H* h, const double* hay_re, const double* hay_im, double2** rhs) {
+  const size_t nv = h->nvec();
+  double2* tmp = nullptr;
+  OG_TRY(dmalloc(rhs, nv));
+  int rc = dmalloc(&tmp, nv);
+  if (rc == 0) rc = put_planes(h, hay_re, hay_im, tmp, nv);
+  if (rc == 0) {
+    k_mask<<<grid_for(nv), kBlock, 0, h->stream>>>(*rhs, tmp, h->allow, nv);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(tmp);
+  if (rc) dfree(*rhs);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+}  // extern "C"
+
+namespace {
+
+// Per-energy part of the spectral gradient (see ocean_gpu_spectral_grad).
+template <typename T2, int NA>
+int grad_t(H* h, int nE, const double2* xs, const double2* ys, const std::vector<double>& r, double kpref,
+           bool want_s, bool want_w, double* gG, double* gD, double* gS, double* Q) {
+  const size_t nv = h->nvec();
+  const int f = flags_of(h, 0, nullptr);
+  T2 *PX = nullptr, *PY = nullptr;
+  double2* v = nullptr;
+  int rc = OCEAN_GPU_OK;
+  if (want_w) {
+    rc = dmalloc(&PX, h->nphi());
+    if (rc == 0) rc = dmalloc(&PY, h->nphi());
+  }
+  if (rc == 0 && want_s) rc = dmalloc(&v, nv);
+  const double s = h->d.inter_scale;
+  for (int e = 0; e < nE && rc == 0; ++e) {
+    const double2* x = xs + (size_t)e * nv;
+    const double2* y = ys + (size_t)e * nv;
+    const double c = 2.0 * r[e] * kpref;
+    double zr = 0.0, zi = 0.0;
+    if ((rc = dev_dot(h, y, x, &zr, &zi))) break;
+    *gG += c * zr;   // d sigma / d Gamma = kpref Re <y|x>
+    *gD -= c * zi;   // d sigma / d Delta = -kpref Im <y|x>
+    if (want_s) {
+      // (H_lr + H_mult) x: the action with the energy and spin-orbit terms off
+      if ((rc = xact_dev(h, x, v, f & (og::F_LR | og::F_FG)))) break;
+      if ((rc = dev_dot(h, y, v, &zr, &zi))) break;
+      *gS -= c * zi / s;
+    }
+    if (want_w && (f & og::F_LR)) {
+      og::ActArgs<T2> a;
+      Geom<T2> g;
+      const T2* u = StateOf<T2>::u(h);
+      const auto* W = StateOf<T2>::W(h);
+      for (int side = 0; side < 2 && rc == 0; ++side) {
+        T2* P = side == 0 ? PX : PY;
+        rc = make_args<T2, NA>(h, side == 0 ? x : y, nullptr, og::F_LR, h->tiles, u, StateOf<T2>::uT(h), P, W, a, g);
+        if (rc) break;
+        og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), h->tiles.proj, g.sm_proj, h->stream>>>(a);
+        const size_t tile = (size_t)h->nk * a.G;
+        const size_t smf = (3 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2);
+        if (g.lmax == 8)
+          og::k_fft_fwd<T2, 8><<<a.nfft, kBlock, smf, h->stream>>>(a);
+        else
+          og::k_fft_fwd<T2, 16><<<a.nfft, kBlock, smf, h->stream>>>(a);
+        if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+      }
+      if (rc) break;
+      const size_t tot = (size_t)h->nk * h->nx;
+      og::k_wgrad<T2><<<grid_for(tot), kBlock, 0, h->stream>>>(PX, PY, h->nk, h->nx, h->na, h->ldphi,
+                                                              c / (double)h->nk, Q);
+      if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+    }
+  }
+  if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  dfree(PX);
+  dfree(PY);
+  dfree(v);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const double* hay_im, int32_t nE,
+                            const double* energies_eV, double gres_eV, double gprc_eV, double ffff,
+                            int32_t depth, double kpref, const double* target, double* loss, double* sigma,
+                            double* g_ptab, double* g_scale, double* g_gamma_eV, double* g_scissor_eV) {
+  if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !target || !loss)
+    return OCEAN_GPU_ERR_ARG;
+  if (g_ptab && (h->nptab <= 0 || !h->w_i0)) return OCEAN_GPU_ERR_STATE;  // W must come from build_w
+  const size_t nv = h->nvec();
+  double2* rhs = nullptr;
+  OG_TRY(gmres_rhs(h, hay_re, hay_im, &rhs));
+  double2 *xs = nullptr, *ys = nullptr;
+  double* Q = nullptr;
+  int rc = dmalloc(&xs, (size_t)nE * nv);
+  if (rc == 0) rc = dmalloc(&ys, (size_t)nE * nv);
+  const size_t nw = (size_t)h->nk * h->nx;
+  if (rc == 0 && g_ptab) {
+    rc = dmalloc(&Q, nw);
+    if (rc == 0) {
+      og::k_fill_d<<<grid_for(nw), kBlock, 0, h->stream>>>(Q, 0.0, nw);
+      if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+    }
+  }
+  // forward solves at E + i gres and adjoint solves at the conjugate energy
+  std::vector<double> rows_f(6 * (size_t)nE), rows_b(6 * (size_t)nE);
+  if (rc == 0) rc = gmres_run(h, rhs, nE, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref, rows_f.data(), xs,
+                              nullptr, nullptr);
+  if (rc == 0) rc = gmres_run(h, rhs, nE, energies_eV, -gres_eV, gprc_eV, ffff, depth, kpref, rows_b.data(), ys,
+                              nullptr, nullptr);
+  std::vector<double> r((size_t)nE);
+  double L = 0.0;
+  for (int e = 0; e < nE; ++e) {
+    const double sg = rows_f[6 * (size_t)e + 2];
+    if (sigma) sigma[e] = sg;
+    r[e] = sg - target[e];
+    L += r[e] * r[e];
+  }
+  *loss = L;
+  double gG = 0.0, gD = 0.0, gS = 0.0;
+  if (rc == 0 && (g_ptab || g_scale || g_gamma_eV || g_scissor_eV)) {
+    const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
+    const int NA = na_tmpl(acnt);
+    const bool mixed = h->precision == OCEAN_GPU_MIXED;
+    const bool ws = g_scale != nullptr, ww = g_ptab != nullptr;
+#define OG_GRAD(NAV)                                                                                   \
+    if (NA == NAV)                                                                                     \
+      rc = mixed ? grad_t<float2, NAV>(h, nE, xs, ys, r, kpref, ws, ww, &gG, &gD, &gS, Q)             \
+                 : grad_t<double2, NAV>(h, nE, xs, ys, r, kpref, ws, ww, &gG, &gD, &gS, Q);
+    OG_GRAD(2)
+    OG_GRAD(4)
+    OG_GRAD(6)
+    OG_GRAD(12)
+#undef OG_GRAD
+  }
+  // dL/dptab_i = sum_(k,x) Q(k,x) dW(k,x)/dptab_i with the hat weights of build_w
+  if (rc == 0 && g_ptab) {
+    std::vector<double> q(nw), w0(nw), w1(nw);
+    std::vector<int32_t> i0(nw);
+    if (hipMemcpy(q.data(), Q, nw * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(i0.data(), h->w_i0, nw * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(w0.data(), h->w_w0, nw * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(w1.data(), h->w_w1, nw * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+    if (rc == 0) {
+      for (int i = 0; i < h->nptab; ++i) g_ptab[i] = 0.0;
+      for (int k = 0; k < h->nk; ++k)
+        for (int x = 0; x < h->nx; ++x) {
+          const size_t eq = (size_t)k * h->nx + x, ew = (size_t)x * h->nk + k;  // Q [k][x], W [x][k]
+          const int j = i0[ew];
+          g_ptab[j] += q[eq] * w0[ew];
+          if (j + 1 < h->nptab) g_ptab[j + 1] += q[eq] * w1[ew];
+        }
+    }
+  }
+  if (g_gamma_eV) *g_gamma_eV = gG * kEV2Hartree;
+  if (g_scissor_eV) *g_scissor_eV = gD * kEV2Hartree;
+  if (g_scale) *g_scale = gS;
+  dfree(rhs); dfree(xs); dfree(ys); dfree(Q);
+  return rc;
+}
+
+int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im, int32_t nE,
+                    const double* energies_eV, double gres_eV, double gprc_eV, double ffff, int32_t depth,
+                    double kpref, double* out, double* x_re, double* x_im) {
+  if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !out) return OCEAN_GPU_ERR_ARG;
+  double2* rhs = nullptr;
+  OG_TRY(gmres_rhs(h, hay_re, hay_im, &rhs));
+  const int rc = gmres_run(h, rhs, nE, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref, out, nullptr, x_re, x_im);
+  dfree(rhs);
   return rc;
 }
 
@@ -1178,20 +1378,6 @@ struct ocean_gpu_batch {
 namespace {
 
 using B = ocean_gpu_batch;
-
-template <typename T2> struct StateOf;
-template <> struct StateOf<double2> {
-  static const double2* u(const H* h) { return h->u64; }
-  static const double2* uT(const H* h) { return h->uT64; }
-  static double2* phi(const H* h) { return h->phi64; }
-  static const double* W(const H* h) { return h->W64; }
-};
-template <> struct StateOf<float2> {
-  static const float2* u(const H* h) { return h->u32; }
-  static const float2* uT(const H* h) { return h->uT32; }
-  static float2* phi(const H* h) { return h->phi32; }
-  static const float* W(const H* h) { return h->W32; }
-};
 
 // Device tables of one group for one set of (in, out) vectors.
 struct Tables {

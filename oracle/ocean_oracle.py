@@ -31,6 +31,8 @@ Pinning (what checks each function; "parity unpinned" where none does):
   populate_W/intval    closed forms of the geometry and the interpolant
   bloch_phase          closed form (phase of a plane wave)
   gmres_do_recycle     dense direct solve of (E + iG - H) x = b
+  spectral_grad        central finite differences of spectral_sigma (no reference
+                       gradient exists; the loss itself rests on the pins above)
 """
 from __future__ import annotations
 
@@ -1137,3 +1139,78 @@ def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, d
         xs.append(x.copy())
     out = np.array(rows)
     return (out, xs) if return_x else out
+
+
+# ----------------------------------------------------------------------------
+# Differentiation form (no reference counterpart: OCEAN2 computes no gradients).
+# The spectrum is the GMRES one (OCEAN_gmres.f90:761, :864-869):
+#   sigma(E) = -kpref Im<b|x_E>,  ((E + i gam) - H) x_E = b,  b = allow . hay
+# The restatement solves densely (tiny sizes); its analytic gradient is pinned
+# by central finite differences of spectral_sigma in tests/test_oracle.py.
+# ----------------------------------------------------------------------------
+def spectral_sigma(P: Problem, hay_vec, energies_eV, gam_eV, kpref, H=None):
+    """sigma(E) by dense solves of the resolvent system."""
+    if H is None:
+        H = dense_H(P)
+    b = (P.allow * np.asarray(hay_vec, np.complex128)).ravel(order="F")
+    N = b.size
+    gam = gam_eV * EV2HARTREE
+    out = np.zeros(len(energies_eV))
+    for i, E_eV in enumerate(energies_eV):
+        x = np.linalg.solve((E_eV * EV2HARTREE + 1j * gam) * np.eye(N) - H, b)
+        out[i] = -kpref * np.vdot(b, x).imag
+    return out
+
+
+def spectral_grad(P: Problem, hay_vec, energies_eV, gam_eV, kpref, target, wweights=None, nptab=None):
+    """Loss sum_E (sigma - target)^2 and its gradient by adjoint (conjugate-energy) solves.
+
+    With M = (E + i gam) - H, x = M^-1 b and y = M^-H b:
+      d sigma/d gam   =  kpref Re<y|x>          (dM/dgam = i)
+      d sigma/d Delta = -kpref Im<y|x>          (H -> H + Delta on the energy term)
+      d sigma/d s     = -kpref Im<y|(H_lr + H_mult) x> / s   (W and mul scale with s)
+      d sigma/d W(k,x) = (kpref / N_k) Im sum_a conj(P_y) P_x,  P_v = F_+ U (A v)
+    (H_lr = -sum_x A U_x^+ N_k^-1 F_- W_x F_+ U_x A, long_range.f90:897-961).  Gamma and
+    Delta derivatives are returned per eV; wweights = (i0, w0, w1) from
+    populate_W_weights gives dL/dptab.
+    """
+    H = dense_H(P)
+    import copy
+    P2 = copy.copy(P)
+    P2.e0 = False
+    H_int = dense_H(P2, hflag=(1, 0, 0, 0, 1, 0))  # H_lr + H_mult
+    shape = (P.nb, P.nk, P.nalpha)
+    b = (P.allow * np.asarray(hay_vec, np.complex128)).ravel(order="F")
+    N = b.size
+    gam = gam_eV * EV2HARTREE
+    vs = val_spin_of_alpha(P.lc, P.nspn)
+    sig = np.zeros(len(energies_eV))
+    gG = gD = gS = 0.0
+    Q = np.zeros((P.nk, P.nx))
+    for i, E_eV in enumerate(energies_eV):
+        M = (E_eV * EV2HARTREE + 1j * gam) * np.eye(N) - H
+        x = np.linalg.solve(M, b)
+        y = np.linalg.solve(M.conj().T, b)
+        sig[i] = -kpref * np.vdot(b, x).imag
+        c = 2.0 * (sig[i] - target[i]) * kpref
+        z = np.vdot(y, x)
+        gG += c * z.real
+        gD -= c * z.imag
+        gS -= c * np.vdot(y, H_int @ x).imag / P.inter_scale
+        xa = P.allow * x.reshape(shape, order="F")
+        ya = P.allow * y.reshape(shape, order="F")
+        for a in range(P.nalpha):
+            us = P.u[:, :, :, vs[a]]
+            px = fft_backward(np.einsum("nkx,nk->kx", us, xa[:, :, a]), P.kmesh, axis=0)
+            py = fft_backward(np.einsum("nkx,nk->kx", us, ya[:, :, a]), P.kmesh, axis=0)
+            Q += c / P.nk * (np.conj(py) * px).imag
+    loss = float(np.sum((sig - np.asarray(target)) ** 2))
+    out = dict(loss=loss, sigma=sig, g_gamma_eV=gG * EV2HARTREE, g_scissor_eV=gD * EV2HARTREE,
+               g_scale=gS, dL_dW=Q)
+    if wweights is not None:
+        i0, w0, w1 = wweights
+        g = np.zeros(nptab)
+        np.add.at(g, i0.ravel(), (Q * w0).ravel())
+        np.add.at(g, i0.ravel() + 1, (Q * w1).ravel())
+        out["g_ptab"] = g
+    return out

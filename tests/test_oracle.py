@@ -454,3 +454,44 @@ def test_c_restatement_haydock(c_oracle_mod):
     assert rc == 0
     assert np.abs(ra[:30] - a[:30]).max() < 1e-12 * np.abs(a).max()
     assert np.abs(rb[1:31] - b[1:31]).max() < 1e-12 * np.abs(b).max()
+
+
+# ---------------------------------------------------------------------------
+# differentiation form: the oracle's adjoint gradient against finite differences
+# ---------------------------------------------------------------------------
+def test_spectral_grad_matches_central_differences():
+    from grad_common import oracle_problem, target_params
+    from ocean_amd import synth
+
+    inp = synth.make_inputs(nb=4, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=31, inter_scale=0.9)
+    E = np.linspace(-2.0, 32.0, 6)
+    kp = inp.kpref
+    tp, ts, tg, td = target_params(inp)
+    target = orc.spectral_sigma(oracle_problem(inp, tp, ts, td), inp.psi0, E, tg, kp)
+    p0, s0, g0, d0 = inp.ptab.copy(), inp.inter_scale, 0.5, 0.0
+    P0 = oracle_problem(inp, p0, s0, d0)
+    ww = orc.populate_W_weights(inp.avec, inp.kmesh, inp.xmesh, inp.tau, inp.xshift, inp.rtab, s0)
+    G = orc.spectral_grad(P0, inp.psi0, E, g0, kp, target, wweights=ww, nptab=len(p0))
+
+    def loss(p=p0, s=s0, g=g0, d=d0):
+        sig = orc.spectral_sigma(oracle_problem(inp, p, s, d), inp.psi0, E, g, kp)
+        return float(np.sum((sig - target) ** 2))
+
+    h = 1e-6
+    assert abs(G["loss"] - loss()) <= 1e-12 * abs(G["loss"])
+    fd = {
+        "g_gamma_eV": (loss(g=g0 + h) - loss(g=g0 - h)) / (2 * h),
+        "g_scissor_eV": (loss(d=d0 + h) - loss(d=d0 - h)) / (2 * h),
+        "g_scale": (loss(s=s0 + h) - loss(s=s0 - h)) / (2 * h),
+    }
+    for k, v in fd.items():
+        assert abs(G[k] - v) <= 2e-5 * abs(v) + 1e-12, (k, G[k], v)
+    # the knots that carry most of the W gradient
+    for i in np.argsort(-np.abs(G["g_ptab"]))[:4]:
+        dp = np.zeros_like(p0)
+        dp[i] = h
+        v = (loss(p=p0 + dp) - loss(p=p0 - dp)) / (2 * h)
+        assert abs(G["g_ptab"][i] - v) <= 2e-5 * abs(v) + 1e-12, (i, G["g_ptab"][i], v)
+    # knots beyond every distance carry no gradient
+    far = inp.rtab > orc._w_geometry(inp.avec, inp.kmesh, inp.xmesh, inp.tau, inp.xshift)[0].max() + 0.1
+    assert np.all(G["g_ptab"][far] == 0.0)
