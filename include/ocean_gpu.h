@@ -40,6 +40,9 @@ extern "C" {
 #define OCEAN_GPU_ERR_STATE    1002     /* required module state not loaded */
 #define OCEAN_GPU_ERR_HIP      1003     /* HIP runtime failure */
 #define OCEAN_GPU_ERR_NOMEM    1004     /* device allocation failed */
+#define OCEAN_GPU_ERR_NOCONV   1005     /* GMRES did not reach ffff within 50 restarts at one
+                                           energy (the reference would loop on; an FP32 action
+                                           cannot reach FP64-sized thresholds) */
 #define OCEAN_GPU_ERR_UNSUPPORTED 9963  /* as OCEAN_gmres.f90:538-542 for backf */
 /* GMRES least-squares failure returns the zhptrf-style pivot index (> 0),
  * as OCEAN2/OCEAN_gmres.f90:312-329 returns LAPACK's info. */

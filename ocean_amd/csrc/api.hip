@@ -460,6 +460,7 @@ const char* ocean_gpu_strerror(int code) {
     case OCEAN_GPU_ERR_HIP: return "HIP runtime error";
     case OCEAN_GPU_ERR_NOMEM: return "device allocation failed";
     case OCEAN_GPU_ERR_UNSUPPORTED: return "unsupported option";
+    case OCEAN_GPU_ERR_NOCONV: return "GMRES did not converge within its iteration cap";
     default: return code > 0 ? "GMRES least-squares pivot is zero (zhptrf info)" : "unknown error";
   }
 }
@@ -1129,6 +1130,10 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
       }
     }
     while (!converged && rc == 0) {
+      if (complete_iter >= 50 * depth) {  // 50 restarts at one energy
+        rc = OCEAN_GPU_ERR_NOCONV;
+        break;
+      }
       for (int it = prev_iter + 1; it <= depth; ++it) {
         ++complete_iter;
         ++global_iter;

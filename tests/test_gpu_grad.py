@@ -42,9 +42,9 @@ def gpu_at(inp, rdist, ptab, s, scissor_eV, precision=core.FP64):
     return g
 
 
-def run(inp, rdist, target, ptab, s, gam, d, precision=core.FP64):
+def run(inp, rdist, target, ptab, s, gam, d, precision=core.FP64, ffff=FFFF):
     g = gpu_at(inp, rdist, ptab, s, d, precision)
-    out = g.spectral_grad(inp.psi0, E, gam, GPRC, FFFF, DEPTH, target)
+    out = g.spectral_grad(inp.psi0, E, gam, GPRC, ffff, DEPTH, target)
     g.close()
     return out
 
@@ -86,7 +86,8 @@ def test_gradient_matches_oracle_and_finite_differences(setup):
 def test_mixed_gradient_close_to_fp64(setup):
     inp, target, rdist = setup
     G64 = run(inp, rdist, target, inp.ptab, inp.inter_scale, 0.5, 0.0)
-    G32 = run(inp, rdist, target, inp.ptab, inp.inter_scale, 0.5, 0.0, precision=core.MIXED)
+    # an FP32 action bounds the reachable residual: |g|^2 < 1e-12 (|g| ~ 1e-6)
+    G32 = run(inp, rdist, target, inp.ptab, inp.inter_scale, 0.5, 0.0, precision=core.MIXED, ffff=1e-12)
     for k in ("g_gamma_eV", "g_scissor_eV", "g_scale"):
         print(k, G64[k], G32[k], flush=True)
         assert abs(G32[k] - G64[k]) <= 1e-3 * abs(G64[k])
@@ -126,4 +127,4 @@ def test_autograd_function(setup):
     with torch.no_grad():
         step = 1e-3 / max(float(gamma.grad.abs()), 1e-30)
         l2 = SpectralMisfit.apply(ptab, scale, gamma - step * gamma.grad, scissor)
-    assert float(l2) < float(loss)
+    assert l2.item() < loss.item()
