@@ -138,6 +138,53 @@ def batch_workload(n, niter, reps, log_every=8):
     return out
 
 
+def gradient_workload(log_prefix="grad"):
+    """Config 4: config-1 shapes, GMRES at 128 energies (Gamma 0.5 eV), misfit to a
+    target spectrum made with perturbed parameters (W table x 1.1 plus a Gaussian
+    bump, Gamma 0.7 eV, scissor +0.3 eV); gradient w.r.t. the 200-knot W table,
+    the screening scale, Gamma and the scissor, FP64 and mixed."""
+    from ocean_amd import core, setup_host as sh, synth
+
+    inp = synth.config(4)
+    rdist = sh.w_geometry(inp.avec, inp.kmesh, inp.xmesh, inp.tau, inp.xshift)[0]
+    E = np.linspace(-5.0, 40.0, 128)
+    p = inp.params
+    out = {}
+    # target: the same problem at the perturbed parameters (FP64 GMRES spectrum)
+    bump = 0.05 * np.exp(-((inp.rtab - 2.0) / 0.5) ** 2)
+    gt = core.OceanGPU.from_inputs(inp)
+    gt.build_w(rdist, inp.rtab, inp.ptab * 1.1 + bump, inp.epsilon0, inp.inter_scale)
+    gt.load_energies(np.asfortranarray(inp.energy + 0.3 * core.EV2HARTREE), inp.allow)
+    target = gt.gmres(inp.psi0, E, 0.7, 13.605, 1e-16, 20)[:, 2]
+    gt.close()
+    res = {}
+    for prec, name, ffff in ((core.FP64, "fp64", 1e-16), (core.MIXED, "mixed", 1e-12)):
+        g = core.OceanGPU.from_inputs(inp, precision=prec)
+        g.build_w(rdist, inp.rtab, inp.ptab, inp.epsilon0, inp.inter_scale)
+        t0 = time.perf_counter()
+        r = g.spectral_grad(inp.psi0, E, p["gam"], 13.605, ffff, 20, target)
+        dt = time.perf_counter() - t0
+        g.close()
+        res[name] = r
+        out[f"grad_seconds_{name}"] = round(dt, 2)
+        log(f"{log_prefix} {name}: loss {r['loss']:.6e}, dL/dGamma {r['g_gamma_eV']:.6e}, {dt:.2f} s")
+    a, b = res["fp64"], res["mixed"]
+    gv = lambda r: np.concatenate([r["g_ptab"], [r["g_scale"], r["g_gamma_eV"], r["g_scissor_eV"]]])
+    out["grad_energies"] = len(E)
+    out["grad_mixed_rel_diff"] = float(np.linalg.norm(gv(b) - gv(a)) / np.linalg.norm(gv(a)))
+    return out
+
+
+def gmres_workload(inp, g, n=64):
+    """Config 1 GMRES: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
+    E = np.linspace(-5.0, 40.0, n)
+    g.gmres(inp.psi0, E[:2], 0.5, 13.605, 1e-16, 20)
+    t0 = time.perf_counter()
+    rows = g.gmres(inp.psi0, E, 0.5, 13.605, 1e-16, 20)
+    dt = time.perf_counter() - t0
+    return {"gmres_energies_per_s_fp64": round(n / dt, 2), "gmres_matvecs": int(rows[-1, 5])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +195,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batched form")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient (differentiation) form")
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-niter", type=int, default=200)
     ap.add_argument("--calib-copy", action="store_true",
@@ -238,9 +286,17 @@ def main():
         extras["mixed_speedup_hpsi"] = round(extras["hpsi_per_s_mixed"] / hpsi_fp64, 3)
         g.set_precision(core.FP64)
 
-    if not args.no_extras and not args.no_batch:
+    if not args.no_extras:
+        extras.update(gmres_workload(inp, g))
+        log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})
+    if not args.no_extras and not args.no_grad:
         g.close()
         g = None
+        extras.update(gradient_workload())
+    if not args.no_extras and not args.no_batch:
+        if g is not None:
+            g.close()
+            g = None
         extras.update(batch_workload(args.batch_n, args.batch_niter, reps=5))
 
     cpu = None

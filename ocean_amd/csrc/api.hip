@@ -1122,7 +1122,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         const double rel = -gval / ival;
         if (rel < 0.001 || gval < ffff / 10.0) converged = true;
       }
-      if (!converged && std::fabs(prev_energy - E) > gprc / 4.0) {
+      if (!converged && std::fabs(prev_energy - E) > std::fabs(gprc) / 4.0) {
         prev_iter = 0;
         prev_energy = E;
         k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
@@ -1293,7 +1293,8 @@ int ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const dou
   std::vector<double> rows_f(6 * (size_t)nE), rows_b(6 * (size_t)nE);
   if (rc == 0) rc = gmres_run(h, rhs, nE, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref, rows_f.data(), xs,
                               nullptr, nullptr);
-  if (rc == 0) rc = gmres_run(h, rhs, nE, energies_eV, -gres_eV, gprc_eV, ffff, depth, kpref, rows_b.data(), ys,
+  // (the adjoint's preconditioner is the conjugate one: gprc -> -gprc)
+  if (rc == 0) rc = gmres_run(h, rhs, nE, energies_eV, -gres_eV, -gprc_eV, ffff, depth, kpref, rows_b.data(), ys,
                               nullptr, nullptr);
   std::vector<double> r((size_t)nE);
   double L = 0.0;

@@ -120,5 +120,45 @@ module ocean_gpu_iface
       integer(c_int32_t), value :: ne, literal_cmplx
       real(c_double), intent(out) :: out(*)
     end function
+
+    ! ---- batches (many sites / edges / polarisations at once) ----
+    integer(c_int) function ocean_gpu_batch_create(hs, n, b) bind(C, name='ocean_gpu_batch_create')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function ocean_gpu_batch_destroy(b) bind(C, name='ocean_gpu_batch_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    integer(c_int) function ocean_gpu_batch_autotune(b, reps) bind(C, name='ocean_gpu_batch_autotune')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: reps
+    end function
+
+    integer(c_int) function ocean_gpu_batch_haydock(b, psi_re, psi_im, niter, real_a, real_b, imag_a, &
+        niter_done) bind(C, name='ocean_gpu_batch_haydock')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      type(c_ptr), intent(in) :: psi_re(*), psi_im(*), real_a(*), real_b(*), imag_a(*)
+      integer(c_int32_t), value :: niter
+      integer(c_int32_t), intent(out) :: niter_done(*)
+    end function
+
+    ! ---- gradient of a GMRES spectral misfit ----
+    integer(c_int) function ocean_gpu_spectral_grad(h, hay_re, hay_im, ne, energies_ev, gres_ev, gprc_ev, &
+        ffff, depth, kpref, target, loss, sigma, g_ptab, g_scale, g_gamma_ev, g_scissor_ev) &
+        bind(C, name='ocean_gpu_spectral_grad')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: hay_re(*), hay_im(*), energies_ev(*), target(*)
+      integer(c_int32_t), value :: ne, depth
+      real(c_double), value :: gres_ev, gprc_ev, ffff, kpref
+      real(c_double), intent(out) :: loss, sigma(*), g_ptab(*), g_scale, g_gamma_ev, g_scissor_ev
+    end function
   end interface
 end module ocean_gpu_iface
