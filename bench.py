@@ -42,11 +42,24 @@ def kernel_bytes(inp):
     def fam(ub, cb):
         u = ub * nb * nk * nx * ns
         return {
-            "proj": u + 24 * N + cb * nphi,                 # u, psi+allow, phi out
+            "proj": u + 24 * N + 16 * nb * nk * P * ns + cb * nphi,  # u, psi+allow, c (multiplet partials), phi out
             "fft": 2 * cb * nphi + (ub // 2) * nk * nx,     # phi in/out, W
             "adj": u + cb * nphi + 16 * nb * nk * P * ns + 16 * N * 3 + 8 * N,  # u, phi', c, psi+E+out, allow
         }
     return {"fp64": fam(16, 16), "mixed": fam(8, 8)}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_kmesh8_xmesh8_nb64_fp64.json")
+
+
+def measured_traffic(kname):
+    """HBM bytes per launch of kname from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes over this workload (scaled by the copy calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["kernels"][kname]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(inp, budget_s):
@@ -247,7 +260,7 @@ def main():
     kname = {"proj": "k_proj", "fft": "k_fft_w", "adj": "k_adj"}[fam]
     ach = bytes_["fp64"][fam] / (prof[fam] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
             "algorithmic_bytes": bytes_["fp64"][fam],
             "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()}}
 
