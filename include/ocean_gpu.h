@@ -161,7 +161,7 @@ int  ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double*
  * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
  * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,
  * rel_error, complete_iter, global_iter) as written to absspct (:864-869).
- * x_re/x_im (nb, nk, nalpha, nE) may be NULL. */
+ * x_re/x_im (nb, nk, nalpha, nE) may be NULL.  depth <= 48. */
 int  ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im,
                      int32_t nE, const double* energies_eV, double gres_eV, double gprc_eV,
                      double ffff, int32_t depth, double kpref, double* out,

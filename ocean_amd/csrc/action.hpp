@@ -61,7 +61,8 @@ __device__ __forceinline__ int find_problem(const int* __restrict__ off, int np,
 
 __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const double2* __restrict__ psi,
                                         const double* __restrict__ allow, double2* __restrict__ apart,
-                                        int nb, int nk, int na, int ns, int P, int k, int s);
+                                        int nb, int nk, int na, int ns, int P, int k, int s, int c0,
+                                        int cstep);
 
 // ---------------------------------------------------------------------------
 // Projection: phi(k,x,a) = sum_n u(n,k,x) (A psi)(n,k,a)
@@ -87,8 +88,8 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
     ps[i] = cmake<T2>((T)(p.x * m), (T)(p.y * m));
   }
   __syncthreads();
-  if ((A.flags & F_FG) && bx == A.px - 1)
-    mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s);
+  // the projector chunks of this k are shared out over its x-tile workgroups
+  if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s, bx, A.px);
   const int x = bx * blockDim.x + threadIdx.x;
   if (x >= A.nx) return;
   T2 acc[NA];
@@ -157,7 +158,35 @@ __device__ __forceinline__ int lidx(int e) { return e + (e >> 3); }
 // loaded in bit-reversed order; otherwise the direct sum.  ld(k, g) / st(k, g, v)
 // move one element, so the first pass reads HBM straight into the pencil
 // registers and the last writes HBM from them.
-template <typename T2, int L, typename Ld, typename St>
+// cos and sin of 2 pi m / 16, correctly rounded (c1 = cos(pi/8), s1 = sin(pi/8),
+// r = sqrt(2)/2): the power-of-two twiddles
+#define OG_C1 0.92387953251128674
+#define OG_S1 0.38268343236508978
+#define OG_R2 0.70710678118654752
+__device__ constexpr double kCos16[16] = {1.0, OG_C1, OG_R2, OG_S1, 0.0, -OG_S1, -OG_R2, -OG_C1,
+                                          -1.0, -OG_C1, -OG_R2, -OG_S1, 0.0, OG_S1, OG_R2, OG_C1};
+__device__ constexpr double kSin16[16] = {0.0, OG_S1, OG_R2, OG_C1, 1.0, OG_C1, OG_R2, OG_S1,
+                                          0.0, -OG_S1, -OG_R2, -OG_C1, -1.0, -OG_C1, -OG_R2, -OG_S1};
+#undef OG_C1
+#undef OG_S1
+#undef OG_R2
+
+// v * w^m, w = exp(SIGN 2 pi i / L), L a power of two <= 16.  After unrolling m is
+// a constant, so the multiplications by 1, -1 and +-i fold away and the others
+// use literal twiddles.
+template <int SIGN, int L, typename T2>
+__device__ __forceinline__ T2 tw_mul(int m, T2 v) {
+  using T = typename Scalar<T2>::T;
+  const int idx = (m * (16 / L)) & 15;
+  if (idx == 0) return v;
+  if (idx == 8) return cmake<T2>(-v.x, -v.y);
+  if (idx == 4) return SIGN > 0 ? cmake<T2>(-v.y, v.x) : cmake<T2>(v.y, -v.x);
+  if (idx == 12) return SIGN > 0 ? cmake<T2>(v.y, -v.x) : cmake<T2>(-v.y, v.x);
+  const T c = (T)kCos16[idx], sn = (T)(SIGN * kSin16[idx]);
+  return cmake<T2>(c * v.x - sn * v.y, c * v.y + sn * v.x);
+}
+
+template <typename T2, int L, int SIGN, typename Ld, typename St>
 __device__ __forceinline__ void dft_pass(const T2* __restrict__ tw, int G, int nk, int S, Ld ld, St st) {
   const int npen = (G * nk) / L;
   for (int q = threadIdx.x; q < npen; q += blockDim.x) {
@@ -174,7 +203,7 @@ __device__ __forceinline__ void dft_pass(const T2* __restrict__ tw, int G, int n
         for (int i = 0; i < L; i += len) {
 #pragma unroll
           for (int j = 0; j < half; ++j) {
-            const T2 t = cmul(tw[j * step], v[i + j + half]);
+            const T2 t = tw_mul<SIGN, L>(j * step, v[i + j + half]);
             const T2 u = v[i + j];
             v[i + j] = cmake<T2>(u.x + t.x, u.y + t.y);
             v[i + j + half] = cmake<T2>(u.x - t.x, u.y - t.y);
@@ -197,24 +226,24 @@ __device__ __forceinline__ void dft_pass(const T2* __restrict__ tw, int G, int n
   }
 }
 
-template <typename T2, int LMAX, typename Ld, typename St>
+template <typename T2, int LMAX, int SIGN, typename Ld, typename St>
 __device__ __forceinline__ void dft_run(const T2* tw, int G, int nk, int L, int S, Ld ld, St st) {
   switch (L) {
-    case 1: dft_pass<T2, 1>(tw, G, nk, S, ld, st); break;
-    case 2: dft_pass<T2, 2>(tw, G, nk, S, ld, st); break;
-    case 3: dft_pass<T2, 3>(tw, G, nk, S, ld, st); break;
-    case 4: dft_pass<T2, 4>(tw, G, nk, S, ld, st); break;
-    case 5: dft_pass<T2, 5>(tw, G, nk, S, ld, st); break;
-    case 6: dft_pass<T2, 6>(tw, G, nk, S, ld, st); break;
-    case 7: dft_pass<T2, 7>(tw, G, nk, S, ld, st); break;
-    case 8: dft_pass<T2, 8>(tw, G, nk, S, ld, st); break;
+    case 1: dft_pass<T2, 1, SIGN>(tw, G, nk, S, ld, st); break;
+    case 2: dft_pass<T2, 2, SIGN>(tw, G, nk, S, ld, st); break;
+    case 3: dft_pass<T2, 3, SIGN>(tw, G, nk, S, ld, st); break;
+    case 4: dft_pass<T2, 4, SIGN>(tw, G, nk, S, ld, st); break;
+    case 5: dft_pass<T2, 5, SIGN>(tw, G, nk, S, ld, st); break;
+    case 6: dft_pass<T2, 6, SIGN>(tw, G, nk, S, ld, st); break;
+    case 7: dft_pass<T2, 7, SIGN>(tw, G, nk, S, ld, st); break;
+    case 8: dft_pass<T2, 8, SIGN>(tw, G, nk, S, ld, st); break;
     default:
       if constexpr (LMAX > 8) {
         switch (L) {
-          case 9: dft_pass<T2, 9>(tw, G, nk, S, ld, st); break;
-          case 10: dft_pass<T2, 10>(tw, G, nk, S, ld, st); break;
-          case 12: dft_pass<T2, 12>(tw, G, nk, S, ld, st); break;
-          case 16: dft_pass<T2, 16>(tw, G, nk, S, ld, st); break;
+          case 9: dft_pass<T2, 9, SIGN>(tw, G, nk, S, ld, st); break;
+          case 10: dft_pass<T2, 10, SIGN>(tw, G, nk, S, ld, st); break;
+          case 12: dft_pass<T2, 12, SIGN>(tw, G, nk, S, ld, st); break;
+          case 16: dft_pass<T2, 16, SIGN>(tw, G, nk, S, ld, st); break;
           default: break;  // rejected on the host
         }
       }
@@ -297,7 +326,7 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
       gphi[(size_t)k * ldphi + g] = v;
     }
   };
-  dft_run<T2, LMAX>(twl + 0 * LMAX, G, nk, K3, 1, ld_hbm, st_lds);
+  dft_run<T2, LMAX, 1>(twl + 0 * LMAX, G, nk, K3, 1, ld_hbm, st_lds);
 #pragma unroll
   for (int i = 0; i < kWR; ++i) {
     const int idx = threadIdx.x + i * blockDim.x;
@@ -305,15 +334,15 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   }
   for (int idx = threadIdx.x + kWR * blockDim.x; idx < nwl; idx += blockDim.x) wl[idx] = wsrc[idx];
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 2 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
+  dft_run<T2, LMAX, 1>(twl + 2 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 4 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_w);
+  dft_run<T2, LMAX, 1>(twl + 4 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_w);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 1 * LMAX, G, nk, K3, 1, ld_lds, st_lds);
+  dft_run<T2, LMAX, -1>(twl + 1 * LMAX, G, nk, K3, 1, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 3 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
+  dft_run<T2, LMAX, -1>(twl + 3 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
+  dft_run<T2, LMAX, -1>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
 }
 
 template <typename T2, int LMAX>
@@ -370,11 +399,11 @@ __global__ __launch_bounds__(256) void k_fft_fwd(const ActArgs<T2> A) {
   auto st_hbm = [&](int k, int g, T2 v) {
     if (g < gcnt) gphi[(size_t)k * ldphi + g] = v;
   };
-  dft_run<T2, LMAX>(twl + 0 * LMAX, G, nk, A.K3, 1, ld_hbm, st_lds);
+  dft_run<T2, LMAX, 1>(twl + 0 * LMAX, G, nk, A.K3, 1, ld_hbm, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 1 * LMAX, G, nk, A.K2, A.K3, ld_lds, st_lds);
+  dft_run<T2, LMAX, 1>(twl + 1 * LMAX, G, nk, A.K2, A.K3, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX>(twl + 2 * LMAX, G, nk, A.K1, A.K2 * A.K3, ld_lds, st_hbm);
+  dft_run<T2, LMAX, 1>(twl + 2 * LMAX, G, nk, A.K1, A.K2 * A.K3, ld_lds, st_hbm);
 }
 
 // Q[k][x] += coef * Im sum_a conj(PY(k,x,a)) PX(k,x,a); one thread per (k, x), each
@@ -406,14 +435,15 @@ constexpr int kProjChunk = 8;
 
 __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const double2* __restrict__ psi,
                                         const double* __restrict__ allow, double2* __restrict__ apart,
-                                        int nb, int nk, int na, int ns, int P, int k, int s) {
+                                        int nb, int nk, int na, int ns, int P, int k, int s, int c0,
+                                        int cstep) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
   const double2* cb = c + ((size_t)s * nk + k) * P * nb;
   for (int j = wid; j < acnt; j += nw) {
     const int a = s + j * astep;
-    for (int p0 = 0; p0 < P; p0 += kProjChunk) {
+    for (int p0 = c0 * kProjChunk; p0 < P; p0 += cstep * kProjChunk) {
       double v[2 * kProjChunk];
 #pragma unroll
       for (int q = 0; q < 2 * kProjChunk; ++q) v[q] = 0.0;
@@ -447,7 +477,7 @@ __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const dou
 __global__ __launch_bounds__(kBlock) void k_mproj(
     const double2* __restrict__ c, const double2* __restrict__ psi, const double* __restrict__ allow,
     double2* __restrict__ apart, int nb, int nk, int na, int ns, int P) {
-  mproj_k(c, psi, allow, apart, nb, nk, na, ns, P, blockIdx.x, blockIdx.y);
+  mproj_k(c, psi, allow, apart, nb, nk, na, ns, P, blockIdx.x, blockIdx.y, 0, 1);
 }
 
 // amp(j) = sum_k apart(j, k) for j in [j0, j1): one wave per amplitude, fixed order.

@@ -307,26 +307,42 @@ int dev_nrm(H* h, const double2* p, double* v) {
 // ---------------------------------------------------------------------------
 // GMRES helpers
 // ---------------------------------------------------------------------------
-// <W_i|v1>, <W_i|v2> for i < m: partials [blk][i][4]
+// <W_i|v1>, <W_i|v2> for i < m: partials [blk][i][4].  Eight W rows per pass with
+// all their loads in flight; the 32 sums of a pass reduce by one butterfly per
+// wave and one LDS round across the waves.
 __global__ __launch_bounds__(kBlock) void k_multidot2(const double2* const* __restrict__ Wl, int m,
                                                       const double2* __restrict__ v1,
                                                       const double2* __restrict__ v2, size_t n,
                                                       double* __restrict__ part) {
-  __shared__ double lds[4 * 4];
-  for (int i = 0; i < m; ++i) {
-    const double2* w = Wl[i];
-    double v[4] = {0, 0, 0, 0};
+  constexpr int R = 8;
+  __shared__ double lds[(kBlock / 64) * 4 * R];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i0 = 0; i0 < m; i0 += R) {
+    double v[4 * R];
+#pragma unroll
+    for (int q = 0; q < 4 * R; ++q) v[q] = 0.0;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-      const double2 a = w[e], b = v1[e], c = v2[e];
-      v[0] = fma(a.x, b.x, v[0]); v[0] = fma(a.y, b.y, v[0]);
-      v[1] = fma(a.x, b.y, v[1]); v[1] = fma(-a.y, b.x, v[1]);
-      v[2] = fma(a.x, c.x, v[2]); v[2] = fma(a.y, c.y, v[2]);
-      v[3] = fma(a.x, c.y, v[3]); v[3] = fma(-a.y, c.x, v[3]);
+      const double2 b = v1[e], c = v2[e];
+      double2 a[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) a[q] = Wl[min(i0 + q, m - 1)][e];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        v[4 * q + 0] = fma(a[q].x, b.x, v[4 * q + 0]); v[4 * q + 0] = fma(a[q].y, b.y, v[4 * q + 0]);
+        v[4 * q + 1] = fma(a[q].x, b.y, v[4 * q + 1]); v[4 * q + 1] = fma(-a[q].y, b.x, v[4 * q + 1]);
+        v[4 * q + 2] = fma(a[q].x, c.x, v[4 * q + 2]); v[4 * q + 2] = fma(a[q].y, c.y, v[4 * q + 2]);
+        v[4 * q + 3] = fma(a[q].x, c.y, v[4 * q + 3]); v[4 * q + 3] = fma(-a[q].y, c.x, v[4 * q + 3]);
+      }
     }
-    og::block_sum<4>(v, lds);
-    if (threadIdx.x == 0) {
-      double* o = part + ((size_t)blockIdx.x * m + i) * 4;
-      o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    int owner = 0;
+    const double tot = og::wave_multi_sum<double, 4 * R>(v, lane, &owner);
+    if ((lane & (64 / (4 * R) - 1)) == 0) lds[wid * 4 * R + owner] = tot;
+    __syncthreads();
+    if (threadIdx.x < 4 * R) {
+      double sum = 0.0;
+      for (int w = 0; w < nw; ++w) sum += lds[w * 4 * R + threadIdx.x];
+      const int q = threadIdx.x >> 2;
+      if (i0 + q < m) part[((size_t)blockIdx.x * m + i0 + q) * 4 + (threadIdx.x & 3)] = sum;
     }
     __syncthreads();
   }
@@ -340,18 +356,100 @@ __global__ void k_multifinish(const double* __restrict__ part, int nblk, int nv,
   }
 }
 
-// y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341)
+// y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341),
+// eight W rows per step with their loads in flight, summed in i order.
 __global__ void k_multiaxpy(const double2* const* __restrict__ Wl, const double2* __restrict__ c, int m,
                             double2* __restrict__ y, size_t n) {
+  constexpr int R = 8;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     double2 yv = y[e];
-    for (int i = 0; i < m; ++i) {
-      const double2 w = Wl[i][e];
-      const double2 ci = c[i];
-      yv.x += ci.x * w.x; yv.y += ci.x * w.y;
-      yv.x += -ci.y * w.y; yv.y += ci.y * w.x;
+    for (int i0 = 0; i0 < m; i0 += R) {
+      double2 w[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) w[q] = Wl[min(i0 + q, m - 1)][e];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (i0 + q < m) {
+          const double2 ci = c[i0 + q];
+          yv.x += ci.x * w[q].x; yv.y += ci.x * w[q].y;
+          yv.x += -ci.y * w[q].y; yv.y += ci.y * w[q].x;
+        }
+      }
     }
     y[e] = yv;
+  }
+}
+
+constexpr int kMaxGmresDepth = 48;  // the least-squares factor lives in one workgroup's LDS
+
+__global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin, double2* __restrict__ Cup,
+                                                  double2* __restrict__ coef, int cur, int depth,
+                                                  int* __restrict__ info) {
+  extern __shared__ double2 sm[];          // L [depth][depth], rhs [depth]
+  __shared__ double D[64];
+  double2* L = sm;
+  double2* r = sm + (size_t)depth * depth;
+  const int t = threadIdx.x;
+  const size_t col = (size_t)(cur - 1) * cur / 2;
+  for (int i = t; i < cur; i += 64) {
+    Cup[col + i] = (i == cur - 1) ? make_double2(fin[4 * i], 0.0) : make_double2(fin[4 * i], fin[4 * i + 1]);
+    r[i] = make_double2(-fin[4 * i + 2], -fin[4 * i + 3]);
+  }
+  __syncthreads();
+  // C(i, j): upper triangle stored, lower by conjugation
+  auto C = [&](int i, int j) -> double2 {
+    if (i <= j) return Cup[(size_t)j * (j + 1) / 2 + i];
+    const double2 v = Cup[(size_t)i * (i + 1) / 2 + j];
+    return make_double2(v.x, -v.y);
+  };
+  // sum_k a_k conj(b_k) d_k, as a * conj(b) * d in the host's left-to-right order
+  auto mcd = [](double2 a, double2 b, double d) {
+    const double2 p = make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_double2(p.x * d, p.y * d);
+  };
+  int bad = 0;
+  for (int j = 0; j < cur; ++j) {
+    if (t == 0) {
+      double2 sj = C(j, j);
+      for (int k = 0; k < j; ++k) {
+        const double2 m = mcd(L[j * depth + k], L[j * depth + k], D[k]);
+        sj.x -= m.x; sj.y -= m.y;
+      }
+      D[j] = sj.x;
+      L[j * depth + j] = make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    if (D[j] == 0.0) { bad = j + 1; break; }
+    for (int i = j + 1 + t; i < cur; i += 64) {
+      double2 ti = C(i, j);
+      for (int k = 0; k < j; ++k) {
+        const double2 m = mcd(L[i * depth + k], L[j * depth + k], D[k]);
+        ti.x -= m.x; ti.y -= m.y;
+      }
+      L[i * depth + j] = make_double2(ti.x / D[j], ti.y / D[j]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (bad) {
+      *info = bad;
+      for (int i = 0; i < cur; ++i) coef[i] = make_double2(0.0, 0.0);
+    } else {
+      for (int i = 0; i < cur; ++i)
+        for (int k = 0; k < i; ++k) {
+          const double2 l = L[i * depth + k], v = r[k];
+          r[i].x -= l.x * v.x - l.y * v.y;
+          r[i].y -= l.x * v.y + l.y * v.x;
+        }
+      for (int i = 0; i < cur; ++i) { r[i].x /= D[i]; r[i].y /= D[i]; }
+      for (int i = cur - 1; i >= 0; --i)
+        for (int k = i + 1; k < cur; ++k) {
+          const double2 l = L[k * depth + i], v = r[k];  // conj(l) * v
+          r[i].x -= l.x * v.x + l.y * v.y;
+          r[i].y -= l.x * v.y - l.y * v.x;
+        }
+      for (int i = 0; i < cur; ++i) coef[i] = r[i];
+    }
   }
 }
 
@@ -412,38 +510,6 @@ __global__ void k_mask(double2* __restrict__ y, const double2* __restrict__ x, c
     v.x *= m[e]; v.y *= m[e];
     y[e] = v;
   }
-}
-
-// Packed Hermitian solve by LDL^H (no pivoting), returning the 1-based index of a
-// zero pivot as zhptrf's info would (OCEAN_gmres.f90:303-321).
-int hermitian_solve(const std::vector<cplx>& Cup, int m, std::vector<cplx>& rhs) {
-  // Cup(i,j) for i <= j at j*(j+1)/2 + i
-  auto C = [&](int i, int j) -> cplx {
-    if (i <= j) return Cup[(size_t)j * (j + 1) / 2 + i];
-    return std::conj(Cup[(size_t)i * (i + 1) / 2 + j]);
-  };
-  std::vector<cplx> L((size_t)m * m, cplx(0, 0));
-  std::vector<double> D(m, 0.0);
-  for (int j = 0; j < m; ++j) {
-    cplx s = C(j, j);
-    for (int k = 0; k < j; ++k) s -= L[(size_t)j * m + k] * std::conj(L[(size_t)j * m + k]) * D[k];
-    D[j] = s.real();
-    if (D[j] == 0.0) return j + 1;
-    L[(size_t)j * m + j] = 1.0;
-    for (int i = j + 1; i < m; ++i) {
-      cplx t = C(i, j);
-      for (int k = 0; k < j; ++k) t -= L[(size_t)i * m + k] * std::conj(L[(size_t)j * m + k]) * D[k];
-      L[(size_t)i * m + j] = t / D[j];
-    }
-  }
-  for (int i = 0; i < m; ++i) {
-    for (int k = 0; k < i; ++k) rhs[i] -= L[(size_t)i * m + k] * rhs[k];
-  }
-  for (int i = 0; i < m; ++i) rhs[i] /= D[i];
-  for (int i = m - 1; i >= 0; --i) {
-    for (int k = i + 1; k < m; ++k) rhs[i] -= std::conj(L[(size_t)k * m + i]) * rhs[k];
-  }
-  return 0;
 }
 
 }  // namespace
@@ -1003,6 +1069,7 @@ namespace {
 int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, double gres_eV,
               double gprc_eV, double ffff, int depth, double kpref, double* out, double2* xs_dev,
               double* x_re, double* x_im) {
+  if (depth > kMaxGmresDepth) return OCEAN_GPU_ERR_ARG;
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
   const size_t nv = h->nvec();
@@ -1020,9 +1087,18 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = dmalloc(&dlist, 2 * (size_t)depth + 2);
   if (rc == 0) rc = dmalloc(&dcoef, (size_t)depth);
   if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 16);
+  double2* dcup = nullptr;
+  int* dinfo = nullptr;
+  double* hpin = nullptr;
+  if (rc == 0) rc = dmalloc(&dcup, (size_t)depth * (depth + 1) / 2);
+  if (rc == 0) rc = dmalloc(&dinfo, (size_t)2);
+  if (rc == 0 && hipHostMalloc((void**)&hpin, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = OCEAN_GPU_ERR_NOMEM;
+  if (rc == 0 && hipMemsetAsync(dinfo, 0, 2 * sizeof(int), h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   auto cleanup = [&]() {
     for (auto& p : V) dfree(p);
-    dfree(dlist); dfree(dcoef); dfree(part);
+    dfree(dlist); dfree(dcoef); dfree(part); dfree(dcup); dfree(dinfo);
+    if (hpin) (void)hipHostFree(hpin);
   };
   if (rc) { cleanup(); return rc; }
   double2 *rhs = V[0], *h1 = V[1], *x = V[2], *ax = V[3], *g = V[4], *pc = V[5];
@@ -1030,8 +1106,6 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   double2** AU = &V[nfix + depth];   // AU[i]
   std::vector<double2*> hlist(2 * depth);
   for (int i = 0; i < depth; ++i) { hlist[i] = AU[i]; hlist[depth + i] = U[i]; }
-  std::vector<cplx> Cup((size_t)depth * (depth + 1) / 2);
-  std::vector<double> pin(4 * (size_t)depth + 8);
 
   auto sync_check = [&]() -> int {
     if (hipGetLastError() != hipSuccess) return OCEAN_GPU_ERR_HIP;
@@ -1051,42 +1125,32 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     k_axmy<<<grid, kBlock, 0, h->stream>>>(in, outv, E, gres, nv);
     return sync_check();
   };
+  // The only host synchronisation per iteration: the residual norm, <rhs|x> and
+  // the least-squares pivot flag of the updates since the last one.
   auto resid = [&]() -> int {  // g = ax - rhs ; gval ; (rval, ival) = <rhs|x>
+    double* fin = part + (size_t)kRedBlocks * 4 * depth;
     k_resid<<<kRedBlocks, kBlock, 0, h->stream>>>(ax, rhs, x, g, nv, part);
-    og::k_finish<<<1, 64, 0, h->stream>>>(part, kRedBlocks, 3, part + kRedBlocks * 4 * depth);
+    og::k_finish<<<1, 64, 0, h->stream>>>(part, kRedBlocks, 3, fin);
     OG_TRY(sync_check());
-    OG_CHECK(hipMemcpyAsync(pin.data(), part + kRedBlocks * 4 * depth, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                            h->stream));
+    OG_CHECK(hipMemcpyAsync(hpin, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    OG_CHECK(hipMemcpyAsync(hpin + 3, dinfo, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     OG_CHECK(hipStreamSynchronize(h->stream));
-    gval = pin[0]; rval = pin[1]; ival = pin[2];
-    return OCEAN_GPU_OK;
+    gval = hpin[0]; rval = hpin[1]; ival = hpin[2];
+    int info = 0;
+    std::memcpy(&info, hpin + 3, sizeof(int));
+    return info;  // zhptrf's info of a failed least-squares step, else 0
   };
-  // update_gmres(cur): Gram column cur and rhs, LS solve, x/ax updates (OCEAN_gmres.f90:241-344)
+  // update_gmres(cur): Gram column cur, LS solve and x/ax updates, all on the
+  // device (OCEAN_gmres.f90:241-344)
   auto update = [&](int cur) -> int {
     k_multidot2<<<kRedBlocks, kBlock, 0, h->stream>>>((const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
     k_multifinish<<<(4 * cur + 255) / 256, 256, 0, h->stream>>>(part, kRedBlocks, 4 * cur, fin);
-    OG_TRY(sync_check());
-    OG_CHECK(hipMemcpyAsync(pin.data(), fin, 4 * cur * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    OG_CHECK(hipStreamSynchronize(h->stream));
-    const size_t col = (size_t)(cur - 1) * cur / 2;
-    for (int i = 0; i < cur; ++i) {
-      // C(i, cur) = <au_i|au_cur> ; diagonal from the squared norm
-      Cup[col + i] = (i == cur - 1) ? cplx(pin[4 * i], 0.0) : cplx(pin[4 * i], pin[4 * i + 1]);
-    }
-    std::vector<cplx> coef(cur);
-    for (int i = 0; i < cur; ++i) coef[i] = -cplx(pin[4 * i + 2], pin[4 * i + 3]);
-    std::vector<cplx> Cs(Cup.begin(), Cup.begin() + (size_t)cur * (cur + 1) / 2);
-    const int info = hermitian_solve(Cs, cur, coef);
-    if (info != 0) return info;
-    std::vector<double2> cd(cur);
-    for (int i = 0; i < cur; ++i) cd[i] = make_double2(coef[i].real(), coef[i].imag());
-    OG_CHECK(hipMemcpyAsync(dcoef, cd.data(), cur * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    k_ls_update<<<1, 64, (size_t)(depth * depth + depth) * sizeof(double2), h->stream>>>(fin, dcup, dcoef, cur, depth,
+                                                                                     dinfo);
     k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)(dlist + depth), dcoef, cur, x, nv);
     k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)dlist, dcoef, cur, ax, nv);
-    OG_TRY(sync_check());
-    OG_CHECK(hipStreamSynchronize(h->stream));  // cd must stay valid until the copy ran
-    return OCEAN_GPU_OK;
+    return sync_check();
   };
 
   int global_iter = 0, prev_iter = 0;
