@@ -348,12 +348,14 @@ __global__ __launch_bounds__(kBlock) void k_multidot2(const double2* const* __re
   }
 }
 
+// out[j] = sum_b part[b*nv + j]: one wave per value, lane-strided partial sums
+// then a butterfly (fixed order, so every run gives the same bits).
 __global__ void k_multifinish(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * nv + j];
-    out[j] = s;
-  }
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= nv) return;
+  const double s = og::wave_sum_parts(part, nblk, nv, j, lane);
+  if (lane == 0) out[j] = s;
 }
 
 // y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341),
@@ -380,77 +382,65 @@ __global__ void k_multiaxpy(const double2* const* __restrict__ Wl, const double2
   }
 }
 
-constexpr int kMaxGmresDepth = 48;  // the least-squares factor lives in one workgroup's LDS
+// Least-squares step of update_gmres on the device (OCEAN_gmres.f90:241-344).
+// The Gram matrix grows by one bordered row per call (cur = 1, 2, ... after every
+// restart or recycled energy), so its LDL^H factor (no pivoting) grows by one row:
+//   L(j,k) = (C(j,k) - sum_{m<k} L(j,m) conj(L(k,m)) D(m)) / D(k),  k < j = cur-1
+//   D(j)   =  C(j,j) - sum_{k<j} L(j,k) conj(L(j,k)) D(k)
+// with L and D kept in global memory between calls; then coef = -C^-1 <au|g> by the
+// two triangular solves.  One thread: cur is at most kMaxGmresDepth and the work is
+// O(cur^2).  A zero pivot leaves its 1-based index in *info (zhptrf's info) and the
+// coefficients at 0.
+constexpr int kMaxGmresDepth = 48;
 
-__global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin, double2* __restrict__ Cup,
-                                                  double2* __restrict__ coef, int cur, int depth,
-                                                  int* __restrict__ info) {
-  extern __shared__ double2 sm[];          // L [depth][depth], rhs [depth]
-  __shared__ double D[64];
-  double2* L = sm;
-  double2* r = sm + (size_t)depth * depth;
-  const int t = threadIdx.x;
-  const size_t col = (size_t)(cur - 1) * cur / 2;
-  for (int i = t; i < cur; i += 64) {
-    Cup[col + i] = (i == cur - 1) ? make_double2(fin[4 * i], 0.0) : make_double2(fin[4 * i], fin[4 * i + 1]);
-    r[i] = make_double2(-fin[4 * i + 2], -fin[4 * i + 3]);
+__global__ void k_ls_update(const double* __restrict__ fin, double2* __restrict__ L, double* __restrict__ D,
+                            double2* __restrict__ coef, int cur, int depth, int* __restrict__ info) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int j = cur - 1;
+  double2 r[kMaxGmresDepth];
+  // C(j, k) = conj(<au_k|au_j>) for k < j (the lower triangle of the Hermitian
+  // matrix whose upper column j holds <au_k|au_j>); C(j, j) = |au_j|^2
+  for (int k = 0; k < j; ++k) {
+    double2 t = make_double2(fin[4 * k], -fin[4 * k + 1]);
+    for (int m = 0; m < k; ++m) {
+      const double2 a = L[j * depth + m], b = L[k * depth + m];
+      const double d = D[m];
+      // a * conj(b) * d
+      t.x -= (a.x * b.x + a.y * b.y) * d;
+      t.y -= (a.y * b.x - a.x * b.y) * d;
+    }
+    L[j * depth + k] = make_double2(t.x / D[k], t.y / D[k]);
   }
-  __syncthreads();
-  // C(i, j): upper triangle stored, lower by conjugation
-  auto C = [&](int i, int j) -> double2 {
-    if (i <= j) return Cup[(size_t)j * (j + 1) / 2 + i];
-    const double2 v = Cup[(size_t)i * (i + 1) / 2 + j];
-    return make_double2(v.x, -v.y);
-  };
-  // sum_k a_k conj(b_k) d_k, as a * conj(b) * d in the host's left-to-right order
-  auto mcd = [](double2 a, double2 b, double d) {
-    const double2 p = make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-    return make_double2(p.x * d, p.y * d);
-  };
+  double dj = fin[4 * j];
+  for (int k = 0; k < j; ++k) {
+    const double2 a = L[j * depth + k];
+    dj -= (a.x * a.x + a.y * a.y) * D[k];
+  }
+  D[j] = dj;
+  L[j * depth + j] = make_double2(1.0, 0.0);
   int bad = 0;
-  for (int j = 0; j < cur; ++j) {
-    if (t == 0) {
-      double2 sj = C(j, j);
-      for (int k = 0; k < j; ++k) {
-        const double2 m = mcd(L[j * depth + k], L[j * depth + k], D[k]);
-        sj.x -= m.x; sj.y -= m.y;
-      }
-      D[j] = sj.x;
-      L[j * depth + j] = make_double2(1.0, 0.0);
-    }
-    __syncthreads();
-    if (D[j] == 0.0) { bad = j + 1; break; }
-    for (int i = j + 1 + t; i < cur; i += 64) {
-      double2 ti = C(i, j);
-      for (int k = 0; k < j; ++k) {
-        const double2 m = mcd(L[i * depth + k], L[j * depth + k], D[k]);
-        ti.x -= m.x; ti.y -= m.y;
-      }
-      L[i * depth + j] = make_double2(ti.x / D[j], ti.y / D[j]);
-    }
-    __syncthreads();
+  for (int k = 0; k <= j; ++k)
+    if (D[k] == 0.0) { bad = k + 1; break; }
+  if (bad) {
+    *info = bad;
+    for (int i = 0; i < cur; ++i) coef[i] = make_double2(0.0, 0.0);
+    return;
   }
-  if (t == 0) {
-    if (bad) {
-      *info = bad;
-      for (int i = 0; i < cur; ++i) coef[i] = make_double2(0.0, 0.0);
-    } else {
-      for (int i = 0; i < cur; ++i)
-        for (int k = 0; k < i; ++k) {
-          const double2 l = L[i * depth + k], v = r[k];
-          r[i].x -= l.x * v.x - l.y * v.y;
-          r[i].y -= l.x * v.y + l.y * v.x;
-        }
-      for (int i = 0; i < cur; ++i) { r[i].x /= D[i]; r[i].y /= D[i]; }
-      for (int i = cur - 1; i >= 0; --i)
-        for (int k = i + 1; k < cur; ++k) {
-          const double2 l = L[k * depth + i], v = r[k];  // conj(l) * v
-          r[i].x -= l.x * v.x + l.y * v.y;
-          r[i].y -= l.x * v.y - l.y * v.x;
-        }
-      for (int i = 0; i < cur; ++i) coef[i] = r[i];
+  for (int i = 0; i < cur; ++i) r[i] = make_double2(-fin[4 * i + 2], -fin[4 * i + 3]);
+  for (int i = 0; i < cur; ++i)
+    for (int k = 0; k < i; ++k) {
+      const double2 l = L[i * depth + k], v = r[k];
+      r[i].x -= l.x * v.x - l.y * v.y;
+      r[i].y -= l.x * v.y + l.y * v.x;
     }
-  }
+  for (int i = 0; i < cur; ++i) { r[i].x /= D[i]; r[i].y /= D[i]; }
+  for (int i = cur - 1; i >= 0; --i)
+    for (int k = i + 1; k < cur; ++k) {
+      const double2 l = L[k * depth + i], v = r[k];  // conj(l) * v
+      r[i].x -= l.x * v.x + l.y * v.y;
+      r[i].y -= l.x * v.y - l.y * v.x;
+    }
+  for (int i = 0; i < cur; ++i) coef[i] = r[i];
 }
 
 // y = (er + i ei) x - y  (OCEAN_psi_axmy, OCEAN_psi.f90:1468-1562)
@@ -1087,17 +1077,19 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = dmalloc(&dlist, 2 * (size_t)depth + 2);
   if (rc == 0) rc = dmalloc(&dcoef, (size_t)depth);
   if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 16);
-  double2* dcup = nullptr;
+  double2* dL = nullptr;
+  double* dD = nullptr;
   int* dinfo = nullptr;
   double* hpin = nullptr;
-  if (rc == 0) rc = dmalloc(&dcup, (size_t)depth * (depth + 1) / 2);
+  if (rc == 0) rc = dmalloc(&dL, (size_t)depth * depth);
+  if (rc == 0) rc = dmalloc(&dD, (size_t)depth);
   if (rc == 0) rc = dmalloc(&dinfo, (size_t)2);
   if (rc == 0 && hipHostMalloc((void**)&hpin, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = OCEAN_GPU_ERR_NOMEM;
   if (rc == 0 && hipMemsetAsync(dinfo, 0, 2 * sizeof(int), h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   auto cleanup = [&]() {
     for (auto& p : V) dfree(p);
-    dfree(dlist); dfree(dcoef); dfree(part); dfree(dcup); dfree(dinfo);
+    dfree(dlist); dfree(dcoef); dfree(part); dfree(dL); dfree(dD); dfree(dinfo);
     if (hpin) (void)hipHostFree(hpin);
   };
   if (rc) { cleanup(); return rc; }
@@ -1145,9 +1137,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   auto update = [&](int cur) -> int {
     k_multidot2<<<kRedBlocks, kBlock, 0, h->stream>>>((const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
-    k_multifinish<<<(4 * cur + 255) / 256, 256, 0, h->stream>>>(part, kRedBlocks, 4 * cur, fin);
-    k_ls_update<<<1, 64, (size_t)(depth * depth + depth) * sizeof(double2), h->stream>>>(fin, dcup, dcoef, cur, depth,
-                                                                                     dinfo);
+    k_multifinish<<<(4 * cur + 3) / 4, 256, 0, h->stream>>>(part, kRedBlocks, 4 * cur, fin);
+    k_ls_update<<<1, 64, 0, h->stream>>>(fin, dL, dD, dcoef, cur, depth, dinfo);
     k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)(dlist + depth), dcoef, cur, x, nv);
     k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)dlist, dcoef, cur, ax, nv);
     return sync_check();
