@@ -188,6 +188,41 @@ def gradient_workload(log_prefix="grad"):
     return out
 
 
+def precision_workload(reps=20):
+    """Config 2 (L2,3 edge, nalpha 12, nb 128, k-mesh 8^3, x-mesh 10^3, spin-orbit
+    10 eV): the mixed form against the FP64 drop-in over 500 Haydock iterations and
+    4000 energy points (max error of peak and L2 relative error of the spectrum),
+    plus both H.psi rates."""
+    from ocean_amd import core, synth
+
+    t0 = time.perf_counter()
+    inp = synth.config(2)
+    log(f"config 2 inputs in {time.perf_counter() - t0:.1f}s")
+    p = inp.params
+    E2H = core.EV2HARTREE
+    g = core.OceanGPU.from_inputs(inp)
+    vin, vout = g.vec_new(inp.psi0), g.vec_new()
+    out = {}
+    spec = {}
+    for prec, name in ((core.FP64, "fp64"), (core.MIXED, "mixed")):
+        g.set_precision(prec)
+        g.xact_repeat(vin, vout, 2)
+        ms = g.xact_repeat(vin, vout, reps)
+        out[f"cfg2_hpsi_per_s_{name}"] = round(1e3 / ms, 1)
+        t1 = time.perf_counter()
+        a, b, _ = g.haydock(inp.psi0, p["niter"])
+        sp = core.spectrum(a, b, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"] - 1)
+        out[f"cfg2_spectra_per_s_{name}"] = round(1.0 / (time.perf_counter() - t1), 2)
+        spec[name] = sp[:, 2]
+        log(f"config 2 {name}: {ms:.4f} ms per H.psi")
+    g.close()
+    s64, smx = spec["fp64"], spec["mixed"]
+    out["cfg2_points"] = int(s64.size)
+    out["cfg2_mixed_max_err_of_peak"] = float(np.abs(smx - s64).max() / np.abs(s64).max())
+    out["cfg2_mixed_l2_rel_err"] = float(np.linalg.norm(smx - s64) / np.linalg.norm(s64))
+    return out
+
+
 def gmres_workload(inp, g, n=64):
     """Config 1 GMRES: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
     E = np.linspace(-5.0, 40.0, n)
@@ -209,6 +244,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batched form")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient (differentiation) form")
+    ap.add_argument("--no-cfg2", action="store_true", help="skip the L2,3 precision configuration")
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-niter", type=int, default=200)
     ap.add_argument("--calib-copy", action="store_true",
@@ -302,6 +338,8 @@ def main():
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})
+    if not args.no_extras and not args.no_cfg2:
+        extras.update(precision_workload())
     if not args.no_extras and not args.no_grad:
         g.close()
         g = None

@@ -138,6 +138,13 @@ int  ocean_gpu_vec_axpy(ocean_gpu_handle* h, double rval, double ival, int32_t x
 /* x *= rval (OCEAN_psi.f90:1876-1917) */
 int  ocean_gpu_vec_scal(ocean_gpu_handle* h, double rval, int32_t x);
 int  ocean_gpu_vec_copy(ocean_gpu_handle* h, int32_t dst, int32_t src);
+/* y = (rval + i ival) x - y  (OCEAN_psi_axmy, OCEAN2/OCEAN_psi.f90:1468-1562) */
+int  ocean_gpu_vec_axmy(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y);
+/* y = (rval + i ival) x - z  (OCEAN_psi_axmz, OCEAN2/OCEAN_psi.f90:1571-1670) */
+int  ocean_gpu_vec_axmz(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y, int32_t z);
+/* x = x / (rval + i ival)  (OCEAN_psi_divide, OCEAN2/OCEAN_psi.f90:1764-1874); which
+ * selects the reference's optional arguments: 1 rval only, 2 ival only, 3 both. */
+int  ocean_gpu_vec_divide(ocean_gpu_handle* h, int32_t x, double rval, double ival, int32_t which);
 /* xact between device vectors; the form bench.py times (inputs resident in HBM). */
 int  ocean_gpu_xact_vec(ocean_gpu_handle* h, int32_t in, int32_t out, int32_t backwards,
                         const int32_t* hflag);

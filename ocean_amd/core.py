@@ -72,6 +72,9 @@ _SIGS = {
     "ocean_gpu_vec_axpy": (C.c_int, [_H, C.c_double, C.c_double, C.c_int32, C.c_int32]),
     "ocean_gpu_vec_scal": (C.c_int, [_H, C.c_double, C.c_int32]),
     "ocean_gpu_vec_copy": (C.c_int, [_H, C.c_int32, C.c_int32]),
+    "ocean_gpu_vec_axmy": (C.c_int, [_H, C.c_double, C.c_double, C.c_int32, C.c_int32]),
+    "ocean_gpu_vec_axmz": (C.c_int, [_H, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]),
+    "ocean_gpu_vec_divide": (C.c_int, [_H, C.c_int32, C.c_double, C.c_double, C.c_int32]),
     "ocean_gpu_xact_vec": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32, _IP]),
     "ocean_gpu_xact_vec_repeat": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
     "ocean_gpu_profile_xact": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
@@ -312,6 +315,19 @@ class OceanGPU:
 
     def vec_copy(self, dst, src):
         _chk(_lib().ocean_gpu_vec_copy(self.h, dst, src), "vec_copy")
+
+    def vec_axmy(self, a, x, y):
+        """y = a x - y (OCEAN_psi_axmy)."""
+        _chk(_lib().ocean_gpu_vec_axmy(self.h, a.real, a.imag, x, y), "vec_axmy")
+
+    def vec_axmz(self, a, x, y, z):
+        """y = a x - z (OCEAN_psi_axmz)."""
+        _chk(_lib().ocean_gpu_vec_axmz(self.h, a.real, a.imag, x, y, z), "vec_axmz")
+
+    def vec_divide(self, x, rval=None, ival=None):
+        """x = x / (rval + i ival) with the reference's optional-argument branches."""
+        which = (1 if rval is not None else 0) + (2 if ival is not None else 0)
+        _chk(_lib().ocean_gpu_vec_divide(self.h, x, rval or 0.0, ival or 0.0, which), "vec_divide")
 
     def xact_vec(self, vin, vout, backwards=False, hflag=None):
         hf = None if hflag is None else np.ascontiguousarray(hflag, np.int32)

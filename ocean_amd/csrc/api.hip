@@ -895,6 +895,29 @@ int ocean_gpu_vec_scal(ocean_gpu_handle* h, double rval, int32_t x) {
   return OCEAN_GPU_OK;
 }
 
+int ocean_gpu_vec_axmy(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y) {
+  if (!h || !vec_ptr(h, x) || !vec_ptr(h, y)) return OCEAN_GPU_ERR_ARG;
+  og::k_axmz<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(rval, ival, vec_ptr(h, x), vec_ptr(h, y), vec_ptr(h, y),
+                                                          h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_axmz(ocean_gpu_handle* h, double rval, double ival, int32_t x, int32_t y, int32_t z) {
+  if (!h || !vec_ptr(h, x) || !vec_ptr(h, y) || !vec_ptr(h, z)) return OCEAN_GPU_ERR_ARG;
+  og::k_axmz<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(rval, ival, vec_ptr(h, x), vec_ptr(h, z), vec_ptr(h, y),
+                                                          h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_vec_divide(ocean_gpu_handle* h, int32_t x, double rval, double ival, int32_t which) {
+  if (!h || !vec_ptr(h, x) || which < 1 || which > 3) return OCEAN_GPU_ERR_ARG;
+  og::k_divide<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(rval, ival, which, vec_ptr(h, x), h->nvec());
+  OG_CHECK(hipGetLastError());
+  return OCEAN_GPU_OK;
+}
+
 int ocean_gpu_vec_copy(ocean_gpu_handle* h, int32_t dst, int32_t src) {
   if (!h || !vec_ptr(h, dst) || !vec_ptr(h, src)) return OCEAN_GPU_ERR_ARG;
   og::k_copy<<<grid_for(h->nvec()), kBlock, 0, h->stream>>>(vec_ptr(h, dst), vec_ptr(h, src), h->nvec());

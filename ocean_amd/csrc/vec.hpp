@@ -64,6 +64,39 @@ __global__ void k_scal(double a, double2* __restrict__ x, size_t n) {
   }
 }
 
+// y = (r + i i) x - z  (OCEAN_psi_axmz, OCEAN_psi.f90:1571-1670; z = y gives OCEAN_psi_axmy,
+// :1468-1562, whose buffer order (r x - y) -/+ i x is the same arithmetic)
+__global__ void k_axmz(double r, double i, const double2* __restrict__ x, const double2* z, double2* y, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 xv = x[e], zv = z[e];
+    double2 o;
+    o.x = (r * xv.x - zv.x) - i * xv.y;
+    o.y = (r * xv.y - zv.y) + i * xv.x;
+    y[e] = o;
+  }
+}
+
+// x = x / (r + i i) with the reference's branches (OCEAN_psi_divide, OCEAN_psi.f90:1764-1874):
+// which = 1: x * (1/r); 2: (x_i/i, -x_r/i); 3: x * conj(r + i i) / (r^2 + i^2)
+__global__ void k_divide(double r, double i, int which, double2* __restrict__ x, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = x[e];
+    double2 o;
+    if (which == 1) {
+      const double rn = 1.0 / r;
+      o = make_double2(rn * v.x, rn * v.y);
+    } else if (which == 2) {
+      const double in = 1.0 / i;
+      o = make_double2(v.y * in, -in * v.x);
+    } else {
+      const double den = r * r + i * i, rn = r / den, in = i / den;
+      o.x = rn * v.x + in * v.y;
+      o.y = -in * v.x + rn * v.y;
+    }
+    x[e] = o;
+  }
+}
+
 __global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];

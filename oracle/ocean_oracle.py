@@ -868,6 +868,35 @@ def psi_dot(p, q):
     return d.real, d.imag
 
 
+def psi_axmz(x, z, rval=1.0, ival=0.0):
+    """y = (rval + i ival) x - z in the reference's operation order.
+
+    OCEAN_psi.f90:1571-1670 (axmz: y = r x - z - i x_i, r x_i - z_i + i x_r) and
+    :1468-1562 (axmy: the buffer (r x - y) then -/+ i x, the same arithmetic with z = y).
+    """
+    xr, xi, zr, zi = x.real, x.imag, z.real, z.imag
+    yr = (rval * xr - zr) - ival * xi
+    yi = (rval * xi - zi) + ival * xr
+    return np.asfortranarray(yr + 1j * yi)
+
+
+def psi_divide(x, rval=None, ival=None):
+    """x / (rval + i ival) with the branches of OCEAN_psi_divide (OCEAN_psi.f90:1764-1874)."""
+    xr, xi = x.real, x.imag
+    if rval is not None and ival is not None:
+        den = rval * rval + ival * ival
+        rn, inn = rval / den, ival / den
+        outr = rn * xr + inn * xi
+        outi = -inn * xr + rn * xi
+    elif rval is not None:
+        rn = 1.0 / rval
+        outr, outi = rn * xr, rn * xi
+    else:
+        inn = 1.0 / ival
+        outr, outi = xi * inn, -inn * xr
+    return np.asfortranarray(outr + 1j * outi)
+
+
 def psi_nrm(p):
     """Squared norm.  OCEAN_psi.f90:1673-1760."""
     return float(np.sum(p.real ** 2 + p.imag ** 2))

@@ -120,6 +120,18 @@ def test_vector_ops(cfg0):
     vo = g.vec_new()
     g.xact_vec(vp, vo)
     assert relerr(g.vec_get(vo), orc.xact(to_problem(cfg0), 2.5 * p)) < 1e-12
+    # axmy / axmz / divide in the reference's operation order (device FMA contraction aside)
+    P2, Q2 = g.vec_get(vp), g.vec_get(vq)
+    vy = g.vec_new(Q2)
+    g.vec_axmy(0.7 - 1.3j, vp, vy)
+    assert relerr(g.vec_get(vy), orc.psi_axmz(P2, Q2, 0.7, -1.3)) < 1e-15
+    vz = g.vec_new()
+    g.vec_axmz(-0.2 + 0.9j, vp, vz, vq)
+    assert relerr(g.vec_get(vz), orc.psi_axmz(P2, Q2, -0.2, 0.9)) < 1e-15
+    for kw in (dict(rval=1.7), dict(ival=-0.6), dict(rval=0.4, ival=2.2)):
+        vd = g.vec_new(P2)
+        g.vec_divide(vd, **kw)
+        assert relerr(g.vec_get(vd), orc.psi_divide(P2, **kw)) < 1e-15, kw
     g.close()
 
 
