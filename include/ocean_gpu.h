@@ -164,6 +164,18 @@ int  ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double*
                        int32_t niter, double* real_a, double* real_b, double* imag_a,
                        int32_t* niter_done);
 
+/* The same recursion with the reference's early exit (sys%earlyExit,
+ * OCEAN2/OCEAN_haydock.f90:272-291): after each iteration past `spacing`, relArea
+ * between the spectra of iterations i and i - spacing (check_convergence,
+ * calc_spect_core, :1608-1661, :1991-2027: ne midpoint energies over [el, eh],
+ * broadening gam, all in Hartree) is compared with `threshold`; two successes in a
+ * row stop the loop.  niter_done returns the iteration it stopped at; entries past
+ * it are zero. */
+int  ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im,
+                             int32_t niter, int32_t spacing, double threshold, double kpref,
+                             double gam, double el, double eh, int32_t ne, double* real_a,
+                             double* real_b, double* imag_a, int32_t* niter_done);
+
 /* Recycled preconditioned GMRES over energies (OCEAN2/OCEAN_gmres.f90:508-908).
  * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
  * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,

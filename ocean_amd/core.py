@@ -79,6 +79,9 @@ _SIGS = {
     "ocean_gpu_xact_vec_repeat": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
     "ocean_gpu_profile_xact": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
     "ocean_gpu_haydock": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, _DP, _DP, _IP]),
+    "ocean_gpu_haydock_early": (C.c_int, [_H, _DP, _DP, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_int32, _DP, _DP,
+                                          _DP, _IP]),
     "ocean_gpu_gmres": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
                                   C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
@@ -386,6 +389,17 @@ class OceanGPU:
         _chk(rc, "spectral_grad")
         return dict(loss=loss.value, sigma=sig, g_ptab=gp, g_scale=gs.value if scale else None,
                     g_gamma_eV=gg.value if gamma else None, g_scissor_eV=gd.value if scissor else None)
+
+    def haydock_early(self, psi0, niter, spacing, threshold, gam, el, eh, ne, kpref=None):
+        """Haydock with the reference's early exit; returns (a, b, ai, niter_done)."""
+        kpref = self.kpref if kpref is None else kpref
+        pr, pi = _planes(psi0)
+        a, b, ai = np.zeros(niter + 1), np.zeros(niter + 1), np.zeros(niter + 1)
+        done = C.c_int32()
+        _chk(_lib().ocean_gpu_haydock_early(self.h, _p(pr), _p(pi), niter, spacing, threshold, kpref,
+                                            gam, el, eh, ne, _p(a), _p(b), _p(ai), C.byref(done)),
+             "haydock_early")
+        return a, b, ai, done.value
 
     def gmres(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref=None,
               return_x=False):

@@ -281,7 +281,9 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   T2* lds = twl + 6 * LMAX;                               // [k][g], padded
   T* wl = reinterpret_cast<T*>(lds + lidx(nk * G - 1) + 1);  // W(x, :) of the strip
   const int Ks[3] = {K3, K2, K1};
-  for (int t = threadIdx.x; t < 6 * LMAX; t += blockDim.x) {
+  // the LDS twiddle table serves the direct sums of non-power-of-two edges only
+  const bool direct = ((K1 & (K1 - 1)) | (K2 & (K2 - 1)) | (K3 & (K3 - 1))) != 0;
+  for (int t = threadIdx.x; direct && t < 6 * LMAX; t += blockDim.x) {
     const int d = t / (2 * LMAX), rem = t - d * 2 * LMAX, sg = rem / LMAX, m = rem - sg * LMAX;
     const int L = Ks[d];
     T sn = 0, cs = 1;
@@ -373,7 +375,8 @@ __global__ __launch_bounds__(256) void k_fft_fwd(const ActArgs<T2> A) {
   T2* lds = twl + 3 * LMAX;
   const int nk = A.nk, ncol = A.ncol, G = A.G, ldphi = A.ldphi;
   const int Ks[3] = {A.K3, A.K2, A.K1};
-  for (int t = threadIdx.x; t < 3 * LMAX; t += blockDim.x) {
+  const bool direct = ((A.K1 & (A.K1 - 1)) | (A.K2 & (A.K2 - 1)) | (A.K3 & (A.K3 - 1))) != 0;
+  for (int t = threadIdx.x; direct && t < 3 * LMAX; t += blockDim.x) {
     const int d = t / LMAX, m = t - d * LMAX;
     const int L = Ks[d];
     T sn = 0, cs = 1;

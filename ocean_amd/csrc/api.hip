@@ -992,41 +992,87 @@ int ocean_gpu_last_timing(ocean_gpu_handle* h, const char* name, double* ms) {
 }
 
 // ---- Haydock ----
-int ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
-                      double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
-  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Early-exit settings of the Haydock loop (sys%earlyExit, haydockConvergeSpacing,
+// haydockConvergeThreshold; spectrum grid of calc_spect_core in Hartree).
+struct EarlyExit {
+  int spacing = 0;
+  double threshold = 0.0, kpref = 0.0, gam = 0.0, el = 0.0, eh = 0.0;
+  int ne = 0;
+};
+
+int haydock_run(H* h, const double* psi0_re, const double* psi0_im, int niter, const EarlyExit* ex,
+                double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
   const size_t nv = h->nvec();
-  OG_TRY(ensure_scal(h, 3 * (size_t)(niter + 1) + 2 * kRedBlocks + kRedBlocks));
+  const size_t n1 = (size_t)niter + 1;
+  constexpr int kConvBlocks = 64;
+  OG_TRY(ensure_scal(h, 3 * n1 + 3 * (size_t)kRedBlocks + 2 * n1 + 4 * (size_t)kConvBlocks));
   double* da = h->scal;
-  double* dai = da + (niter + 1);
-  double* db = dai + (niter + 1);
-  double* part1 = db + (niter + 1);
+  double* dai = da + n1;
+  double* db = dai + n1;
+  double* part1 = db + n1;
   double* part2 = part1 + 2 * kRedBlocks;
+  double* conv = part2 + kRedBlocks;      // [iter][relArea, maxDiff]
+  double* cpart = conv + 2 * n1;
   double2 *v0 = nullptr, *v1 = nullptr, *v2 = nullptr;
   OG_TRY(dmalloc(&v0, nv));
   if (dmalloc(&v1, nv) || dmalloc(&v2, nv)) { dfree(v0); dfree(v1); return OCEAN_GPU_ERR_NOMEM; }
   int rc = put_planes(h, psi0_re, psi0_im, v0, nv);
   double2 *psi = v0, *old = v1, *hv = v2;
+  int done = niter;
+  // early exit: the convergence figures are formed on the device every iteration
+  // and read back every kCheck iterations; the reference's decision is replayed in
+  // order, so the iteration it stops at is the reference's (later ones are dropped)
+  constexpr int kCheck = 8;
+  std::vector<double> hconv(2 * n1, 0.0);
+  bool prev_conv = false, stop = false;
+  int checked = 0;
+  auto replay = [&](int upto) -> int {
+    OG_CHECK(hipMemcpyAsync(hconv.data(), conv, 2 * n1 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    OG_CHECK(hipStreamSynchronize(h->stream));
+    for (int it = checked + 1; it <= upto && !stop; ++it) {
+      if (it <= ex->spacing) continue;
+      if (hconv[2 * (size_t)it] < ex->threshold) {
+        if (prev_conv) { stop = true; done = it; }
+        else prev_conv = true;
+      } else {
+        prev_conv = false;
+      }
+    }
+    checked = upto;
+    return OCEAN_GPU_OK;
+  };
   if (rc == 0) {
     og::k_fill<<<grid_for(nv), kBlock, 0, h->stream>>>(old, make_double2(0.0, 0.0), nv);
-    og::k_fill_d<<<grid_for(3 * (size_t)(niter + 1)), kBlock, 0, h->stream>>>(da, 0.0, 3 * (size_t)(niter + 1));
-    for (int it = 1; it <= niter && rc == 0; ++it) {
+    og::k_fill_d<<<grid_for(5 * n1), kBlock, 0, h->stream>>>(da, 0.0, 3 * n1);
+    og::k_fill_d<<<grid_for(2 * n1), kBlock, 0, h->stream>>>(conv, 0.0, 2 * n1);
+    for (int it = 1; it <= niter && rc == 0 && !stop; ++it) {
       rc = xact_dev(h, psi, hv, f);
       if (rc) break;
       og::k_hay1<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, old, psi, db + (it - 1), nv, part1);
       og::k_hay2<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, psi, part1, kRedBlocks, nv, da + (it - 1),
                                                       dai + (it - 1), part2);
       og::k_hay3<<<kRedBlocks, kBlock, 0, h->stream>>>(hv, part2, kRedBlocks, nv, db + it);
+      if (ex && it > ex->spacing) {
+        const int nb = std::min(kConvBlocks, std::max(1, (ex->ne + kBlock - 1) / kBlock));
+        og::k_conv_part<<<nb, kBlock, 0, h->stream>>>(da, db, dai, it, it - ex->spacing, ex->kpref, ex->gam,
+                                                      ex->el, ex->eh, ex->ne, cpart);
+        og::k_conv_fin<<<1, 64, 0, h->stream>>>(cpart, nb, conv + 2 * (size_t)it);
+      }
       if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
       double2* t = old;
       old = psi;
       psi = hv;
       hv = t;
+      if (rc == 0 && ex && (it % kCheck == 0 || it == niter)) rc = replay(it);
     }
   }
-  std::vector<double> hb(3 * (size_t)(niter + 1));
+  std::vector<double> hb(3 * n1);
   if (rc == 0) {
     if (hipMemcpyAsync(hb.data(), da, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
@@ -1034,16 +1080,42 @@ int ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double* 
   }
   dfree(v0); dfree(v1); dfree(v2);
   if (rc) return rc;
-  int done = niter;
-  for (int i = 0; i < niter; ++i)
+  for (int i = 0; i < done; ++i)
     if (hb[i] != hb[i]) { done = i + 1; rc = OCEAN_GPU_ERR_NAN; break; }
-  for (int i = 0; i <= niter; ++i) {
-    real_a[i] = hb[i];
-    real_b[i] = hb[2 * (size_t)(niter + 1) + i];
-    if (imag_a) imag_a[i] = hb[(size_t)(niter + 1) + i];
+  // a(0:done-1) and b(0:done) belong to the recursion that stopped at `done`
+  for (size_t i = 0; i < n1; ++i) {
+    real_a[i] = (int)i < done ? hb[i] : 0.0;
+    real_b[i] = (int)i <= done ? hb[2 * n1 + i] : 0.0;
+    if (imag_a) imag_a[i] = (int)i < done ? hb[n1 + i] : 0.0;
   }
   if (niter_done) *niter_done = done;
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocean_gpu_haydock(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
+                      double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
+  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
+  return haydock_run(h, psi0_re, psi0_im, niter, nullptr, real_a, real_b, imag_a, niter_done);
+}
+
+int ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
+                            int32_t spacing, double threshold, double kpref, double gam, double el, double eh,
+                            int32_t ne, double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
+  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !real_b || spacing < 1 || ne < 1)
+    return OCEAN_GPU_ERR_ARG;
+  EarlyExit ex;
+  ex.spacing = spacing;
+  ex.threshold = threshold;
+  ex.kpref = kpref;
+  ex.gam = gam;
+  ex.el = el;
+  ex.eh = eh;
+  ex.ne = ne;
+  return haydock_run(h, psi0_re, psi0_im, niter, &ex, real_a, real_b, imag_a, niter_done);
 }
 
 // ---- spectrum ----

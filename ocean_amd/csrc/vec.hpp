@@ -314,12 +314,10 @@ __device__ __forceinline__ double2 c_div(double2 a, double2 b) {
 
 __device__ __forceinline__ double f32r(double v) { return (double)(float)v; }
 
-__global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict__ rb,
-                        const double* __restrict__ ia, int n, double kpref, double gam, double el,
-                        double eh, int ne, int literal, double hartree2ev, double* __restrict__ out) {
-  const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ie > ne) return;
-  const double e = el + (eh - el) * (double)(2 * ie) / (double)(2 * ne);
+// delta(e) of the n-level fraction with its square-root terminator; the spectrum
+// is kpref |Im delta| / |delta|^2 and SPKK kpref Re delta / |delta|^2.
+__device__ double2 cf_delta(const double* __restrict__ ra, const double* __restrict__ rb,
+                            const double* __restrict__ ia, int n, double e, double gam, int literal) {
   double2 ct = make_double2(e - ra[n - 1], gam + ia[n - 1]);
   double fourbc;
   if (literal) {
@@ -346,12 +344,74 @@ __global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict_
     const double2 q = c_div(make_double2(bc, 0.0), delta);
     delta = make_double2(z.x - q.x, z.y - q.y);
   }
+  return delta;
+}
+
+__global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict__ rb,
+                        const double* __restrict__ ia, int n, double kpref, double gam, double el,
+                        double eh, int ne, int literal, double hartree2ev, double* __restrict__ out) {
+  const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ie > ne) return;
+  const double e = el + (eh - el) * (double)(2 * ie) / (double)(2 * ne);
+  const double2 delta = cf_delta(ra, rb, ia, n, e, gam, literal);
   const double dr = delta.x, di = fabs(delta.y);
   const double den = dr * dr + di * di;
   const double sp = kpref * di / den, kk = kpref * dr / den;
   double* o = out + (size_t)ie * 5;
   o[0] = hartree2ev * e;
   o[1] = sp; o[2] = sp; o[3] = kk; o[4] = kk;
+}
+
+// Early-exit test of the Haydock loop (check_convergence + calc_spect_core,
+// OCEAN2/OCEAN_haydock.f90:1608-1661, :1991-2027): the spectra of iterations n1 and
+// n2 at the ne midpoint energies e = el + (eh - el)(2 ie + 1)/(2 ne).  Partials per
+// workgroup: sum |sp1 - sp2|, sum |sp1|, sum |sp2|, max |sp1 - sp2|.
+__global__ __launch_bounds__(kBlock) void k_conv_part(const double* __restrict__ ra, const double* __restrict__ rb,
+                                                      const double* __restrict__ ia, int n1, int n2,
+                                                      double kpref, double gam, double el, double eh, int ne,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4 * 4];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int ie = blockIdx.x * blockDim.x + threadIdx.x; ie < ne; ie += gridDim.x * blockDim.x) {
+    const double e = el + (eh - el) * (double)(2 * ie + 1) / (double)(2 * ne);
+    const double2 d1 = cf_delta(ra, rb, ia, n1, e, gam, 1);
+    const double2 d2 = cf_delta(ra, rb, ia, n2, e, gam, 1);
+    const double i1 = fabs(d1.y), i2 = fabs(d2.y);
+    const double s1 = kpref * i1 / (d1.x * d1.x + i1 * i1);
+    const double s2 = kpref * i2 / (d2.x * d2.x + i2 * i2);
+    const double df = fabs(s1 - s2);
+    v[0] += df;
+    v[1] += fabs(s1);
+    v[2] += fabs(s2);
+    v[3] = fmax(v[3], df);
+  }
+  // max via the sum helper would be wrong: reduce it on its own
+  double mx = v[3];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  double s3[3] = {v[0], v[1], v[2]};
+  block_sum<3>(s3, lds);
+  __shared__ double mw[4];
+  if ((threadIdx.x & 63) == 0) mw[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mx = fmax(mx, mw[w]);
+    double* o = part + 4 * (size_t)blockIdx.x;
+    o[0] = s3[0]; o[1] = s3[1]; o[2] = s3[2]; o[3] = fmax(mw[0], mx);
+  }
+}
+
+// conv[0] = relArea = 2 sum|diff| / (sum|sp1| + sum|sp2|), conv[1] = maxDiff
+__global__ void k_conv_fin(const double* __restrict__ part, int nblk, double* __restrict__ conv) {
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0, c = 0.0, m = 0.0;
+  for (int i = 0; i < nblk; ++i) {
+    a += part[4 * i];
+    b += part[4 * i + 1];
+    c += part[4 * i + 2];
+    m = fmax(m, part[4 * i + 3]);
+  }
+  conv[0] = 2.0 * a / (b + c);
+  conv[1] = m;
 }
 
 // ---------------------------------------------------------------------------

@@ -1025,6 +1025,46 @@ def write_core(a, b, niter, kpref, gam0, el, eh, ne, imag_a=None, literal_cmplx=
     return np.stack([ener, spct, spct, spkk, spkk], axis=1)
 
 
+def calc_spect_core(a, b, n, kpref, gam0, el, eh, ne, imag_a=None):
+    """calc_spect_core (OCEAN_haydock.f90:1991-2027): the n-level fraction at the ne
+    midpoints e = el + (eh - el)(2 ie - 1)/(2 ne); rows (E_eV, SPKK, Spect).  Its
+    kind-less cmplx sites round like write_core's."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    ia = np.zeros_like(a) if imag_a is None else np.asarray(imag_a, np.float64)
+    e = el + (eh - el) * (2.0 * np.arange(1, ne + 1) - 1.0) / (2.0 * ne)
+    ctmp = (e - a[n - 1]) + 1j * (gam0 + ia[n - 1])
+    four_bc = np.float64(np.float32(np.float32(4.0 * np.float32(b[n])) * np.float32(b[n])))
+    disc = np.sqrt(ctmp ** 2 - four_bc)
+    delta = np.where(disc.imag > 0.0, (ctmp + disc) / 2.0, (ctmp - disc) / 2.0)
+    for jj in range(n - 1, -1, -1):
+        t = np.float64(np.float32(np.float32(b[jj + 1]) * np.float32(b[jj + 1])))
+        z = np.float32(e - a[jj]).astype(np.float64) + 1j * np.float64(np.float32(gam0 + ia[jj]))
+        delta = z - t / delta
+    dr = delta.real
+    di = np.abs(delta.imag)
+    return np.stack([HARTREE2EV * e, kpref * dr / (dr ** 2 + di ** 2), kpref * di / (dr ** 2 + di ** 2)])
+
+
+def haydock_early_exit(a, b, niter, spacing, threshold, kpref, gam0, el, eh, ne, imag_a=None):
+    """The stopping iteration of OCEAN_haydock_Herm_do's early exit (OCEAN_haydock.f90:272-291,
+    check_convergence :1608-1661) replayed over a finished recursion's coefficients."""
+    prev = False
+    for it in range(1, niter + 1):
+        if it <= spacing:
+            continue
+        s1 = calc_spect_core(a, b, it, kpref, gam0, el, eh, ne, imag_a)[2]
+        s2 = calc_spect_core(a, b, it - spacing, kpref, gam0, el, eh, ne, imag_a)[2]
+        rel = 2.0 * np.sum(np.abs(s1 - s2)) / (np.sum(np.abs(s1)) + np.sum(np.abs(s2)))
+        if rel < threshold:
+            if prev:
+                return it
+            prev = True
+        else:
+            prev = False
+    return niter
+
+
 def redtrid_eigs(a, b, n):
     """Eigenvalues of the (n+1) x (n+1) tridiagonal Lanczos matrix.
 

@@ -147,3 +147,37 @@ def test_gmres_small():
     assert np.abs(got[:, 2] - ref[:, 2]).max() / pk < 1e-8
     assert np.array_equal(got[:, 4], ref[:, 4])
     g.close()
+
+
+def test_haydock_early_exit(cfg0):
+    """The device loop stops where the reference's early exit would (replayed by the
+    oracle over the same coefficients) and keeps the coefficients up to there."""
+    g = _gpu(cfg0)
+    niter, spacing, ne = cfg0.params["niter"], 10, 500
+    E2H = orc.EV2HARTREE
+    gam, el, eh = 0.5 * E2H, -5.0 * E2H, 40.0 * E2H
+    a, b, ai = g.haydock(cfg0.psi0, niter)
+    rel = []
+    for it in range(spacing + 1, niter + 1):
+        s1 = orc.calc_spect_core(a, b, it, cfg0.kpref, gam, el, eh, ne, ai)[2]
+        s2 = orc.calc_spect_core(a, b, it - spacing, cfg0.kpref, gam, el, eh, ne, ai)[2]
+        rel.append(2 * np.abs(s1 - s2).sum() / (np.abs(s1).sum() + np.abs(s2).sum()))
+    rel = np.array(rel)
+    # a threshold well away from every relArea value that stops the loop mid-way
+    cands = np.sort(rel)
+    thr = None
+    for lo, hi in zip(cands[:-1], cands[1:]):
+        t = np.sqrt(lo * hi)
+        if hi / lo < 1.01:
+            continue
+        stop = orc.haydock_early_exit(a, b, niter, spacing, t, cfg0.kpref, gam, el, eh, ne, ai)
+        if spacing + 2 < stop < niter - 10:
+            thr = t
+            break
+    assert thr is not None
+    ae, be, aie, done = g.haydock_early(cfg0.psi0, niter, spacing, thr, gam, el, eh, ne)
+    print("threshold", thr, "stop", stop, "device", done, flush=True)
+    assert done == stop
+    assert np.array_equal(ae[:done], a[:done]) and np.array_equal(be[:done + 1], b[:done + 1])
+    assert not ae[done:].any() and not be[done + 1:].any()
+    g.close()

@@ -495,3 +495,33 @@ def test_spectral_grad_matches_central_differences():
     # knots beyond every distance carry no gradient
     far = inp.rtab > orc._w_geometry(inp.avec, inp.kmesh, inp.xmesh, inp.tau, inp.xshift)[0].max() + 0.1
     assert np.all(G["g_ptab"][far] == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# early exit of the Haydock loop
+# ---------------------------------------------------------------------------
+def test_calc_spect_core_is_write_core_at_midpoints():
+    """calc_spect_core (:1991-2027) is write_core's fraction (pinned by the Fe fixture)
+    on the midpoint grid e = el + (eh - el)(2 ie - 1)/(2 ne)."""
+    n, kpref, a, b, _ = load_lanc(os.path.join(GOLD, "abslanc_Fe.0001_1s_01"))
+    el, eh, ne, gam = -1.5, 1.7, 400, 0.018
+    cs = orc.calc_spect_core(a, b, n, kpref, gam, el, eh, ne)
+    h = (eh - el) / (2 * ne)
+    wc = orc.write_core(a, b, n, kpref, gam, el + h, eh - h, ne - 1)
+    assert np.allclose(cs[0], wc[:, 0], rtol=1e-13, atol=0)
+    assert np.abs(cs[2] - wc[:, 2]).max() <= 1e-9 * np.abs(wc[:, 2]).max()
+    assert np.abs(cs[1] - wc[:, 4]).max() <= 1e-9 * np.abs(wc[:, 4]).max()
+
+
+def test_early_exit_on_a_constant_chain_stops_at_the_first_pair():
+    """With a_i = a, b_i = b the square-root terminator makes every truncation the same
+    fraction, so relArea vanishes (up to the single-precision rounding of the kind-less
+    cmplx sites, ~1e-8) from the first check and the loop stops at spacing + 2."""
+    n = 60
+    a = np.full(n + 1, 0.4)
+    b = np.full(n + 1, 0.25)
+    b[0] = 0.0
+    for spacing in (3, 10):
+        assert orc.haydock_early_exit(a, b, n, spacing, 1e-6, 1.0, 0.02, -0.5, 1.5, 300) == spacing + 2
+    # and a threshold of zero never stops it
+    assert orc.haydock_early_exit(a, b, n, 5, 0.0, 1.0, 0.02, -0.5, 1.5, 300) == n
