@@ -97,8 +97,8 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
   for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
   const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
   // Batches of kB band rows: all kB loads are issued before the first use, so
-  // each wave keeps kB 1-KB row segments in flight.
-  constexpr int kB = 8;
+  // each wave keeps kB row segments in flight (8 KB per wave in both precisions).
+  constexpr int kB = sizeof(T) == 8 ? 8 : 16;
   for (int n0 = 0; n0 < nb; n0 += kB) {
     T2 uv[kB];
 #pragma unroll
@@ -694,10 +694,13 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
               const int xl = min(xb + i * XS, xc - 1);
               uv[i] = ld_stream(ub + (size_t)(x0 + xl) * nb + n);
             }
+            // one row's NA phi' values at a time: with many alphas, letting the
+            // compiler hoist the LDS reads of several rows costs occupancy
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
               const int xl = xb + i * XS;
               if (xl < xc) {
+                if constexpr (NA >= 6) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int j = 0; j < NA; ++j) cfmac(acc[j], uv[i], ph[xl * NA + j]);
               }
