@@ -248,6 +248,32 @@ int  ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, co
                              int32_t niter, double* const* real_a, double* const* real_b,
                              double* const* imag_a, int32_t* niter_done);
 
+/* write_core as OCEAN 3.2.1 writes absspct (OCEAN2/OCEAN_haydock.f90:1801-1894) and,
+ * when gam_gauss > gam0/10, write_core_Voigt (:1896-1989).  Energies in Hartree except
+ * ebase (eV, added to the printed energy as the reference does). */
+typedef struct ocean_spect_desc {
+  double  gam0;          /* Lorentzian half width (Hartree), bse.in line 4 */
+  double  el, eh;        /* energy window (Hartree) */
+  int32_t ne;            /* ne + 1 output energies */
+  double  ebase_ev;      /* energy offset of the printed energy (eV) */
+  double  gam_gauss;     /* Gaussian width sys%gaussBroaden (Hartree); 0 = off */
+  int32_t old_broaden;   /* sys%oldXASbroaden: Spect/SPKK with gamfcn broadening */
+  double  nval, eps;     /* gamfcn inputs: nval.h (bohr^-3) and epsilon */
+  double  ucvol;         /* cell volume (bohr^3) for the cross-section columns */
+  int32_t literal_cmplx; /* 1: the kind-less cmplx sites rounded to single precision */
+} ocean_spect_desc;
+/* out: (ne+1) rows of *ncols values: 9 columns (E eV, Spect, Spect(0), SPKK, SPKK(0),
+ * sigma a.u.^2, sigma cm^2, sigma^2 a.u.^5, sigma^2 cm^4 s), or 5 for the Voigt form
+ * (E, Spect, Spect(0), SPKK, SPKK(0), Spect and SPKK with gamfcn).  out holds 9*(ne+1). */
+int  ocean_gpu_write_core(const double* real_a, const double* real_b, const double* imag_a,
+                          int32_t niter, double kpref, const ocean_spect_desc* sd, double* out,
+                          int32_t* ncols);
+
+/* Eigenvalues, ascending, of the (n+1) x (n+1) tridiagonal Lanczos matrix with
+ * diagonal real_a(0:n) and off-diagonals real_b(1:n): the block redtrid appends to
+ * abslanc (OCEAN2/OCEAN_haydock.f90:2273-2316, :2354-2356).  w has n+1 entries. */
+int  ocean_gpu_tridiag_eigs(const double* real_a, const double* real_b, int32_t n, double* w);
+
 /* Average durations (ms) measured with HIP events on the library's stream:
  * "total" from the last ocean_gpu_xact_vec_repeat, "proj", "fft", "mult", "adj"
  * from the last ocean_gpu_profile_xact. */

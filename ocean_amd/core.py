@@ -86,6 +86,8 @@ _SIGS = {
                                   C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
+    "ocean_gpu_write_core": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP]),
+    "ocean_gpu_tridiag_eigs": (C.c_int, [_DP, _DP, C.c_int32, _DP]),
     "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
     "ocean_gpu_load_bloch_dev": (C.c_int, [_H, C.c_void_p, C.c_void_p]),
     "ocean_gpu_spectral_grad": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
@@ -150,6 +152,42 @@ def spectrum(real_a, real_b, niter, kpref, gam, el, eh, ne, imag_a=None, literal
     _chk(lib.ocean_gpu_spectrum(_p(ra), _p(rb), _p(ia), niter, kpref, gam, el, eh, ne,
                                 1 if literal_cmplx else 0, out.ctypes.data_as(_DP)), "spectrum")
     return out
+
+
+class SpectDesc(C.Structure):
+    """ocean_spect_desc of include/ocean_gpu.h."""
+    _fields_ = [("gam0", C.c_double), ("el", C.c_double), ("eh", C.c_double), ("ne", C.c_int32),
+                ("ebase_ev", C.c_double), ("gam_gauss", C.c_double), ("old_broaden", C.c_int32),
+                ("nval", C.c_double), ("eps", C.c_double), ("ucvol", C.c_double),
+                ("literal_cmplx", C.c_int32)]
+
+
+def write_core(real_a, real_b, niter, kpref, gam0, el, eh, ne, ebase_eV=0.0, old_broaden=False,
+               nval=0.0, eps=1.0, ucvol=1.0, gam_gauss=0.0, imag_a=None, literal_cmplx=True):
+    """write_core (nine columns, or the five of its Voigt form) on the GPU."""
+    lib = _lib()
+    ra = np.ascontiguousarray(real_a, np.float64)
+    rb = np.ascontiguousarray(real_b, np.float64)
+    ia = None if imag_a is None else np.ascontiguousarray(imag_a, np.float64)
+    sd = SpectDesc(gam0, el, eh, ne, ebase_eV, gam_gauss, 1 if old_broaden else 0, nval, eps, ucvol,
+                   1 if literal_cmplx else 0)
+    out = np.zeros(9 * (ne + 1))
+    ncols = C.c_int32(0)
+    _chk(lib.ocean_gpu_write_core(_p(ra), _p(rb), _p(ia), niter, kpref, C.byref(sd),
+                                  out.ctypes.data_as(_DP), C.byref(ncols)), "write_core")
+    nc = ncols.value
+    return out[:nc * (ne + 1)].reshape(ne + 1, nc)
+
+
+def tridiag_eigs(real_a, real_b, n):
+    """Ascending eigenvalues of the (n+1)-square Lanczos matrix (a(0:n), b(1:n))."""
+    lib = _lib()
+    ra = np.ascontiguousarray(real_a, np.float64)
+    rb = np.ascontiguousarray(real_b, np.float64)
+    assert ra.size >= n + 1 and rb.size >= n + 1
+    w = np.zeros(n + 1)
+    _chk(lib.ocean_gpu_tridiag_eigs(_p(ra), _p(rb), n, w.ctypes.data_as(_DP)), "tridiag_eigs")
+    return w
 
 
 class OceanGPU:

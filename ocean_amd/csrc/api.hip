@@ -16,6 +16,10 @@ namespace {
 
 constexpr double kEV2Hartree = 0.036749322175666;   // OCEAN2/OCEAN_constants.f90:15
 constexpr double kHartree2eV = 27.211386245980;     // OCEAN2/OCEAN_constants.f90:16
+constexpr double kAlpha = 0.0072973525643;           // OCEAN2/OCEAN_constants.f90:18
+constexpr double kBohr = 0.529177210544;             // OCEAN2/OCEAN_constants.f90:20 (Angstrom)
+constexpr double kAu2sec = 2.4188843265864e-17;      // OCEAN2/OCEAN_constants.f90:22
+constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;  // :29
 
 int grid_for(size_t n) {
   size_t g = (n + kBlock - 1) / kBlock;
@@ -1116,6 +1120,93 @@ int ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const do
   ex.eh = eh;
   ex.ne = ne;
   return haydock_run(h, psi0_re, psi0_im, niter, &ex, real_a, real_b, imag_a, niter_done);
+}
+
+// ---- spectrum, OCEAN 3.2.1 write_core (nine columns / Voigt) ----
+int ocean_gpu_write_core(const double* real_a, const double* real_b, const double* imag_a, int32_t niter,
+                         double kpref, const ocean_spect_desc* sd, double* out, int32_t* ncols) {
+  if (!real_a || !real_b || !out || !sd || niter <= 0 || sd->ne <= 0 || !(sd->eh > sd->el)) return OCEAN_GPU_ERR_ARG;
+  const int ne = sd->ne;
+  const size_t n1 = (size_t)niter + 1;
+  const bool voigt = sd->gam_gauss > sd->gam0 / 10.0;
+  const int nc = voigt ? 5 : 9;
+  if (ncols) *ncols = nc;
+  // write_core_Voigt's fine grid (OCEAN_haydock.f90:1910-1917), in its order
+  double estep = 0.0, prefactor = 0.0, inv2s2 = 0.0;
+  int nratio = 1, nstep = 0;
+  if (voigt) {
+    const double de = (sd->eh - sd->el) / (double)ne;
+    estep = std::min(sd->gam0, sd->gam_gauss) / 2.0;
+    nratio = (int)((de - estep) / estep) + 1;
+    estep = de / (double)nratio;
+    nstep = (int)((2.99999 * sd->gam_gauss) / estep) + 1;
+    prefactor = estep / (sd->gam_gauss * 2.506628274631);
+    inv2s2 = 1.0 / (2.0 * sd->gam_gauss * sd->gam_gauss);
+    // a grid coarser than six widths would shift past the reference's fine array
+    if (nratio > 2 * nstep || nratio < 1) return OCEAN_GPU_ERR_ARG;
+  }
+  double* d = nullptr;
+  OG_TRY(dmalloc(&d, 3 * n1 + (size_t)nc * (ne + 1)));
+  std::vector<double> ia(n1, 0.0);
+  if (imag_a) std::copy(imag_a, imag_a + n1, ia.begin());
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(d, real_a, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + n1, real_b, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + 2 * n1, ia.data(), n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    if (voigt)
+      og::k_write_core_voigt<<<(ne + 1 + 63) / 64, 64>>>(d, d + n1, d + 2 * n1, niter, kpref, sd->gam0, sd->el, sd->eh,
+                                                         ne, sd->ebase_ev, sd->nval, sd->eps, sd->literal_cmplx,
+                                                         estep, nratio, nstep, prefactor, inv2s2, kHartree2eV,
+                                                         d + 3 * n1);
+    else
+      og::k_write_core<<<(ne + 1 + 127) / 128, 128>>>(d, d + n1, d + 2 * n1, niter, kpref, sd->gam0, sd->el, sd->eh,
+                                                      ne, sd->ebase_ev, sd->old_broaden, sd->nval, sd->eps,
+                                                      sd->ucvol, sd->literal_cmplx,
+                                                      og::SpectConst{kHartree2eV, kEV2Hartree, kAlpha, kBohr, kPi,
+                                                                     kAu2sec},
+                                                      d + 3 * n1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(out, d + 3 * n1, (size_t)nc * (ne + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(d);
+  return rc;
+}
+
+// ---- Lanczos eigenvalues (redtrid) ----
+int ocean_gpu_tridiag_eigs(const double* real_a, const double* real_b, int32_t n, double* w) {
+  if (!real_a || !real_b || !w || n < 0) return OCEAN_GPU_ERR_ARG;
+  const int m = n + 1;
+  // b(1:n) couple rows i-1 and i; b(0) is unused
+  std::vector<double> bb((size_t)m, 0.0);
+  for (int i = 1; i < m; ++i) bb[i] = real_b[i];
+  double lo = real_a[0], hi = real_a[0], amax = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(bb[i]) : 0.0) + (i + 1 < m ? std::fabs(bb[i + 1]) : 0.0);
+    lo = std::min(lo, real_a[i] - r);
+    hi = std::max(hi, real_a[i] + r);
+    amax = std::max(amax, std::max(std::fabs(real_a[i]), std::fabs(bb[i])));
+  }
+  const double span = std::max(hi - lo, 1e-300);
+  lo -= 1e-14 * span;
+  hi += 1e-14 * span;
+  const double pivmin = std::max(amax * amax, 1e-300) * 2.2250738585072014e-308 / 1.1102230246251565e-16;
+  double* d = nullptr;
+  OG_TRY(dmalloc(&d, 3 * (size_t)m));
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(d, real_a, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + m, bb.data(), m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_tridiag_eigs<<<(m + 63) / 64, 64>>>(d, d + m, m, lo, hi, pivmin, d + 2 * m);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(w, d + 2 * m, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(d);
+  return rc;
 }
 
 // ---- spectrum ----

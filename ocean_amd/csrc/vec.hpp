@@ -362,6 +362,137 @@ __global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict_
   o[1] = sp; o[2] = sp; o[3] = kk; o[4] = kk;
 }
 
+// Plasmon-pole lifetime broadening (CNBSE/zcmsnmult/zf90/gamfcn.f90:8-42): energy e
+// (Hartree) above the Fermi level, valence density n (bohr^-3), dielectric e0.
+__device__ double gamfcn_dev(double e, double n, double e0) {
+  const double pi = 3.14159265358979;
+  const double kf = pow(3.0 * pi * pi * n, 1.0 / 3.0);
+  const double wp = sqrt(4.0 * pi * n);
+  double k = kf * kf / 2.0 + e;
+  if (k < kf * kf / 2.0) return 0.0;
+  k = sqrt(2.0 * k);
+  const double wf = kf * kf / 2.0;
+  const double lam = sqrt(wp * wp / (wf * wf * (e0 - 1.0)));
+  const double dq = 0.005 * kf;
+  double q = 0.5 * dq, su = 0.0;
+  bool test1 = true;
+  while (test1) {
+    const double qq = q / kf;
+    const double xp = qq * (2.0 + qq) / lam, xm = qq * (2.0 - qq) / lam;
+    const double q3 = qq * qq * qq, q5 = q3 * qq * qq;
+    const double term1 = lam / (2.0 * q3) * (atan(xp) + atan(xm));
+    const double term2 = (0.125 * lam * lam / q5 + 0.5 / q3 - 0.125 / qq) * log((1.0 + xp * xp) / (1.0 + xm * xm));
+    const double epsq = 1.0 + 2.0 / (pi * kf) * (1.0 / (qq * qq) - term1 + term2);
+    const double wq = wp / sqrt(1.0 - 1.0 / epsq);
+    test1 = k * k - kf * kf - 2.0 * wq > 0.0;
+    const bool test2 = k * q - wq - q * q / 2.0 > 0.0;
+    if (test1 && test2) su += dq / (q * wq);
+    q += dq;
+  }
+  return wp * wp * su / (2.0 * k);
+}
+
+// Spectrum rows as OCEAN 3.2.1's write_core writes them (OCEAN_haydock.f90:1801-1894),
+// one thread per energy: (E, Spect, Spect(0), SPKK, SPKK(0), sigma a.u.^2, sigma cm^2,
+// sigma^2 a.u.^5, sigma^2 cm^4 s).  Spect/SPKK carry gamfcn when old_broaden.
+struct SpectConst {
+  double hartree2ev, ev2hartree, alpha, bohr, pi, au2sec;
+};
+__global__ void k_write_core(const double* __restrict__ ra, const double* __restrict__ rb,
+                             const double* __restrict__ ia, int n, double kpref, double gam0, double el,
+                             double eh, int ne, double ebase_ev, int old_broaden, double nval, double eps,
+                             double ucvol, int literal, SpectConst K, double* __restrict__ out) {
+  const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ie > ne) return;
+  const double e = el + (eh - el) * (double)(2 * ie) / (double)(2 * ne);
+  double spct[2], spkk[2];
+  for (int jd = 0; jd <= (old_broaden ? 1 : 0); ++jd) {
+    const double gam = gam0 + (jd ? gamfcn_dev(e, nval, eps) : 0.0);
+    const double2 d = cf_delta(ra, rb, ia, n, e, gam, literal);
+    const double dr = d.x, di = fabs(d.y), den = dr * dr + di * di;
+    spct[jd] = kpref * di / den;
+    spkk[jd] = kpref * dr / den;
+  }
+  if (!old_broaden) { spct[1] = spct[0]; spkk[1] = spkk[0]; }
+  const double ener = ebase_ev + K.hartree2ev * e;
+  const double a_by_v = K.ev2hartree * K.alpha * ucvol;
+  const double bsq = K.bohr * K.bohr * 1.e-16;
+  const double tpa = (K.alpha * K.alpha * 2.0 * K.pi) * ucvol;
+  const double tpacm4 = tpa * bsq * bsq * K.au2sec;
+  const double sigma = a_by_v * ener * spct[0];
+  double* o = out + (size_t)ie * 9;
+  o[0] = ener; o[1] = spct[1]; o[2] = spct[0]; o[3] = spkk[1]; o[4] = spkk[0];
+  o[5] = sigma; o[6] = sigma * bsq; o[7] = spct[0] * tpa; o[8] = spct[0] * tpacm4;
+}
+
+// write_core_Voigt (OCEAN_haydock.f90:1896-1989), one thread per output point p.  The
+// reference walks a fine grid of step estep around each output energy and reuses it,
+// shifted by nratio, for the next point: the value at offset ii of point p was last
+// computed at the point q whose freshly computed window [nstep - nratio, nstep] (all
+// of [-nstep, nstep] at q = 0) held offset o = ii + (p - q) nratio.  Each thread
+// evaluates exactly those (q, o) energies and sums in the reference's ii order, so
+// the rows equal the sequential reuse.  The printed energy is the last fine energy
+// of the point's loop, ee_p + nstep estep, as the reference prints it.
+__global__ void k_write_core_voigt(const double* __restrict__ ra, const double* __restrict__ rb,
+                                   const double* __restrict__ ia, int n, double kpref, double gam0,
+                                   double el, double eh, int ne, double ebase_ev, double nval, double eps,
+                                   int literal, double estep, int nratio, int nstep, double prefactor,
+                                   double inv2s2, double hartree2ev, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > ne) return;
+  double s[2] = {0.0, 0.0}, k[2] = {0.0, 0.0};
+  for (int ii = -nstep; ii <= nstep; ++ii) {
+    int d = 0;
+    if (nstep - nratio - ii > 0) d = (nstep - nratio - ii + nratio - 1) / nratio;
+    int q = p - d, o = ii + d * nratio;
+    if (q < 1) { q = 0; o = ii + p * nratio; }
+    const double eeq = el + (eh - el) * (double)(2 * q) / (double)(2 * ne);
+    const double e = eeq + (double)o * estep;
+    const double pref = exp(-(double)(ii * ii) * (estep * estep) * inv2s2);
+    for (int jd = 0; jd <= 1; ++jd) {
+      const double gam = gam0 + (jd ? gamfcn_dev(e, nval, eps) : 0.0);
+      const double2 dl = cf_delta(ra, rb, ia, n, e, gam, literal);
+      const double dr = dl.x, di = fabs(dl.y), den = dr * dr + di * di;
+      s[jd] += pref * (prefactor * kpref * di / den);
+      k[jd] += pref * (prefactor * kpref * dr / den);
+    }
+  }
+  const double ee = el + (eh - el) * (double)(2 * p) / (double)(2 * ne);
+  double* o5 = out + (size_t)p * 5;
+  o5[0] = ebase_ev + hartree2ev * (ee + (double)nstep * estep);
+  o5[1] = s[1]; o5[2] = s[0]; o5[3] = k[1]; o5[4] = k[0];
+}
+
+// Eigenvalues of the (n+1) x (n+1) Lanczos matrix redtrid builds (diagonal a(0:n),
+// off-diagonals b(1:n); OCEAN_haydock.f90:2273-2316, solved there by EISPACK's elsch),
+// by Sturm-sequence bisection: thread j brackets the j-th smallest eigenvalue inside
+// the Gershgorin interval until the bracket stops shrinking in FP64.
+__device__ int sturm_below(const double* __restrict__ a, const double* __restrict__ b, int m, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    q = (a[i] - x) - b[i] * b[i] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+__global__ void k_tridiag_eigs(const double* __restrict__ a, const double* __restrict__ b, int m, double lo0,
+                               double hi0, double pivmin, double* __restrict__ w) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double lo = lo0, hi = hi0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (!(mid > lo && mid < hi)) break;  // the bracket is two adjacent doubles
+    if (sturm_below(a, b, m, mid, pivmin) > j) hi = mid; else lo = mid;
+  }
+  w[j] = 0.5 * (lo + hi);
+}
+
 // Early-exit test of the Haydock loop (check_convergence + calc_spect_core,
 // OCEAN2/OCEAN_haydock.f90:1608-1661, :1991-2027): the spectra of iterations n1 and
 // n2 at the ne midpoint energies e = el + (eh - el)(2 ie + 1)/(2 ne).  Partials per

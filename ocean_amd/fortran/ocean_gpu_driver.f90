@@ -83,10 +83,10 @@ program ocean_gpu_driver
     call OCEAN_haydock_do_gpu(p_re, p_im, niter, ra, rb, ia, done, ierr)
     if (ierr == 0) then
       open(unit=12, file=trim(prefix)//'.absspct', form='formatted', status='unknown')
-      call write_core_gpu(12, niter, sc(3), sc(4), sc(5), sc(6), ne, ra, rb, ia, ierr)
+      call write_core_gpu(12, niter, sc(3), sc(4), sc(5), sc(6), ne, ra, rb, ia, ierr, ucvol=sc(1))
       close(12)
       open(unit=13, file=trim(prefix)//'.abslanc', form='formatted', status='unknown')
-      call write_lanczos_gpu(13, niter, sc(3), ra, rb)
+      if (ierr == 0) call write_lanczos_gpu(13, niter, sc(3), ra, rb, .true., ierr)
       close(13)
     end if
   end if

@@ -111,31 +111,77 @@ contains
     end if
   end subroutine OCEAN_gmres_do_recycle_gpu
 
-  ! absspct writer in write_core's layout (Lorentzian, gaussian broadening off).
-  subroutine write_core_gpu(fh, niter, kpref, gam0, el, eh, ne, real_a, real_b, imag_a, ierr)
+  ! absspct writer (write_core, OCEAN_haydock.f90:1801-1894): nine columns, or the five
+  ! of write_core_Voigt (:1896-1989) when gam_gauss > gam0/10.  Optional arguments
+  ! default to sys%ebase = 0, oldXASbroaden off, gaussBroaden off, ucVol = 1.
+  subroutine write_core_gpu(fh, niter, kpref, gam0, el, eh, ne, real_a, real_b, imag_a, ierr, &
+                            ebase, old_broaden, nval, eps, ucvol, gam_gauss)
     integer, intent(in) :: fh, niter, ne
     real(DP), intent(in) :: kpref, gam0, el, eh
     real(DP), intent(in) :: real_a(0:niter), real_b(0:niter), imag_a(0:niter)
     integer, intent(inout) :: ierr
+    real(DP), intent(in), optional :: ebase, nval, eps, ucvol, gam_gauss
+    logical, intent(in), optional :: old_broaden
+    type(ocean_spect_desc) :: sd
     real(DP), allocatable :: out(:, :)
+    integer(c_int32_t) :: nc
     integer :: ie
-    allocate(out(5, 0:ne))
-    ierr = ocean_gpu_spectrum(real_a, real_b, imag_a, int(niter, c_int32_t), kpref, gam0, el, eh, &
-                              int(ne, c_int32_t), 1_c_int32_t, out)
+    sd%gam0 = gam0
+    sd%el = el
+    sd%eh = eh
+    sd%ne = int(ne, c_int32_t)
+    sd%ebase_ev = 0.0_DP
+    if (present(ebase)) sd%ebase_ev = ebase
+    sd%gam_gauss = 0.0_DP
+    if (present(gam_gauss)) sd%gam_gauss = gam_gauss
+    sd%old_broaden = 0
+    if (present(old_broaden)) sd%old_broaden = merge(1, 0, old_broaden)
+    sd%nval = 0.0_DP
+    if (present(nval)) sd%nval = nval
+    sd%eps = 1.0_DP
+    if (present(eps)) sd%eps = eps
+    sd%ucvol = 1.0_DP
+    if (present(ucvol)) sd%ucvol = ucvol
+    sd%literal_cmplx = 1
+    allocate(out(9, 0:ne))
+    ierr = ocean_gpu_write_core(real_a, real_b, imag_a, int(niter, c_int32_t), kpref, sd, out, nc)
     if (ierr /= 0) return
-    write(fh, '(A,1i5,A,1e15.8,A,1e15.8)') '#   iter=', niter, '   gam=', gam0, '   kpref=', kpref
-    write(fh, '(5(A15,1x))') '#   Energy', 'Spect', 'Spect(0)', 'SPKK', 'SPKK(0)'
-    do ie = 0, ne
-      write(fh, '(5(1e15.8,1x))') out(1, ie), out(2, ie), out(3, ie), out(4, ie), out(5, ie)
-    end do
+    if (nc == 9) then
+      write(fh, '(A,1i5,A,1e15.8,A,1e15.8,A,1e15.8)') '#   iter=', niter, '   gam=', gam0, '   kpref=', kpref, &
+            'vol=', sd%ucvol
+      write(fh, '(9(A15,1x))') '#   Energy', 'Spect', 'Spect(0)', 'SPKK', 'SPKK(0)', 'sigma (a.u.^2)', &
+            'sigma( cm^2 )', 'sig.^2 (a.u.^5)', 'sig.^2 cm^4 sec'
+    else
+      write(fh, '(A,1i5,A,1e15.8,A,1e15.8)') '#   iter=', niter, '   gam=', gam0, '   kpref=', kpref
+      write(fh, '(5(A15,1x))') '#   Energy', 'Spect', 'Spect(0)', 'SPKK', 'SPKK(0)'
+    end if
+    ! the C side packs rows of nc values contiguously
+    call emit(out, nc)
     deallocate(out)
+  contains
+    subroutine emit(buf, ncol)
+      real(DP), intent(in) :: buf(*)
+      integer(c_int32_t), intent(in) :: ncol
+      integer :: r
+      do r = 0, ne
+        if (ncol == 9) then
+          write(fh, '(9(1e15.8,1x))') buf(9 * r + 1:9 * r + 9)
+        else
+          write(fh, '(5(1e15.8,1x))') buf(5 * r + 1:5 * r + 5)
+        end if
+      end do
+    end subroutine emit
   end subroutine write_core_gpu
 
-  ! abslanc writer (write_lanczos without the eigenvalue block).
-  subroutine write_lanczos_gpu(fh, n, kpref, real_a, real_b)
+  ! abslanc writer (write_lanczos, OCEAN_haydock.f90:2318-2358); with_eigs appends the
+  ! eigenvalue block redtrid passes in (:2273-2316), computed on the device.
+  subroutine write_lanczos_gpu(fh, n, kpref, real_a, real_b, with_eigs, ierr)
     integer, intent(in) :: fh, n
     real(DP), intent(in) :: kpref, real_a(0:n), real_b(0:n)
-    integer :: i
+    logical, intent(in), optional :: with_eigs
+    integer, intent(inout), optional :: ierr
+    real(DP), allocatable :: w(:)
+    integer :: i, rc
     write(fh, '(1i8,1x,1ES24.17)') n, kpref
     do i = 0, n
       if (i == 0) then
@@ -144,6 +190,15 @@ contains
         write(fh, '(2(2x,ES24.17))') real_a(i), real_b(i)
       end if
     end do
+    if (present(with_eigs)) then
+      if (with_eigs) then
+        allocate(w(n + 1))
+        rc = ocean_gpu_tridiag_eigs(real_a, real_b, int(n, c_int32_t), w)
+        if (present(ierr)) ierr = rc
+        if (rc == 0) write(fh, '(2x,2i5,1f20.10)') (i, n + 1, w(i), i = 1, n + 1)
+        deallocate(w)
+      end if
+    end if
   end subroutine write_lanczos_gpu
 
   subroutine OCEAN_gpu_finalize(ierr)

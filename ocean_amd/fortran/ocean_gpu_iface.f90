@@ -33,6 +33,21 @@ module ocean_gpu_iface
     integer(c_int32_t) :: precision
   end type ocean_bse_desc
 
+  ! ocean_spect_desc of ocean_gpu.h: write_core's inputs beyond the coefficients
+  type, bind(C) :: ocean_spect_desc
+    real(c_double) :: gam0
+    real(c_double) :: el
+    real(c_double) :: eh
+    integer(c_int32_t) :: ne
+    real(c_double) :: ebase_ev
+    real(c_double) :: gam_gauss
+    integer(c_int32_t) :: old_broaden
+    real(c_double) :: nval
+    real(c_double) :: eps
+    real(c_double) :: ucvol
+    integer(c_int32_t) :: literal_cmplx
+  end type ocean_spect_desc
+
   interface
     integer(c_int) function ocean_gpu_create(desc, h) bind(C, name='ocean_gpu_create')
       import :: c_int, c_ptr, ocean_bse_desc
@@ -119,6 +134,24 @@ module ocean_gpu_iface
       real(c_double), value :: kpref, gam, el, eh
       integer(c_int32_t), value :: ne, literal_cmplx
       real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_int) function ocean_gpu_write_core(real_a, real_b, imag_a, niter, kpref, sd, out, ncols) &
+        bind(C, name='ocean_gpu_write_core')
+      import :: c_int, c_double, c_int32_t, ocean_spect_desc
+      real(c_double), intent(in) :: real_a(*), real_b(*), imag_a(*)
+      integer(c_int32_t), value :: niter
+      real(c_double), value :: kpref
+      type(ocean_spect_desc), intent(in) :: sd
+      real(c_double), intent(out) :: out(*)
+      integer(c_int32_t), intent(out) :: ncols
+    end function
+
+    integer(c_int) function ocean_gpu_tridiag_eigs(real_a, real_b, n, w) bind(C, name='ocean_gpu_tridiag_eigs')
+      import :: c_int, c_double, c_int32_t
+      real(c_double), intent(in) :: real_a(*), real_b(*)
+      integer(c_int32_t), value :: n
+      real(c_double), intent(out) :: w(*)
     end function
 
     ! ---- batches (many sites / edges / polarisations at once) ----

@@ -22,6 +22,13 @@ spin fastest (OCEAN_multiplet.f90:343-355).  Energies are in Hartree.
 
 Pinning (what checks each function; "parity unpinned" where none does):
   write_core           TEST/Fe/Results abslanc->absspct fixture, Lorentzian closed form
+  write_core_full      same fixture (Spect/SPKK columns with gamfcn at the Fe run's own
+                       valence density 16/acell^3 and diemac 1e5), cross-section columns
+                       against the CODATA products; Voigt form against scipy's
+                       voigt_profile for a one-level fraction (within the +-3 sigma
+                       truncation) and its fine-grid reuse against direct evaluation
+  gamfcn               the same fixture (its only consumer), zero below the Fermi level
+  calc_spect_core      write_core at the midpoint grid
   redtrid              TEST/Fe/Results abslanc eigenvalue block
   haydock              dense Lanczos/eigendecomposition brute force (tests/test_oracle.py)
   lr_act               dense operator brute force, loop-literal lr_kernel restatement
@@ -1023,6 +1030,147 @@ def write_core(a, b, niter, kpref, gam0, el, eh, ne, imag_a=None, literal_cmplx=
     ener = ebase + hartree2ev * e
     del f32
     return np.stack([ener, spct, spct, spkk, spkk], axis=1)
+
+
+# OCEAN_constants.f90:17-29 (2022 CODATA)
+ALPHA = 0.0072973525643
+BOHR = 0.529177210544
+AU2SEC = 2.4188843265864e-17
+PI_DP = 3.1415926535897932384626433832795028841971693993751
+
+
+def gamfcn(e, n, e0):
+    """Plasmon-pole lifetime broadening of an excited electron at energy e (Hartree)
+    in a gas of density n (bohr^-3) with dielectric constant e0.
+
+    CNBSE/zcmsnmult/zf90/gamfcn.f90:8-42 (q stepped by kf/200 until the plasmon
+    threshold test fails).
+    """
+    pi = 3.14159265358979
+    kf = (3.0 * pi * pi * n) ** (1.0 / 3.0)
+    wp = math.sqrt(4.0 * pi * n)
+    k = kf ** 2 / 2.0 + e
+    if k < kf ** 2 / 2.0:
+        return 0.0
+    k = math.sqrt(2.0 * k)
+    wf = kf ** 2 / 2.0
+    lam = math.sqrt(wp ** 2 / (wf ** 2 * (e0 - 1.0)))
+    dq = 0.005 * kf
+    q = 0.5 * dq
+    su = 0.0
+    test1 = True
+    while test1:
+        qq = q / kf
+        xp = qq * (2.0 + qq) / lam
+        xm = qq * (2.0 - qq) / lam
+        term1 = lam / (2.0 * qq ** 3) * (math.atan(xp) + math.atan(xm))
+        term2 = (0.125 * lam ** 2 / qq ** 5 + 0.5 / qq ** 3 - 0.125 / qq) * math.log((1.0 + xp ** 2) / (1.0 + xm ** 2))
+        epsq = 1.0 + 2.0 / (pi * kf) * (1.0 / qq ** 2 - term1 + term2)
+        wq = wp / math.sqrt(1.0 - 1.0 / epsq)
+        test1 = k ** 2 - kf ** 2 - 2.0 * wq > 0.0
+        test2 = k * q - wq - q ** 2 / 2.0 > 0.0
+        if test1 and test2:
+            su = su + dq / (q * wq)
+        q = q + dq
+    return wp * wp * su / (2.0 * k)
+
+
+def _cf_delta(a, b, ia, n, e, gam, literal):
+    """delta(e) of write_core's fraction (one energy array, one gamma array)."""
+    ctmp = (e - a[n - 1]) + 1j * (gam + ia[n - 1])
+    if literal:
+        four_bc = np.float64(np.float32(np.float32(4.0 * np.float32(b[n])) * np.float32(b[n])))
+    else:
+        four_bc = 4.0 * b[n] * b[n]
+    disc = np.sqrt(ctmp ** 2 - four_bc)
+    delta = np.where(disc.imag > 0.0, (ctmp + disc) / 2.0, (ctmp - disc) / 2.0)
+    for jj in range(n - 1, -1, -1):
+        if literal:
+            t = np.float64(np.float32(np.float32(b[jj + 1]) * np.float32(b[jj + 1])))
+            z = np.float32(e - a[jj]).astype(np.float64) + 1j * np.float32(gam + ia[jj]).astype(np.float64)
+        else:
+            t = b[jj + 1] * b[jj + 1]
+            z = (e - a[jj]) + 1j * (gam + ia[jj])
+        delta = z - t / delta
+    return delta
+
+
+def write_core_full(a, b, niter, kpref, gam0, el, eh, ne, ebase_eV=0.0, old_broaden=False, nval=0.0,
+                    eps=1.0, ucvol=1.0, gam_gauss=0.0, imag_a=None, literal_cmplx=True,
+                    hartree2ev=HARTREE2EV):
+    """write_core as OCEAN 3.2.1 writes it (OCEAN_haydock.f90:1801-1894): the nine
+    columns (E, Spect, Spect(0), SPKK, SPKK(0), sigma a.u.^2, sigma cm^2, two-photon
+    a.u.^5 and cm^4 s), Spect/SPKK with gamfcn broadening when oldXASbroaden, and the
+    Voigt variant write_core_Voigt (:1896-1989) when gamGauss > gam0/10 (five
+    columns, its fine-grid reuse and printed-energy convention included).
+    """
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    ia = np.zeros_like(a) if imag_a is None else np.asarray(imag_a, np.float64)
+    n = niter
+    if gam_gauss > gam0 / 10.0:
+        return _write_core_voigt(a, b, ia, n, kpref, gam0, el, eh, ne, ebase_eV, nval, eps, gam_gauss,
+                                 literal_cmplx, hartree2ev)
+    e = el + (eh - el) * np.arange(0, 2 * ne + 1, 2, dtype=np.float64) / (2 * ne)
+    d0 = _cf_delta(a, b, ia, n, e, np.full_like(e, gam0), literal_cmplx)
+    spct0 = kpref * np.abs(d0.imag) / (d0.real ** 2 + d0.imag ** 2)
+    spkk0 = kpref * d0.real / (d0.real ** 2 + d0.imag ** 2)
+    if old_broaden:
+        g1 = gam0 + np.array([gamfcn(v, nval, eps) for v in e])
+        d1 = _cf_delta(a, b, ia, n, e, g1, literal_cmplx)
+        spct1 = kpref * np.abs(d1.imag) / (d1.real ** 2 + d1.imag ** 2)
+        spkk1 = kpref * d1.real / (d1.real ** 2 + d1.imag ** 2)
+    else:
+        spct1, spkk1 = spct0, spkk0
+    ener = ebase_eV + hartree2ev * e
+    a_by_v = EV2HARTREE * ALPHA * ucvol
+    bohr_sq = BOHR * BOHR * 1e-16
+    tpa = (ALPHA * ALPHA * 2.0 * PI_DP) * ucvol
+    tpacm4 = tpa * bohr_sq * bohr_sq * AU2SEC
+    sigma = a_by_v * ener * spct0
+    return np.stack([ener, spct1, spct0, spkk1, spkk0, sigma, sigma * bohr_sq, spct0 * tpa, spct0 * tpacm4],
+                    axis=1)
+
+
+def _write_core_voigt(a, b, ia, n, kpref, gam0, el, eh, ne, ebase_eV, nval, eps, gam_gauss, literal,
+                      hartree2ev=HARTREE2EV):
+    """write_core_Voigt line by line: the fine grid of step estep around each output
+    energy, reused (shifted by nratio) from one output point to the next."""
+    de = (eh - el) / float(ne)
+    estep = min(gam0, gam_gauss) / 2.0
+    nratio = int((de - estep) / estep) + 1
+    estep = de / float(nratio)
+    nstep = int((2.99999 * gam_gauss) / estep) + 1
+    prefactor = estep / (gam_gauss * 2.506628274631)
+    inv2s2 = 1.0 / (2.0 * gam_gauss * gam_gauss)
+    sa = np.zeros((2 * nstep + 1, 2))
+    sk = np.zeros((2 * nstep + 1, 2))
+    rows = []
+    e = 0.0
+    for ie in range(0, 2 * ne + 1, 2):
+        ee = el + (eh - el) * float(ie) / float(2 * ne)
+        if ie == 0:
+            istart = -nstep
+        else:
+            istart = nstep - nratio
+            sa[0:2 * nstep + 1 - nratio] = sa[nratio:2 * nstep + 1].copy()
+            sk[0:2 * nstep + 1 - nratio] = sk[nratio:2 * nstep + 1].copy()
+        for ii in range(istart, nstep + 1):
+            e = ee + float(ii) * estep
+            for jd in (0, 1):
+                g = gam0 + gamfcn(e, nval, eps) * jd
+                d = _cf_delta(a, b, ia, n, np.array([e]), np.array([g]), literal)[0]
+                dr, di = d.real, abs(d.imag)
+                sa[ii + nstep, jd] = prefactor * kpref * di / (dr ** 2 + di ** 2)
+                sk[ii + nstep, jd] = prefactor * kpref * dr / (dr ** 2 + di ** 2)
+        s = np.zeros(2)
+        k = np.zeros(2)
+        for ii in range(-nstep, nstep + 1):
+            pref = math.exp(-float(ii ** 2) * estep ** 2 * inv2s2)
+            s += pref * sa[ii + nstep]
+            k += pref * sk[ii + nstep]
+        rows.append([ebase_eV + hartree2ev * e, s[1], s[0], k[1], k[0]])
+    return np.array(rows)
 
 
 def calc_spect_core(a, b, n, kpref, gam0, el, eh, ne, imag_a=None):

@@ -36,11 +36,15 @@ def test_fortran_host_xact_haydock_gmres(tmp_path):
     assert relerr(h, orc.xact(P, inp.psi0)) < 1e-12
     ra, rb, ia, _ = orc.haydock(P, inp.psi0, niter)
     assert np.max(np.abs(a[:niter] - ra[:niter]) / np.abs(ra[:niter])) < 1e-10
-    sp = orc.write_core(ra, rb, niter, inp.kpref, gam, el, eh, ne)
+    sp = orc.write_core_full(ra, rb, niter, inp.kpref, gam, el, eh, ne, ucvol=inp.celvol)
     txt = np.loadtxt(tmp_path / "out.absspct", comments="#")
-    assert txt.shape == (ne + 1, 5)
-    assert np.abs(txt[:, 2] - sp[:, 2]).max() / np.abs(sp[:, 2]).max() < 1e-7  # 8 printed digits
-    lanc = open(tmp_path / "out.abslanc").read().split()
-    assert int(lanc[0]) == niter
+    assert txt.shape == (ne + 1, 9)
+    for c in range(1, 9):  # 8 printed digits
+        assert np.abs(txt[:, c] - sp[:, c]).max() / np.abs(sp[:, c]).max() < 1e-7, c
+    lanc = open(tmp_path / "out.abslanc").read().split("\n")
+    assert int(lanc[0].split()[0]) == niter
+    eig = np.array([float(l.split()[2]) for l in lanc[niter + 2:] if l.strip()])
+    assert eig.shape == (niter + 1,)
+    assert np.abs(eig - orc.redtrid_eigs(ra, rb, niter)).max() < 2e-10  # f20.10
     ref_rows = orc.gmres_do_recycle(P, inp.psi0, energies, 0.5, 13.605, 1e-12, 30, inp.kpref)
     assert np.abs(rows[:, 2] - ref_rows[:, 2]).max() / np.abs(ref_rows[:, 2]).max() < 1e-6

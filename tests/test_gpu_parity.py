@@ -103,6 +103,74 @@ def test_spectrum_kernel_matches_oracle():
         assert np.allclose(got[:, 0], ref[:, 0], rtol=0, atol=1e-12)
 
 
+def _fe_lanc(site="01"):
+    import os
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    lines = open(os.path.join(gold, f"abslanc_Fe.0001_1s_{site}")).read().split("\n")
+    n, kpref = int(lines[0].split()[0]), float(lines[0].split()[1])
+    a, b = np.zeros(n + 1), np.zeros(n + 1)
+    a[0] = float(lines[1])
+    for i in range(1, n + 1):
+        a[i], b[i] = (float(t) for t in lines[1 + i].split()[:2])
+    eig = np.array([float(l.split()[2]) for l in lines[n + 2:] if l.strip()])
+    return n, kpref, a, b, eig
+
+
+FE_NVAL = 16.0 / 5.423513587 ** 3
+FE_EPS = 1.0e5
+
+
+def test_write_core_nine_columns_matches_oracle():
+    """ocean_gpu_write_core vs write_core_full: Spect(0)/SPKK(0) and the cross sections
+    to 1e-12 of peak, the gamfcn-broadened columns to 1e-9 (device atan/log)."""
+    from ocean_amd import core
+    n, kpref, a, b, _ = _fe_lanc()
+    for old in (False, True):
+        for lit in (True, False):
+            kw = dict(ebase_eV=7112.0, old_broaden=old, nval=FE_NVAL, eps=FE_EPS, ucvol=159.5,
+                      literal_cmplx=lit)
+            ref = orc.write_core_full(a, b, n, kpref, 0.0184, -1.5, 2.5, 400, **kw)
+            got = core.write_core(a, b, n, kpref, 0.0184, -1.5, 2.5, 400, **kw)
+            assert got.shape == ref.shape == (401, 9)
+            assert np.allclose(got[:, 0], ref[:, 0], rtol=1e-15, atol=0)
+            for c in (2, 4, 5, 6, 7, 8):
+                assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-12 * np.abs(ref[:, c]).max(), (old, lit, c)
+            for c in (1, 3):
+                assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-9 * np.abs(ref[:, c]).max(), (old, lit, c)
+
+
+def test_write_core_voigt_matches_oracle():
+    """The Voigt form (fine grid reused along the output energies in the reference)
+    evaluated independently per output energy on the device."""
+    from ocean_amd import core
+    n, kpref, a, b, _ = _fe_lanc("02")
+    for (el, eh, ne, sig) in ((-0.3, 1.6, 57, 0.012), (-0.1, 0.2, 30, 0.05)):
+        kw = dict(ebase_eV=1.0, nval=FE_NVAL, eps=FE_EPS, gam_gauss=sig)
+        ref = orc.write_core_full(a, b, n, kpref, 0.0184, el, eh, ne, **kw)
+        got = core.write_core(a, b, n, kpref, 0.0184, el, eh, ne, **kw)
+        assert got.shape == ref.shape == (ne + 1, 5)
+        assert np.allclose(got[:, 0], ref[:, 0], rtol=1e-15, atol=1e-12)
+        for c in (2, 4):
+            assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-11 * np.abs(ref[:, c]).max(), c
+        for c in (1, 3):
+            assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-9 * np.abs(ref[:, c]).max(), c
+
+
+def test_tridiag_eigs_match_fixture_block():
+    """Sturm bisection on the device vs the abslanc eigenvalue block (elsch) and LAPACK."""
+    from ocean_amd import core
+    for site in ("01", "02"):
+        n, _, a, b, eig = _fe_lanc(site)
+        w = core.tridiag_eigs(a, b, n)
+        assert np.abs(w - np.sort(eig)).max() < 1e-8
+        assert np.abs(w - orc.redtrid_eigs(a, b, n)).max() < 1e-12
+    # a zero coupling splits the matrix; repeated eigenvalues come out twice
+    a = np.array([1.0, 1.0, 0.5, 1.0])
+    b = np.array([0.0, 0.0, 0.3, 0.0])
+    w = core.tridiag_eigs(a, b, 3)
+    assert np.abs(w - orc.redtrid_eigs(a, b, 3)).max() < 1e-14
+
+
 def test_vector_ops(cfg0):
     g = _gpu(cfg0)
     rng = np.random.default_rng(5)

@@ -525,3 +525,126 @@ def test_early_exit_on_a_constant_chain_stops_at_the_first_pair():
         assert orc.haydock_early_exit(a, b, n, spacing, 1e-6, 1.0, 0.02, -0.5, 1.5, 300) == spacing + 2
     # and a threshold of zero never stops it
     assert orc.haydock_early_exit(a, b, n, 5, 0.0, 1.0, 0.02, -0.5, 1.5, 300) == n
+
+
+# ------------------------------------------------------------ write_core (nine columns)
+def _fe_fixture(site):
+    n, kpref, a, b, _ = load_lanc(os.path.join(GOLD, f"abslanc_Fe.0001_1s_{site}"))
+    gam, d = load_spct(os.path.join(GOLD, f"absspct_Fe.0001_1s_{site}"))
+    h2e = orc.HARTREE2EV_OLD
+    return n, kpref, a, b, gam, d, h2e
+
+
+FE_NVAL = 16.0 / 5.423513587 ** 3  # TEST/Fe/Fe.in: zion 16 (fe.fhi), one atom, acell^3
+FE_EPS = 1.0e5                     # TEST/Fe/Fe.in: diemac
+
+
+@pytest.mark.parametrize("site", ["01", "02"])
+def test_write_core_gamfcn_columns_reproduce_fe_fixture(site):
+    """write_core with oldXASbroaden (OCEAN_haydock.f90:1801-1894): Spect and SPKK use
+    gam0 + gamfcn(E) (CNBSE/zcmsnmult/zf90/gamfcn.f90) at the Fe run's valence density
+    and dielectric constant; Spect(0)/SPKK(0) keep gam0.  The fixture's first five
+    columns are reproduced to the printed precision."""
+    n, kpref, a, b, gam, d, h2e = _fe_fixture(site)
+    ne = d.shape[0] - 1
+    el, eh = d[0, 0] / h2e, d[-1, 0] / h2e
+    out = orc.write_core_full(a, b, n, kpref, gam, el, eh, ne, old_broaden=True, nval=FE_NVAL, eps=FE_EPS,
+                              hartree2ev=h2e)
+    assert out.shape == (ne + 1, 9)
+    assert np.abs(out[:, 0] - d[:, 0]).max() < 1e-6
+    for c in (1, 2):
+        assert np.abs(out[:, c] - d[:, c]).max() < 1e-6 * np.abs(d[:, c]).max(), c
+    for c in (3, 4):
+        assert np.abs(out[:, c] - d[:, c]).max() < 1e-6 * np.abs(d[:, c]).max(), c
+    # the broadened columns differ from the unbroadened ones only above E_F (E > 0)
+    below = out[:, 0] / h2e < -1e-3
+    assert np.array_equal(out[below, 1], out[below, 2])
+    assert np.abs(out[~below, 1] - out[~below, 2]).max() > 1e-3 * np.abs(out[:, 2]).max()
+
+
+def test_gamfcn_vanishes_below_fermi_level_and_grows_above():
+    """gamfcn.f90:13-15: no decay channel below E_F.  Plasmon emission needs
+    k^2 - kf^2 > 2 w_q, with w_q >= wp = sqrt(4 pi n) (the q -> 0 pole of a metal), so the
+    rate is exactly zero up to E = wp, then positive and rising just above it."""
+    wp = math.sqrt(4 * math.pi * FE_NVAL)
+    assert orc.gamfcn(-0.01, FE_NVAL, FE_EPS) == 0.0
+    assert orc.gamfcn(0.98 * wp, FE_NVAL, FE_EPS) == 0.0
+    g = [orc.gamfcn(e * wp, FE_NVAL, FE_EPS) for e in (1.3, 1.8, 2.5)]
+    assert 0.0 < g[0] < g[1] < g[2]
+
+
+def test_write_core_cross_section_columns():
+    """Columns 6-9 of write_core (OCEAN_haydock.f90:1855-1880): sigma = E alpha V Spect(0)
+    with E in Hartree (a.u.^2), times bohr^2 in cm^2; the two-photon columns
+    2 pi alpha^2 V Spect(0) (a.u.^5) and that times bohr^4 cm^4 and the atomic time."""
+    n, kpref, a, b, gam, d, h2e = _fe_fixture("01")
+    V = 5.423513587 ** 3
+    out = orc.write_core_full(a, b, n, kpref, gam, -0.5, 1.0, 40, ebase_eV=3.0, ucvol=V)
+    E_h = out[:, 0] * orc.EV2HARTREE
+    assert np.allclose(out[:, 5], E_h * orc.ALPHA * V * out[:, 2], rtol=1e-14, atol=0)
+    assert np.allclose(out[:, 6], out[:, 5] * (0.529177210544e-8) ** 2, rtol=1e-12, atol=0)
+    assert np.allclose(out[:, 7], 2 * np.pi * orc.ALPHA ** 2 * V * out[:, 2], rtol=1e-14, atol=0)
+    assert np.allclose(out[:, 8], out[:, 7] * (0.529177210544e-8) ** 4 * 2.4188843265864e-17, rtol=1e-12,
+                       atol=0)
+    # printed energy: ebase + Hartree2eV * (el + (eh - el) ie / ne)
+    assert np.allclose(out[:, 0], 3.0 + orc.HARTREE2EV * np.linspace(-0.5, 1.0, 41), rtol=0, atol=1e-12)
+
+
+def test_write_core_voigt_is_gaussian_convolved_lorentzian():
+    """write_core_Voigt (OCEAN_haydock.f90:1896-1989) for a one-level fraction (b1 = 0:
+    an exact Lorentzian of half width gam0) is pi kpref times scipy's Voigt profile,
+    up to the +-3 sigma truncation and the Riemann sum on the fine grid; the printed
+    energy is the last fine energy, ee + nstep estep."""
+    from scipy.special import voigt_profile
+    a0, gam0, sig, kpref = 0.2, 0.02, 0.03, 1.7
+    a = np.array([a0, 0.0])
+    b = np.array([0.0, 0.0])
+    el, eh, ne = -0.1, 0.5, 60
+    out = orc.write_core_full(a, b, 1, kpref, gam0, el, eh, ne, gam_gauss=sig, nval=FE_NVAL, eps=FE_EPS,
+                              literal_cmplx=False)
+    assert out.shape == (ne + 1, 5)
+    ee = el + (eh - el) * np.arange(ne + 1) / ne
+    ref = np.pi * kpref * voigt_profile(ee - a0, sig, gam0)
+    peak = ref.max()
+    assert np.abs(out[:, 2] - ref).max() < 6e-3 * peak
+    # a half-step shift of the grid would already break the bound
+    shifted = np.pi * kpref * voigt_profile(ee - a0 + 0.5 * (eh - el) / ne, sig, gam0)
+    assert np.abs(out[:, 2] - shifted).max() > 2e-2 * peak
+    de = (eh - el) / ne
+    estep = min(gam0, sig) / 2
+    nratio = int((de - estep) / estep) + 1
+    estep = de / nratio
+    nstep = int(2.99999 * sig / estep) + 1
+    assert np.allclose(out[:, 0], orc.HARTREE2EV * (ee + nstep * estep), rtol=0, atol=1e-10)
+    # below E_F the gamfcn columns equal the plain ones
+    below = ee + nstep * estep < -1e-9
+    assert np.array_equal(out[below, 1], out[below, 2])
+
+
+def test_write_core_voigt_reuse_equals_direct_fine_grid():
+    """The reference reuses the fine grid from one output energy to the next (shift by
+    nratio).  Its sums equal a direct evaluation of every fine energy ee + ii estep
+    when the fine energy is formed the same way (point q's ee plus its offset), which
+    this rebuilds independently from the reuse bookkeeping."""
+    n, kpref, a, b, gam, d, h2e = _fe_fixture("01")
+    el, eh, ne, sig = -0.2, 0.3, 12, 0.01
+    out = orc.write_core_full(a, b, n, kpref, gam, el, eh, ne, gam_gauss=sig, nval=FE_NVAL, eps=FE_EPS)
+    de = (eh - el) / ne
+    estep = min(gam, sig) / 2
+    nratio = int((de - estep) / estep) + 1
+    estep = de / nratio
+    nstep = int(2.99999 * sig / estep) + 1
+    pref = estep / (sig * 2.506628274631)
+    def writer(p, ii):  # point p's own loop writes ii >= nstep - nratio; the rest came from p - 1
+        if p == 0 or ii >= nstep - nratio:
+            return p, ii
+        return writer(p - 1, ii + nratio)
+
+    for p in (0, 1, 5, ne):
+        acc = np.zeros(2)
+        for ii in range(-nstep, nstep + 1):
+            q, o = writer(p, ii)
+            e = el + (eh - el) * (2 * q) / (2 * ne) + o * estep
+            dl = orc._cf_delta(a, b, np.zeros_like(a), n, np.array([e]), np.array([gam]), True)[0]
+            acc[0] += math.exp(-ii * ii * estep * estep / (2 * sig * sig)) * pref * kpref * abs(dl.imag) / abs(dl) ** 2
+        assert abs(acc[0] - out[p, 2]) <= 1e-12 * abs(out[p, 2]), p
