@@ -269,6 +269,22 @@ int  ocean_gpu_write_core(const double* real_a, const double* real_b, const doub
                           int32_t niter, double kpref, const ocean_spect_desc* sd, double* out,
                           int32_t* ncols);
 
+/* o_spect.x (OCEAN2/o_spect.f90:26-266) for one abslanc run: spect.in's values (eV),
+ * epsilon, nval.h and the cell volume.  Core runs (is_val = 0) give (ne+1) rows of
+ * (E, Spect, Spect(0), SPKK, SPKK(0)) as :228-266 writes them; valence runs give ne
+ * rows of (omega eV, eps1, eps2, n, kappa, mu, R, loss) as :166-222.  real_a/real_b
+ * hold a(0:iter), b(0:iter) (b(0) unused); out holds 8*(ne+1) values. */
+typedef struct ocean_ospect_desc {
+  int32_t ne;
+  double  el_ev, eh_ev, gam0_ev, ebase_ev;
+  double  eps0;          /* epsilon file */
+  double  nval;          /* nval.h */
+  double  ucvol;         /* |det avecsinbohr| */
+  int32_t is_val;        /* 1 for VAL/RXS runs (opcons / rxsspct) */
+} ocean_ospect_desc;
+int  ocean_gpu_ospect(const double* real_a, const double* real_b, int32_t iter, double kpref,
+                      const ocean_ospect_desc* od, double* out, int32_t* nrows, int32_t* ncols);
+
 /* Eigenvalues, ascending, of the (n+1) x (n+1) tridiagonal Lanczos matrix with
  * diagonal real_a(0:n) and off-diagonals real_b(1:n): the block redtrid appends to
  * abslanc (OCEAN2/OCEAN_haydock.f90:2273-2316, :2354-2356).  w has n+1 entries. */

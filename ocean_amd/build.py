@@ -49,11 +49,18 @@ def build_fortran(verbose=True):
     srcs = [s for s in srcs if os.path.exists(s)]
     if len(srcs) < 3:
         return None
-    cmd = [fc, "-O2", "-J", fdir] + srcs + ["-o", out, "-L" + HERE, "-locean_gpu",
-                                            "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE]
+    link = ["-L" + HERE, "-locean_gpu", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE]
+    cmd = [fc, "-O2", "-J", fdir] + srcs + ["-o", out] + link
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    # o_spect.x over the same C-ABI
+    osrc = os.path.join(fdir, "ocean_gpu_ospect.f90")
+    if os.path.exists(osrc):
+        cmd = [fc, "-O2", "-J", fdir, srcs[0], osrc, "-o", os.path.join(fdir, "ocean_gpu_ospect.x")] + link
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     return out
 
 

@@ -88,6 +88,7 @@ _SIGS = {
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_write_core": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP]),
     "ocean_gpu_tridiag_eigs": (C.c_int, [_DP, _DP, C.c_int32, _DP]),
+    "ocean_gpu_ospect": (C.c_int, [_DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP, _IP]),
     "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
     "ocean_gpu_load_bloch_dev": (C.c_int, [_H, C.c_void_p, C.c_void_p]),
     "ocean_gpu_spectral_grad": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
@@ -177,6 +178,28 @@ def write_core(real_a, real_b, niter, kpref, gam0, el, eh, ne, ebase_eV=0.0, old
                                   out.ctypes.data_as(_DP), C.byref(ncols)), "write_core")
     nc = ncols.value
     return out[:nc * (ne + 1)].reshape(ne + 1, nc)
+
+
+class OspectDesc(C.Structure):
+    """ocean_ospect_desc of include/ocean_gpu.h."""
+    _fields_ = [("ne", C.c_int32), ("el_ev", C.c_double), ("eh_ev", C.c_double), ("gam0_ev", C.c_double),
+                ("ebase_ev", C.c_double), ("eps0", C.c_double), ("nval", C.c_double), ("ucvol", C.c_double),
+                ("is_val", C.c_int32)]
+
+
+def ospect(real_a, real_b, it, kpref, ne, el_eV, eh_eV, gam0_eV, ebase_eV=0.0, eps0=1.0, nval=0.0,
+           ucvol=1.0, is_val=False):
+    """o_spect.x's spectrum for one Lanczos run on the GPU (rows as o_spect writes them)."""
+    lib = _lib()
+    ra = np.ascontiguousarray(real_a, np.float64)
+    rb = np.ascontiguousarray(real_b, np.float64)
+    assert ra.size >= it + 1 and rb.size >= it + 1
+    od = OspectDesc(ne, el_eV, eh_eV, gam0_eV, ebase_eV, eps0, nval, ucvol, 1 if is_val else 0)
+    out = np.zeros(8 * (ne + 1))
+    nr, nc = C.c_int32(0), C.c_int32(0)
+    _chk(lib.ocean_gpu_ospect(_p(ra), _p(rb), it, kpref, C.byref(od), out.ctypes.data_as(_DP), C.byref(nr),
+                              C.byref(nc)), "ospect")
+    return out[:nr.value * nc.value].reshape(nr.value, nc.value)
 
 
 def tridiag_eigs(real_a, real_b, n):

@@ -17,6 +17,7 @@ namespace {
 constexpr double kEV2Hartree = 0.036749322175666;   // OCEAN2/OCEAN_constants.f90:15
 constexpr double kHartree2eV = 27.211386245980;     // OCEAN2/OCEAN_constants.f90:16
 constexpr double kAlpha = 0.0072973525643;           // OCEAN2/OCEAN_constants.f90:18
+constexpr double kAlphaInv = 137.035999177;           // OCEAN2/OCEAN_constants.f90:19
 constexpr double kBohr = 0.529177210544;             // OCEAN2/OCEAN_constants.f90:20 (Angstrom)
 constexpr double kAu2sec = 2.4188843265864e-17;      // OCEAN2/OCEAN_constants.f90:22
 constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;  // :29
@@ -1169,6 +1170,39 @@ int ocean_gpu_write_core(const double* real_a, const double* real_b, const doubl
                                                       d + 3 * n1);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpy(out, d + 3 * n1, (size_t)nc * (ne + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(d);
+  return rc;
+}
+
+// ---- o_spect (OCEAN2/o_spect.f90) ----
+int ocean_gpu_ospect(const double* real_a, const double* real_b, int32_t iter, double kpref,
+                     const ocean_ospect_desc* od, double* out, int32_t* nrows, int32_t* ncols) {
+  if (!real_a || !real_b || !od || !out || iter <= 0 || od->ne <= 0) return OCEAN_GPU_ERR_ARG;
+  const int ne = od->ne;
+  const int nr = od->is_val ? ne : ne + 1, nc = od->is_val ? 8 : 5;
+  if (nrows) *nrows = nr;
+  if (ncols) *ncols = nc;
+  // spect.in is in eV (o_spect.f90:30-33)
+  const double el = od->el_ev * kEV2Hartree, eh = od->eh_ev * kEV2Hartree;
+  const double gam0 = od->gam0_ev * kEV2Hartree, ebase = od->ebase_ev * kEV2Hartree;
+  const size_t n1 = (size_t)iter + 1;
+  double* d = nullptr;
+  OG_TRY(dmalloc(&d, 2 * n1 + (size_t)nr * nc));
+  int rc = OCEAN_GPU_OK;
+  if (hipMemcpy(d, real_a, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + n1, real_b, n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    if (od->is_val)
+      og::k_ospect_val<<<(nr + 63) / 64, 64>>>(d, d + n1, iter, kpref, od->ucvol, el, eh, ne, gam0, kHartree2eV,
+                                               kBohr * kAlphaInv * 1000.0, d + 2 * n1);
+    else
+      og::k_ospect_core<<<(nr + 63) / 64, 64>>>(d, d + n1, iter, kpref, el, eh, ne, gam0, ebase, od->nval,
+                                                od->eps0, kHartree2eV, d + 2 * n1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(out, d + 2 * n1, (size_t)nr * nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       rc = OCEAN_GPU_ERR_HIP;
   }
   dfree(d);

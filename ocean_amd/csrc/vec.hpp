@@ -463,6 +463,84 @@ __global__ void k_write_core_voigt(const double* __restrict__ ra, const double* 
   o5[1] = s[1]; o5[2] = s[0]; o5[3] = k[1]; o5[4] = k[0];
 }
 
+// o_spect's core branch (OCEAN2/o_spect.f90:228-266), one thread per output energy:
+// the it-level fraction in full FP64 complex arithmetic, gam0 and gam0 + gamfcn, and
+// with ebase > 0.01 the anti-resonant fraction md (poles mirrored to -(a + 2 ebase)).
+// Both SPKK columns carry -Re(eps) of the broadened pass.
+__global__ void k_ospect_core(const double* __restrict__ ra, const double* __restrict__ rb, int it,
+                              double kpref, double el, double eh, int ne, double gam0, double ebase,
+                              double nval, double eps0, double hartree2ev, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > ne) return;
+  const double e = el + (eh - el) * (double)(2 * p) / (double)(2 * ne);
+  double sp[2] = {0.0, 0.0};
+  double2 eps = make_double2(0.0, 0.0);
+  for (int jd = 0; jd <= 1; ++jd) {
+    const double gam = gam0 + (jd ? gamfcn_dev(e, nval, eps0) : 0.0);
+    const double2 ct = make_double2(e - ra[it - 1], gam);
+    const double2 disc = c_sqrt(make_double2(ct.x * ct.x - ct.y * ct.y - 4.0 * (rb[it] * rb[it]), 2.0 * ct.x * ct.y));
+    double2 delta = (disc.y > 0.0) ? make_double2(0.5 * (ct.x + disc.x), 0.5 * (ct.y + disc.y))
+                                   : make_double2(0.5 * (ct.x - disc.x), 0.5 * (ct.y - disc.y));
+    double2 md = make_double2(delta.x - e - 2.0 * ebase, delta.y - gam);
+    for (int jj = it - 1; jj >= 0; --jj) {
+      const double bb = rb[jj + 1] * rb[jj + 1];
+      const double2 q = c_div(make_double2(bb, 0.0), delta);
+      const double2 qm = c_div(make_double2(bb, 0.0), md);
+      delta = make_double2((e - ra[jj]) - q.x, gam - q.y);
+      md = make_double2((-e - ra[jj] - 2.0 * ebase) - qm.x, -gam - qm.y);
+    }
+    eps = c_div(make_double2(-kpref, 0.0), delta);
+    if (ebase > 0.01) {
+      const double2 t = c_div(make_double2(kpref, 0.0), md);
+      eps = make_double2(eps.x - t.x, eps.y - t.y);
+    }
+    sp[jd] = eps.y;
+  }
+  double* o5 = out + (size_t)p * 5;
+  o5[0] = (e + ebase) * hartree2ev;
+  o5[1] = sp[1]; o5[2] = sp[0]; o5[3] = -eps.x; o5[4] = -eps.x;
+}
+
+// o_spect's valence branch (OCEAN2/o_spect.f90:166-222) at the ne midpoint energies:
+// eps = 1 - f/al - f/be with the resonant (al) and anti-resonant (be) fractions, then
+// n, kappa, mu, reflectivity and the loss function.
+__global__ void k_ospect_val(const double* __restrict__ ra, const double* __restrict__ rb, int it,
+                             double kpref, double ucvol, double el, double eh, int ne, double gam0,
+                             double hartree2ev, double mu_den, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double w = el + (eh - el) * (double)(2 * i + 1) / (double)(2 * ne);
+  const double fact = kpref * ucvol * 2.0;
+  const double at = ra[it - 1];
+  const double2 arg = c_sqrt(make_double2((w - at) * (w - at) - 4.0 * (rb[it] * rb[it]), 0.0));
+  const double2 rp = make_double2(0.5 * (w - at + arg.x), 0.5 * arg.y);
+  const double2 rm = make_double2(0.5 * (w - at - arg.x), -0.5 * arg.y);
+  const double2 rrr = (rp.y < 0.0) ? rp : rm;
+  double2 al = make_double2((w - at) - rrr.x, gam0 - rrr.y);
+  double2 be = make_double2((-w - at) - rrr.x, -gam0 - rrr.y);
+  for (int j = it - 1; j >= 0; --j) {
+    const double bb = rb[j + 1] * rb[j + 1];
+    const double2 qa = c_div(make_double2(bb, 0.0), al);
+    const double2 qb = c_div(make_double2(bb, 0.0), be);
+    al = make_double2((w - ra[j]) - qa.x, gam0 - qa.y);
+    be = make_double2((-w - ra[j]) - qb.x, -gam0 - qb.y);
+  }
+  const double2 fa = c_div(make_double2(fact, 0.0), al), fb = c_div(make_double2(fact, 0.0), be);
+  const double re = (1.0 - fa.x) - fb.x, im = (-fa.y) - fb.y;
+  const double loss = im / (re * re + im * im);
+  const double rad = sqrt(re * re + im * im);
+  const double th = acos(re / rad) / 2.0;
+  const double nref = sqrt(rad) * cos(th), kap = sqrt(rad) * sin(th);
+  const double2 N = c_sqrt(make_double2(re, im));
+  const double2 r = c_div(make_double2(N.x - 1.0, N.y), make_double2(N.x + 1.0, N.y));
+  double* o8 = out + (size_t)i * 8;
+  o8[0] = w * hartree2ev;
+  o8[1] = re; o8[2] = im; o8[3] = nref; o8[4] = kap;
+  o8[5] = 2.0 * w * hartree2ev * N.y / mu_den;
+  o8[6] = r.x * r.x + r.y * r.y;
+  o8[7] = loss;
+}
+
 // Eigenvalues of the (n+1) x (n+1) Lanczos matrix redtrid builds (diagonal a(0:n),
 // off-diagonals b(1:n); OCEAN_haydock.f90:2273-2316, solved there by EISPACK's elsch),
 // by Sturm-sequence bisection: thread j brackets the j-th smallest eigenvalue inside

@@ -48,6 +48,19 @@ module ocean_gpu_iface
     integer(c_int32_t) :: literal_cmplx
   end type ocean_spect_desc
 
+  ! ocean_ospect_desc of ocean_gpu.h: o_spect.x's run-directory inputs
+  type, bind(C) :: ocean_ospect_desc
+    integer(c_int32_t) :: ne
+    real(c_double) :: el_ev
+    real(c_double) :: eh_ev
+    real(c_double) :: gam0_ev
+    real(c_double) :: ebase_ev
+    real(c_double) :: eps0
+    real(c_double) :: nval
+    real(c_double) :: ucvol
+    integer(c_int32_t) :: is_val
+  end type ocean_ospect_desc
+
   interface
     integer(c_int) function ocean_gpu_create(desc, h) bind(C, name='ocean_gpu_create')
       import :: c_int, c_ptr, ocean_bse_desc
@@ -145,6 +158,17 @@ module ocean_gpu_iface
       type(ocean_spect_desc), intent(in) :: sd
       real(c_double), intent(out) :: out(*)
       integer(c_int32_t), intent(out) :: ncols
+    end function
+
+    integer(c_int) function ocean_gpu_ospect(real_a, real_b, it, kpref, od, out, nrows, ncols) &
+        bind(C, name='ocean_gpu_ospect')
+      import :: c_int, c_double, c_int32_t, ocean_ospect_desc
+      real(c_double), intent(in) :: real_a(*), real_b(*)
+      integer(c_int32_t), value :: it
+      real(c_double), value :: kpref
+      type(ocean_ospect_desc), intent(in) :: od
+      real(c_double), intent(out) :: out(*)
+      integer(c_int32_t), intent(out) :: nrows, ncols
     end function
 
     integer(c_int) function ocean_gpu_tridiag_eigs(real_a, real_b, n, w) bind(C, name='ocean_gpu_tridiag_eigs')

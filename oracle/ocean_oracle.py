@@ -29,6 +29,10 @@ Pinning (what checks each function; "parity unpinned" where none does):
                        truncation) and its fine-grid reuse against direct evaluation
   gamfcn               the same fixture (its only consumer), zero below the Fermi level
   calc_spect_core      write_core at the midpoint grid
+  o_spect_core         the Fe absspct fixture (Spect, Spect(0) and SPKK columns), the
+                       resonant-minus-anti-resonant Lorentzian closed form for one level
+  o_spect_val          the Lorentz-oscillator closed form of eps for one level; n + i kappa
+                       squared equals eps, R and the loss function from their definitions
   redtrid              TEST/Fe/Results abslanc eigenvalue block
   haydock              dense Lanczos/eigendecomposition brute force (tests/test_oracle.py)
   lr_act               dense operator brute force, loop-literal lr_kernel restatement
@@ -1170,6 +1174,76 @@ def _write_core_voigt(a, b, ia, n, kpref, gam0, el, eh, ne, ebase_eV, nval, eps,
             s += pref * sa[ii + nstep]
             k += pref * sk[ii + nstep]
         rows.append([ebase_eV + hartree2ev * e, s[1], s[0], k[1], k[0]])
+    return np.array(rows)
+
+
+ALPHAINV = 137.035999177  # OCEAN_constants.f90:19
+
+
+def o_spect_core(a, b, it, kpref, ne, el, eh, gam0, ebase, nval, eps0):
+    """o_spect's core branch (OCEAN2/o_spect.f90:228-266) for one abslanc run: el, eh,
+    gam0, ebase in Hartree (spect.in values times eV2Hartree, :30-33); it levels.
+
+    Full FP64 complex arithmetic (rm1 is complex(dp)); with ebase > 0.01 Hartree the
+    anti-resonant fraction mdelta (poles at -(a + 2 ebase)) is subtracted; both SPKK
+    columns print -Re(eps) of the gamfcn-broadened (jdamp = 1) pass, as the whole-array
+    assignment at :257 leaves them.  Rows (E_eV, Spect, Spect(0), SPKK, SPKK(0))."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    rows = []
+    for ie in range(0, 2 * ne + 1, 2):
+        e = el + (eh - el) * float(ie) / float(2 * ne)
+        spct = [0.0, 0.0]
+        for jd in (0, 1):
+            gam = gam0 + gamfcn(e, nval, eps0) * jd
+            ct = complex(e - a[it - 1], gam)
+            disc = np.sqrt(ct * ct - 4.0 * b[it] ** 2 + 0j)
+            delta = (ct + disc) / 2 if disc.imag > 0.0 else (ct - disc) / 2
+            md = complex(delta.real - e - 2.0 * ebase, delta.imag - gam)
+            for jj in range(it - 1, -1, -1):
+                delta = complex(e - a[jj], gam) - b[jj + 1] ** 2 / delta
+                md = complex(-e - a[jj] - 2.0 * ebase, -gam) - b[jj + 1] ** 2 / md
+            eps = -kpref / delta
+            if ebase > 0.01:
+                eps = eps - kpref / md
+            spct[jd] = eps.imag
+        spkk = -eps.real
+        rows.append([(e + ebase) * HARTREE2EV, spct[1], spct[0], spkk, spkk])
+    return np.array(rows)
+
+
+def o_spect_val(a, b, it, kpref, ucvol, ne, el, eh, gam0):
+    """o_spect's valence branch (OCEAN2/o_spect.f90:166-222): eps(w) = 1 - f/al - f/be
+    with f = 2 kpref ucvol, al/be the resonant and anti-resonant fractions at the ne
+    midpoints w = el + (eh - el)(2i - 1)/(2 ne), then n, kappa, mu (cm^-1), R and the
+    loss function.  Rows (w_eV, eps1, eps2, n, kappa, mu, R, loss)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    fact = kpref * ucvol * 2.0
+    rows = []
+    for ie in range(1, 2 * ne, 2):
+        w = el + (eh - el) * float(ie) / float(2 * ne)
+        ct = complex(w, gam0)
+        arg = np.sqrt(complex((w - a[it - 1]) ** 2 - 4.0 * b[it] ** 2, 0.0))
+        rp = 0.5 * (w - a[it - 1] + arg)
+        rm = 0.5 * (w - a[it - 1] - arg)
+        rrr = rp if rp.imag < 0.0 else rm
+        al = ct - a[it - 1] - rrr
+        be = -ct - a[it - 1] - rrr
+        for i in range(it - 1, -1, -1):
+            al = ct - a[i] - b[i + 1] ** 2 / al
+            be = -ct - a[i] - b[i + 1] ** 2 / be
+        eps = 1.0 - fact / al - fact / be
+        re, im = eps.real, eps.imag
+        loss = im / (re ** 2 + im ** 2)
+        rad = math.sqrt(re ** 2 + im ** 2)
+        th = math.acos(re / rad) / 2
+        nref, kap = math.sqrt(rad) * math.cos(th), math.sqrt(rad) * math.sin(th)
+        N = np.sqrt(eps)
+        r = (N - 1.0) / (N + 1.0)
+        refl = (r * np.conj(r)).real
+        mu = 2.0 * w * HARTREE2EV * N.imag / (BOHR * ALPHAINV * 1000)
+        rows.append([w * HARTREE2EV, re, im, nref, kap, mu, refl, loss])
     return np.array(rows)
 
 

@@ -249,3 +249,80 @@ def test_haydock_early_exit(cfg0):
     assert np.array_equal(ae[:done], a[:done]) and np.array_equal(be[:done + 1], b[:done + 1])
     assert not ae[done:].any() and not be[done + 1:].any()
     g.close()
+
+
+def test_ospect_kernels_match_oracle():
+    """ocean_gpu_ospect (core with and without the anti-resonant term, and the valence
+    branch) vs the oracle's o_spect restatement on the Fe Lanczos coefficients."""
+    from ocean_amd import core
+    n, kpref, a, b, _ = _fe_lanc("01")
+    E2H = orc.EV2HARTREE
+    for ebase in (0.0, 3.0):
+        got = core.ospect(a, b, n, kpref, 300, -20.0, 60.0, 0.5, ebase_eV=ebase, eps0=FE_EPS, nval=FE_NVAL)
+        ref = orc.o_spect_core(a, b, n, kpref, 300, -20.0 * E2H, 60.0 * E2H, 0.5 * E2H, ebase * E2H, FE_NVAL,
+                               FE_EPS)
+        assert got.shape == ref.shape == (301, 5)
+        assert np.allclose(got[:, 0], ref[:, 0], rtol=1e-15, atol=1e-12)
+        for c in (2,):
+            assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-12 * np.abs(ref[:, c]).max(), (ebase, c)
+        for c in (1, 3, 4):
+            assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-9 * np.abs(ref[:, c]).max(), (ebase, c)
+    got = core.ospect(a, b, 60, kpref, 200, 1.0, 60.0, 0.3, ucvol=159.5, is_val=True)
+    ref = orc.o_spect_val(a, b, 60, kpref, 159.5, 200, 1.0 * E2H, 60.0 * E2H, 0.3 * E2H)
+    assert got.shape == ref.shape == (200, 8)
+    _check_opcons(got, ref)
+
+
+def _check_opcons(got, ref):
+    # n and kappa come from acos(eps1/|eps|)/2, which loses digits where eps is nearly
+    # real, so they are held to 1e-11 of |N| = sqrt|eps| rather than of themselves
+    absN = np.hypot(ref[:, 3], ref[:, 4]).max()
+    for c in range(8):
+        scale = absN if c in (3, 4) else np.abs(ref[:, c]).max()
+        assert np.abs(got[:, c] - ref[:, c]).max() <= 1e-11 * scale, c
+
+
+def test_fortran_ospect_run_directory(tmp_path):
+    """ocean_gpu_ospect.x in a run directory holding the Fe abslanc files: both runs of
+    runlist, the photon-weighted TotAvg, and an opcons run from oplanc."""
+    import os
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "ocean_amd", "fortran", "ocean_gpu_ospect.x")
+    if not os.path.exists(exe):
+        pytest.fail("ocean_gpu_ospect.x not built (run __graft_entry__.build())")
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    for site in ("01", "02"):
+        shutil.copy(os.path.join(gold, f"abslanc_Fe.0001_1s_{site}"), tmp_path)
+    shutil.copy(os.path.join(gold, "abslanc_Fe.0001_1s_01"), tmp_path / "oplanc")
+    (tmp_path / "spect.in").write_text("400 -20.0 60.0 0.5 0.0\n")
+    (tmp_path / "epsilon").write_text("100000\n")
+    (tmp_path / "nval.h").write_text(f"{FE_NVAL!r}\n")
+    acell = 5.423513587
+    (tmp_path / "avecsinbohr.ipt").write_text(f"{acell} 0 0\n0 {acell} 0\n0 0 {acell}\n")
+    (tmp_path / "runlist").write_text("3\n26 1 0 Fe 1s 1 1 XAS\n26 1 0 Fe 1s 1 2 XAS\n26 1 0 Fe 1s 1 1 VAL\n")
+    (tmp_path / "do_avg.ipt").write_text(".true.\n")
+    for ph, w in ((1, 0.25), (2, 0.75)):
+        (tmp_path / f"photon{ph}").write_text(f"quad\ncartesian 1 0 0\nend\ncartesian 0 1 0\nend\n8990\n{w}\n")
+    print("fortran o_spect case: Fe abslanc x2 + oplanc, ne=400", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    E2H = orc.EV2HARTREE
+    tot = 0
+    for site, w in (("01", 0.25), ("02", 0.75)):
+        n, kpref, a, b, _ = _fe_lanc(site)
+        ref = orc.o_spect_core(a, b, n, kpref, 400, -20.0 * E2H, 60.0 * E2H, 0.5 * E2H, 0.0, FE_NVAL, FE_EPS)
+        txt = np.loadtxt(tmp_path / f"absspct_Fe.0001_1s_{site}", comments="#")
+        assert txt.shape == (401, 5)
+        for c in range(1, 5):  # eight printed digits
+            assert np.abs(txt[:, c] - ref[:, c]).max() <= 1e-7 * np.abs(ref[:, c]).max(), (site, c)
+        tot = tot + w * ref[:, [4, 2, 3, 1]]
+    avg = np.loadtxt(tmp_path / "TotAvg")
+    assert avg.shape == (400, 5)
+    assert np.abs(avg[:, 1:] - tot[:400]).max() <= 1e-9 * np.abs(tot).max()  # gamfcn columns: device atan/log
+    n, kpref, a, b, _ = _fe_lanc("01")
+    ref = orc.o_spect_val(a, b, n, kpref, acell ** 3, 400, -20.0 * E2H, 60.0 * E2H, 0.5 * E2H)
+    txt = np.loadtxt(tmp_path / "opcons", comments="#")
+    assert txt.shape == (400, 8)
+    _check_opcons(txt, ref)

@@ -648,3 +648,58 @@ def test_write_core_voigt_reuse_equals_direct_fine_grid():
             dl = orc._cf_delta(a, b, np.zeros_like(a), n, np.array([e]), np.array([gam]), True)[0]
             acc[0] += math.exp(-ii * ii * estep * estep / (2 * sig * sig)) * pref * kpref * abs(dl.imag) / abs(dl) ** 2
         assert abs(acc[0] - out[p, 2]) <= 1e-12 * abs(out[p, 2]), p
+
+
+# ------------------------------------------------------------ o_spect
+def test_o_spect_core_reproduces_fe_fixture():
+    """o_spect's core branch on the Fe abslanc (ebase 0: no anti-resonant term) gives the
+    fixture's Spect, Spect(0) and, in both SPKK columns, its broadened SPKK."""
+    n, kpref, a, b, gam, d, h2e = _fe_fixture("01")
+    ne = d.shape[0] - 1
+    el, eh = d[0, 0] / h2e, d[-1, 0] / h2e
+    out = orc.o_spect_core(a, b, n, kpref, ne, el, eh, gam, 0.0, FE_NVAL, FE_EPS)
+    assert out.shape == (ne + 1, 5)
+    assert np.abs(out[:, 0] * h2e / orc.HARTREE2EV - d[:, 0]).max() < 1e-6
+    for c, fc in ((1, 1), (2, 2), (3, 3), (4, 3)):
+        assert np.abs(out[:, c] - d[:, fc]).max() < 1e-6 * np.abs(d[:, fc]).max(), c
+
+
+def test_o_spect_core_anti_resonant_closed_form():
+    """One level a0 (b1 = 0) with ebase > 0.01: eps = -kpref/(E - a0 + i g) + kpref/(E + a0 +
+    2 ebase + i g), the resonant pole minus its mirror; below E_F gamfcn adds nothing."""
+    a0, g, kp, eb = -0.3, 0.02, 0.7, 0.2
+    a = np.array([a0, 0.0])
+    b = np.zeros(2)
+    out = orc.o_spect_core(a, b, 1, kp, 50, -0.6, -0.01, g, eb, FE_NVAL, FE_EPS)
+    E = -0.6 + 0.59 * np.arange(51) / 50
+    eps = -kp / (E - a0 + 1j * g) + kp / (E + a0 + 2 * eb + 1j * g)
+    assert np.allclose(out[:, 0], (E + eb) * orc.HARTREE2EV, rtol=1e-15, atol=1e-12)
+    for c in (1, 2):
+        assert np.allclose(out[:, c], eps.imag, rtol=1e-12, atol=1e-14)
+    for c in (3, 4):
+        assert np.allclose(out[:, c], -eps.real, rtol=1e-12, atol=1e-14)
+    # without the anti-resonant term (ebase below 0.01) only the resonant pole remains
+    out0 = orc.o_spect_core(a, b, 1, kp, 50, -0.6, -0.01, g, 0.005, FE_NVAL, FE_EPS)
+    assert np.allclose(out0[:, 2], (-kp / (E - a0 + 1j * g)).imag, rtol=1e-12, atol=1e-14)
+
+
+def test_o_spect_val_lorentz_oscillator():
+    """One level: eps(w) = 1 + 2 a0 f / (a0^2 - (w + i g)^2), f = 2 kpref ucvol; then
+    (n + i kappa)^2 = eps, R = |(N - 1)/(N + 1)|^2, loss = Im eps / |eps|^2, and mu is
+    proportional to w kappa."""
+    a0, g, kp, V = 0.35, 0.01, 0.02, 40.0
+    a = np.array([a0, 0.0])
+    b = np.zeros(2)
+    ne, el, eh = 80, 0.05, 0.8
+    out = orc.o_spect_val(a, b, 1, kp, V, ne, el, eh, g)
+    w = el + (eh - el) * (2 * np.arange(1, ne + 1) - 1) / (2 * ne)
+    f = 2 * kp * V
+    eps = 1 + 2 * a0 * f / (a0 ** 2 - (w + 1j * g) ** 2)
+    assert np.allclose(out[:, 0], w * orc.HARTREE2EV, rtol=1e-15)
+    assert np.allclose(out[:, 1] + 1j * out[:, 2], eps, rtol=1e-12, atol=1e-13)
+    N = out[:, 3] + 1j * out[:, 4]
+    assert np.allclose(N ** 2, eps, rtol=1e-12, atol=1e-13)
+    assert np.allclose(out[:, 6], np.abs((N - 1) / (N + 1)) ** 2, rtol=1e-12)
+    assert np.allclose(out[:, 7], eps.imag / np.abs(eps) ** 2, rtol=1e-12)
+    ratio = out[:, 5] / (w * out[:, 4])
+    assert np.allclose(ratio, ratio[0], rtol=1e-12)
