@@ -71,50 +71,236 @@ __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const dou
 // projection partials of that k (a 16-KB read beside the 512-KB u slab), so the
 // multiplet term needs no launch or stream of its own.
 // ---------------------------------------------------------------------------
+// FP32 projection on the matrix cores, for many alphas (L2,3 edges), where the FMA
+// count per byte of u would otherwise bound the streaming loop.  Per (k, s) it is the
+// real GEMM phi[x][c] = sum_{n, r} U[x][(n, r)] B[(n, r)][c] with the complex parts
+// unrolled: r = re/im part of u(n, x), columns c < acnt the real parts of phi(x, a),
+// c - acnt the imaginary parts, so B[(n, re)] = (psi_r, psi_i), B[(n, im)] =
+// (-psi_i, psi_r) of A psi(n, a).  v_mfma_f32_16x16x4f32: lane l supplies
+// U[x(l % 16)][k = l / 16] and B[k = l / 16][c = l % 16] and receives D[4 (l / 16) + r]
+// [l % 16] in register r.  Each lane loads 16 bytes, u(n, x) and u(n, x + 1), which
+// feed two tiles (even and odd x) of 16 x points; one wave covers 32 x points, four
+// band rows per step, and two column tiles (acnt <= 12, 2 acnt <= 32).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBRow = 80;  // floats per B row: re part 32, im part 32, 16 pad (no bank conflicts)
+template <int NA>
+__device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx, int k, int s) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* bt = reinterpret_cast<float*>(smem);  // [nb4][kBRow]: B rows (n, re) then (n, im)
+  const int nb = A.nb, nk = A.nk, na = A.na, ns = A.ns;
+  const int astep = (ns == 2) ? 2 : 1;
+  const int acnt = (ns == 2) ? na / 2 : na;
+  const int nb4 = (nb + 3) & ~3;
+  // each (n, alpha) element is read once (kF loads in flight per thread) and fills
+  // its four B entries; columns past 2 acnt and rows past nb are zero
+  const int nel = nb4 * acnt;
+  constexpr int kF = 8;
+  for (int i0 = threadIdx.x; i0 < nel; i0 += kF * blockDim.x) {
+    double2 pv[kF];
+    double mv[kF];
+#pragma unroll
+    for (int f = 0; f < kF; ++f) {
+      const int i = i0 + f * blockDim.x;
+      const int n = i / acnt, c = i - n * acnt;
+      pv[f] = make_double2(0.0, 0.0);
+      mv[f] = 0.0;
+      if (i < nel && n < nb) {
+        const size_t off = ((size_t)(s + c * astep) * nk + k) * nb + n;
+        pv[f] = A.psi[off];
+        mv[f] = A.allow[off];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < kF; ++f) {
+      const int i = i0 + f * blockDim.x;
+      if (i < nel) {
+        const int n = i / acnt, c = i - n * acnt;
+        float* row = bt + n * kBRow;
+        const float pr = (float)(pv[f].x * mv[f]), pim = (float)(pv[f].y * mv[f]);
+        row[c] = pr;
+        row[32 + c] = -pim;
+        row[acnt + c] = pim;
+        row[32 + acnt + c] = pr;
+      }
+    }
+  }
+  const int nz = 32 - 2 * acnt;
+  for (int i = threadIdx.x; i < nb4 * nz; i += blockDim.x) {
+    const int n = i / nz, c = 2 * acnt + (i - n * nz);
+    bt[n * kBRow + c] = 0.f;
+    bt[n * kBRow + 32 + c] = 0.f;
+  }
+  __syncthreads();
+  if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s, bx, A.px);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int xb = (bx * (int)(blockDim.x >> 6) + wave) * 32;
+  if (xb >= A.nx) return;
+  const int m16 = lane & 15, kq = lane >> 4;
+  // x + 1 <= xb + 31 < nx + 31 < ldu: the pair stays inside the padded row
+  const float2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + xb + 2 * m16;
+  constexpr int NT = NA > 8 ? 2 : 1;  // column tiles (2 acnt <= 16 fits one)
+  f32x4 ce[NT], co[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { ce[t] = f32x4{0, 0, 0, 0}; co[t] = f32x4{0, 0, 0, 0}; }
+  // one four-row step: tile t takes re(u) x B(n, re) + im(u) x B(n, im); the four
+  // chains are interleaved so that dependent products are NT * 2 issues apart
+  auto step = [&](const float4& q, const float* br) {
+    float b[NT], c[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { b[t] = br[16 * t]; c[t] = br[32 + 16 * t]; }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x, b[t], ce[t], 0, 0, 0);
+      co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z, b[t], co[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y, c[t], ce[t], 0, 0, 0);
+      co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w, c[t], co[t], 0, 0, 0);
+    }
+  };
+  // batches of kS four-row steps, double-buffered: the loads of batch b + 1 are
+  // issued before the products of batch b, so each wave keeps HBM busy while its
+  // matrix-core work runs
+  constexpr int kS = 8;
+  const int nfull = nb / (4 * kS);
+  float4 qa[kS], qb[kS];
+  auto load_batch = [&](float4* q, int b) {
+#pragma unroll
+    for (int i = 0; i < kS; ++i) q[i] = ld_stream4(ub + (size_t)(b * 4 * kS + 4 * i + kq) * A.ldu);
+  };
+  auto use_batch = [&](const float4* q, int b) {
+#pragma unroll
+    for (int i = 0; i < kS; ++i) step(q[i], bt + (b * 4 * kS + 4 * i + kq) * kBRow + m16);
+  };
+  if (nfull > 0) load_batch(qa, 0);
+  for (int b = 0; b < nfull; b += 2) {
+    if (b + 1 < nfull) load_batch(qb, b + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    use_batch(qa, b);
+    if (b + 1 < nfull) {
+      if (b + 2 < nfull) load_batch(qa, b + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      use_batch(qb, b + 1);
+    }
+  }
+  // remaining rows, four at a time; rows past nb read zero B rows (nb4 covers them)
+  for (int n0 = nfull * 4 * kS; n0 < nb; n0 += 4) {
+    const float4 q = ld_stream4(ub + (size_t)min(n0 + kq, nb - 1) * A.ldu);
+    step(q, bt + (n0 + kq) * kBRow + m16);
+  }
+  // D[4 kq + r][m16]: x = xb + 2 (4 kq + r) (+1 for the odd tile), column 16 t + m16
+  float* ph = reinterpret_cast<float*>(A.phi + (size_t)k * A.ldphi);
+  auto put = [&](const f32x4& d, int t, int odd) {
+    const int c = 16 * t + m16;
+    if (c >= 2 * acnt) return;
+    const int j = c < acnt ? c : c - acnt, part = c < acnt ? 0 : 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int x = xb + 2 * (4 * kq + r) + odd;
+      if (x < A.nx) ph[2 * ((size_t)x * na + s + j * astep) + part] = d[r];
+    }
+  };
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    put(ce[t], t, 0);
+    put(co[t], t, 1);
+  }
+}
+
 template <typename T2, int NA>
 __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, int s) {
+  if constexpr (sizeof(T2) == 8 && NA >= 6) {
+    proj_mfma_body<NA>(A, bx, k, s);
+    return;
+  }
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T2* ps = reinterpret_cast<T2*>(smem);  // [acnt][nb]
   const int nb = A.nb, nk = A.nk, na = A.na, ns = A.ns;
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
-  for (int i = threadIdx.x; i < acnt * nb; i += blockDim.x) {
-    const int j = i / nb, n = i - j * nb;
-    const int a = s + j * astep;
-    const size_t off = ((size_t)a * nk + k) * nb + n;
-    const double2 p = A.psi[off];
-    const double m = A.allow[off];
-    ps[i] = cmake<T2>((T)(p.x * m), (T)(p.y * m));
+  // A psi of this k in LDS, with up to kF loads in flight per thread
+  constexpr int kF = 4;
+  for (int i0 = threadIdx.x; i0 < acnt * nb; i0 += kF * blockDim.x) {
+    double2 pv[kF];
+    double mv[kF];
+#pragma unroll
+    for (int f = 0; f < kF; ++f) {
+      const int i = min(i0 + f * (int)blockDim.x, acnt * nb - 1);
+      const int j = i / nb, n = i - j * nb;
+      const size_t off = ((size_t)(s + j * astep) * nk + k) * nb + n;
+      pv[f] = A.psi[off];
+      mv[f] = A.allow[off];
+    }
+#pragma unroll
+    for (int f = 0; f < kF; ++f) {
+      const int i = i0 + f * blockDim.x;
+      if (i < acnt * nb) ps[i] = cmake<T2>((T)(pv[f].x * mv[f]), (T)(pv[f].y * mv[f]));
+    }
   }
   __syncthreads();
   // the projector chunks of this k are shared out over its x-tile workgroups
   if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s, bx, A.px);
-  const int x = bx * blockDim.x + threadIdx.x;
+  // 16 bytes per lane and row: one x point in FP64, two adjacent ones in FP32 (the
+  // row stride ldu is even, so the pair is 16-byte aligned)
+  constexpr int VX = sizeof(T) == 8 ? 1 : 2;
+  const int x = (bx * blockDim.x + threadIdx.x) * VX;
   if (x >= A.nx) return;
-  T2 acc[NA];
+  T2 acc[VX][NA];
 #pragma unroll
-  for (int j = 0; j < NA; ++j) acc[j] = cmake<T2>(0, 0);
+  for (int v = 0; v < VX; ++v)
+#pragma unroll
+    for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
   const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
   // Batches of kB band rows: all kB loads are issued before the first use, so
-  // each wave keeps kB row segments in flight (8 KB per wave in both precisions).
-  constexpr int kB = sizeof(T) == 8 ? 8 : 16;
-  for (int n0 = 0; n0 < nb; n0 += kB) {
-    T2 uv[kB];
+  // each wave keeps kB row segments in flight (8 KB per wave in FP64; FP32 runs
+  // half as many threads per workgroup and keeps 16 KB per wave in flight).
+  // Full batches carry no per-row predicate: with one, the compiler sinks the FP32
+  // loads into per-row branches and waits on each row in turn.
+  constexpr int kB = VX == 1 ? 8 : 16;
+  auto load_row = [&](int n, T2* dst) {
+    const T2* row = ub + (size_t)n * A.ldu;
+    if constexpr (VX == 1) {
+      dst[0] = ld_stream(row);
+    } else {
+      const float4 q = ld_stream4(reinterpret_cast<const float2*>(row));
+      dst[0] = cmake<T2>(q.x, q.y);
+      dst[VX - 1] = cmake<T2>(q.z, q.w);
+    }
+  };
+  auto use_row = [&](int n, const T2* uvr) {
 #pragma unroll
-    for (int i = 0; i < kB; ++i) uv[i] = ld_stream(ub + (size_t)min(n0 + i, nb - 1) * A.ldu);
+    for (int j = 0; j < NA; ++j) {
+      const T2 pv = ps[j * nb + n];
 #pragma unroll
-    for (int i = 0; i < kB; ++i) {
-      if (n0 + i < nb) {
+      for (int v = 0; v < VX; ++v) cfma(acc[v][j], uvr[v], pv);
+    }
+  };
+  int n0 = 0;
+  for (; n0 + kB <= nb; n0 += kB) {
+    T2 uv[kB][VX];
 #pragma unroll
-        for (int j = 0; j < NA; ++j) cfma(acc[j], uv[i], ps[j * nb + n0 + i]);
-      }
+    for (int i = 0; i < kB; ++i) load_row(n0 + i, uv[i]);
+    // keep the kB loads together ahead of the arithmetic
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) use_row(n0 + i, uv[i]);
+  }
+  for (; n0 < nb; ++n0) {
+    T2 uv[VX];
+    load_row(n0, uv);
+    use_row(n0, uv);
+  }
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    if (x + v < A.nx) {
+      T2* out = A.phi + (size_t)k * A.ldphi + (size_t)(x + v) * na;
+#pragma unroll
+      for (int j = 0; j < NA; ++j)
+        if (j < acnt) out[s + j * astep] = acc[v][j];
     }
   }
-  T2* out = A.phi + (size_t)k * A.ldphi + (size_t)x * na;
-#pragma unroll
-  for (int j = 0; j < NA; ++j)
-    if (j < acnt) out[s + j * astep] = acc[j];
 }
 
 // One problem: grid (px, nk, ns).
@@ -685,8 +871,9 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
         }
         __syncthreads();
         if (active) {
-          // kB independent row loads in flight per wave (addresses clamped, use predicated)
-          constexpr int kB = 8;
+          // kB independent row loads in flight per wave (addresses clamped, use
+          // predicated): 128 bytes per lane in both precisions
+          constexpr int kB = sizeof(T2) == 16 ? 8 : 16;
           for (int xb = xs; xb < xc; xb += kB * XS) {
             T2 uv[kB];
 #pragma unroll
@@ -694,6 +881,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
               const int xl = min(xb + i * XS, xc - 1);
               uv[i] = ld_stream(ub + (size_t)(x0 + xl) * nb + n);
             }
+            __builtin_amdgcn_sched_barrier(0);
             // one row's NA phi' values at a time: with many alphas, letting the
             // compiler hoist the LDS reads of several rows costs occupancy
 #pragma unroll

@@ -158,8 +158,12 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   const bool lr = (flags & og::F_LR) != 0, fg = (flags & og::F_FG) != 0;
   // projection
   if (t.proj < 64 || t.proj > 512 || (t.proj & 63)) return OCEAN_GPU_ERR_ARG;
-  a.px = (nx + t.proj - 1) / t.proj;
-  g.sm_proj = (size_t)acnt * nb * sizeof(T2);
+  // FP32 streaming workgroups cover two x points per thread; the FP32 matrix-core
+  // form (many alphas) covers 32 per wave and stages B as nb4 rows of kBRow floats
+  const bool mf = sizeof(T) == 4 && NA >= 6;
+  const int xper = mf ? 32 * (t.proj / 64) : t.proj * (sizeof(T) == 4 ? 2 : 1);
+  a.px = (nx + xper - 1) / xper;
+  g.sm_proj = mf ? (size_t)((nb + 3) & ~3) * og::kBRow * sizeof(float) : (size_t)acnt * nb * sizeof(T2);
   g.nblk_proj = a.px * nk * ns;
   // transform
   a.ncol = nx * na;
@@ -209,15 +213,29 @@ template <> struct StateOf<float2> {
   static const float* W(const H* h) { return h->W32; }
 };
 
+// A handle's tiles for one precision: FP32 streaming projection threads take two x
+// points each, so half as many threads keep the x points per workgroup of FP64; the
+// FP32 matrix-core form for many alphas (32 x points per wave) takes twice as many.
+template <typename T2, int NA>
+Tiles tiles_for(const H* h) {
+  Tiles t = h->tiles;
+  if (sizeof(typename og::Scalar<T2>::T) == 4) {
+    if (NA < 6) t.proj = std::max(64, t.proj / 2);
+    else t.proj = std::min(512, 2 * t.proj);  // fewer, wider workgroups per k share the B staging
+  }
+  return t;
+}
+
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
            const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
   og::ActArgs<T2> a;
   Geom<T2> g;
-  OG_TRY((make_args<T2, NA>(h, in, out, flags, h->tiles, u, uT, phi, W, a, g)));
+  const Tiles tl = tiles_for<T2, NA>(h);
+  OG_TRY((make_args<T2, NA>(h, in, out, flags, tl, u, uT, phi, W, a, g)));
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
-    og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), h->tiles.proj, g.sm_proj, h->stream>>>(a);
+    og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), tl.proj, g.sm_proj, h->stream>>>(a);
     OG_CHECK(hipGetLastError());
     if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
     if (g.lmax == 8)
@@ -543,7 +561,7 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   h->nx = d->xmesh[0] * d->xmesh[1] * d->xmesh[2];
   h->na = 4 * (2 * d->lc + 1);
   h->ns = d->nspn;
-  h->ldu = h->nx + 32;
+  h->ldu = (h->nx + 33) & ~1;  // even, so FP32 rows pair up into 16-byte loads
   h->ldphi = h->nx * h->na + 32;
   h->ldT = h->nx * h->nb + 32;
   // the adjoint keeps one k's result (nalpha x nb complex) in LDS
@@ -1513,9 +1531,10 @@ int grad_t(H* h, int nE, const double2* xs, const double2* ys, const std::vector
       const auto* W = StateOf<T2>::W(h);
       for (int side = 0; side < 2 && rc == 0; ++side) {
         T2* P = side == 0 ? PX : PY;
-        rc = make_args<T2, NA>(h, side == 0 ? x : y, nullptr, og::F_LR, h->tiles, u, StateOf<T2>::uT(h), P, W, a, g);
+        const Tiles tl = tiles_for<T2, NA>(h);
+        rc = make_args<T2, NA>(h, side == 0 ? x : y, nullptr, og::F_LR, tl, u, StateOf<T2>::uT(h), P, W, a, g);
         if (rc) break;
-        og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), h->tiles.proj, g.sm_proj, h->stream>>>(a);
+        og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), tl.proj, g.sm_proj, h->stream>>>(a);
         const size_t tile = (size_t)h->nk * a.G;
         const size_t smf = (3 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2);
         if (g.lmax == 8)
@@ -1822,6 +1841,7 @@ int ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_bat
       return g.prec == key.prec && g.NA == key.NA && g.lmax == key.lmax;
     });
     if (it == b->groups.end()) {
+      if (key.prec != OCEAN_GPU_FP64 && key.NA < 6) key.tiles.proj = Tiles{}.proj / 2;  // same x per workgroup
       b->groups.push_back(key);
       it = b->groups.end() - 1;
     }

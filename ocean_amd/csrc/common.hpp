@@ -75,6 +75,12 @@ __device__ __forceinline__ double2 ld_stream(const double2* p) {
   const v2d t = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
   return make_double2(t.x, t.y);
 }
+// two adjacent complex floats (16-byte aligned)
+__device__ __forceinline__ float4 ld_stream4(const float2* p) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
   typedef float v2f __attribute__((ext_vector_type(2)));
   const v2f t = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(p));

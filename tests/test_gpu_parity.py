@@ -53,6 +53,25 @@ def test_xact_matches_oracle(name):
     g.close()
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_xact_close_to_oracle(name):
+    """The mixed form (FP32 u, FFT and W; FP32 matrix cores for the many-alpha
+    projection) against the FP64 oracle: H.psi to 1e-5 of its largest element, both
+    directions (FP32 rounding of u and of sums over up to a few hundred terms)."""
+    inp = synth.make_inputs(**CASES[name])
+    P = to_problem(inp)
+    from ocean_amd import core
+    print("mixed xact case", name, flush=True)
+    with _gpu(inp, precision=core.MIXED) as g:
+        rng = np.random.default_rng(11)
+        psi = np.asfortranarray(rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
+        for back in (False, True):
+            got = g.xact(psi, backwards=back)
+            ref = orc.xact(P, psi, backwards=back)
+            print("  backwards", back, "relerr", relerr(got, ref), flush=True)
+            assert relerr(got, ref) < 1e-5, (name, back, relerr(got, ref))
+
+
 def test_xact_config0(cfg0):
     P = to_problem(cfg0)
     g = _gpu(cfg0)
