@@ -67,9 +67,9 @@ __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const dou
 // ---------------------------------------------------------------------------
 // Projection: phi(k,x,a) = sum_n u(n,k,x) (A psi)(n,k,a)
 //   long_range.f90:897-906 (no conjugation of u).  Grid (ceil(nx/256), nk, ns).
-// With fg set, the last x-tile workgroup of each (k, s) also forms the multiplet
-// projection partials of that k (a 16-KB read beside the 512-KB u slab), so the
-// multiplet term needs no launch or stream of its own.
+// With fg set, the slab workgroups of each k also form its multiplet projection
+// partials (a 16-KB read of c beside the 512-KB u slab), so the multiplet term needs
+// no launch or stream of its own.
 // ---------------------------------------------------------------------------
 // FP32 projection on the matrix cores, for many alphas (L2,3 edges), where the FMA
 // count per byte of u would otherwise bound the streaming loop.  Per (k, s) it is the
@@ -131,80 +131,80 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
     bt[n * kBRow + 32 + c] = 0.f;
   }
   __syncthreads();
-  if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s, bx, A.px);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xb = (bx * (int)(blockDim.x >> 6) + wave) * 32;
-  if (xb >= A.nx) return;
-  const int m16 = lane & 15, kq = lane >> 4;
-  // x + 1 <= xb + 31 < nx + 31 < ldu: the pair stays inside the padded row
-  const float2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + xb + 2 * m16;
-  constexpr int NT = NA > 8 ? 2 : 1;  // column tiles (2 acnt <= 16 fits one)
-  f32x4 ce[NT], co[NT];
+  if (xb < A.nx) {  // waves past nx stay for the multiplet partials after the slab
+    const int m16 = lane & 15, kq = lane >> 4;
+    // x + 1 <= xb + 31 < nx + 31 < ldu: the pair stays inside the padded row
+    const float2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + xb + 2 * m16;
+    constexpr int NT = NA > 8 ? 2 : 1;  // column tiles (2 acnt <= 16 fits one)
+    f32x4 ce[NT], co[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) { ce[t] = f32x4{0, 0, 0, 0}; co[t] = f32x4{0, 0, 0, 0}; }
-  // one four-row step: tile t takes re(u) x B(n, re) + im(u) x B(n, im); the four
-  // chains are interleaved so that dependent products are NT * 2 issues apart
-  auto step = [&](const float4& q, const float* br) {
-    float b[NT], c[NT];
+    for (int t = 0; t < NT; ++t) { ce[t] = f32x4{0, 0, 0, 0}; co[t] = f32x4{0, 0, 0, 0}; }
+    // one four-row step: tile t takes re(u) x B(n, re) + im(u) x B(n, im); the four
+    // chains are interleaved so that dependent products are NT * 2 issues apart
+    auto step = [&](const float4& q, const float* br) {
+      float b[NT], c[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) { b[t] = br[16 * t]; c[t] = br[32 + 16 * t]; }
+      for (int t = 0; t < NT; ++t) { b[t] = br[16 * t]; c[t] = br[32 + 16 * t]; }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x, b[t], ce[t], 0, 0, 0);
-      co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z, b[t], co[t], 0, 0, 0);
-    }
+      for (int t = 0; t < NT; ++t) {
+        ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x, b[t], ce[t], 0, 0, 0);
+        co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z, b[t], co[t], 0, 0, 0);
+      }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y, c[t], ce[t], 0, 0, 0);
-      co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w, c[t], co[t], 0, 0, 0);
-    }
-  };
-  // batches of kS four-row steps, double-buffered: the loads of batch b + 1 are
-  // issued before the products of batch b, so each wave keeps HBM busy while its
-  // matrix-core work runs
-  constexpr int kS = 8;
-  const int nfull = nb / (4 * kS);
-  float4 qa[kS], qb[kS];
-  auto load_batch = [&](float4* q, int b) {
+      for (int t = 0; t < NT; ++t) {
+        ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y, c[t], ce[t], 0, 0, 0);
+        co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w, c[t], co[t], 0, 0, 0);
+      }
+    };
+    // batches of kS four-row steps, double-buffered: the loads of batch b + 1 are
+    // issued before the products of batch b, so each wave keeps HBM busy while its
+    // matrix-core work runs
+    constexpr int kS = 8;
+    const int nfull = nb / (4 * kS);
+    float4 qa[kS], qb[kS];
+    auto load_batch = [&](float4* q, int b) {
 #pragma unroll
-    for (int i = 0; i < kS; ++i) q[i] = ld_stream4(ub + (size_t)(b * 4 * kS + 4 * i + kq) * A.ldu);
-  };
-  auto use_batch = [&](const float4* q, int b) {
+      for (int i = 0; i < kS; ++i) q[i] = ld_stream4(ub + (size_t)(b * 4 * kS + 4 * i + kq) * A.ldu);
+    };
+    auto use_batch = [&](const float4* q, int b) {
 #pragma unroll
-    for (int i = 0; i < kS; ++i) step(q[i], bt + (b * 4 * kS + 4 * i + kq) * kBRow + m16);
-  };
-  if (nfull > 0) load_batch(qa, 0);
-  for (int b = 0; b < nfull; b += 2) {
-    if (b + 1 < nfull) load_batch(qb, b + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    use_batch(qa, b);
-    if (b + 1 < nfull) {
-      if (b + 2 < nfull) load_batch(qa, b + 2);
+      for (int i = 0; i < kS; ++i) step(q[i], bt + (b * 4 * kS + 4 * i + kq) * kBRow + m16);
+    };
+    if (nfull > 0) load_batch(qa, 0);
+    for (int b = 0; b < nfull; b += 2) {
+      if (b + 1 < nfull) load_batch(qb, b + 1);
       __builtin_amdgcn_sched_barrier(0);
-      use_batch(qb, b + 1);
+      use_batch(qa, b);
+      if (b + 1 < nfull) {
+        if (b + 2 < nfull) load_batch(qa, b + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        use_batch(qb, b + 1);
+      }
     }
-  }
-  // remaining rows, four at a time; rows past nb read zero B rows (nb4 covers them)
-  for (int n0 = nfull * 4 * kS; n0 < nb; n0 += 4) {
-    const float4 q = ld_stream4(ub + (size_t)min(n0 + kq, nb - 1) * A.ldu);
-    step(q, bt + (n0 + kq) * kBRow + m16);
-  }
-  // D[4 kq + r][m16]: x = xb + 2 (4 kq + r) (+1 for the odd tile), column 16 t + m16
-  float* ph = reinterpret_cast<float*>(A.phi + (size_t)k * A.ldphi);
-  auto put = [&](const f32x4& d, int t, int odd) {
-    const int c = 16 * t + m16;
-    if (c >= 2 * acnt) return;
-    const int j = c < acnt ? c : c - acnt, part = c < acnt ? 0 : 1;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int x = xb + 2 * (4 * kq + r) + odd;
-      if (x < A.nx) ph[2 * ((size_t)x * na + s + j * astep) + part] = d[r];
+    // remaining rows, four at a time; rows past nb read zero B rows (nb4 covers them)
+    for (int n0 = nfull * 4 * kS; n0 < nb; n0 += 4) {
+      const float4 q = ld_stream4(ub + (size_t)min(n0 + kq, nb - 1) * A.ldu);
+      step(q, bt + (n0 + kq) * kBRow + m16);
     }
-  };
+    // D[4 kq + r][m16]: x = xb + 2 (4 kq + r) (+1 for the odd tile), column 16 t + m16
+    float* ph = reinterpret_cast<float*>(A.phi + (size_t)k * A.ldphi);
+    auto put = [&](const f32x4& d, int t, int odd) {
+      const int c = 16 * t + m16;
+      if (c >= 2 * acnt) return;
+      const int j = c < acnt ? c : c - acnt, part = c < acnt ? 0 : 1;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    put(ce[t], t, 0);
-    put(co[t], t, 1);
+      for (int r = 0; r < 4; ++r) {
+        const int x = xb + 2 * (4 * kq + r) + odd;
+        if (x < A.nx) ph[2 * ((size_t)x * na + s + j * astep) + part] = d[r];
+      }
+    };
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      put(ce[t], t, 0);
+      put(co[t], t, 1);
+    }
   }
 }
 
@@ -240,73 +240,82 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
     }
   }
   __syncthreads();
-  // the projector chunks of this k are shared out over its x-tile workgroups
-  if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, nb, nk, na, ns, A.P, k, s, bx, A.px);
   // 16 bytes per lane and row: one x point in FP64, two adjacent ones in FP32 (the
   // row stride ldu is even, so the pair is 16-byte aligned)
   constexpr int VX = sizeof(T) == 8 ? 1 : 2;
   const int x = (bx * blockDim.x + threadIdx.x) * VX;
-  if (x >= A.nx) return;
-  T2 acc[VX][NA];
+  if (x < A.nx) {  // threads past nx stay for the multiplet partials after the slab
+    T2 acc[VX][NA];
 #pragma unroll
-  for (int v = 0; v < VX; ++v)
+    for (int v = 0; v < VX; ++v)
 #pragma unroll
-    for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
-  const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
-  // Batches of kB band rows: all kB loads are issued before the first use, so
-  // each wave keeps kB row segments in flight (8 KB per wave in FP64; FP32 runs
-  // half as many threads per workgroup and keeps 16 KB per wave in flight).
-  // Full batches carry no per-row predicate: with one, the compiler sinks the FP32
-  // loads into per-row branches and waits on each row in turn.
-  constexpr int kB = VX == 1 ? 8 : 16;
-  auto load_row = [&](int n, T2* dst) {
-    const T2* row = ub + (size_t)n * A.ldu;
-    if constexpr (VX == 1) {
-      dst[0] = ld_stream(row);
-    } else {
-      const float4 q = ld_stream4(reinterpret_cast<const float2*>(row));
-      dst[0] = cmake<T2>(q.x, q.y);
-      dst[VX - 1] = cmake<T2>(q.z, q.w);
+      for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
+    const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
+    // Batches of kB band rows: all kB loads are issued before the first use, so
+    // each wave keeps kB row segments in flight (8 KB per wave in FP64; FP32 runs
+    // half as many threads per workgroup and keeps 16 KB per wave in flight).
+    // Full batches carry no per-row predicate: with one, the compiler sinks the FP32
+    // loads into per-row branches and waits on each row in turn.
+    constexpr int kB = VX == 1 ? 8 : 16;
+    auto load_row = [&](int n, T2* dst) {
+      const T2* row = ub + (size_t)n * A.ldu;
+      if constexpr (VX == 1) {
+        dst[0] = ld_stream(row);
+      } else {
+        const float4 q = ld_stream4(reinterpret_cast<const float2*>(row));
+        dst[0] = cmake<T2>(q.x, q.y);
+        dst[VX - 1] = cmake<T2>(q.z, q.w);
+      }
+    };
+    auto use_row = [&](int n, const T2* uvr) {
+#pragma unroll
+      for (int j = 0; j < NA; ++j) {
+        const T2 pv = ps[j * nb + n];
+#pragma unroll
+        for (int v = 0; v < VX; ++v) cfma(acc[v][j], uvr[v], pv);
+      }
+    };
+    int n0 = 0;
+    for (; n0 + kB <= nb; n0 += kB) {
+      T2 uv[kB][VX];
+#pragma unroll
+      for (int i = 0; i < kB; ++i) load_row(n0 + i, uv[i]);
+      // keep the kB loads together ahead of the arithmetic
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < kB; ++i) use_row(n0 + i, uv[i]);
     }
-  };
-  auto use_row = [&](int n, const T2* uvr) {
-#pragma unroll
-    for (int j = 0; j < NA; ++j) {
-      const T2 pv = ps[j * nb + n];
-#pragma unroll
-      for (int v = 0; v < VX; ++v) cfma(acc[v][j], uvr[v], pv);
+    for (; n0 < nb; ++n0) {
+      T2 uv[VX];
+      load_row(n0, uv);
+      use_row(n0, uv);
     }
-  };
-  int n0 = 0;
-  for (; n0 + kB <= nb; n0 += kB) {
-    T2 uv[kB][VX];
 #pragma unroll
-    for (int i = 0; i < kB; ++i) load_row(n0 + i, uv[i]);
-    // keep the kB loads together ahead of the arithmetic
-    __builtin_amdgcn_sched_barrier(0);
+    for (int v = 0; v < VX; ++v) {
+      if (x + v < A.nx) {
+        T2* out = A.phi + (size_t)k * A.ldphi + (size_t)(x + v) * na;
 #pragma unroll
-    for (int i = 0; i < kB; ++i) use_row(n0 + i, uv[i]);
-  }
-  for (; n0 < nb; ++n0) {
-    T2 uv[VX];
-    load_row(n0, uv);
-    use_row(n0, uv);
-  }
-#pragma unroll
-  for (int v = 0; v < VX; ++v) {
-    if (x + v < A.nx) {
-      T2* out = A.phi + (size_t)k * A.ldphi + (size_t)(x + v) * na;
-#pragma unroll
-      for (int j = 0; j < NA; ++j)
-        if (j < acnt) out[s + j * astep] = acc[v][j];
+        for (int j = 0; j < NA; ++j)
+          if (j < acnt) out[s + j * astep] = acc[v][j];
+      }
     }
   }
 }
 
-// One problem: grid (px, nk, ns).
+// Workgroup b of a projection launch: slab tile bx of (k, s).  With the multiplet
+// on, each tile then forms its share (chunks bx, bx + px, ...) of that k's multiplet
+// partials (16 KB of c per k), after its slab so that the stream starts at once.
+template <typename T2, int NA>
+__device__ __forceinline__ void proj_dispatch(const ActArgs<T2>& A, int b) {
+  const int r = b / A.px, bx = b - r * A.px, k = r % A.nk, s = r / A.nk;
+  proj_body<T2, NA>(A, bx, k, s);
+  if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, A.nb, A.nk, A.na, A.ns, A.P, k, s, bx, A.px);
+}
+
+// One problem: a 1-D grid of px * nk * ns slab tiles.
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) void k_proj(const ActArgs<T2> A) {
-  proj_body<T2, NA>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+  proj_dispatch<T2, NA>(A, blockIdx.x);
 }
 
 // Many problems: workgroup b belongs to the problem whose range holds it.
@@ -316,8 +325,7 @@ __global__ __launch_bounds__(512) void k_proj_batch(const ActArgs<T2>* __restric
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2> A = As[p];
   const int local = blockIdx.x - off[p];
-  const int bx = local % A.px, r = local / A.px;
-  proj_body<T2, NA>(A, bx, r % A.nk, r / A.nk);
+  proj_dispatch<T2, NA>(A, local);
 }
 
 // ---------------------------------------------------------------------------

@@ -235,7 +235,7 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
   OG_TRY((make_args<T2, NA>(h, in, out, flags, tl, u, uT, phi, W, a, g)));
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
-    og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), tl.proj, g.sm_proj, h->stream>>>(a);
+    og::k_proj<T2, NA><<<g.nblk_proj, tl.proj, g.sm_proj, h->stream>>>(a);
     OG_CHECK(hipGetLastError());
     if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
     if (g.lmax == 8)
@@ -1534,7 +1534,7 @@ int grad_t(H* h, int nE, const double2* xs, const double2* ys, const std::vector
         const Tiles tl = tiles_for<T2, NA>(h);
         rc = make_args<T2, NA>(h, side == 0 ? x : y, nullptr, og::F_LR, tl, u, StateOf<T2>::uT(h), P, W, a, g);
         if (rc) break;
-        og::k_proj<T2, NA><<<dim3(a.px, h->nk, h->ns), tl.proj, g.sm_proj, h->stream>>>(a);
+        og::k_proj<T2, NA><<<g.nblk_proj, tl.proj, g.sm_proj, h->stream>>>(a);
         const size_t tile = (size_t)h->nk * a.G;
         const size_t smf = (3 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2);
         if (g.lmax == 8)
