@@ -19,6 +19,10 @@ CASES = {
     "l23_so": dict(nb=16, kmesh=(4, 4, 4), xmesh=(4, 4, 4), lc=1, seed=5, xi_eV=6.0),
     "k_6cube": dict(nb=32, kmesh=(6, 6, 6), xmesh=(6, 6, 6), lc=0, seed=6),
     "nspn2": dict(nb=8, kmesh=(2, 2, 4), xmesh=(3, 3, 3), lc=1, seed=7, nspn=2, xi_eV=3.0),
+    # odd band count and odd x count: the row tails of both projection forms, the
+    # padded even row stride of the FP32 slab, the last partial 32-point MFMA tile
+    "l23_ragged": dict(nb=13, kmesh=(3, 2, 2), xmesh=(3, 3, 5), lc=1, seed=8, xi_eV=4.0),
+    "k_ragged": dict(nb=23, kmesh=(2, 3, 3), xmesh=(5, 3, 3), lc=0, seed=9),
 }
 
 
