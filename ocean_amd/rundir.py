@@ -1,0 +1,434 @@
+"""Reader of an OCEAN2 run directory for a core-level (XAS) run.
+
+Reads the files OCEAN2 reads when it sets up the core-level BSE for the first core
+run of ``runlist`` and turns them into the arrays the C-ABI loaders take; the two
+setup steps with a device form (the Bloch phase and W) run on the GPU through
+``ocean_gpu_load_bloch_raw`` and ``ocean_gpu_build_w``.  File formats and the
+derived quantities follow the reference routines cited at each reader (paths
+relative to the OCEAN 3.2.1 tree).
+
+Not read (a run that needs them raises ``NotImplementedError``): XES / valence /
+RIXS runs, GW corrections, doping and ``findNewFermi.ipt``, occupation types other
+than the default, ``core_broaden.ipt``, ``rpa``, ``tdlda``, ``isolated.inp``,
+``pbc.inp``, ``epsilon3D``, ``xshift_override.ipt``, ``file_brange.ipt`` and the
+legacy (sequential) cks files.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import setup_host as sh
+from .synth import MultipletInputs
+
+EV2HARTREE = sh.EV2HARTREE
+
+
+def _p(d, name):
+    return os.path.join(d, name)
+
+
+def _tokens(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def _fbool(tok):
+    """Fortran list-directed logical: .true./.false./T/F."""
+    t = tok.strip().lower().lstrip(".")
+    if t.startswith("t"):
+        return True
+    if t.startswith("f"):
+        return False
+    raise ValueError(f"not a Fortran logical: {tok!r}")
+
+
+def _ffloat(tok):
+    return float(tok.lower().replace("d", "e"))
+
+
+def _refuse(d, names):
+    for n in names:
+        if os.path.exists(_p(d, n)):
+            raise NotImplementedError(f"run directory uses {n}, which this reader does not support")
+
+
+def fortran_records(path):
+    """Records of a Fortran unformatted sequential file (4-byte length markers)."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        n = int(np.frombuffer(data, np.int32, 1, pos)[0])
+        rec = data[pos + 4:pos + 4 + n]
+        if int(np.frombuffer(data, np.int32, 1, pos + 4 + n)[0]) != n:
+            raise ValueError(f"{path}: broken record marker at byte {pos}")
+        out.append(rec)
+        pos += 8 + n
+    return out
+
+
+@dataclass
+class CoreRun:
+    """One line of ``runlist`` (OCEAN2/OCEAN_system.f90:655-705)."""
+    Z: int
+    n: int
+    l: int
+    elname: str
+    corelevel: str
+    indx: int
+    photon: int
+    calc_type: str
+
+
+@dataclass
+class RunSystem:
+    """o_system fields of a core run (OCEAN2/OCEAN_system.f90:160-471)."""
+    xmesh: tuple
+    kmesh: tuple
+    nspn: int
+    ZNL: tuple
+    nb: int
+    brange: tuple
+    avec: np.ndarray          # columns are the lattice vectors (bohr)
+    celvol: float
+    run: CoreRun
+    tau: np.ndarray           # site, reduced coordinates (xyz.wyck)
+    inter_scale: float
+    epsilon0: float
+    mult: bool
+    lvl: int
+    lvh: int
+    xi_eV: float
+    solver: dict = field(default_factory=dict)
+
+    @property
+    def nk(self):
+        return int(np.prod(self.kmesh))
+
+    @property
+    def nx(self):
+        return int(np.prod(self.xmesh))
+
+    @property
+    def nalpha(self):
+        return 4 * (2 * self.ZNL[2] + 1)
+
+    @property
+    def bands_in_file(self):
+        return self.brange[3] - self.brange[2] + 1
+
+
+def read_system(d):
+    """Meshes, spins, ZNL, bands, lattice, first core run, site, mode, epsilon,
+    bse.in (OCEAN2/OCEAN_system.f90:166-231, :310-335, :644-726;
+    OCEAN_multiplet.f90:298-303; OCEAN_haydock.f90:2109-2116)."""
+    _refuse(d, ["force_val_ham_spin.ipt", "file_brange.ipt", "rpa", "tdlda", "core_broaden.ipt",
+                "isolated.inp", "pbc.inp", "epsilon3D", "xshift_override.ipt"])
+    xmesh = tuple(int(t) for t in _tokens(_p(d, "xmesh.ipt"))[:3])
+    kmesh = tuple(int(t) for t in _tokens(_p(d, "kmesh.ipt"))[:3])
+    nspn = int(_tokens(_p(d, "nspin"))[0])
+    ZNL = tuple(int(t) for t in _tokens(_p(d, "ZNL"))[:3])
+    nb = int(_tokens(_p(d, "nbuse.ipt"))[0])
+    brange = tuple(int(t) for t in _tokens(_p(d, "brange.ipt"))[:4])
+    av = np.array([_ffloat(t) for t in _tokens(_p(d, "avecsinbohr.ipt"))[:9]])
+    avec = np.asfortranarray(av.reshape(3, 3, order="F"))
+    celvol = abs(float(np.linalg.det(avec)))
+    if not _fbool(_tokens(_p(d, "cks.normal"))[0]):
+        raise NotImplementedError("cks.normal .false. (XES) runs are not supported")
+    inter_scale = _ffloat(_tokens(_p(d, "mode"))[0])
+    epsilon0 = _ffloat(_tokens(_p(d, "epsilon"))[0])
+    mult = True
+    if os.path.exists(_p(d, "mult.ipt")):
+        mult = _fbool(_tokens(_p(d, "mult.ipt"))[0])
+    # runlist: the first core run
+    with open(_p(d, "runlist")) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip()]
+    nruns = int(lines[0].split()[0])
+    run = None
+    for ln in lines[1:1 + nruns]:
+        t = ln.split()
+        r = CoreRun(int(t[0]), int(t[1]), int(t[2]), t[3].strip("'\""), t[4].strip("'\""), int(t[5]),
+                    int(t[6]), t[7].strip("'\"").upper())
+        if r.calc_type in ("XAS",):
+            run = r
+            break
+        raise NotImplementedError(f"run type {r.calc_type} is not supported")
+    if run is None:
+        raise ValueError("runlist holds no core run")
+    if tuple(ZNL) != (run.Z, run.n, run.l):
+        raise ValueError("ZNL and the runlist entry disagree")
+    # site: the indx-th atom of that element in xyz.wyck
+    with open(_p(d, "xyz.wyck")) as f:
+        wl = [ln for ln in f.read().splitlines() if ln.strip()]
+    ntot = int(wl[0].split()[0])
+    seen = 0
+    tau = None
+    for ln in wl[1:1 + ntot]:
+        t = ln.split()
+        if t[0].strip("'\"") == run.elname:
+            seen += 1
+            if seen == run.indx:
+                tau = np.array([_ffloat(v) for v in t[1:4]])
+                break
+    if tau is None:
+        raise ValueError(f"site {run.elname} {run.indx} not in xyz.wyck")
+    # bse.in: lc lvl lvh / xi (eV) / hay / niter ne el eh gam ebase (eV)
+    with open(_p(d, "bse.in")) as f:
+        bl = [ln for ln in f.read().splitlines() if ln.strip()]
+    t = bl[0].split()
+    lc, lvl, lvh = int(t[0]), int(t[1]), int(t[2])
+    if lc != run.l:
+        raise ValueError("bse.in core l disagrees with ZNL")
+    xi = _ffloat(bl[1].split()[0])
+    solver = {"kind": bl[2].split()[0].strip("'\"")}
+    if solver["kind"] == "hay" and len(bl) > 3:
+        v = bl[3].split()
+        solver.update(niter=int(v[0]), ne=int(v[1]), el=_ffloat(v[2]), eh=_ffloat(v[3]),
+                      gam=_ffloat(v[4]), ebase=_ffloat(v[5]) if len(v) > 5 else 0.0)
+    return RunSystem(xmesh, kmesh, nspn, ZNL, nb, brange, avec, celvol, run, tau, inter_scale,
+                     epsilon0, mult, lvl, lvh, xi, solver)
+
+
+def read_u2(d, S: RunSystem):
+    """con.u2.dat: stream complex128, [spin][k][band 1..bandsInFile][x], x fastest;
+    the first nb bands of each k are kept (OCEAN2/OCEAN_bloch.f90:998-1115)."""
+    nb, nk, nx, ns, bif = S.nb, S.nk, S.nx, S.nspn, S.bands_in_file
+    if bif < nb:
+        raise ValueError("brange holds fewer conduction bands than nbuse.ipt")
+    raw = np.fromfile(_p(d, "con.u2.dat"), dtype=np.complex128)
+    if raw.size != ns * nk * bif * nx:
+        raise ValueError(f"con.u2.dat holds {raw.size} values, expected {ns * nk * bif * nx}")
+    a = raw.reshape(ns, nk, bif, nx)[:, :, :nb, :]
+    return np.asfortranarray(a.transpose(2, 1, 3, 0))  # (nb, nk, nx, nspn)
+
+
+def read_energies(d, S: RunSystem):
+    """wvfcninfo (Hartree, first nb bands), the Fermi level, the core-level shift and
+    the allow mask, spread over alpha by valence spin (OCEAN2/OCEAN_energies.f90:
+    303-368 load, :371-487 allow (default occupation), :629-700 read_energies,
+    :777-825 read_coreLevelShift, :829-887 stubby, :1340-1430 Fermi level)."""
+    _refuse(d, ["doping", "findNewFermi.ipt", "occupation.ipt", "GW", "gw_control"])
+    recs = fortran_records(_p(d, "wvfcninfo"))
+    hdr = np.frombuffer(recs[0], np.int32)
+    nbd, nq = int(hdr[0]), int(hdr[1])
+    nspn = int(hdr[2]) if S.nspn == 2 else 1
+    if nbd < S.nb or nq != S.nk or nspn != S.nspn:
+        raise ValueError("wvfcninfo does not match the run's bands, k points or spins")
+    E = np.frombuffer(recs[1], np.float64).reshape(nbd, nq, nspn, order="F")[:S.nb]
+    # Fermi level: efermiinrydberg.ipt (Rydberg) / 2
+    efermi = _ffloat(_tokens(_p(d, "efermiinrydberg.ipt"))[0]) / 2.0
+    # core-level shift: cls.z<El><indx>_n<nn>l<ll> (eV), else eshift.ipt unless noshift_lumo
+    r = S.run
+    cls = _p(d, "cls.z%2s%04d_n%02dl%02d" % (r.elname, r.indx, S.ZNL[1], S.ZNL[2]))
+    if os.path.exists(cls):
+        shift = _ffloat(_tokens(cls)[0]) * EV2HARTREE
+    else:
+        noshift = os.path.exists(_p(d, "noshift_lumo")) and _fbool(_tokens(_p(d, "noshift_lumo"))[0])
+        shift = 0.0 if noshift else _ffloat(_tokens(_p(d, "eshift.ipt"))[0]) * EV2HARTREE
+    # core_allow_fixed with the default occupation: XAS allows E >= E_F, else 0
+    allow_b = np.where(E >= efermi, 1.0, 0.0)
+    na = S.nalpha
+    vs = np.zeros(na, np.int64) if S.nspn == 1 else np.arange(na) % 2
+    energy = np.zeros((S.nb, S.nk, na), np.complex128, order="F")
+    allow = np.zeros((S.nb, S.nk, na), np.float64, order="F")
+    for a in range(na):
+        energy[:, :, a] = E[:, :, vs[a]] + shift
+        allow[:, :, a] = allow_b[:, :, vs[a]]
+    return energy, allow, efermi, shift
+
+
+def read_rpottrim(d, S: RunSystem):
+    """rpottrim.z<El><indx>_n<nn>l<ll>: a header (word, count), then (ptab, rtab) rows
+    (OCEAN2/long_range.f90:1688-1696)."""
+    r = S.run
+    name = "rpottrim.z%2s%04d_n%02dl%02d" % (r.elname, r.indx, S.ZNL[1], S.ZNL[2])
+    with open(_p(d, name)) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip()]
+    n = int(lines[0].split()[1])
+    vals = np.array([[_ffloat(t) for t in ln.split()[:2]] for ln in lines[1:1 + n]])
+    return np.ascontiguousarray(vals[:, 1]), np.ascontiguousarray(vals[:, 0])  # rtab, ptab
+
+
+def read_prjfile(d, S: RunSystem):
+    """prjfilez<Z>: lmin lmax, then nproj(lmin:lmax) (OCEAN2/OCEAN_multiplet.f90:309-316)."""
+    t = _tokens(_p(d, "prjfilez%03d" % S.ZNL[0]))
+    lmin, lmax = int(t[0]), int(t[1])
+    nproj = [int(v) for v in t[2:2 + lmax - lmin + 1]]
+    if S.lvl < lmin or S.lvh > lmax:
+        raise ValueError("bad lvl lvh")
+    return lmin, lmax, nproj
+
+
+def read_parcksc(d, S: RunSystem, nptot):
+    """parcksc.<El><indx>: stream int32 (nptot, ntot, nspn), then complex128
+    (nptot, ntot, nspn) (OCEAN2/OCEAN_multiplet.f90:426-451, OCEAN_psi.f90:5920-5937)."""
+    r = S.run
+    path = _p(d, "parcksc.%2s%04d" % (r.elname, r.indx))
+    hdr = np.fromfile(path, dtype=np.int32, count=3)
+    if int(hdr[0]) != nptot or int(hdr[2]) != S.nspn:
+        raise ValueError("parcksc header does not match the projectors or spins")
+    ntot = int(hdr[1])
+    nbd = ntot // S.nk
+    if nbd * S.nk != ntot or nbd < S.nb:
+        raise ValueError("Badly formed cks")
+    cks = np.fromfile(path, dtype=np.complex128, offset=12)
+    if cks.size != nptot * ntot * S.nspn:
+        raise ValueError("parcksc is truncated")
+    return np.asfortranarray(cks.reshape(nptot, ntot, S.nspn, order="F")), nbd
+
+
+def projector_array(cks, S: RunSystem, lmin, lmax, nproj, bands_per_k):
+    """mpcr/mpci(nb, nk, nu, m, l, s) from the cks projector order (l, m, nu)
+    (OCEAN2/OCEAN_multiplet.f90:451-462); bands past nb are skipped."""
+    npmax = max(nproj)
+    nl = lmax - lmin + 1
+    c = np.zeros((S.nb, S.nk, npmax, 2 * lmax + 1, nl, S.nspn), np.complex128, order="F")
+    byk = cks.reshape(cks.shape[0], bands_per_k, S.nk, S.nspn, order="F")[:, :S.nb]
+    ip = 0
+    for li, l in enumerate(range(lmin, lmax + 1)):
+        for m in range(-l, l + 1):
+            for nu in range(nproj[li]):
+                c[:, :, nu, m + lmax, li, :] = byk[ip]
+                ip += 1
+    return c
+
+
+def read_cmel(d, lmin, lmax, nproj):
+    """cmel<l>: nproj(l)^2 values (eV) in column order -> mpm (Hartree)
+    (OCEAN2/OCEAN_multiplet.f90:486-504)."""
+    npmax = max(nproj)
+    mpm = np.zeros((npmax, npmax, lmax - lmin + 1), order="F")
+    for li, l in enumerate(range(lmin, lmax + 1)):
+        npl = nproj[li]
+        v = np.array([_ffloat(t) for t in _tokens(_p(d, "cmel%d" % l))[:npl * npl]])
+        mpm[:npl, :npl, li] = v.reshape(npl, npl, order="F")
+    return mpm * EV2HARTREE
+
+
+def read_slater(d, S: RunSystem, lv, npl):
+    """fk<lc><lv><k>z<Z>n<nn>l<ll> / gk...: np x np Slater integrals and their scale,
+    kept pre-multiplied; scfac (default 1) (OCEAN2/OCEAN_multiplet.f90:2064-2114)."""
+    lc = S.ZNL[2]
+    add10 = "z%03dn%02dl%02d" % S.ZNL
+    scfac = 1.0
+    if os.path.exists(_p(d, "scfac")):
+        scfac = _ffloat(_tokens(_p(d, "scfac"))[0])
+
+    def one(prefix, k):
+        t = _tokens(_p(d, "%2s%1d%1d%1d%s" % (prefix, lc, lv, k, add10)))
+        v = np.array([_ffloat(x) for x in t[:npl * npl + 1]])
+        return v[:npl * npl].reshape(npl, npl, order="F") * v[npl * npl]
+
+    fk = {k: one("fk", k) for k in range(2, min(2 * lc, 2 * lv) + 1, 2)}
+    gk = {k: one("gk", k) for k in range(abs(lc - lv), lc + lv + 1, 2)}
+    return fk, gk, scfac
+
+
+def read_mels(d, S: RunSystem, nptot):
+    """mels.z<Z>n<nn>l<ll>p<photon>: (re, im) per (m_c, p), m_c outer
+    (OCEAN2/OCEAN_psi.f90:5964-5975)."""
+    r = S.run
+    t = _tokens(_p(d, "mels.z%03dn%02dl%02dp%02d" % (S.ZNL[0], S.ZNL[1], S.ZNL[2], r.photon)))
+    lc = S.ZNL[2]
+    v = np.array([_ffloat(x) for x in t[:2 * nptot * (2 * lc + 1)]]).reshape(2 * lc + 1, nptot, 2)
+    return np.asfortranarray((v[:, :, 0] + 1j * v[:, :, 1]).T)  # (nptot, 2lc+1)
+
+
+def starting_vector(mels, cks, S: RunSystem, bands_per_k):
+    """psi0 from the dipole elements and projector overlaps, then normalised with
+    kpref = 4 pi |psi|^2 / (N_k Omega^2) (OCEAN2/OCEAN_psi.f90:5887-6031, :5589-5638)."""
+    lc, nb, nk, na = S.ZNL[2], S.nb, S.nk, S.nalpha
+    psi = np.zeros((nb, nk, na), np.complex128, order="F")
+    ia = 0
+    for icms in (-1, 1):
+        for icml in range(-lc, lc + 1):
+            for ivms in (-1, 1):
+                if icms == ivms:
+                    sp = 0 if S.nspn == 1 else (0 if ivms == -1 else 1)
+                    byk = cks[:, :, sp].reshape(cks.shape[0], bands_per_k, nk, order="F")[:, :nb]
+                    me = mels[:, icml + lc]
+                    # rr - ii + i(-ri - ir) = conj(me) . conj(pc) summed over projectors
+                    psi[:, :, ia] = np.einsum("p,pnk->nk", np.conj(me), np.conj(byk))
+                ia += 1
+    return sh.normalise_psi0(psi, nk, S.celvol)
+
+
+@dataclass
+class RunInputs:
+    """Everything a core run's handle is loaded with, read from the directory."""
+    system: RunSystem
+    u_raw: np.ndarray
+    xoff: np.ndarray          # int32 (3, nx) wrapped x offsets for the device phase
+    tau_c: np.ndarray
+    xshift: np.ndarray
+    rdist: np.ndarray         # r(k, x) for the device W
+    rtab: np.ndarray
+    ptab: np.ndarray
+    energy: np.ndarray
+    allow: np.ndarray
+    mult: MultipletInputs
+    psi0: np.ndarray
+    kpref: float
+    efermi: float
+    shift: float
+
+
+def read_rundir(d):
+    """Read a core XAS run directory into the loader arrays."""
+    S = read_system(d)
+    u_raw = read_u2(d, S)
+    xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau)
+    rdist, _ = sh.w_geometry(S.avec, S.kmesh, S.xmesh, tau_c, xshift)
+    rtab, ptab = read_rpottrim(d, S)
+    energy, allow, efermi, shift = read_energies(d, S)
+    lmin, lmax, nproj = read_prjfile(d, S)
+    nptot = sum((2 * l + 1) * nproj[i] for i, l in enumerate(range(lmin, lmax + 1)))
+    cks, bpk = read_parcksc(d, S, nptot)
+    c = projector_array(cks, S, lmin, lmax, nproj, bpk)
+    mpm = read_cmel(d, lmin, lmax, nproj)
+    X = []
+    for lv in range(S.lvl, S.lvh + 1):
+        npl = nproj[lv - lmin]
+        fk, gk, scfac = read_slater(d, S, lv, npl)
+        X.append(np.asfortranarray(sh.exchange_matrix(S.ZNL[2], lv, npl, fk, gk, scfac) * EV2HARTREE))
+    sr, si, have_so = sh.spin_orbit(S.ZNL[2], S.xi_eV)
+    mult = MultipletInputs(lmin, lmax, nproj, S.lvl, S.lvh, c, mpm, X, sr, si, have_so)
+    mels = read_mels(d, S, nptot)
+    psi0, kpref = starting_vector(mels, cks, S, bpk)
+    return RunInputs(S, u_raw, xoff, tau_c, xshift, rdist, rtab, ptab, energy, allow, mult, psi0,
+                     kpref, efermi, shift)
+
+
+def open_rundir(d, precision=None):
+    """A loaded handle for the run directory: u phased and W built on the device.
+    Returns (OceanGPU, RunInputs)."""
+    from . import core
+    R = read_rundir(d)
+    S = R.system
+    m = R.mult
+    prec = core.FP64 if precision is None else precision
+    g = core.OceanGPU(S.nb, S.kmesh, S.xmesh, S.ZNL[2], S.nspn, S.celvol, S.inter_scale, m.lmin, m.lmax,
+                      tuple(m.nproj), m.lvl, m.lvh, True, S.mult, S.inter_scale >= 0.000001, prec)  # inter_min, OCEAN_system.f90:157
+    g.load_bloch_raw(R.u_raw, R.xoff)
+    g.build_w(R.rdist, R.rtab, R.ptab, S.epsilon0, S.inter_scale)
+    g.load_mult(m)
+    g.load_energies(R.energy, R.allow)
+    g.kpref = R.kpref
+    return g, R
+
+
+def hartree_window(S: RunSystem):
+    """The Haydock settings of bse.in line 4 (eV) converted as OCEAN_haydock.f90:
+    2109-2116 does: (niter, ne, el, eh, gam, ebase) with energies in Hartree."""
+    s = S.solver
+    return (s["niter"], s["ne"], s["el"] * EV2HARTREE, s["eh"] * EV2HARTREE, s["gam"] * EV2HARTREE,
+            s["ebase"])
+
+
+__all__ = ["read_system", "read_u2", "read_energies", "read_rpottrim", "read_prjfile", "read_parcksc",
+           "projector_array", "read_cmel", "read_slater", "read_mels", "starting_vector", "read_rundir",
+           "open_rundir", "fortran_records", "RunInputs", "RunSystem", "CoreRun", "hartree_window"]

@@ -33,10 +33,11 @@ def alpha_table(lc):
     return [(s, m, v) for s in (-1, 1) for m in range(-lc, lc + 1) for v in (-1, 1)]
 
 
-def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
-    """Phase u(n,k,x) by exp(2 pi i sum_d (x_d + wrap_d) k_d / (Nx_d Nk_d)).
+def x_offsets(xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
+    """The site centring and wrapped x offsets of OCEAN_bloch_lrLOAD.
 
-    Returns (u_phased, tau_centred, xshift).  Vectorised over (k, x).
+    Returns (xoff int32 [x, 3] with ix fastest over x, tau_centred, xshift); xoff
+    is the layout ocean_gpu_load_bloch_raw takes (xoff(3, nx) in Fortran order).
     """
     xm = np.array(xmesh, np.int64)
     km = np.array(kmesh, np.int64)
@@ -50,7 +51,18 @@ def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
     idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1)  # [x, 3], 1-based
     targ = idx - xshift[None, :]
     wrap = np.where(targ > xm[None, :], -xm[None, :], np.where(targ < 1, xm[None, :], 0))
-    xoff = (wrap + idx - 1).astype(np.float64)  # [x, 3]
+    return np.ascontiguousarray(wrap + idx - 1, dtype=np.int32), tau_c, xshift
+
+
+def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
+    """Phase u(n,k,x) by exp(2 pi i sum_d (x_d + wrap_d) k_d / (Nx_d Nk_d)).
+
+    Returns (u_phased, tau_centred, xshift).  Vectorised over (k, x).
+    """
+    xm = np.array(xmesh, np.int64)
+    km = np.array(kmesh, np.int64)
+    xo, tau_c, xshift = x_offsets(xmesh, kmesh, tau, xshift_override)
+    xoff = xo.astype(np.float64)  # [x, 3]
     k1, k2, k3 = np.meshgrid(np.arange(km[0]), np.arange(km[1]), np.arange(km[2]), indexing="ij")
     kid = np.stack([k1.ravel(), k2.ravel(), k3.ravel()], axis=1).astype(np.float64)  # [k, 3]
     ph = np.zeros((kid.shape[0], xoff.shape[0]))

@@ -29,6 +29,8 @@ Pinning (what checks each function; "parity unpinned" where none does):
                        truncation) and its fine-grid reuse against direct evaluation
   gamfcn               the same fixture (its only consumer), zero below the Fermi level
   calc_spect_core      write_core at the midpoint grid
+  energies_core_xas    the allow mask's defining inequality on a seeded directory
+                       (parity of the branch itself is by restatement only)
   o_spect_core         the Fe absspct fixture (Spect, Spect(0) and SPKK columns), the
                        resonant-minus-anti-resonant Lorentzian closed form for one level
   o_spect_val          the Lorentz-oscillator closed form of eps for one level; n + i kappa
@@ -921,6 +923,31 @@ def core_pnorm(psi, nk, celvol):
     val = math.sqrt(val)
     kpref = 4.0 * math.pi * val ** 2 / (nk * celvol ** 2)
     return np.asfortranarray(psi * (1.0 / val)), kpref
+
+
+def energies_core_xas(E, efermi, shift, lc, nspn):
+    """p_energy and allow of a core XAS run with the default occupation.
+
+    OCEAN_energies.f90:303-368 (load), :421-487 (core_allow_fixed without a
+    temperature: allow 1 for E >= E_F, 0 below), :829-887 (stubby: alpha takes the
+    band energies of its valence spin, shifted by the core-level shift).  E is
+    (nb, nk, nspn) in Hartree.  Returns (energy complex (nb, nk, na), allow).
+    """
+    nb, nk = E.shape[0], E.shape[1]
+    na = 4 * (2 * lc + 1)
+    energy = np.zeros((nb, nk, na), np.complex128, order="F")
+    allow = np.zeros((nb, nk, na), order="F")
+    ialpha = 0
+    for _icms in range(2):
+        for _icml in range(-lc, lc + 1):
+            for ivms in range(2):
+                vs = 0 if nspn == 1 else ivms
+                for k in range(nk):
+                    for n in range(nb):
+                        energy[n, k, ialpha] = E[n, k, vs] + shift
+                        allow[n, k, ialpha] = 1.0 if E[n, k, vs] >= efermi else 0.0
+                ialpha += 1
+    return energy, allow
 
 
 def psi_dotter(mels, cks, lc, nb, nk, nspn, bands_in_file=None):

@@ -1,0 +1,60 @@
+"""A core run read from its run directory, phased and W-built on the device, against
+the oracle's action and recursion on the same files."""
+import numpy as np
+import pytest
+
+import ocean_oracle as orc
+from helpers import relerr
+from rundir_files import EV2H, write_rundir
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {
+    "k_edge": dict(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=1, nproj=(2, 1, 1)),
+    "l23_spin": dict(nb=6, kmesh=(2, 1, 3), xmesh=(3, 2, 3), lc=1, nspn=2, seed=2, xi_eV=3.0, lmin=1, lmax=2,
+                     nproj=(1, 2)),
+}
+
+
+def oracle_problem(gen, R):
+    S = R.system
+    u, tau_c, xshift = orc.bloch_phase(R.u_raw, S.xmesh, S.kmesh, S.tau)
+    W = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, R.rtab, R.ptab, S.epsilon0, S.inter_scale)
+    energy, allow = orc.energies_core_xas(gen["E"][:gen["nb"]], gen["efermi"], gen["eshift"], gen["lc"],
+                                          gen["nspn"])
+    m = R.mult
+    X = []
+    for lv in range(m.lvl, m.lvh + 1):
+        npl = m.nproj[lv - m.lmin]
+        fk = {k: gen["slater"][("fk", lv, k)][0] for k in range(2, min(2 * gen["lc"], 2 * lv) + 1, 2)}
+        sf = {k: gen["slater"][("fk", lv, k)][1] for k in fk}
+        gk = {k: gen["slater"][("gk", lv, k)][0] for k in range(abs(gen["lc"] - lv), gen["lc"] + lv + 1, 2)}
+        sg = {k: gen["slater"][("gk", lv, k)][1] for k in gk}
+        X.append(orc.nbsemhsetup2(gen["lc"], lv, npl, fk, sf, gk, sg) * EV2H)
+    sr, si, have_so = orc.soprep(gen["lc"], gen["xi_eV"])
+    M = orc.Multiplet(m.lmin, m.lmax, list(m.nproj), m.lvl, m.lvh, m.c, m.mpm, X, sr, si, have_so)
+    P = orc.Problem(S.nb, S.kmesh, S.xmesh, gen["lc"], S.nspn, S.celvol, S.inter_scale, u, W, energy, allow, M)
+    psi = orc.psi_dotter(gen["mels"], gen["cks"], gen["lc"], gen["nb"], S.nk, gen["nspn"],
+                         bands_in_file=gen["bif"])
+    psi0, kpref = orc.core_pnorm(psi, S.nk, S.celvol)
+    return P, psi0, kpref
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_rundir_action_and_recursion(tmp_path, name):
+    from ocean_amd import rundir
+    d = str(tmp_path / name)
+    gen = write_rundir(d, **SHAPES[name])
+    print("run directory case", name, flush=True)
+    g, R = rundir.open_rundir(d)
+    P, psi0, kpref = oracle_problem(gen, R)
+    assert abs(g.kpref - kpref) <= 1e-14 * kpref
+    for back in (False, True):
+        assert relerr(g.xact(R.psi0, backwards=back), orc.xact(P, psi0, backwards=back)) < 1e-12, back
+    niter = 20
+    a, b, _ = g.haydock(R.psi0, niter)
+    ra, rb, _, ierr = orc.haydock(P, psi0, niter)
+    assert ierr == 0
+    assert np.max(np.abs(a[:niter] - ra[:niter]) / np.abs(ra[:niter])) < 1e-10
+    assert np.max(np.abs(b[1:niter + 1] - rb[1:niter + 1]) / np.abs(rb[1:niter + 1])) < 1e-10
+    g.close()
