@@ -188,7 +188,7 @@ def gradient_workload(log_prefix="grad"):
     return out
 
 
-def precision_workload(reps=20):
+def precision_workload(reps=20, with_haydock=True):
     """Config 2 (L2,3 edge, nalpha 12, nb 128, k-mesh 8^3, x-mesh 10^3, spin-orbit
     10 eV): the mixed form against the FP64 drop-in over 500 Haydock iterations and
     4000 energy points (max error of peak and L2 relative error of the spectrum),
@@ -209,13 +209,19 @@ def precision_workload(reps=20):
         g.xact_repeat(vin, vout, 2)
         ms = g.xact_repeat(vin, vout, reps)
         out[f"cfg2_hpsi_per_s_{name}"] = round(1e3 / ms, 1)
+        out[f"cfg2_per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in
+                                             g.profile_xact(vin, vout, reps).items()}
+        log(f"config 2 {name}: {ms:.4f} ms per H.psi")
+        if not with_haydock:
+            continue
         t1 = time.perf_counter()
         a, b, _ = g.haydock(inp.psi0, p["niter"])
         sp = core.spectrum(a, b, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"] - 1)
         out[f"cfg2_spectra_per_s_{name}"] = round(1.0 / (time.perf_counter() - t1), 2)
         spec[name] = sp[:, 2]
-        log(f"config 2 {name}: {ms:.4f} ms per H.psi")
     g.close()
+    if not with_haydock:
+        return out
     s64, smx = spec["fp64"], spec["mixed"]
     out["cfg2_points"] = int(s64.size)
     out["cfg2_mixed_max_err_of_peak"] = float(np.abs(smx - s64).max() / np.abs(s64).max())
@@ -247,6 +253,8 @@ def main():
     ap.add_argument("--no-cfg2", action="store_true", help="skip the L2,3 precision configuration")
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-niter", type=int, default=200)
+    ap.add_argument("--cfg2-only", action="store_true",
+                    help="only the L2,3 configuration's H.psi rates and kernel times (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -264,6 +272,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.cfg2_only:
+        print(json.dumps(precision_workload(reps=args.steps, with_haydock=False)), flush=True)
+        return
 
     from ocean_amd import core, synth
 
