@@ -81,10 +81,10 @@ def cpu_baseline(inp, budget_s):
 
 
 def batch_workload(n, niter, reps, log_every=8):
-    """Config 3: n uneven solves (K and L2,3 edges, nb 32..160, k-mesh 4^3..8^3,
+    """The uneven batch: n solves (K and L2,3 edges, nb 32..160, k-mesh 4^3..8^3,
     x-mesh 8^3).  u is a complex Gaussian / sqrt(2 N_x) drawn in FP32 and widened
     (not QR'd: same shapes and bytes as the QR'd generator, which would take
-    minutes at 20 GB); the other inputs come from the config-0 generators.
+    minutes at 20 GB); the other inputs come from the same seeded generators.
     Returns the batched-form figures for the JSON line."""
     from ocean_amd import core, synth
 
@@ -152,13 +152,13 @@ def batch_workload(n, niter, reps, log_every=8):
 
 
 def gradient_workload(log_prefix="grad"):
-    """Config 4: config-1 shapes, GMRES at 128 energies (Gamma 0.5 eV), misfit to a
+    """The spectral gradient at the 64-band K-edge shapes: GMRES at 128 energies (Gamma 0.5 eV), misfit to a
     target spectrum made with perturbed parameters (W table x 1.1 plus a Gaussian
     bump, Gamma 0.7 eV, scissor +0.3 eV); gradient w.r.t. the 200-knot W table,
     the screening scale, Gamma and the scissor, FP64 and mixed."""
     from ocean_amd import core, setup_host as sh, synth
 
-    inp = synth.config(4)
+    inp = synth.workload("k_nb64_grad")
     rdist = sh.w_geometry(inp.avec, inp.kmesh, inp.xmesh, inp.tau, inp.xshift)[0]
     E = np.linspace(-5.0, 40.0, 128)
     p = inp.params
@@ -189,15 +189,15 @@ def gradient_workload(log_prefix="grad"):
 
 
 def precision_workload(reps=20, with_haydock=True):
-    """Config 2 (L2,3 edge, nalpha 12, nb 128, k-mesh 8^3, x-mesh 10^3, spin-orbit
+    """The L2,3 workload (L2,3 edge, nalpha 12, nb 128, k-mesh 8^3, x-mesh 10^3, spin-orbit
     10 eV): the mixed form against the FP64 drop-in over 500 Haydock iterations and
     4000 energy points (max error of peak and L2 relative error of the spectrum),
     plus both H.psi rates."""
     from ocean_amd import core, synth
 
     t0 = time.perf_counter()
-    inp = synth.config(2)
-    log(f"config 2 inputs in {time.perf_counter() - t0:.1f}s")
+    inp = synth.workload("l23_nb128")
+    log(f"L2,3 inputs in {time.perf_counter() - t0:.1f}s")
     p = inp.params
     E2H = core.EV2HARTREE
     g = core.OceanGPU.from_inputs(inp)
@@ -208,29 +208,29 @@ def precision_workload(reps=20, with_haydock=True):
         g.set_precision(prec)
         g.xact_repeat(vin, vout, 2)
         ms = g.xact_repeat(vin, vout, reps)
-        out[f"cfg2_hpsi_per_s_{name}"] = round(1e3 / ms, 1)
-        out[f"cfg2_per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in
+        out[f"l23_hpsi_per_s_{name}"] = round(1e3 / ms, 1)
+        out[f"l23_per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in
                                              g.profile_xact(vin, vout, reps).items()}
-        log(f"config 2 {name}: {ms:.4f} ms per H.psi")
+        log(f"L2,3 {name}: {ms:.4f} ms per H.psi")
         if not with_haydock:
             continue
         t1 = time.perf_counter()
         a, b, _ = g.haydock(inp.psi0, p["niter"])
         sp = core.spectrum(a, b, p["niter"], inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"] - 1)
-        out[f"cfg2_spectra_per_s_{name}"] = round(1.0 / (time.perf_counter() - t1), 2)
+        out[f"l23_spectra_per_s_{name}"] = round(1.0 / (time.perf_counter() - t1), 2)
         spec[name] = sp[:, 2]
     g.close()
     if not with_haydock:
         return out
     s64, smx = spec["fp64"], spec["mixed"]
-    out["cfg2_points"] = int(s64.size)
-    out["cfg2_mixed_max_err_of_peak"] = float(np.abs(smx - s64).max() / np.abs(s64).max())
-    out["cfg2_mixed_l2_rel_err"] = float(np.linalg.norm(smx - s64) / np.linalg.norm(s64))
+    out["l23_points"] = int(s64.size)
+    out["l23_mixed_max_err_of_peak"] = float(np.abs(smx - s64).max() / np.abs(s64).max())
+    out["l23_mixed_l2_rel_err"] = float(np.linalg.norm(smx - s64) / np.linalg.norm(s64))
     return out
 
 
 def gmres_workload(inp, g, n=64):
-    """Config 1 GMRES: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
+    """GMRES at the 64-band K-edge workload: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
     E = np.linspace(-5.0, 40.0, n)
     g.gmres(inp.psi0, E[:2], 0.5, 13.605, 1e-16, 20)
     t0 = time.perf_counter()
@@ -250,10 +250,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 H.psi rate")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batched form")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient (differentiation) form")
-    ap.add_argument("--no-cfg2", action="store_true", help="skip the L2,3 precision configuration")
+    ap.add_argument("--no-l23", action="store_true", help="skip the L2,3 precision configuration")
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-niter", type=int, default=200)
-    ap.add_argument("--cfg2-only", action="store_true",
+    ap.add_argument("--l23-only", action="store_true",
                     help="only the L2,3 configuration's H.psi rates and kernel times (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -272,14 +272,14 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
-    if args.cfg2_only:
+    if args.l23_only:
         print(json.dumps(precision_workload(reps=args.steps, with_haydock=False)), flush=True)
         return
 
     from ocean_amd import core, synth
 
     t0 = time.perf_counter()
-    inp = synth.config(1)
+    inp = synth.workload("k_nb64")
     log(f"inputs ready in {time.perf_counter() - t0:.1f}s")
     g = core.OceanGPU.from_inputs(inp, precision=core.FP64)
     vin = g.vec_new(inp.psi0)
@@ -349,7 +349,7 @@ def main():
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})
-    if not args.no_extras and not args.no_cfg2:
+    if not args.no_extras and not args.no_l23:
         extras.update(precision_workload())
     if not args.no_extras and not args.no_grad:
         g.close()

@@ -195,23 +195,27 @@ def make_inputs(nb, kmesh, xmesh, lc=0, nspn=1, seed=0, xi_eV=0.0, lmin=0, lmax=
                      dict(params or {}))
 
 
-# Solver settings per named configuration (energies in eV)
-CONFIGS = {
-    0: dict(nb=16, kmesh=(4, 4, 4), xmesh=(6, 6, 6), lc=0,
-            solver=dict(niter=100, gam=0.5, el=-5.0, eh=40.0, ne=2000)),
-    1: dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
-            solver=dict(niter=300, gam=0.5, el=-5.0, eh=40.0, ne=2000,
-                        gmres_n=64, gmres_tol=1e-8)),
-    2: dict(nb=128, kmesh=(8, 8, 8), xmesh=(10, 10, 10), lc=1, xi=10.0 / 1.5,
-            solver=dict(niter=500, gam=0.5, el=-5.0, eh=40.0, ne=4000)),
-    4: dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
-            solver=dict(gam=0.5, el=-5.0, eh=40.0, gmres_n=128, gmres_tol=1e-8)),
+# Named workloads with their solver settings (energies in eV)
+WORKLOADS = {
+    # K edge, 16 bands, k-mesh 4^3, x-mesh 6^3: the size the CPU reference runs quickly
+    "k_nb16": dict(nb=16, kmesh=(4, 4, 4), xmesh=(6, 6, 6), lc=0,
+                   solver=dict(niter=100, gam=0.5, el=-5.0, eh=40.0, ne=2000)),
+    # K edge, 64 bands, k-mesh 8^3, x-mesh 8^3: the drop-in timing workload
+    "k_nb64": dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
+                   solver=dict(niter=300, gam=0.5, el=-5.0, eh=40.0, ne=2000,
+                               gmres_n=64, gmres_tol=1e-8)),
+    # L2,3 edge (nalpha 12), 128 bands, x-mesh 10^3, spin-orbit splitting 10 eV
+    "l23_nb128": dict(nb=128, kmesh=(8, 8, 8), xmesh=(10, 10, 10), lc=1, xi=10.0 / 1.5,
+                      solver=dict(niter=500, gam=0.5, el=-5.0, eh=40.0, ne=4000)),
+    # the K-edge 64-band shapes with GMRES at 128 energies (the spectral-gradient workload)
+    "k_nb64_grad": dict(nb=64, kmesh=(8, 8, 8), xmesh=(8, 8, 8), lc=0,
+                        solver=dict(gam=0.5, el=-5.0, eh=40.0, gmres_n=128, gmres_tol=1e-8)),
 }
 
 
-def config(i, seed=0):
-    """Inputs for named configuration ``i`` (0, 1, 2 or 4)."""
-    c = CONFIGS[i]
+def workload(name, seed=0):
+    """Inputs of a named workload (a key of WORKLOADS)."""
+    c = WORKLOADS[name]
     return make_inputs(c["nb"], c["kmesh"], c["xmesh"], lc=c["lc"], seed=seed,
                        xi_eV=c.get("xi", 0.0), params=dict(c["solver"]))
 

@@ -32,8 +32,8 @@ def _gpu(inp, **kw):
 
 
 @pytest.fixture(scope="module")
-def cfg0():
-    return synth.config(0)
+def k16():
+    return synth.workload("k_nb16")
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -76,33 +76,33 @@ def test_mixed_xact_close_to_oracle(name):
             assert relerr(got, ref) < 1e-5, (name, back, relerr(got, ref))
 
 
-def test_xact_config0(cfg0):
-    P = to_problem(cfg0)
-    g = _gpu(cfg0)
-    ref = orc.xact(P, cfg0.psi0)
-    got = g.xact(cfg0.psi0)
+def test_xact_k_nb16(k16):
+    P = to_problem(k16)
+    g = _gpu(k16)
+    ref = orc.xact(P, k16.psi0)
+    got = g.xact(k16.psi0)
     assert relerr(got, ref) < 1e-12
     g.close()
 
 
-def test_haydock_config0(cfg0):
-    P = to_problem(cfg0)
-    g = _gpu(cfg0)
-    niter = cfg0.params["niter"]
-    a_ref, b_ref, ia_ref, ierr = orc.haydock(P, cfg0.psi0, niter)
-    a, b, ai = g.haydock(cfg0.psi0, niter)
+def test_haydock_k_nb16(k16):
+    P = to_problem(k16)
+    g = _gpu(k16)
+    niter = k16.params["niter"]
+    a_ref, b_ref, ia_ref, ierr = orc.haydock(P, k16.psi0, niter)
+    a, b, ai = g.haydock(k16.psi0, niter)
     assert ierr == 0
     ra = np.abs(a[:50] - a_ref[:50]) / np.abs(a_ref[:50])
     rb = np.abs(b[1:51] - b_ref[1:51]) / np.abs(b_ref[1:51])
     print("max rel a", ra.max(), "b", rb.max())
     assert ra.max() < 1e-10 and rb.max() < 1e-10
     # spectra (literal write_core) within 1e-8 of peak
-    p = cfg0.params
+    p = k16.params
     E2H = orc.EV2HARTREE
-    sp_ref = orc.write_core(a_ref, b_ref, niter, cfg0.kpref, p["gam"] * E2H, p["el"] * E2H,
+    sp_ref = orc.write_core(a_ref, b_ref, niter, k16.kpref, p["gam"] * E2H, p["el"] * E2H,
                             p["eh"] * E2H, p["ne"])
     from ocean_amd import core
-    sp = core.spectrum(a, b, niter, cfg0.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])
+    sp = core.spectrum(a, b, niter, k16.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])
     peak = np.abs(sp_ref[:, 2]).max()
     err = np.abs(sp[:, 2] - sp_ref[:, 2]).max() / peak
     print("spectrum err/peak", err)
@@ -194,8 +194,8 @@ def test_tridiag_eigs_match_fixture_block():
     assert np.abs(w - orc.redtrid_eigs(a, b, 3)).max() < 1e-14
 
 
-def test_vector_ops(cfg0):
-    g = _gpu(cfg0)
+def test_vector_ops(k16):
+    g = _gpu(k16)
     rng = np.random.default_rng(5)
     p = np.asfortranarray(rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
     q = np.asfortranarray(rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
@@ -210,7 +210,7 @@ def test_vector_ops(cfg0):
     assert relerr(g.vec_get(vp), 2.5 * p) < 1e-15
     vo = g.vec_new()
     g.xact_vec(vp, vo)
-    assert relerr(g.vec_get(vo), orc.xact(to_problem(cfg0), 2.5 * p)) < 1e-12
+    assert relerr(g.vec_get(vo), orc.xact(to_problem(k16), 2.5 * p)) < 1e-12
     # axmy / axmz / divide in the reference's operation order (device FMA contraction aside)
     P2, Q2 = g.vec_get(vp), g.vec_get(vq)
     vy = g.vec_new(Q2)
@@ -240,18 +240,18 @@ def test_gmres_small():
     g.close()
 
 
-def test_haydock_early_exit(cfg0):
+def test_haydock_early_exit(k16):
     """The device loop stops where the reference's early exit would (replayed by the
     oracle over the same coefficients) and keeps the coefficients up to there."""
-    g = _gpu(cfg0)
-    niter, spacing, ne = cfg0.params["niter"], 10, 500
+    g = _gpu(k16)
+    niter, spacing, ne = k16.params["niter"], 10, 500
     E2H = orc.EV2HARTREE
     gam, el, eh = 0.5 * E2H, -5.0 * E2H, 40.0 * E2H
-    a, b, ai = g.haydock(cfg0.psi0, niter)
+    a, b, ai = g.haydock(k16.psi0, niter)
     rel = []
     for it in range(spacing + 1, niter + 1):
-        s1 = orc.calc_spect_core(a, b, it, cfg0.kpref, gam, el, eh, ne, ai)[2]
-        s2 = orc.calc_spect_core(a, b, it - spacing, cfg0.kpref, gam, el, eh, ne, ai)[2]
+        s1 = orc.calc_spect_core(a, b, it, k16.kpref, gam, el, eh, ne, ai)[2]
+        s2 = orc.calc_spect_core(a, b, it - spacing, k16.kpref, gam, el, eh, ne, ai)[2]
         rel.append(2 * np.abs(s1 - s2).sum() / (np.abs(s1).sum() + np.abs(s2).sum()))
     rel = np.array(rel)
     # a threshold well away from every relArea value that stops the loop mid-way
@@ -261,12 +261,12 @@ def test_haydock_early_exit(cfg0):
         t = np.sqrt(lo * hi)
         if hi / lo < 1.01:
             continue
-        stop = orc.haydock_early_exit(a, b, niter, spacing, t, cfg0.kpref, gam, el, eh, ne, ai)
+        stop = orc.haydock_early_exit(a, b, niter, spacing, t, k16.kpref, gam, el, eh, ne, ai)
         if spacing + 2 < stop < niter - 10:
             thr = t
             break
     assert thr is not None
-    ae, be, aie, done = g.haydock_early(cfg0.psi0, niter, spacing, thr, gam, el, eh, ne)
+    ae, be, aie, done = g.haydock_early(k16.psi0, niter, spacing, thr, gam, el, eh, ne)
     print("threshold", thr, "stop", stop, "device", done, flush=True)
     assert done == stop
     assert np.array_equal(ae[:done], a[:done]) and np.array_equal(be[:done + 1], b[:done + 1])
