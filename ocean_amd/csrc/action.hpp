@@ -312,10 +312,15 @@ __device__ __forceinline__ void proj_dispatch(const ActArgs<T2>& A, int b) {
   if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, A.nb, A.nk, A.na, A.ns, A.P, k, s, bx, A.px);
 }
 
-// One problem: a 1-D grid of px * nk * ns slab tiles.
+// One problem: a 1-D grid of px * nk * ns slab tiles.  Workgroups are dispatched
+// round-robin over the 8 XCDs; the remap gives each XCD a contiguous range of tiles,
+// so the px tiles of one k (which stage the same A psi) share an L2.
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) void k_proj(const ActArgs<T2> A) {
-  proj_dispatch<T2, NA>(A, blockIdx.x);
+  int b = blockIdx.x;
+  const int n = gridDim.x;
+  if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
+  proj_dispatch<T2, NA>(A, b);
 }
 
 // Many problems: workgroup b belongs to the problem whose range holds it.
