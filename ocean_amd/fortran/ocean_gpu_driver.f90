@@ -2,7 +2,8 @@
 ! vector), then runs OCEAN_xact_gpu, the Haydock recursion with its absspct /
 ! abslanc output, and optionally GMRES, all through the HIP C-ABI.
 !
-! usage: ocean_gpu_driver.x <input> <output-prefix>
+! usage: ocean_gpu_driver.x <input> <output-prefix> [<spacing> <threshold> <ne>]
+!   (the optional triple runs the recursion with the reference's early exit)
 ! input (stream, little endian):
 !   int32  nb k1 k2 k3 x1 x2 x3 nspn lc lmin lmax lvl lvh nproj(8) have_so niter ne nen depth precision
 !   real8  celvol inter_scale kpref gam el eh gres gprc ffff
@@ -16,7 +17,10 @@ program ocean_gpu_driver
   use ocean_gpu_host
   implicit none
   integer, parameter :: DP = kind(1.0d0)
-  character(len=512) :: fin, prefix
+  character(len=512) :: fin, prefix, arg
+  integer :: conv_spacing, conv_ne
+  real(DP) :: conv_thresh
+  logical :: early
   integer :: hdr(27), nb, nk, nx, nspn, lc, lmin, lmax, lvl, lvh, npmax, nl, na, niter, ne, nen, depth
   integer :: ierr, done, i, l, mhsum, have_so, prec
   real(DP) :: sc(9)
@@ -27,6 +31,15 @@ program ocean_gpu_driver
 
   call get_command_argument(1, fin)
   call get_command_argument(2, prefix)
+  early = command_argument_count() >= 5
+  if (early) then
+    call get_command_argument(3, arg)
+    read(arg, *) conv_spacing
+    call get_command_argument(4, arg)
+    read(arg, *) conv_thresh
+    call get_command_argument(5, arg)
+    read(arg, *) conv_ne
+  end if
   open(unit=11, file=trim(fin), form='unformatted', access='stream', status='old')
   read(11) hdr
   nb = hdr(1); nk = hdr(2) * hdr(3) * hdr(4); nx = hdr(5) * hdr(6) * hdr(7)
@@ -80,13 +93,20 @@ program ocean_gpu_driver
                                      e_re, e_im, allow, ierr)
   if (ierr == 0) call OCEAN_xact_gpu(p_re, p_im, h_re, h_im, ierr)
   if (ierr == 0 .and. niter > 0) then
-    call OCEAN_haydock_do_gpu(p_re, p_im, niter, ra, rb, ia, done, ierr)
+    if (early) then
+      call OCEAN_haydock_do_gpu(p_re, p_im, niter, ra, rb, ia, done, ierr, conv_spacing, conv_thresh, &
+                                sc(3), sc(4), sc(5), sc(6), conv_ne)
+    else
+      call OCEAN_haydock_do_gpu(p_re, p_im, niter, ra, rb, ia, done, ierr)
+    end if
     if (ierr == 0) then
       open(unit=12, file=trim(prefix)//'.absspct', form='formatted', status='unknown')
-      call write_core_gpu(12, niter, sc(3), sc(4), sc(5), sc(6), ne, ra, rb, ia, ierr, ucvol=sc(1))
+      ! the files describe the iterations actually run (haydump / redtrid take that count)
+      call write_core_gpu(12, done, sc(3), sc(4), sc(5), sc(6), ne, ra(0:done), rb(0:done), ia(0:done), ierr, &
+                          ucvol=sc(1))
       close(12)
       open(unit=13, file=trim(prefix)//'.abslanc', form='formatted', status='unknown')
-      if (ierr == 0) call write_lanczos_gpu(13, niter, sc(3), ra, rb, .true., ierr)
+      if (ierr == 0) call write_lanczos_gpu(13, done, sc(3), ra(0:done), rb(0:done), .true., ierr)
       close(13)
     end if
   end if

@@ -79,14 +79,27 @@ contains
   end subroutine OCEAN_xact_gpu
 
   ! Hermitian Haydock recursion; real_a/real_b/imag_a are (0:niter) as in the reference.
-  subroutine OCEAN_haydock_do_gpu(psi_r, psi_i, niter, real_a, real_b, imag_a, niter_done, ierr)
+  ! With conv_spacing/conv_thresh present (haydockconv.ipt), the recursion stops where
+  ! the reference's early exit does (OCEAN_haydock.f90:272-291): the relArea test of
+  ! check_convergence over ne midpoint energies in [el, eh] (Hartree), broadening gam.
+  subroutine OCEAN_haydock_do_gpu(psi_r, psi_i, niter, real_a, real_b, imag_a, niter_done, ierr, &
+                                  conv_spacing, conv_thresh, kpref, gam, el, eh, ne)
     real(DP), intent(in) :: psi_r(*), psi_i(*)
     integer, intent(in) :: niter
     real(DP), intent(out) :: real_a(0:niter), real_b(0:niter), imag_a(0:niter)
     integer, intent(out) :: niter_done
     integer, intent(inout) :: ierr
+    integer, intent(in), optional :: conv_spacing, ne
+    real(DP), intent(in), optional :: conv_thresh, kpref, gam, el, eh
     integer(c_int32_t) :: done
-    ierr = ocean_gpu_haydock(handle, psi_r, psi_i, int(niter, c_int32_t), real_a, real_b, imag_a, done)
+    if (present(conv_spacing) .and. present(conv_thresh) .and. present(kpref) .and. present(gam) &
+        .and. present(el) .and. present(eh) .and. present(ne)) then
+      ierr = ocean_gpu_haydock_early(handle, psi_r, psi_i, int(niter, c_int32_t), int(conv_spacing, c_int32_t), &
+                                     conv_thresh, kpref, gam, el, eh, int(ne, c_int32_t), real_a, real_b, &
+                                     imag_a, done)
+    else
+      ierr = ocean_gpu_haydock(handle, psi_r, psi_i, int(niter, c_int32_t), real_a, real_b, imag_a, done)
+    end if
     niter_done = done
   end subroutine OCEAN_haydock_do_gpu
 

@@ -125,6 +125,18 @@ module ocean_gpu_iface
       integer(c_int32_t), intent(out) :: niter_done
     end function
 
+    integer(c_int) function ocean_gpu_haydock_early(h, psi_re, psi_im, niter, spacing, threshold, kpref, &
+        gam, el, eh, ne, real_a, real_b, imag_a, niter_done) bind(C, name='ocean_gpu_haydock_early')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: psi_re(*), psi_im(*)
+      integer(c_int32_t), value :: niter, spacing
+      real(c_double), value :: threshold, kpref, gam, el, eh
+      integer(c_int32_t), value :: ne
+      real(c_double), intent(out) :: real_a(*), real_b(*), imag_a(*)
+      integer(c_int32_t), intent(out) :: niter_done
+    end function
+
     integer(c_int) function ocean_gpu_gmres(h, hay_re, hay_im, ne, energies_ev, gres_ev, gprc_ev, ffff, &
         depth, kpref, out, x_re, x_im) bind(C, name='ocean_gpu_gmres')
       import :: c_int, c_ptr, c_double, c_int32_t
