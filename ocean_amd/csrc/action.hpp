@@ -220,6 +220,35 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
   const int nb = A.nb, nk = A.nk, na = A.na, ns = A.ns;
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
+  // 16 bytes per lane and row: one x point in FP64, two adjacent ones in FP32 (the
+  // row stride ldu is even, so the pair is 16-byte aligned)
+  constexpr int VX = sizeof(T) == 8 ? 1 : 2;
+  const int x = (bx * blockDim.x + threadIdx.x) * VX;
+  const bool on = x < A.nx;  // threads past nx stay for the multiplet partials after the slab
+  const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + min(x, (A.nx - 1) & ~(VX - 1));
+  // Batches of kB band rows: all kB loads are issued before the first use, so
+  // each wave keeps kB row segments in flight (8 KB per wave in FP64; FP32 runs
+  // half as many threads per workgroup and keeps 16 KB per wave in flight).
+  // Full batches carry no per-row predicate: with one, the compiler sinks the FP32
+  // loads into per-row branches and waits on each row in turn.
+  constexpr int kB = VX == 1 ? 8 : 16;
+  auto load_row = [&](int n, T2* dst) {
+    const T2* row = ub + (size_t)n * A.ldu;
+    if constexpr (VX == 1) {
+      dst[0] = ld_stream(row);
+    } else {
+      const float4 q = ld_stream4(reinterpret_cast<const float2*>(row));
+      dst[0] = cmake<T2>(q.x, q.y);
+      dst[VX - 1] = cmake<T2>(q.z, q.w);
+    }
+  };
+  // the first batch is in flight while A psi is staged
+  T2 uv0[kB][VX];
+  const bool pre = on && nb >= kB;
+  if (pre) {
+#pragma unroll
+    for (int i = 0; i < kB; ++i) load_row(i, uv0[i]);
+  }
   // A psi of this k in LDS, with up to kF loads in flight per thread
   constexpr int kF = 4;
   for (int i0 = threadIdx.x; i0 < acnt * nb; i0 += kF * blockDim.x) {
@@ -240,33 +269,12 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
     }
   }
   __syncthreads();
-  // 16 bytes per lane and row: one x point in FP64, two adjacent ones in FP32 (the
-  // row stride ldu is even, so the pair is 16-byte aligned)
-  constexpr int VX = sizeof(T) == 8 ? 1 : 2;
-  const int x = (bx * blockDim.x + threadIdx.x) * VX;
-  if (x < A.nx) {  // threads past nx stay for the multiplet partials after the slab
+  if (on) {
     T2 acc[VX][NA];
 #pragma unroll
     for (int v = 0; v < VX; ++v)
 #pragma unroll
       for (int j = 0; j < NA; ++j) acc[v][j] = cmake<T2>(0, 0);
-    const T2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + x;
-    // Batches of kB band rows: all kB loads are issued before the first use, so
-    // each wave keeps kB row segments in flight (8 KB per wave in FP64; FP32 runs
-    // half as many threads per workgroup and keeps 16 KB per wave in flight).
-    // Full batches carry no per-row predicate: with one, the compiler sinks the FP32
-    // loads into per-row branches and waits on each row in turn.
-    constexpr int kB = VX == 1 ? 8 : 16;
-    auto load_row = [&](int n, T2* dst) {
-      const T2* row = ub + (size_t)n * A.ldu;
-      if constexpr (VX == 1) {
-        dst[0] = ld_stream(row);
-      } else {
-        const float4 q = ld_stream4(reinterpret_cast<const float2*>(row));
-        dst[0] = cmake<T2>(q.x, q.y);
-        dst[VX - 1] = cmake<T2>(q.z, q.w);
-      }
-    };
     auto use_row = [&](int n, const T2* uvr) {
 #pragma unroll
       for (int j = 0; j < NA; ++j) {
@@ -276,6 +284,11 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
       }
     };
     int n0 = 0;
+    if (pre) {
+#pragma unroll
+      for (int i = 0; i < kB; ++i) use_row(i, uv0[i]);
+      n0 = kB;
+    }
     for (; n0 + kB <= nb; n0 += kB) {
       T2 uv[kB][VX];
 #pragma unroll
