@@ -264,7 +264,10 @@ typedef struct ocean_spect_desc {
 } ocean_spect_desc;
 /* out: (ne+1) rows of *ncols values: 9 columns (E eV, Spect, Spect(0), SPKK, SPKK(0),
  * sigma a.u.^2, sigma cm^2, sigma^2 a.u.^5, sigma^2 cm^4 s), or 5 for the Voigt form
- * (E, Spect, Spect(0), SPKK, SPKK(0), Spect and SPKK with gamfcn).  out holds 9*(ne+1). */
+ * (E, Spect, Spect(0), SPKK, SPKK(0), Spect and SPKK with gamfcn).  out holds 9*(ne+1).
+ * The Voigt form needs the output step (eh-el)/ne to be at most six Gaussian widths
+ * (the reference's fine-grid reuse assumes it); a coarser grid returns
+ * OCEAN_GPU_ERR_ARG. */
 int  ocean_gpu_write_core(const double* real_a, const double* real_b, const double* imag_a,
                           int32_t niter, double kpref, const ocean_spect_desc* sd, double* out,
                           int32_t* ncols);
