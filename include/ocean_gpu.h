@@ -85,14 +85,35 @@ int  ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision);
  * OCEAN_bloch_lrLOAD (OCEAN2/OCEAN_bloch.f90:54-219; long_range.f90:186-187). */
 int  ocean_gpu_load_bloch(ocean_gpu_handle* h, const double* u_re, const double* u_im);
 
-/* Raw Bloch functions (con.u2.dat order: OCEAN2/OCEAN_bloch.f90:998-1115) phased
- * on the device as OCEAN_bloch_lrLOAD does; xoff(3, nx) are the wrapped integer x
- * offsets (xph + ix - 1, ...) of OCEAN2/OCEAN_bloch.f90:133-203. */
 /* As ocean_gpu_load_bloch with the two planes already in device memory (same
  * layout); for callers that build or keep u on the GPU. */
 int  ocean_gpu_load_bloch_dev(ocean_gpu_handle* h, const double* u_re_dev, const double* u_im_dev);
+/* Raw Bloch functions (con.u2.dat order: OCEAN2/OCEAN_bloch.f90:998-1115) phased
+ * on the device as OCEAN_bloch_lrLOAD does; xoff(3, nx) are the wrapped integer x
+ * offsets (xph + ix - 1, ...) of OCEAN2/OCEAN_bloch.f90:133-203. */
 int  ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const double* u_im,
                               const int32_t* xoff);
+
+/* Orbital-basis-function (OBF) form of the Bloch functions, and from then on the
+ * OBF long-range term lr_act_obf2 (OCEAN2/long_range.f90:262-455), which the
+ * reference selects with bloch_control's use_obf (:81-82, :128-146) and loads in
+ * OCEAN_obf_load (OCEAN2/OCEAN_obf.f90:510-732):
+ *   obf_re/im   re/im_obf(num_obf, nx), x in the Bloch x order (:551-600)
+ *   obf2u_re/im re/im_obf2u(nb, num_obf, nk) (:704-715)
+ *   xoff(3, nx) the wrapped x offsets of OCEAN_obf_make_phase (:267-395), whose
+ *               phase the device forms; for an odd kmesh(1) its x shift along the
+ *               first axis uses xmesh(2) (:300-301).
+ * As in the reference, the OBF term replaces the interaction vector, so the
+ * multiplet term is dropped whenever the long-range term runs (:445-446).  Every
+ * action entry point of the handle then uses it.  FP64 and nspn = 1 only
+ * (OCEAN_GPU_ERR_STATE otherwise, and from ocean_gpu_set_precision while it is
+ * on); ocean_gpu_spectral_grad and ocean_gpu_batch_create return
+ * OCEAN_GPU_ERR_UNSUPPORTED for such a handle.  A later ocean_gpu_load_bloch*
+ * switches the handle back to the Bloch form.  In ocean_gpu_last_timing, "proj"
+ * is the band-side projection, "fft" the x side (both products with obf and the
+ * transform) and "adj" the adjoint. */
+int  ocean_gpu_load_obf(ocean_gpu_handle* h, int32_t num_obf, const double* obf_re, const double* obf_im,
+                        const double* obf2u_re, const double* obf2u_im, const int32_t* xoff);
 
 /* W(nk, nx) as filled by lr_populate_W2 (OCEAN2/long_range.f90:1659-1853). */
 int  ocean_gpu_load_w(ocean_gpu_handle* h, const double* W);

@@ -58,6 +58,7 @@ _SIGS = {
     "ocean_gpu_set_precision": (C.c_int, [_H, C.c_int32]),
     "ocean_gpu_load_bloch": (C.c_int, [_H, _DP, _DP]),
     "ocean_gpu_load_bloch_raw": (C.c_int, [_H, _DP, _DP, _IP]),
+    "ocean_gpu_load_obf": (C.c_int, [_H, C.c_int32, _DP, _DP, _DP, _DP, _IP]),
     "ocean_gpu_load_w": (C.c_int, [_H, _DP]),
     "ocean_gpu_build_w": (C.c_int, [_H, _DP, _DP, _DP, C.c_int32, C.c_double, C.c_double]),
     "ocean_gpu_load_mult": (C.c_int, [_H, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int32]),
@@ -300,6 +301,25 @@ class OceanGPU:
         xo = np.ascontiguousarray(xoff, np.int32)
         _chk(_lib().ocean_gpu_load_bloch_raw(self.h, _p(re), _p(im), xo.ctypes.data_as(_IP)),
              "load_bloch_raw")
+
+    def load_obf(self, obf, obf2u, xoff):
+        """The OBF form (ocean_gpu_load_obf): obf[o, x], obf2u[n, o, k] complex, xoff
+        [x, 3] from setup_host.obf_x_offsets.  The long-range term then runs
+        lr_act_obf2 (OCEAN2/long_range.f90:262-455) until load_bloch* is called."""
+        obf = np.asarray(obf, np.complex128)
+        no = obf.shape[0]
+        fre = np.asfortranarray(obf.real)
+        fim = np.asfortranarray(obf.imag)
+        c = np.asarray(obf2u, np.complex128)
+        if c.shape != (self.nb, no, self.nk) or obf.shape != (no, self.nx):
+            raise ValueError("obf must be (num_obf, nx) and obf2u (nb, num_obf, nk)")
+        cre = np.asfortranarray(c.real)
+        cim = np.asfortranarray(c.imag)
+        xo = np.ascontiguousarray(xoff, np.int32)
+        if xo.shape != (self.nx, 3):
+            raise ValueError("xoff must be (nx, 3)")
+        _chk(_lib().ocean_gpu_load_obf(self.h, no, _p(fre), _p(fim), _p(cre), _p(cim),
+                                       xo.ctypes.data_as(_IP)), "load_obf")
 
     def load_w(self, W):
         Wf = np.asfortranarray(W, np.float64)

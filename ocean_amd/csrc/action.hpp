@@ -47,6 +47,7 @@ struct ActArgs {
   int G, ncol, nfft, nred;    // transform: columns per strip, strips, reduction workgroups
   int K1, K2, K3;
   int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
+  const T2* phs;              // OBF form: supercell phase [x][k] applied around the transform
 };
 
 // Problem p of a batched launch owns workgroups [off[p], off[p+1]).
@@ -473,7 +474,9 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
 //   passes 4-6 (K3, K2, K1, F_-), the last one scaling by 1/N_k into HBM.
 // nred > 0 prepends that many workgroups running the multiplet reduction, which
 // is independent of phi and overlaps the transform.
-template <typename T2, int LMAX>
+// PHS (the OBF form, long_range.f90:388-406): phi is multiplied by phs(k, x) as it
+// is read and the result by conj(phs(k, x)) as it is written.
+template <typename T2, int LMAX, bool PHS = false>
 __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   using T = typename Scalar<T2>::T;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -523,7 +526,9 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   __syncthreads();  // twiddles
   T2* __restrict__ gphi = phi + c0;
   auto ld_hbm = [&](int k, int g) -> T2 {
-    const T2 v = gphi[(size_t)k * ldphi + min(g, gcnt - 1)];
+    const int gc = min(g, gcnt - 1);
+    T2 v = gphi[(size_t)k * ldphi + gc];
+    if constexpr (PHS) v = cmul(v, A.phs[(size_t)((c0 + gc) / na) * nk + k]);
     return g < gcnt ? v : cmake<T2>(0, 0);
   };
   auto ld_lds = [&](int k, int g) -> T2 { return lds[lidx(k * G + g)]; };
@@ -536,6 +541,10 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   const T inv = (T)1 / (T)nk;
   auto st_hbm = [&](int k, int g, T2 v) {
     if (g < gcnt) {
+      if constexpr (PHS) {
+        const T2 p = A.phs[(size_t)((c0 + g) / na) * nk + k];
+        v = cmake<T2>(v.x * p.x + v.y * p.y, v.y * p.x - v.x * p.y);
+      }
       v.x *= inv; v.y *= inv;
       gphi[(size_t)k * ldphi + g] = v;
     }
@@ -559,9 +568,9 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   dft_run<T2, LMAX, -1>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
 }
 
-template <typename T2, int LMAX>
+template <typename T2, int LMAX, bool PHS = false>
 __global__ __launch_bounds__(256) void k_fft_w(const ActArgs<T2> A) {
-  fft_body<T2, LMAX>(A, blockIdx.x);
+  fft_body<T2, LMAX, PHS>(A, blockIdx.x);
 }
 
 template <typename T2, int LMAX>

@@ -5,6 +5,7 @@
 
 #include "action.hpp"
 #include "common.hpp"
+#include "obf.hpp"
 #include "vec.hpp"
 
 using og::kBlock;
@@ -116,7 +117,7 @@ int flags_of(const H* h, int backwards, const int32_t* hflag) {
 
 int check_ready(const H* h, int flags) {
   if (!h->has_e) return OCEAN_GPU_ERR_STATE;
-  if ((flags & og::F_LR) && !(h->has_u() && h->has_w())) return OCEAN_GPU_ERR_STATE;
+  if ((flags & og::F_LR) && !((h->obf_on || h->has_u()) && h->has_w())) return OCEAN_GPU_ERR_STATE;
   if ((flags & (og::F_FG | og::F_SO)) && !h->has_mult) return OCEAN_GPU_ERR_STATE;
   return OCEAN_GPU_OK;
 }
@@ -134,6 +135,12 @@ struct Geom {
   int adj_threads = 0, nblk_proj = 0, nblk_fft = 0, nblk_adj = 0, lmax = 8;
 };
 
+// The x extent and strides the projection/adjoint kernels run over: the x mesh
+// for the Bloch form, the OBF index for the band side of the OBF form.
+struct BandSide {
+  int nx, ldu, ldT, ldphi;
+};
+
 // Fill the kernel arguments and launch geometry of one problem.  The
 // multiplet term rides on the long-range kernels when both are on: its partials
 // in the last x-tile workgroup of k_proj, its k-sum and K apply in workgroups
@@ -141,9 +148,10 @@ struct Geom {
 template <typename T2, int NA>
 int make_args(const H* h, const double2* in, double2* out, int flags, const Tiles& t, const T2* u,
               const T2* uT, T2* phi, const typename og::Scalar<T2>::T* W, og::ActArgs<T2>& a,
-              Geom<T2>& g) {
+              Geom<T2>& g, const BandSide* bs = nullptr) {
   using T = typename og::Scalar<T2>::T;
-  const int nb = h->nb, nk = h->nk, nx = h->nx, na = h->na, ns = h->ns;
+  const int nb = h->nb, nk = h->nk, na = h->na, ns = h->ns;
+  const int nx = bs ? bs->nx : h->nx;
   const int acnt = (ns == 2) ? na / 2 : na;
   a = og::ActArgs<T2>{};
   a.u = u; a.uT = uT; a.phi = phi; a.W = W;
@@ -153,7 +161,9 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   a.mcount = h->mcount; a.S = h->Smat;
   a.mul = h->d.inter_scale / ((double)nk * h->d.celvol);
   a.nb = nb; a.nk = nk; a.nx = nx; a.na = na; a.ns = ns; a.P = h->P; a.nj = na * h->P;
-  a.ldu = h->ldu; a.ldT = h->ldT; a.ldphi = h->ldphi;
+  a.ldu = bs ? bs->ldu : h->ldu;
+  a.ldT = bs ? bs->ldT : h->ldT;
+  a.ldphi = bs ? bs->ldphi : h->ldphi;
   a.flags = flags;
   const bool lr = (flags & og::F_LR) != 0, fg = (flags & og::F_FG) != 0;
   // projection
@@ -262,10 +272,65 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
   return OCEAN_GPU_OK;
 }
 
+// The action with the OBF long-range term (long_range.f90:262-455):
+//   k_proj over obf2u        beta(k,o,a) = sum_n obf2u (A psi)        (:357-360)
+//   k_obf_fwd                phi(k,x,a)  = sum_o beta obf             (:373-382)
+//   k_fft_w with the phase   Y = conj(phs) F_- W F_+ (phs phi) / N_k  (:388-406)
+//   k_obf_bwd                b(k,o,a) = -N_k sum_x conj(obf) Y        (:412-421)
+//   k_adj over obf2u         hpsi = E psi + A (-sum_o conj(obf2u) b) + S psi
+// The -N_k undoes the transform's 1/N_k (cfft is unnormalised, Modules/jlmfft.f:86-89)
+// and turns the adjoint's subtraction into lr_act_obf2's addition (:448-451).  The
+// multiplet term is not added: lr_act_obf2 overwrites the vector it was
+// accumulated into (:445-446).  Timing events: proj = band-side projection, fft =
+// the x side (both obf products and the transform), adj = the adjoint.
+template <int NA>
+int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev) {
+  constexpr int KT = NA >= 6 ? 1 : 2;
+  flags &= ~og::F_FG;
+  og::ActArgs<double2> a, ab;
+  Geom<double2> g, gb;
+  const Tiles tl = tiles_for<double2, NA>(h);
+  OG_TRY((make_args<double2, NA>(h, in, out, flags, tl, nullptr, nullptr, h->phi64, h->W64, a, g)));
+  a.phs = h->obf_phs;
+  const BandSide bs{h->no, h->ldu_o, h->ldT_o, h->ldb};
+  OG_TRY((make_args<double2, NA>(h, in, out, flags, tl, h->obf_u, h->obf_uT, h->obf_beta, nullptr, ab, gb,
+                                 &bs)));
+  const int nk = h->nk, nx = h->nx, no = h->no;
+  if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
+  og::k_proj<double2, NA><<<gb.nblk_proj, tl.proj, gb.sm_proj, h->stream>>>(ab);
+  OG_CHECK(hipGetLastError());
+  if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
+  og::k_obf_fwd<NA, KT><<<dim3((nx + 255) / 256, (nk + KT - 1) / KT), 256, 0, h->stream>>>(
+      h->obf_beta, h->ldb, h->obf_f, h->phi64, h->ldphi, nk, nx, no);
+  OG_CHECK(hipGetLastError());
+  if (g.lmax == 8)
+    og::k_fft_w<double2, 8, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+  else
+    og::k_fft_w<double2, 16, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+  OG_CHECK(hipGetLastError());
+  og::k_obf_bwd<NA, KT><<<dim3((no + 63) / 64, (nk + KT - 1) / KT), 256, 0, h->stream>>>(
+      h->phi64, h->ldphi, h->obf_fT, h->obf_beta, h->ldb, nk, nx, no, -(double)nk);
+  OG_CHECK(hipGetLastError());
+  if (ev) {
+    OG_CHECK(hipEventRecord(ev[2], h->stream));
+    OG_CHECK(hipEventRecord(ev[3], h->stream));
+  }
+  og::k_adj<double2, NA><<<gb.nblk_adj, gb.adj_threads, gb.sm_adj, h->stream>>>(ab);
+  OG_CHECK(hipGetLastError());
+  if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
+  return OCEAN_GPU_OK;
+}
+
 int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr) {
   const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
   const int NA = na_tmpl(acnt);
   const bool mixed = (h->precision == OCEAN_GPU_MIXED);
+  if (h->obf_on && (flags & og::F_LR)) {
+    if (mixed || h->ns != 1) return OCEAN_GPU_ERR_STATE;
+    if (NA == 4 && h->na == 4) return xact_obf_t<4>(h, in, out, flags, ev);
+    if (NA == 12 && h->na == 12) return xact_obf_t<12>(h, in, out, flags, ev);
+    return OCEAN_GPU_ERR_ARG;
+  }
 #define OG_DISPATCH(NAV)                                                              \
   if (NA == NAV)                                                                      \
     return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev) \
@@ -599,6 +664,7 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   dfree(h->u64); dfree(h->u32); dfree(h->uT64); dfree(h->uT32); dfree(h->W64); dfree(h->W32); dfree(h->phi64); dfree(h->phi32);
   dfree(h->w_i0); dfree(h->w_w0); dfree(h->w_w1);
+  dfree(h->obf_u); dfree(h->obf_uT); dfree(h->obf_f); dfree(h->obf_fT); dfree(h->obf_phs); dfree(h->obf_beta);
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
   dfree(h->amp); dfree(h->mcount);
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
@@ -612,6 +678,7 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
 int ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision) {
   if (!h || (precision != OCEAN_GPU_FP64 && precision != OCEAN_GPU_MIXED)) return OCEAN_GPU_ERR_ARG;
   if (precision == h->precision) return OCEAN_GPU_OK;
+  if (h->obf_on) return OCEAN_GPU_ERR_STATE;  // the OBF form is FP64 only
   h->precision = precision;
   OG_TRY(ensure_precision_buffers(h));
   const size_t nu = h->nu();
@@ -688,6 +755,7 @@ static int load_u_common(H* h, const double* re, const double* im, const int32_t
   if (rc == 0) {
     if (h->precision == OCEAN_GPU_MIXED) { h->u32_ok = true; h->u64_ok = false; }
     else { h->u64_ok = true; h->u32_ok = false; }
+    h->obf_on = false;
   }
   return rc;
 }
@@ -704,6 +772,57 @@ int ocean_gpu_load_bloch_raw(ocean_gpu_handle* h, const double* u_re, const doub
                              const int32_t* xoff) {
   if (!xoff) return OCEAN_GPU_ERR_ARG;
   return load_u_common(h, u_re, u_im, xoff);
+}
+
+int ocean_gpu_load_obf(ocean_gpu_handle* h, int32_t num_obf, const double* obf_re, const double* obf_im,
+                       const double* obf2u_re, const double* obf2u_im, const int32_t* xoff) {
+  if (!h || num_obf <= 0 || !obf_re || !obf_im || !obf2u_re || !obf2u_im || !xoff) return OCEAN_GPU_ERR_ARG;
+  // OCEAN_obf_load handles nspn = 1 only (OCEAN_obf.f90:669-673); the form is FP64
+  if (h->ns != 1 || h->precision != OCEAN_GPU_FP64) return OCEAN_GPU_ERR_STATE;
+  const int nb = h->nb, nk = h->nk, nx = h->nx, no = num_obf;
+  h->obf_on = false;
+  dfree(h->obf_u); dfree(h->obf_uT); dfree(h->obf_f); dfree(h->obf_fT); dfree(h->obf_phs); dfree(h->obf_beta);
+  h->no = no;
+  h->ldu_o = (no + 33) & ~1;
+  h->ldT_o = no * nb + 32;
+  h->ldb = no * h->na + 32;
+  const size_t n2u = (size_t)nb * no * nk, nf = (size_t)no * nx;
+  OG_TRY(dmalloc(&h->obf_u, (size_t)nk * nb * h->ldu_o));
+  OG_TRY(dmalloc(&h->obf_uT, (size_t)nk * h->ldT_o));
+  OG_TRY(dmalloc(&h->obf_f, nf));
+  OG_TRY(dmalloc(&h->obf_fT, nf));
+  OG_TRY(dmalloc(&h->obf_phs, (size_t)nk * nx));
+  OG_TRY(dmalloc(&h->obf_beta, (size_t)nk * h->ldb));
+  OG_CHECK(hipMemsetAsync(h->obf_u, 0, (size_t)nk * nb * h->ldu_o * sizeof(double2), h->stream));
+  OG_CHECK(hipMemsetAsync(h->obf_uT, 0, (size_t)nk * h->ldT_o * sizeof(double2), h->stream));
+  OG_CHECK(hipMemsetAsync(h->obf_beta, 0, (size_t)nk * h->ldb * sizeof(double2), h->stream));
+  double* dpl = nullptr;
+  int* dxo = nullptr;
+  const size_t npl = std::max(n2u, nf);
+  OG_TRY(dmalloc(&dpl, 2 * npl));
+  int rc = dmalloc(&dxo, (size_t)3 * nx);
+  if (rc == 0 && (hipMemcpy(dpl, obf2u_re, n2u * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(dpl + npl, obf2u_im, n2u * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(dxo, xoff, 3 * (size_t)nx * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_obf_load2u<<<grid_for(n2u), kBlock, 0, h->stream>>>(dpl, dpl + npl, h->obf_u, h->obf_uT, nb, no, nk,
+                                                             h->ldu_o, h->ldT_o);
+    og::k_obf_phase<<<grid_for((size_t)nk * nx), kBlock, 0, h->stream>>>(dxo, h->obf_phs, nk, nx, h->km[0],
+        h->km[1], h->km[2], h->d.xmesh[0], h->d.xmesh[1], h->d.xmesh[2]);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0 && (hipMemcpy(dpl, obf_re, nf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(dpl + npl, obf_im, nf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_obf_loadf<<<grid_for(nf), kBlock, 0, h->stream>>>(dpl, dpl + npl, h->obf_f, h->obf_fT, no, nx);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(dpl);
+  dfree(dxo);
+  if (rc == 0) h->obf_on = true;
+  return rc;
 }
 
 static int finish_w(H* h, const double* dW64) {
@@ -1567,6 +1686,7 @@ int ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const dou
                             double* g_ptab, double* g_scale, double* g_gamma_eV, double* g_scissor_eV) {
   if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !target || !loss)
     return OCEAN_GPU_ERR_ARG;
+  if (h->obf_on) return OCEAN_GPU_ERR_UNSUPPORTED;  // the W derivative is taken on the Bloch form
   if (g_ptab && (h->nptab <= 0 || !h->w_i0)) return OCEAN_GPU_ERR_STATE;  // W must come from build_w
   const size_t nv = h->nvec();
   double2* rhs = nullptr;
@@ -1818,6 +1938,10 @@ int ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_bat
         std::find(b->hs.begin(), b->hs.end(), hs[i]) != b->hs.end()) {
       delete b;
       return OCEAN_GPU_ERR_ARG;
+    }
+    if (hs[i]->obf_on) {  // batched groups run the Bloch-form kernels
+      delete b;
+      return OCEAN_GPU_ERR_UNSUPPORTED;
     }
     b->hs.push_back(hs[i]);
   }

@@ -212,6 +212,18 @@ struct ocean_gpu_handle {
   double2* energy = nullptr; // [a][k][n]
   double* allow = nullptr;   // [a][k][n]
 
+  // OBF form of the long-range state (long_range.f90:262-455), FP64 only: the
+  // band side runs the projection/adjoint kernels with x -> o over obf_u/obf_uT
+  int no = 0;                   // num_obf
+  int ldu_o = 0, ldT_o = 0, ldb = 0;
+  double2* obf_u = nullptr;     // obf2u [k][n][o], row stride ldu_o
+  double2* obf_uT = nullptr;    // obf2u [k][o][n], k stride ldT_o
+  double2* obf_f = nullptr;     // obf [o][x]
+  double2* obf_fT = nullptr;    // obf [x][o]
+  double2* obf_phs = nullptr;   // make_phase's phase [x][k]
+  double2* obf_beta = nullptr;  // band-side projection / back-projection [k][o][a], k stride ldb
+  bool obf_on = false;
+
   bool u64_ok = false, u32_ok = false, w64_ok = false, w32_ok = false;
   bool has_mult = false, has_e = false;
   bool has_u() const { return precision == OCEAN_GPU_MIXED ? u32_ok : u64_ok; }

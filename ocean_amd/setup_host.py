@@ -54,6 +54,31 @@ def x_offsets(xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
     return np.ascontiguousarray(wrap + idx - 1, dtype=np.int32), tau_c, xshift
 
 
+def obf_x_offsets(xmesh, kmesh, tau):
+    """The site centring and wrapped x offsets of OCEAN_obf_make_phase.
+
+    As x_offsets, except that for an odd kmesh(1) the shift along the first axis is
+    floor(xmesh(2) (tau_1 - 1/2)), as OCEAN2/OCEAN_obf.f90:300-301 has it (the Bloch
+    path, and the OBF path for an even kmesh(1), use xmesh(1)).  Returns
+    (xoff int32 [x, 3], tau_centred, xshift); xoff is what ocean_gpu_load_obf takes.
+    """
+    xm = np.array(xmesh, np.int64)
+    km = np.array(kmesh, np.int64)
+    tau = np.asarray(tau, np.float64)
+    half = np.where(km % 2 == 0, 0.0, 0.5)
+    xs_mesh = xm.copy()
+    if km[0] % 2 == 1:
+        xs_mesh[0] = xm[1]
+    xshift = np.floor(xs_mesh * (tau - half)).astype(np.int64)
+    tau_c = tau - xshift / xm
+    iz, iy, ix = np.meshgrid(np.arange(1, xm[2] + 1), np.arange(1, xm[1] + 1),
+                             np.arange(1, xm[0] + 1), indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1)
+    targ = idx - xshift[None, :]
+    wrap = np.where(targ > xm[None, :], -xm[None, :], np.where(targ < 1, xm[None, :], 0))
+    return np.ascontiguousarray(wrap + idx - 1, dtype=np.int32), tau_c, xshift
+
+
 def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
     """Phase u(n,k,x) by exp(2 pi i sum_d (x_d + wrap_d) k_d / (Nx_d Nk_d)).
 

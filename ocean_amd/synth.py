@@ -195,6 +195,48 @@ def make_inputs(nb, kmesh, xmesh, lc=0, nspn=1, seed=0, xi_eV=0.0, lmin=0, lmax=
                      dict(params or {}))
 
 
+@dataclass
+class ObfInputs:
+    """The OBF form of a problem's Bloch functions (OCEAN2/OCEAN_obf.f90).
+
+    obf[o, x] (re/im_obf), obf2u[n, o, k] (re/im_obf2u), the make_phase x offsets
+    and the W of the OBF centring (lr_populate_W2 runs on make_phase's tau)."""
+    obf: np.ndarray
+    obf2u: np.ndarray
+    xoff: np.ndarray
+    tau: np.ndarray
+    xshift: np.ndarray
+    W: np.ndarray
+
+    @property
+    def num_obf(self):
+        return int(self.obf.shape[0])
+
+
+def make_obf(inp, num_obf, seed=0):
+    """Seeded OBF inputs for a problem from make_inputs: basis functions over x that
+    are orthogonal with norm N_k^-1/2 (rows of a QR factor, scaled) and Gaussian band
+    coefficients of unit norm per (n, k).  lr_act_obf2 omits the 1/N_k of the
+    Bloch form's transform pair, so the scale puts the OBF long-range term at the
+    Bloch form's magnitude; with unit-norm basis functions it is N_k times larger
+    and a Haydock recursion on it loses all digits within 50 steps (a 1e-15
+    perturbation of psi0 alone changes a by 30 %)."""
+    rng = np.random.default_rng(seed)
+    nx, nk, nb = inp.nx, inp.nk, inp.nb
+    if num_obf > nx:
+        raise ValueError("num_obf must not exceed the number of x points")
+    q, _ = np.linalg.qr(_gaussian_complex(rng, (nx, num_obf)))
+    obf = np.ascontiguousarray(q.T) / np.sqrt(nk)
+    c = _gaussian_complex(rng, (nb, num_obf, nk))
+    c = c / np.sqrt(np.sum(np.abs(c) ** 2, axis=1, keepdims=True))
+    obf2u = np.asfortranarray(c)
+    tau0 = np.asarray(inp.tau, np.float64) + np.asarray(inp.xshift, np.float64) / np.asarray(inp.xmesh, np.float64)
+    xoff, tau_c, xshift = sh.obf_x_offsets(inp.xmesh, inp.kmesh, tau0)
+    W = sh.build_w(inp.avec, inp.kmesh, inp.xmesh, tau_c, xshift, inp.rtab, inp.ptab,
+                   inp.epsilon0, inp.inter_scale)
+    return ObfInputs(obf, obf2u, xoff, tau_c, xshift, W)
+
+
 # Named workloads with their solver settings (energies in eV)
 WORKLOADS = {
     # K edge, 16 bands, k-mesh 4^3, x-mesh 6^3: the size the CPU reference runs quickly

@@ -38,6 +38,12 @@ Pinning (what checks each function; "parity unpinned" where none does):
   redtrid              TEST/Fe/Results abslanc eigenvalue block
   haydock              dense Lanczos/eigendecomposition brute force (tests/test_oracle.py)
   lr_act               dense operator brute force, loop-literal lr_kernel restatement
+  lr_act_obf2          -N_k lr_act with u = sum_o obf2u obf phs (the Bloch functions the
+                       reference's own use_fake_obf path builds, OCEAN_obf.f90:84-96),
+                       Hermiticity of the masked operator
+  obf_phase            bloch_phase's closed form for even meshes; the odd-kmesh(1)
+                       xshift by hand
+  xact_obf             xact with the multiplet term off, plus the two terms added after
   fg_combo/new_soact   dense operator brute force, Hermiticity
   gaunt/threey         Wigner-3j closed form vs Gauss-Legendre quadrature restatement
   soprep               closed-form <L.S> eigenvalues
@@ -747,6 +753,103 @@ def lr_kernel_loops(P: Problem, psi):
                 hpr[:, k, a] += -ur[:, k] * xr[k] - ui[:, k] * xi[k]
                 hpi[:, k, a] += -ur[:, k] * xi[k] + ui[:, k] * xr[k]
     return hpr + 1j * hpi
+
+
+# ----------------------------------------------------------------------------
+# Orbital-basis-function (OBF) form of the long-range term
+# ----------------------------------------------------------------------------
+def obf_phase(xmesh, kmesh, tau):
+    """OCEAN_obf_make_phase (OCEAN_obf.f90:267-395).
+
+    xshift(d) = floor(X tau_d) for an even K_d and floor(X (tau_d - 1/2)) for an odd
+    one, X = xmesh(d), except that for an odd kmesh(1) the reference takes
+    xmesh(2) (:300-301, kept as written); tau_c = tau - xshift / xmesh (:316-318);
+    the x offsets wrap by -+X_d when i_d - xshift_d leaves 1..X_d (:321-358);
+    obf_phs(q, x) = exp(2 pi i sum_d (wrap_d + i_d - 1)(q_d - 1) / (X_d K_d))
+    (:360-372).  Returns (phs complex (nk, nx), xoff (nx, 3), tau_c, xshift).
+    """
+    xm = tuple(int(v) for v in xmesh)
+    km = tuple(int(v) for v in kmesh)
+    tau = np.array(tau, np.float64)
+    xshift = np.zeros(3, np.int64)
+    for d in range(3):
+        if km[d] % 2 == 0:
+            xshift[d] = math.floor(float(xm[d]) * tau[d])
+        else:
+            X = xm[1] if d == 0 else xm[d]
+            xshift[d] = math.floor(float(X) * (tau[d] - 0.5))
+    tau_c = tau - xshift / np.array(xm, np.float64)
+    nx = xm[0] * xm[1] * xm[2]
+    nk = km[0] * km[1] * km[2]
+    xoff = np.zeros((nx, 3), np.int64)
+    it = 0
+    for iz in range(1, xm[2] + 1):
+        for iy in range(1, xm[1] + 1):
+            for ix in range(1, xm[0] + 1):
+                for d, i in enumerate((ix, iy, iz)):
+                    targ = i - xshift[d]
+                    ph = -xm[d] if targ > xm[d] else (xm[d] if targ < 1 else 0)
+                    xoff[it, d] = ph + i - 1
+                it += 1
+    q = np.array([(a, b, c) for a in range(km[0]) for b in range(km[1]) for c in range(km[2])],
+                 np.int64)
+    ang = np.zeros((nk, nx))
+    for d in range(3):
+        ang = ang + 2.0 * np.pi * np.outer(q[:, d], xoff[:, d]).astype(np.float64) / float(xm[d] * km[d])
+    phs = np.asfortranarray(np.cos(ang) + 1j * np.sin(ang))
+    return phs, xoff, tau_c, xshift
+
+
+def lr_act_obf2(P: Problem, psi, obf, obf2u, phs):
+    """Long-range term in the OBF basis.  long_range.f90:262-455 (lr_act_obf2).
+
+    obf (num_obf, nx) = re/im_obf, obf2u (nb, num_obf, nk) = re/im_obf2u, phs (nk, nx):
+      beta(o,k,a)  = sum_n obf2u(n,o,k) psi(n,k,a)            (:357-360, DDOT pairs)
+      phi(k,x)     = sum_o beta(o,k,a) obf(o,x)                (:373-382, DGEMM 'T','N')
+      X            = cfft(-1)[ W(:,x) cfft(+1)[ phi phs ] ]    (:388-400), unnormalised;
+                     cfft(+1) is exp(+2 pi i ...), Modules/jlmfft.f:247-262
+      beta2(o,k)   = sum_x conj(obf(o,x)) X(k,x) conj(phs(k,x))  (:403-421)
+      hp(n,k,a)    = sum_o conj(obf2u(n,o,k)) beta2(o,k)       (:445-452)
+    hp is zeroed before it is accumulated (:445-446): the result replaces, rather
+    than adds to, whatever the caller's vector held.  Spin-independent (nspn = 1).
+    """
+    nb, nk, na = psi.shape
+    hp = np.zeros(psi.shape, np.complex128, order="F")
+    beta = np.einsum("nok,nka->oka", obf2u, psi)
+    for a in range(na):
+        phi = beta[:, :, a].T @ obf
+        t = fft_backward(phi * phs, P.kmesh, axis=0)
+        t = t * P.W
+        t = fft_forward(t, P.kmesh, axis=0)
+        beta2 = np.conj(obf) @ (t * np.conj(phs)).T
+        hp[:, :, a] = np.einsum("nok,ok->nk", np.conj(obf2u), beta2)
+    return hp
+
+
+def xact_obf(P: Problem, psi, obf, obf2u, phs, backwards=False, hflag=(1, 0, 0, 0, 1, 1),
+             inter=None):
+    """H.psi with bloch_control's use_obf (long_range.f90:81-82 selects lr_act_obf2).
+
+    As xact, except that the OBF long-range term replaces the interaction vector
+    (lr_act_obf2 zeroes hp, long_range.f90:445-446), so the multiplet term that
+    OCEAN_xact accumulated into it first (OCEAN_action.f90:344-351) is dropped
+    whenever the long-range term runs; the spin-orbit term (:368-372) and the
+    energies (:596-598) are added after it as usual.
+    """
+    if inter is None:
+        inter = P.inter_scale
+    psi = np.asarray(psi, np.complex128)
+    psi_o = P.allow * psi
+    h = np.zeros(psi.shape, np.complex128, order="F")
+    if P.do_mult and P.mult is not None and hflag[4] == 1:
+        h += fg_combo(P, psi_o, inter)
+    if P.long_range and hflag[0] == 1:
+        h = lr_act_obf2(P, psi_o, obf, obf2u, phs)
+    h = P.allow * h
+    if hflag[5] == 1 and P.do_mult and P.mult is not None:
+        h += new_soact(P, psi, backwards)
+    out = energies_act(P, psi, backwards) if P.e0 else np.zeros_like(h)
+    return np.asfortranarray(out + h)
 
 
 def fg_combo(P: Problem, psi, inter):

@@ -147,6 +147,77 @@ def test_lr_act_equals_dense_operator():
     assert relerr(orc.lr_kernel_loops(P, psi), got) < 1e-13
 
 
+def _obf_case(no=5, **kw):
+    inp = _tiny(**kw)
+    ob = synth.make_obf(inp, no, seed=7)
+    tau0 = inp.tau + inp.xshift / np.array(inp.xmesh, float)
+    phs, xoff, tau_c, xshift = orc.obf_phase(inp.xmesh, inp.kmesh, tau0)
+    P = to_problem(inp)
+    P.W = ob.W
+    # the Bloch functions the OBFs stand for (the sum OCEAN_obf_lrLOAD forms)
+    u = np.einsum("nok,ox->nkx", ob.obf2u, ob.obf) * phs[None, :, :]
+    P2 = to_problem(inp)
+    P2.W = ob.W
+    P2.u = np.asfortranarray(u[:, :, :, None])
+    return inp, ob, phs, xoff, P, P2
+
+
+def test_lr_act_obf2_is_scaled_traditional_action():
+    """lr_act_obf2 = -N_k lr_act with u = sum_o obf2u obf phs: the OBF path's
+    unnormalised transform pair and its + sign against the pinned Bloch path."""
+    inp, ob, phs, _, P, P2 = _obf_case(kmesh=(2, 3, 2), xmesh=(3, 2, 2), nb=3)
+    rng = np.random.default_rng(5)
+    shp = (inp.nb, inp.nk, inp.nalpha)
+    psi = np.asfortranarray(rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+    got = orc.lr_act_obf2(P, psi, ob.obf, ob.obf2u, phs)
+    assert relerr(got, -inp.nk * orc.lr_act(P2, psi)) < 1e-12
+
+
+def test_obf_phase_closed_forms():
+    """Even meshes: bloch_phase's rotation; odd kmesh(1): xshift(1) from xmesh(2)."""
+    xm, km, tau = (3, 4, 2), (2, 4, 2), np.array([0.9, 0.27, 0.41])
+    phs, xoff, tau_c, xshift = orc.obf_phase(xm, km, tau)
+    nk, nx = int(np.prod(km)), int(np.prod(xm))
+    rot, tau_b, xs_b = orc.bloch_phase(np.ones((1, nk, nx, 1), complex), xm, km, tau)
+    assert np.abs(phs - rot[0, :, :, 0]).max() < 1e-14
+    assert np.array_equal(xshift, xs_b) and np.allclose(tau_c, tau_b)
+    km = (3, 4, 2)
+    xm = (3, 5, 2)
+    phs, xoff, tau_c, xshift = orc.obf_phase(xm, km, tau)
+    assert xshift[0] == math.floor(5 * (0.9 - 0.5)) == 2   # xmesh(2), not xmesh(1) (-> 1)
+    assert abs(tau_c[0] - (0.9 - 2 / 3)) < 1e-15
+    # the product's offsets agree with the restatement
+    xo, tc, xs = sh.obf_x_offsets(xm, km, tau)
+    assert np.array_equal(xo, xoff) and np.array_equal(xs, xshift) and np.allclose(tc, tau_c)
+    # a plane wave: phs(q, x) at q = (1, 0, 0) is exp(2 pi i xoff_1 / (X_1 K_1))
+    q = 1 * km[1] * km[2]
+    assert np.abs(phs[q] - np.exp(2j * np.pi * xoff[:, 0] / (xm[0] * km[0]))).max() < 1e-13
+
+
+def test_xact_obf_drops_multiplet_and_is_hermitian():
+    """xact_obf = xact with the multiplet off, SO on and W -> -N_k W on the OBF
+    Bloch functions; the operator is Hermitian."""
+    inp, ob, phs, _, P, P2 = _obf_case(lc=1, xi_eV=6.0, nb=3, kmesh=(2, 2, 2), xmesh=(2, 2, 3))
+    P2.W = -inp.nk * ob.W
+    rng = np.random.default_rng(6)
+    shp = (inp.nb, inp.nk, inp.nalpha)
+    psi = np.asfortranarray(rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+    for back in (False, True):
+        got = orc.xact_obf(P, psi, ob.obf, ob.obf2u, phs, backwards=back)
+        ref = orc.xact(P2, psi, backwards=back, hflag=(1, 0, 0, 0, 0, 1))
+        assert relerr(got, ref) < 1e-12
+    # without the long-range term the multiplet stays (nothing overwrites it)
+    hf = (0, 0, 0, 0, 1, 1)
+    assert relerr(orc.xact_obf(P, psi, ob.obf, ob.obf2u, phs, hflag=hf), orc.xact(P, psi, hflag=hf)) < 1e-14
+    N = int(np.prod(shp))
+    H = np.zeros((N, N), complex)
+    for j in range(N):
+        e = np.zeros(N, complex)
+        e[j] = 1.0
+        H[:, j] = orc.xact_obf(P, e.reshape(shp, order="F"), ob.obf, ob.obf2u, phs).ravel(order="F")
+    assert np.abs(H - np.conj(H.T)).max() < 1e-12 * np.abs(H).max()
+
+
 def test_fg_combo_equals_dense_operator():
     """H_fg = conj(C) (mul K) C^T with K = -mpm (direct) + X^T block (exchange)."""
     inp = _tiny(lc=1, seed=4)
