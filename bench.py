@@ -229,6 +229,33 @@ def precision_workload(reps=20, with_haydock=True):
     return out
 
 
+def obf_workload(inp=None, num_obf=128, reps=50):
+    """The OBF long-range form (lr_act_obf2) at the 64-band K-edge shapes with num_obf
+    basis functions: H.psi rate and per-kernel times (proj = band-side projection,
+    fft = the x side: both products with the basis and the phased transform)."""
+    import dataclasses
+
+    from ocean_amd import core, synth
+
+    if inp is None:
+        inp = synth.workload("k_nb64")
+    ob = synth.make_obf(inp, num_obf, seed=1)
+    g = core.OceanGPU.from_inputs(dataclasses.replace(inp, u=None))
+    g.load_w(ob.W)
+    g.load_obf(ob.obf, ob.obf2u, ob.xoff)
+    vin, vout = g.vec_new(inp.psi0), g.vec_new()
+    g.xact_repeat(vin, vout, 3)
+    ms = g.xact_repeat(vin, vout, reps)
+    prof = g.profile_xact(vin, vout, reps)
+    g.close()
+    # the two x-side products: 8 flop per complex MAC, nk nx num_obf nalpha MACs each
+    gf = 2 * 8.0 * inp.nk * inp.nx * num_obf * inp.nalpha / 1e9
+    log(f"OBF ({num_obf} basis functions): {ms:.4f} ms per H.psi, {prof}")
+    return {"obf_num_basis": num_obf, "obf_hpsi_per_s_fp64": round(1e3 / ms, 1),
+            "obf_per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
+            "obf_xside_gflop": round(gf, 3)}
+
+
 def gmres_workload(inp, g, n=64):
     """GMRES at the 64-band K-edge workload: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
     E = np.linspace(-5.0, 40.0, n)
@@ -255,6 +282,9 @@ def main():
     ap.add_argument("--batch-niter", type=int, default=200)
     ap.add_argument("--l23-only", action="store_true",
                     help="only the L2,3 configuration's H.psi rates and kernel times (for profiling)")
+    ap.add_argument("--no-obf", action="store_true", help="skip the OBF long-range form")
+    ap.add_argument("--obf-only", action="store_true",
+                    help="only the OBF form's H.psi rate and kernel times (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -274,6 +304,9 @@ def main():
         sys.exit(2)
     if args.l23_only:
         print(json.dumps(precision_workload(reps=args.steps, with_haydock=False)), flush=True)
+        return
+    if args.obf_only:
+        print(json.dumps(obf_workload(reps=args.steps)), flush=True)
         return
 
     from ocean_amd import core, synth
@@ -349,6 +382,8 @@ def main():
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})
+    if not args.no_extras and not args.no_obf:
+        extras.update(obf_workload(inp))
     if not args.no_extras and not args.no_l23:
         extras.update(precision_workload())
     if not args.no_extras and not args.no_grad:
