@@ -285,7 +285,6 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
 // the x side (both obf products and the transform), adj = the adjoint.
 template <int NA>
 int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev) {
-  constexpr int KT = NA >= 6 ? 1 : 2;
   flags &= ~og::F_FG;
   og::ActArgs<double2> a, ab;
   Geom<double2> g, gb;
@@ -300,7 +299,9 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   og::k_proj<double2, NA><<<gb.nblk_proj, tl.proj, gb.sm_proj, h->stream>>>(ab);
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
-  og::k_obf_fwd<NA, KT><<<dim3((nx + 255) / 256, (nk + KT - 1) / KT), 256, 0, h->stream>>>(
+  // 16-column tiles of the flattened (k, alpha, re/im) output columns
+  const int ntiles = (2 * nk * NA + 15) / 16;
+  og::k_obf_fwd<NA><<<dim3((nx + 63) / 64, (ntiles + 3) / 4), 256, 0, h->stream>>>(
       h->obf_beta, h->ldb, h->obf_f, h->phi64, h->ldphi, nk, nx, no);
   OG_CHECK(hipGetLastError());
   if (g.lmax == 8)
@@ -308,7 +309,7 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   else
     og::k_fft_w<double2, 16, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
   OG_CHECK(hipGetLastError());
-  og::k_obf_bwd<NA, KT><<<dim3((no + 63) / 64, (nk + KT - 1) / KT), 256, 0, h->stream>>>(
+  og::k_obf_bwd<NA><<<dim3((no + 63) / 64, ntiles), 256, 0, h->stream>>>(
       h->phi64, h->ldphi, h->obf_fT, h->obf_beta, h->ldb, nk, nx, no, -(double)nk);
   OG_CHECK(hipGetLastError());
   if (ev) {

@@ -12,103 +12,159 @@
 
 namespace og {
 
+// The two x-side products are GEMMs (M = x or o, N = the nk nalpha columns, K = o
+// or x) and run on the FP64 matrix cores: v_mfma_f64_16x16x4f64, where lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[l / 16 + 4 r]
+// [l % 16], r = 0..3 (measured on gfx950; the FP32 16x16x4 form returns rows
+// 4 (l / 16) + r instead).  The complex product is a real GEMM with the complex parts
+// unrolled along K: for every group of four K indices (kk = l / 16) the first
+// instruction takes the real parts of A and the second the imaginary parts, with
+// B[(kk, re)][(col, c)] and B[(kk, im)][(col, c)] formed from the lane's own complex
+// B element, so each lane loads one complex A element per M tile and one complex B
+// element per group.  Output columns are flattened as G = 2 (k nalpha + a) + c
+// (c = real/imaginary part); an N tile is 16 consecutive G.
+typedef double __attribute__((ext_vector_type(4))) og_d4;
+
+struct ObfCol {
+  int k, a, c;
+  bool ok;
+};
+__device__ __forceinline__ ObfCol obf_col(int G, int na, int nk) {
+  ObfCol r;
+  r.k = G / (2 * na);
+  r.a = (G >> 1) % na;
+  r.c = G & 1;
+  r.ok = r.k < nk;
+  r.k = min(r.k, nk - 1);
+  return r;
+}
+
 // phi(k, x, a) = sum_o beta(k, o, a) obf(o, x)              (long_range.f90:373-382)
 // beta [k][o][a] (k stride ldb), obf [o][x], phi [k][x][a] (k stride ldphi).
-// Lanes run over x, so obf rows are read coalesced; beta depends only on the
-// workgroup's k tile and the loop index, so it is read through the scalar cache.
-template <int NA, int KT>
+// Workgroup: 64 x points (four M tiles) by four N tiles, one per wave; the waves
+// share the A panel through L1.  Loads of the next group of four o are in flight
+// while the current group's eight instructions run.
+template <int NA>
 __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ beta, int ldb,
                                                  const double2* __restrict__ obf,
                                                  double2* __restrict__ phi, int ldphi, int nk,
                                                  int nx, int no) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int xc = min(x, nx - 1);
-  const int k0 = blockIdx.y * KT;
-  double2 acc[KT][NA];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  const int x0 = blockIdx.x * 64;
+  const ObfCol col = obf_col((blockIdx.y * 4 + w) * 16 + i, NA, nk);
+  const double2* __restrict__ bp = beta + (size_t)col.k * ldb + col.a;
+  int xr[4];
 #pragma unroll
-  for (int t = 0; t < KT; ++t)
+  for (int m = 0; m < 4; ++m) xr[m] = min(x0 + 16 * m + i, nx - 1);
+  og_d4 acc[4];
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[t][a] = make_double2(0.0, 0.0);
-  const double2* __restrict__ fp = obf + xc;
-#pragma unroll 2
-  for (int o = 0; o < no; ++o) {
-    const double2 f = fp[(size_t)o * nx];
+  for (int m = 0; m < 4; ++m) acc[m] = og_d4{0.0, 0.0, 0.0, 0.0};
+  double2 bn, an[4];
+  auto load = [&](int o0) {
+    const int o = min(o0 + kk, no - 1);
+    bn = bp[(size_t)o * NA];
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      const double2* __restrict__ bp = beta + (size_t)min(k0 + t, nk - 1) * ldb + (size_t)o * NA;
+    for (int m = 0; m < 4; ++m) an[m] = obf[(size_t)o * nx + xr[m]];
+  };
+  load(0);
+  for (int o0 = 0; o0 < no; o0 += 4) {
+    const bool ok = o0 + kk < no;
+    double2 bv = bn, av[4];
 #pragma unroll
-      for (int a = 0; a < NA; ++a) cfma(acc[t][a], bp[a], f);
+    for (int m = 0; m < 4; ++m) av[m] = an[m];
+    load(o0 + 4);
+    if (!ok || !col.ok) bv = make_double2(0.0, 0.0);
+    // C = A B: (ar, ai) x (br, bi) -> re: ar br - ai bi, im: ar bi + ai br
+    const double b0 = col.c ? bv.y : bv.x;
+    const double b1 = col.c ? bv.x : -bv.y;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].x : 0.0, b0, acc[m], 0, 0, 0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].y : 0.0, b1, acc[m], 0, 0, 0);
+  }
+  if (!col.ok) return;
+  double* __restrict__ out = reinterpret_cast<double*>(phi + (size_t)col.k * ldphi + col.a) + col.c;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int x = x0 + 16 * m + kk + 4 * r;
+      if (x < nx) out[(size_t)x * NA * 2] = acc[m][r];
     }
-  }
-  if (x >= nx) return;
-#pragma unroll
-  for (int t = 0; t < KT; ++t) {
-    if (k0 + t >= nk) break;
-    double2* __restrict__ out = phi + (size_t)(k0 + t) * ldphi + (size_t)x * NA;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) out[a] = acc[t][a];
-  }
 }
 
 // b(k, o, a) = scale sum_x conj(obf(o, x)) Y(k, x, a)        (long_range.f90:412-421)
 // Y [k][x][a] is the transform output (already multiplied by conj(phs)); obfT [x][o];
-// b [k][o][a].  Lanes run over o (coalesced obfT rows); the four waves of a
-// workgroup take consecutive quarters of the x range and their partial sums are
-// added in LDS in a fixed order, so the result is reproducible.  Y depends only on
-// the wave's x index and the k tile and is read through the scalar cache.
-template <int NA, int KT>
+// b [k][o][a].  Workgroup: 64 basis functions (four M tiles) by one N tile; its four
+// waves take consecutive quarters of the x range and their partial tiles are
+// added in LDS in a fixed order, so the result is reproducible.
+template <int NA>
 __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, int ldphi,
                                                  const double2* __restrict__ obfT,
                                                  double2* __restrict__ b, int ldb, int nk, int nx,
                                                  int no, double scale) {
-  constexpr int XS = 4;
-  __shared__ double2 part[XS - 1][KT * NA][64];
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int o = blockIdx.x * 64 + lane;
-  const int oc = min(o, no - 1);
-  const int k0 = blockIdx.y * KT;
-  const int xa = (nx * w) / XS, xb = (nx * (w + 1)) / XS;
-  double2 acc[KT][NA];
+  __shared__ double part[3][16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  const int o0 = blockIdx.x * 64;
+  const ObfCol col = obf_col(blockIdx.y * 16 + i, NA, nk);
+  const double2* __restrict__ yp = Y + (size_t)col.k * ldphi + col.a;
+  int orow[4];
 #pragma unroll
-  for (int t = 0; t < KT; ++t)
+  for (int m = 0; m < 4; ++m) orow[m] = min(o0 + 16 * m + i, no - 1);
+  const int xa = (nx * w) >> 2, xb = (nx * (w + 1)) >> 2;
+  og_d4 acc[4];
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[t][a] = make_double2(0.0, 0.0);
-  const double2* __restrict__ fp = obfT + oc;
-#pragma unroll 2
-  for (int x = xa; x < xb; ++x) {
-    const double2 f = fp[(size_t)x * no];
+  for (int m = 0; m < 4; ++m) acc[m] = og_d4{0.0, 0.0, 0.0, 0.0};
+  if (xb > xa) {
+    double2 yn, an[4];
+    auto load = [&](int xs) {
+      const int x = min(xs + kk, xb - 1);
+      yn = yp[(size_t)x * NA];
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      const double2* __restrict__ yp = Y + (size_t)min(k0 + t, nk - 1) * ldphi + (size_t)x * NA;
+      for (int m = 0; m < 4; ++m) an[m] = obfT[(size_t)x * no + orow[m]];
+    };
+    load(xa);
+    for (int xs = xa; xs < xb; xs += 4) {
+      const bool ok = xs + kk < xb;
+      double2 yv = yn, av[4];
 #pragma unroll
-      for (int a = 0; a < NA; ++a) cfmac(acc[t][a], f, yp[a]);
+      for (int m = 0; m < 4; ++m) av[m] = an[m];
+      load(xs + 4);
+      if (!ok || !col.ok) yv = make_double2(0.0, 0.0);
+      // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
+      const double b0 = col.c ? yv.y : yv.x;
+      const double b1 = col.c ? -yv.x : yv.y;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].x : 0.0, b0, acc[m], 0, 0, 0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].y : 0.0, b1, acc[m], 0, 0, 0);
     }
   }
   if (w > 0) {
 #pragma unroll
-    for (int t = 0; t < KT; ++t)
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int a = 0; a < NA; ++a) part[w - 1][t * NA + a][lane] = acc[t][a];
+      for (int r = 0; r < 4; ++r) part[w - 1][4 * m + r][lane] = acc[m][r];
   }
   __syncthreads();
-  if (w != 0 || o >= no) return;
+  if (w != 0 || !col.ok) return;
+  double* __restrict__ out = reinterpret_cast<double*>(b + (size_t)col.k * ldb + col.a) + col.c;
 #pragma unroll
-  for (int t = 0; t < KT; ++t) {
-    if (k0 + t >= nk) break;
-    double2* __restrict__ out = b + (size_t)(k0 + t) * ldb + (size_t)o * NA;
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      double2 v = acc[t][a];
+    for (int r = 0; r < 4; ++r) {
+      const int o = o0 + 16 * m + kk + 4 * r;
+      double v = acc[m][r];
 #pragma unroll
-      for (int j = 0; j < XS - 1; ++j) {
-        const double2 p = part[j][t * NA + a][lane];
-        v.x += p.x;
-        v.y += p.y;
-      }
-      out[a] = make_double2(v.x * scale, v.y * scale);
+      for (int j = 0; j < 3; ++j) v += part[j][4 * m + r][lane];
+      if (o < no) out[(size_t)o * NA * 2] = v * scale;
     }
-  }
 }
 
 // obf_phs(k, x) = exp(i (phsx + phsy + phsz)),
