@@ -301,7 +301,7 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
   // 16-column tiles of the flattened (k, alpha, re/im) output columns
   const int ntiles = (2 * nk * NA + 15) / 16;
-  og::k_obf_fwd<NA><<<dim3((nx + 63) / 64, (ntiles + 3) / 4), 256, 0, h->stream>>>(
+  og::k_obf_fwd<NA, 2><<<dim3((nx + 63) / 64, (ntiles + 7) / 8), 256, 0, h->stream>>>(
       h->obf_beta, h->ldb, h->obf_f, h->phi64, h->ldphi, nk, nx, no);
   OG_CHECK(hipGetLastError());
   if (g.lmax == 8)

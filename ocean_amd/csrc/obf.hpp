@@ -44,7 +44,7 @@ __device__ __forceinline__ ObfCol obf_col(int G, int na, int nk) {
 // Workgroup: 64 x points (four M tiles) by four N tiles, one per wave; the waves
 // share the A panel through L1.  Loads of the next group of four o are in flight
 // while the current group's eight instructions run.
-template <int NA>
+template <int NA, int NT>
 __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ beta, int ldb,
                                                  const double2* __restrict__ obf,
                                                  double2* __restrict__ phi, int ldphi, int nk,
@@ -52,48 +52,61 @@ __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ bet
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = lane & 15, kk = lane >> 4;
   const int x0 = blockIdx.x * 64;
-  const ObfCol col = obf_col((blockIdx.y * 4 + w) * 16 + i, NA, nk);
-  const double2* __restrict__ bp = beta + (size_t)col.k * ldb + col.a;
+  ObfCol col[NT];
+  const double2* bp[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    col[n] = obf_col(((blockIdx.y * 4 + w) * NT + n) * 16 + i, NA, nk);
+    bp[n] = beta + (size_t)col[n].k * ldb + col[n].a;
+  }
   int xr[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) xr[m] = min(x0 + 16 * m + i, nx - 1);
-  og_d4 acc[4];
+  og_d4 acc[NT][4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) acc[m] = og_d4{0.0, 0.0, 0.0, 0.0};
-  double2 bn, an[4];
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[n][m] = og_d4{0.0, 0.0, 0.0, 0.0};
+  double2 bn[NT], an[4];
   auto load = [&](int o0) {
     const int o = min(o0 + kk, no - 1);
-    bn = bp[(size_t)o * NA];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bn[n] = bp[n][(size_t)o * NA];
 #pragma unroll
     for (int m = 0; m < 4; ++m) an[m] = obf[(size_t)o * nx + xr[m]];
   };
   load(0);
   for (int o0 = 0; o0 < no; o0 += 4) {
     const bool ok = o0 + kk < no;
-    double2 bv = bn, av[4];
+    double2 bv[NT], av[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) av[m] = an[m];
+    for (int n = 0; n < NT; ++n) bv[n] = bn[n];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) av[m] = ok ? an[m] : make_double2(0.0, 0.0);
     load(o0 + 4);
-    if (!ok || !col.ok) bv = make_double2(0.0, 0.0);
     // C = A B: (ar, ai) x (br, bi) -> re: ar br - ai bi, im: ar bi + ai br
-    const double b0 = col.c ? bv.y : bv.x;
-    const double b1 = col.c ? bv.x : -bv.y;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-      acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].x : 0.0, b0, acc[m], 0, 0, 0);
+    for (int n = 0; n < NT; ++n) {
+      const double b0 = col[n].c ? bv[n].y : bv[n].x;
+      const double b1 = col[n].c ? bv[n].x : -bv[n].y;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-      acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].y : 0.0, b1, acc[m], 0, 0, 0);
-  }
-  if (!col.ok) return;
-  double* __restrict__ out = reinterpret_cast<double*>(phi + (size_t)col.k * ldphi + col.a) + col.c;
+      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0, acc[n][m], 0, 0, 0);
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int x = x0 + 16 * m + kk + 4 * r;
-      if (x < nx) out[(size_t)x * NA * 2] = acc[m][r];
+      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1, acc[n][m], 0, 0, 0);
     }
+  }
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    if (!col[n].ok) continue;
+    double* __restrict__ out = reinterpret_cast<double*>(phi + (size_t)col[n].k * ldphi + col[n].a) + col[n].c;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 16 * m + kk + 4 * r;
+        if (x < nx) out[(size_t)x * NA * 2] = acc[n][m][r];
+      }
+  }
 }
 
 // b(k, o, a) = scale sum_x conj(obf(o, x)) Y(k, x, a)        (long_range.f90:412-421)
