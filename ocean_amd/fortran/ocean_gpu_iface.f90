@@ -85,6 +85,17 @@ module ocean_gpu_iface
       real(c_double), intent(in) :: u_re(*), u_im(*)
     end function
 
+    ! OBF form (lr_act_obf2): re/im_obf(num_obf, nx), re/im_obf2u(nb, num_obf, nk),
+    ! xoff(3, nx) the wrapped x offsets of OCEAN_obf_make_phase
+    integer(c_int) function ocean_gpu_load_obf(h, num_obf, obf_re, obf_im, obf2u_re, obf2u_im, xoff) &
+        bind(C, name='ocean_gpu_load_obf')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: num_obf
+      real(c_double), intent(in) :: obf_re(*), obf_im(*), obf2u_re(*), obf2u_im(*)
+      integer(c_int32_t), intent(in) :: xoff(*)
+    end function
+
     integer(c_int) function ocean_gpu_load_w(h, w) bind(C, name='ocean_gpu_load_w')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

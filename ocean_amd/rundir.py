@@ -209,6 +209,60 @@ def read_u2(d, S: RunSystem):
     return np.asfortranarray(a.transpose(2, 1, 3, 0))  # (nb, nk, nx, nspn)
 
 
+def read_bloch_control(d):
+    """bloch_control: use_obf, use_fake_obf (OCEAN2/long_range.f90:128-146); without
+    the file both are false and the Bloch functions come from con.u2.dat."""
+    if not os.path.exists(_p(d, "bloch_control")):
+        return False, False
+    t = _tokens(_p(d, "bloch_control"))
+    return _fbool(t[0]), _fbool(t[1])
+
+
+def read_obf(d, S: RunSystem):
+    """The OBF form of the Bloch functions, as OCEAN_obf_init and OCEAN_obf_load read
+    it (OCEAN2/OCEAN_obf.f90:424-508, :510-732):
+
+    obf_control     num_obf, val_num_bands, con_num_bands (:465-470)
+    u1.dat          Fortran sequential, one record of complex(xmesh(3), xmesh(2),
+                    xmesh(1)) per basis function (:551-572), taken in the x order
+                    with ix fastest
+    con_eigvecs.dat stream complex128: the coefficients of k are num_obf * nb values
+                    at element offset (k-1) num_obf con_num_bands + num_obf ivh2, as
+                    (num_obf, nb) (:704-715); ivh2 = ivh of brange.ipt, or for a metal
+                    the second entry of k's ibeg.h line minus one (:650-699)
+
+    Returns (obf [num_obf, nx], obf2u [nb, num_obf, nk])."""
+    if S.nspn != 1:
+        raise NotImplementedError("the OBF form handles nspn = 1 only (OCEAN_obf.f90:669-673)")
+    t = _tokens(_p(d, "obf_control"))
+    no, width = int(t[0]), int(t[2])
+    X1, X2, X3 = S.xmesh
+    recs = fortran_records(_p(d, "u1.dat"))
+    if len(recs) < no:
+        raise ValueError("u1.dat holds fewer basis functions than obf_control")
+    obf = np.zeros((no, S.nx), np.complex128)
+    for o in range(no):
+        a = np.frombuffer(recs[o], np.complex128, count=S.nx)
+        obf[o] = a.reshape(X1, X2, X3).transpose(2, 1, 0).ravel()
+    ivl, ivh, icl, ich = (int(v) for v in _tokens(_p(d, "brange.ipt"))[:4])
+    if S.nb > 1 + (ich - icl):
+        raise ValueError("loadux ... nbd mismatch -- cf brange.ipt")
+    metal = _fbool(_tokens(_p(d, "metal"))[0])
+    if metal:
+        rows = [ln.split() for ln in open(_p(d, "ibeg.h")).read().splitlines() if ln.strip()]
+        ivh2 = [int(r[1]) - 1 for r in rows[:S.nk]]
+    else:
+        ivh2 = [ivh] * S.nk
+    raw = np.fromfile(_p(d, "con_eigvecs.dat"), dtype=np.complex128)
+    obf2u = np.zeros((S.nb, no, S.nk), np.complex128, order="F")
+    for k in range(S.nk):
+        off = k * no * width + no * ivh2[k]
+        if off + no * S.nb > raw.size:
+            raise ValueError("con_eigvecs.dat is truncated")
+        obf2u[:, :, k] = raw[off:off + no * S.nb].reshape(S.nb, no)
+    return obf, obf2u
+
+
 def read_energies(d, S: RunSystem):
     """wvfcninfo (Hartree, first nb bands), the Fermi level, the core-level shift and
     the allow mask, spread over alpha by valence spin (OCEAN2/OCEAN_energies.f90:
@@ -378,13 +432,29 @@ class RunInputs:
     kpref: float
     efermi: float
     shift: float
+    obf_mode: str = "bloch"   # "bloch" (con.u2.dat), "obf" (lr_act_obf2) or "fake" (use_fake_obf)
+    obf: np.ndarray | None = None
+    obf2u: np.ndarray | None = None
 
 
 def read_rundir(d):
     """Read a core XAS run directory into the loader arrays."""
     S = read_system(d)
-    u_raw = read_u2(d, S)
-    xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau)
+    use_obf, use_fake = read_bloch_control(d)
+    obf = obf2u = None
+    mode = "bloch"
+    if use_obf:
+        obf, obf2u = read_obf(d, S)
+        mode = "fake" if use_fake else "obf"
+    if mode == "obf":
+        # lr_act_obf2 on the OBF arrays; OCEAN_obf_make_phase centres the site
+        u_raw = None
+        xoff, tau_c, xshift = sh.obf_x_offsets(S.xmesh, S.kmesh, S.tau)
+    else:
+        # con.u2.dat, or the Bloch functions OCEAN_obf_lrLOAD builds from the OBFs;
+        # both are centred and phased as OCEAN_bloch_lrLOAD does
+        u_raw = read_u2(d, S) if mode == "bloch" else sh.obf_fake_bloch(obf, obf2u)
+        xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau)
     rdist, _ = sh.w_geometry(S.avec, S.kmesh, S.xmesh, tau_c, xshift)
     rtab, ptab = read_rpottrim(d, S)
     energy, allow, efermi, shift = read_energies(d, S)
@@ -403,7 +473,7 @@ def read_rundir(d):
     mels = read_mels(d, S, nptot)
     psi0, kpref = starting_vector(mels, cks, S, bpk)
     return RunInputs(S, u_raw, xoff, tau_c, xshift, rdist, rtab, ptab, energy, allow, mult, psi0,
-                     kpref, efermi, shift)
+                     kpref, efermi, shift, mode, obf, obf2u)
 
 
 def open_rundir(d, precision=None):
@@ -416,7 +486,10 @@ def open_rundir(d, precision=None):
     prec = core.FP64 if precision is None else precision
     g = core.OceanGPU(S.nb, S.kmesh, S.xmesh, S.ZNL[2], S.nspn, S.celvol, S.inter_scale, m.lmin, m.lmax,
                       tuple(m.nproj), m.lvl, m.lvh, True, S.mult, S.inter_scale >= 0.000001, prec)  # inter_min, OCEAN_system.f90:157
-    g.load_bloch_raw(R.u_raw, R.xoff)
+    if R.obf_mode == "obf":
+        g.load_obf(R.obf, R.obf2u, R.xoff)
+    else:
+        g.load_bloch_raw(R.u_raw, R.xoff)
     g.build_w(R.rdist, R.rtab, R.ptab, S.epsilon0, S.inter_scale)
     g.load_mult(m)
     g.load_energies(R.energy, R.allow)
@@ -452,7 +525,8 @@ def _fortran_sequential(path, arrays):
 
 def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn=1, extra_bands=2, seed=0,
                  xi_eV=0.0, lmin=0, lmax=2, nproj=(2, 1, 1), inter_scale=1.0, eps=6.0,
-                 efermi_eV=4.0, eshift_eV=0.7, tau=(0.13, 0.27, 0.41), niter=30, a_bohr=5.5):
+                 efermi_eV=4.0, eshift_eV=0.7, tau=(0.13, 0.27, 0.41), niter=30, a_bohr=5.5,
+                 obf=None):
     """Write a synthetic core XAS run directory in the file formats OCEAN2 reads
     (seeded: Bloch functions orthonormal over x per k, sorted band energies, the
     screened-potential table exp(-r/3)/(4 max(r, 0.2)), random projector overlaps,
@@ -465,6 +539,11 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     (nbsemkcmel.f90:134-141 writes it); prjfile, cmel, fk/gk
     (OCEAN_multiplet.f90:309-316, :486-504, :2064-2114); parcksc
     (OCEAN_multiplet.f90:426-451); mels (OCEAN_psi.f90:5964-5975).
+
+    obf = dict(num=..., fake=False) writes the OBF form instead of con.u2.dat:
+    bloch_control, obf_control, u1.dat and con_eigvecs.dat (OCEAN2/long_range.f90:
+    128-146, OCEAN_obf.f90:465-470, :551-572, :704-715), with orthogonal basis
+    functions of norm N_k^-1/2 and unit-norm coefficient vectors per (n, k).
     """
     rng = np.random.default_rng(seed)
     os.makedirs(d, exist_ok=True)
@@ -498,7 +577,23 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
         for k in range(nk):
             q, _ = np.linalg.qr(rng.standard_normal((nx, bif)) + 1j * rng.standard_normal((nx, bif)))
             u[s, k] = q.T
-    u.tofile(os.path.join(d, "con.u2.dat"))
+    obf_arrays = None
+    if obf is None:
+        u.tofile(os.path.join(d, "con.u2.dat"))
+    else:
+        no, ivh = int(obf["num"]), 4
+        width = ivh + bif
+        _w(os.path.join(d, "bloch_control"), ".true.\n%s\n" % (".true." if obf.get("fake") else ".false."))
+        _w(os.path.join(d, "obf_control"), "%d\n%d\n%d\n" % (no, ivh, width))
+        q, _ = np.linalg.qr(rng.standard_normal((nx, no)) + 1j * rng.standard_normal((nx, no)))
+        f = q.T / np.sqrt(nk)                                           # [o, x], ix fastest
+        # record o: complex(X3, X2, X1), element (iz, iy, ix)
+        recs = [f[o].reshape(xmesh[2], xmesh[1], xmesh[0]) for o in range(no)]
+        _fortran_sequential(os.path.join(d, "u1.dat"), recs)
+        coef = rng.standard_normal((nk, width, no)) + 1j * rng.standard_normal((nk, width, no))
+        coef /= np.sqrt(np.sum(np.abs(coef) ** 2, axis=2, keepdims=True))
+        coef.tofile(os.path.join(d, "con_eigvecs.dat"))                 # per k: (no, width)
+        obf_arrays = dict(obf=f, obf2u=np.asfortranarray(coef[:, ivh:ivh + nb, :].transpose(1, 2, 0)))
     # band energies (Hartree), sorted per k
     E = np.sort(rng.uniform(0.0, 30.0, size=(bif, nk, nspn)), axis=0) * EV2HARTREE
     hdr = np.array([bif, nk] + ([nspn] if nspn == 2 else []), np.int32)
@@ -544,10 +639,11 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     return dict(u=u, E=E, rtab=rtab, ptab=ptab, cmel=cmel, slater=slater, cks=cks, mels=mels, avec=avec,
                 tau=np.array(tau), nb=nb, kmesh=kmesh, xmesh=xmesh, lc=lc, nspn=nspn, bif=bif, lmin=lmin,
                 lmax=lmax, nproj=list(nproj), xi_eV=xi_eV, inter_scale=inter_scale, eps=eps,
-                efermi=efermi_eV * EV2HARTREE, eshift=eshift_eV * EV2HARTREE, nptot=nptot, celvol=vol, niter=niter)
+                efermi=efermi_eV * EV2HARTREE, eshift=eshift_eV * EV2HARTREE, nptot=nptot, celvol=vol, niter=niter,
+                obf=obf_arrays)
 
 
-__all__ = ["read_system", "read_u2", "read_energies", "read_rpottrim", "read_prjfile", "read_parcksc",
+__all__ = ["read_system", "read_u2", "read_bloch_control", "read_obf", "read_energies", "read_rpottrim", "read_prjfile", "read_parcksc",
            "projector_array", "read_cmel", "read_slater", "read_mels", "starting_vector", "read_rundir",
            "open_rundir", "fortran_records", "RunInputs", "RunSystem", "CoreRun", "hartree_window",
            "write_synthetic_rundir"]

@@ -79,6 +79,27 @@ def obf_x_offsets(xmesh, kmesh, tau):
     return np.ascontiguousarray(wrap + idx - 1, dtype=np.int32), tau_c, xshift
 
 
+def obf_fake_bloch(obf, obf2u):
+    """Bloch functions built from the OBF form as OCEAN_obf_lrLOAD does for
+    bloch_control's use_fake_obf (OCEAN2/OCEAN_obf.f90:86-118), before the phase:
+    u(n,k,x) = sum_o obf2u(n,o,k) obf(o,x) / sqrt(su(n,k)), where su adds
+    |partial sum over o|^2 after every basis function (the accumulation sits inside
+    the o loop, :88-97), not only the square of the finished sum.
+
+    obf [o, x], obf2u [n, o, k]; returns u_raw (nb, nk, nx, 1) for load_bloch_raw
+    (the phase and centring of that path are the Bloch form's, :121-262)."""
+    obf = np.asarray(obf, np.complex128)
+    c = np.asarray(obf2u, np.complex128)
+    nb, no, nk = c.shape
+    nx = obf.shape[1]
+    u = np.zeros((nb, nk, nx, 1), np.complex128, order="F")
+    for k in range(nk):
+        part = np.cumsum(c[:, :, k][:, :, None] * obf[None, :, :], axis=1)  # (nb, o, nx)
+        su = np.sum(part.real ** 2 + part.imag ** 2, axis=(1, 2))
+        u[:, k, :, 0] = part[:, -1, :] / np.sqrt(su)[:, None]
+    return u
+
+
 def bloch_phase(u_raw, xmesh, kmesh, tau, xshift_override=(1, 1, 1)):
     """Phase u(n,k,x) by exp(2 pi i sum_d (x_d + wrap_d) k_d / (Nx_d Nk_d)).
 

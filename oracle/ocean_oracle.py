@@ -44,6 +44,8 @@ Pinning (what checks each function; "parity unpinned" where none does):
   obf_phase            bloch_phase's closed form for even meshes; the odd-kmesh(1)
                        xshift by hand
   xact_obf             xact with the multiplet term off, plus the two terms added after
+  obf_lrload_bloch     one basis function (plain normalisation) and a hand-summed
+                       two-function, one-point case
   fg_combo/new_soact   dense operator brute force, Hermiticity
   gaunt/threey         Wigner-3j closed form vs Gauss-Legendre quadrature restatement
   soprep               closed-form <L.S> eigenvalues
@@ -798,6 +800,26 @@ def obf_phase(xmesh, kmesh, tau):
         ang = ang + 2.0 * np.pi * np.outer(q[:, d], xoff[:, d]).astype(np.float64) / float(xm[d] * km[d])
     phs = np.asfortranarray(np.cos(ang) + 1j * np.sin(ang))
     return phs, xoff, tau_c, xshift
+
+
+def obf_lrload_bloch(obf, obf2u):
+    """Loop-literal OCEAN_obf_lrLOAD before the phase (OCEAN_obf.f90:86-118): the
+    band sums and their norm accumulator su, updated inside the o loop (:88-97),
+    then u = sum / sqrt(su) (:110-118).  Tiny sizes only.  Returns (nb, nk, nx)."""
+    nb, no, nk = obf2u.shape
+    nx = obf.shape[1]
+    rbs = np.zeros((nb, nk, nx))
+    ibs = np.zeros((nb, nk, nx))
+    su = np.zeros((nb, nk))
+    for x in range(nx):
+        for k in range(nk):
+            for o in range(no):
+                rbs[:, k, x] += obf2u[:, o, k].real * obf[o, x].real - obf2u[:, o, k].imag * obf[o, x].imag
+                ibs[:, k, x] += obf2u[:, o, k].imag * obf[o, x].real + obf2u[:, o, k].real * obf[o, x].imag
+                for n in range(nb):
+                    su[n, k] += rbs[n, k, x] ** 2 + ibs[n, k, x] ** 2
+    su = 1.0 / np.sqrt(su)
+    return (rbs + 1j * ibs) * su[:, :, None]
 
 
 def lr_act_obf2(P: Problem, psi, obf, obf2u, phs):

@@ -58,3 +58,31 @@ def test_rundir_action_and_recursion(tmp_path, name):
     assert np.max(np.abs(a[:niter] - ra[:niter]) / np.abs(ra[:niter])) < 1e-10
     assert np.max(np.abs(b[1:niter + 1] - rb[1:niter + 1]) / np.abs(rb[1:niter + 1])) < 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("fake", [False, True])
+def test_rundir_obf_forms(tmp_path, fake):
+    """A run directory in the OBF form: lr_act_obf2 (use_obf) or the Bloch functions
+    OCEAN_obf_lrLOAD builds (use_fake_obf), against the oracle."""
+    from ocean_amd import rundir
+    d = str(tmp_path / ("fake" if fake else "obf"))
+    shape = dict(nb=5, kmesh=(3, 2, 2), xmesh=(3, 4, 2), lc=0, seed=3, nproj=(2, 1, 1), tau=(0.9, 0.27, 0.41))
+    gen = write_rundir(d, obf=dict(num=7, fake=fake), **shape)
+    print("OBF run directory, fake", fake, flush=True)
+    g, R = rundir.open_rundir(d)
+    S = R.system
+    f, c = gen["obf"]["obf"], gen["obf"]["obf2u"]
+    if fake:
+        R2 = rundir.RunInputs(**{**R.__dict__, "u_raw": orc.obf_lrload_bloch(f, c)[..., None]})
+        P, psi0, _ = oracle_problem(gen, R2)
+        ref = [orc.xact(P, psi0, backwards=b) for b in (False, True)]
+    else:
+        phs, _, tau_c, xshift = orc.obf_phase(S.xmesh, S.kmesh, S.tau)
+        R2 = rundir.RunInputs(**{**R.__dict__, "u_raw": np.zeros((S.nb, S.nk, S.nx, 1), complex)})
+        P, psi0, _ = oracle_problem(gen, R2)
+        P.W = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, R.rtab, R.ptab, S.epsilon0,
+                             S.inter_scale)
+        ref = [orc.xact_obf(P, psi0, f, c, phs, backwards=b) for b in (False, True)]
+    for back, r in zip((False, True), ref):
+        assert relerr(g.xact(R.psi0, backwards=back), r) < 1e-12, back
+    g.close()

@@ -1,5 +1,6 @@
 """The run-directory reader (ocean_amd.rundir) against the files a synthetic OCEAN2
 core run directory holds and the oracle's restatements of the setup steps."""
+import math
 import os
 
 import numpy as np
@@ -107,5 +108,67 @@ def test_unsupported_inputs_are_refused(tmp_path):
         rundir.read_rundir(d)
     os.remove(os.path.join(d, "core_broaden.ipt"))
     open(os.path.join(d, "doping"), "w").write("0.1\n")
+    with pytest.raises(NotImplementedError):
+        rundir.read_rundir(d)
+
+
+OBF_SHAPE = dict(nb=5, kmesh=(3, 2, 2), xmesh=(3, 4, 2), lc=0, seed=3, nproj=(2, 1, 1), tau=(0.9, 0.27, 0.41))
+
+
+def test_obf_files(tmp_path):
+    """bloch_control/obf_control/u1.dat/con_eigvecs.dat read back into the arrays the
+    writer drew; make_phase's centring (odd kmesh(1): xshift(1) from xmesh(2))."""
+    d = str(tmp_path / "obf")
+    gen = write_rundir(d, obf=dict(num=7), **OBF_SHAPE)
+    assert not os.path.exists(os.path.join(d, "con.u2.dat"))
+    R = rundir.read_rundir(d)
+    assert R.obf_mode == "obf" and R.u_raw is None
+    assert np.array_equal(R.obf, gen["obf"]["obf"])
+    assert np.array_equal(R.obf2u, gen["obf"]["obf2u"])
+    S = R.system
+    _, xoff, tau_c, xshift = orc.obf_phase(S.xmesh, S.kmesh, S.tau)
+    assert np.array_equal(R.xoff, xoff) and np.array_equal(R.xshift, xshift)
+    assert np.allclose(R.tau_c, tau_c, rtol=0, atol=1e-15)
+    assert xshift[0] == math.floor(4 * (0.9 - 0.5))  # xmesh(2) = 4, not xmesh(1) = 3
+    r, _ = sh.w_geometry(S.avec, S.kmesh, S.xmesh, tau_c, xshift)
+    assert np.array_equal(R.rdist, r)
+
+
+def test_fake_obf_bloch_functions(tmp_path):
+    """use_fake_obf: the Bloch functions OCEAN_obf_lrLOAD builds, against the
+    loop-literal restatement, then the Bloch form's centring."""
+    d = str(tmp_path / "fake")
+    gen = write_rundir(d, obf=dict(num=6, fake=True), **OBF_SHAPE)
+    R = rundir.read_rundir(d)
+    assert R.obf_mode == "fake"
+    ref = orc.obf_lrload_bloch(gen["obf"]["obf"], gen["obf"]["obf2u"])
+    assert np.abs(R.u_raw[..., 0] - ref).max() < 1e-14 * np.abs(ref).max()
+    S = R.system
+    _, _, xs = orc.bloch_phase(np.zeros((1, S.nk, S.nx, 1), complex), S.xmesh, S.kmesh, S.tau)
+    assert np.array_equal(R.xshift, xs)
+
+
+def test_obf_lrload_closed_forms():
+    """One basis function: plain normalisation.  Two functions at one x point: the
+    norm accumulates |c1 f1|^2 + |c1 f1 + c2 f2|^2 (the partial sum after each)."""
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((1, 5)) + 1j * rng.standard_normal((1, 5))
+    c = rng.standard_normal((3, 1, 2)) + 1j * rng.standard_normal((3, 1, 2))
+    u = orc.obf_lrload_bloch(f, c)
+    ref = c[:, 0, :, None] * f[0][None, None, :]
+    ref = ref / np.linalg.norm(ref, axis=2, keepdims=True)
+    assert np.abs(u - ref).max() < 1e-14
+    f2 = np.array([[1.0 + 2.0j], [0.5 - 1.0j]])
+    c2 = np.array([[[2.0 - 1.0j], [1.0 + 1.0j]]])  # (nb=1, no=2, nk=1)
+    s1 = (2.0 - 1.0j) * (1.0 + 2.0j)
+    s2 = s1 + (1.0 + 1.0j) * (0.5 - 1.0j)
+    want = s2 / math.sqrt(abs(s1) ** 2 + abs(s2) ** 2)
+    assert abs(orc.obf_lrload_bloch(f2, c2)[0, 0, 0] - want) < 1e-15
+    assert np.abs(sh.obf_fake_bloch(f2, c2)[0, 0, 0, 0] - want) < 1e-15
+
+
+def test_obf_needs_one_spin(tmp_path):
+    d = str(tmp_path / "obf2")
+    write_rundir(d, obf=dict(num=4), **dict(SHAPES["l23_spin"]))
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
