@@ -67,33 +67,45 @@ __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ bet
   for (int n = 0; n < NT; ++n)
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[n][m] = og_d4{0.0, 0.0, 0.0, 0.0};
-  double2 bn[NT], an[4];
-  auto load = [&](int o0) {
+  // two register buffers, alternating: the loads of one group of four o are in
+  // flight while the other group's matrix instructions run
+  double2 bA[NT], aA[4], bB[NT], aB[4];
+  auto load = [&](int o0, double2 (&bn)[NT], double2 (&an)[4]) {
     const int o = min(o0 + kk, no - 1);
 #pragma unroll
     for (int n = 0; n < NT; ++n) bn[n] = bp[n][(size_t)o * NA];
 #pragma unroll
     for (int m = 0; m < 4; ++m) an[m] = obf[(size_t)o * nx + xr[m]];
   };
-  load(0);
-  for (int o0 = 0; o0 < no; o0 += 4) {
+  // C = A B: (ar, ai) x (br, bi) -> re: ar br - ai bi, im: ar bi + ai br
+  auto step = [&](int o0, const double2 (&bv)[NT], const double2 (&an)[4]) {
     const bool ok = o0 + kk < no;
-    double2 bv[NT], av[4];
-#pragma unroll
-    for (int n = 0; n < NT; ++n) bv[n] = bn[n];
+    double2 av[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) av[m] = ok ? an[m] : make_double2(0.0, 0.0);
-    load(o0 + 4);
-    // C = A B: (ar, ai) x (br, bi) -> re: ar br - ai bi, im: ar bi + ai br
+    double b0[NT], b1[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      const double b0 = col[n].c ? bv[n].y : bv[n].x;
-      const double b1 = col[n].c ? bv[n].x : -bv[n].y;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0, acc[n][m], 0, 0, 0);
-#pragma unroll
-      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1, acc[n][m], 0, 0, 0);
+      b0[n] = col[n].c ? bv[n].y : bv[n].x;
+      b1[n] = col[n].c ? bv[n].x : -bv[n].y;
     }
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0[n], acc[n][m], 0, 0, 0);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1[n], acc[n][m], 0, 0, 0);
+  };
+  load(0, bA, aA);
+  for (int o0 = 0; o0 < no; o0 += 8) {
+    load(o0 + 4, bB, aB);
+    asm volatile("" ::: "memory");
+    step(o0, bA, aA);
+    load(o0 + 8, bA, aA);
+    asm volatile("" ::: "memory");
+    step(o0 + 4, bB, aB);
   }
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
@@ -120,7 +132,8 @@ __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, 
                                                  double2* __restrict__ b, int ldb, int nk, int nx,
                                                  int no, double scale) {
   __shared__ double part[3][16][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform x quarter
   const int i = lane & 15, kk = lane >> 4;
   const int o0 = blockIdx.x * 64;
   const ObfCol col = obf_col(blockIdx.y * 16 + i, NA, nk);
@@ -133,30 +146,33 @@ __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, 
 #pragma unroll
   for (int m = 0; m < 4; ++m) acc[m] = og_d4{0.0, 0.0, 0.0, 0.0};
   if (xb > xa) {
-    double2 yn, an[4];
-    auto load = [&](int xs) {
+    // two register buffers, alternating, as in k_obf_fwd
+    double2 yA, aA[4], yB, aB[4];
+    auto load = [&](int xs, double2& yn, double2 (&an)[4]) {
       const int x = min(xs + kk, xb - 1);
       yn = yp[(size_t)x * NA];
 #pragma unroll
       for (int m = 0; m < 4; ++m) an[m] = obfT[(size_t)x * no + orow[m]];
     };
-    load(xa);
-    for (int xs = xa; xs < xb; xs += 4) {
+    // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
+    auto step = [&](int xs, const double2& yv, const double2 (&av)[4]) {
       const bool ok = xs + kk < xb;
-      double2 yv = yn, av[4];
+      const double2 y = (ok && col.ok) ? yv : make_double2(0.0, 0.0);
+      const double b0 = col.c ? y.y : y.x;
+      const double b1 = col.c ? -y.x : y.y;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) av[m] = an[m];
-      load(xs + 4);
-      if (!ok || !col.ok) yv = make_double2(0.0, 0.0);
-      // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
-      const double b0 = col.c ? yv.y : yv.x;
-      const double b1 = col.c ? -yv.x : yv.y;
+      for (int m = 0; m < 4; ++m) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0, acc[m], 0, 0, 0);
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
-        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].x : 0.0, b0, acc[m], 0, 0, 0);
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? av[m].y : 0.0, b1, acc[m], 0, 0, 0);
+      for (int m = 0; m < 4; ++m) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1, acc[m], 0, 0, 0);
+    };
+    load(xa, yA, aA);
+    for (int xs = xa; xs < xb; xs += 8) {
+      load(xs + 4, yB, aB);
+      asm volatile("" ::: "memory");
+      step(xs, yA, aA);
+      load(xs + 8, yA, aA);
+      asm volatile("" ::: "memory");
+      step(xs + 4, yB, aB);
     }
   }
   if (w > 0) {
