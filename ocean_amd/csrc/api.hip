@@ -309,7 +309,7 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   else
     og::k_fft_w<double2, 16, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
   OG_CHECK(hipGetLastError());
-  og::k_obf_bwd<NA><<<dim3((no + 63) / 64, ntiles), 256, 0, h->stream>>>(
+  og::k_obf_bwd<NA, 1><<<dim3((no + 63) / 64, ntiles), 256, 0, h->stream>>>(
       h->phi64, h->ldphi, h->obf_fT, h->obf_beta, h->ldb, nk, nx, no, -(double)nk);
   OG_CHECK(hipGetLastError());
   if (ev) {

@@ -126,44 +126,63 @@ __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ bet
 // b [k][o][a].  Workgroup: 64 basis functions (four M tiles) by one N tile; its four
 // waves take consecutive quarters of the x range and their partial tiles are
 // added in LDS in a fixed order, so the result is reproducible.
-template <int NA>
+template <int NA, int NT>
 __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, int ldphi,
                                                  const double2* __restrict__ obfT,
                                                  double2* __restrict__ b, int ldb, int nk, int nx,
                                                  int no, double scale) {
-  __shared__ double part[3][16][64];
+  __shared__ double part[4][NT * 16][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform x quarter
   const int i = lane & 15, kk = lane >> 4;
   const int o0 = blockIdx.x * 64;
-  const ObfCol col = obf_col(blockIdx.y * 16 + i, NA, nk);
-  const double2* __restrict__ yp = Y + (size_t)col.k * ldphi + col.a;
+  ObfCol col[NT];
+  const double2* yp[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    col[n] = obf_col((blockIdx.y * NT + n) * 16 + i, NA, nk);
+    yp[n] = Y + (size_t)col[n].k * ldphi + col[n].a;
+  }
   int orow[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) orow[m] = min(o0 + 16 * m + i, no - 1);
   const int xa = (nx * w) >> 2, xb = (nx * (w + 1)) >> 2;
-  og_d4 acc[4];
+  og_d4 acc[NT][4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) acc[m] = og_d4{0.0, 0.0, 0.0, 0.0};
-  if (xb > xa) {
-    // two register buffers, alternating, as in k_obf_fwd
-    double2 yA, aA[4], yB, aB[4];
-    auto load = [&](int xs, double2& yn, double2 (&an)[4]) {
-      const int x = min(xs + kk, xb - 1);
-      yn = yp[(size_t)x * NA];
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[n][m] = og_d4{0.0, 0.0, 0.0, 0.0};
+  {
+    // two register buffers, alternating, as in k_obf_fwd (an empty quarter, nx < 4,
+    // loads row 0 and runs no step)
+    double2 yA[NT], aA[4], yB[NT], aB[4];
+    auto load = [&](int xs, double2 (&yn)[NT], double2 (&an)[4]) {
+      const int x = max(min(xs + kk, xb - 1), 0);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) yn[n] = yp[n][(size_t)x * NA];
 #pragma unroll
       for (int m = 0; m < 4; ++m) an[m] = obfT[(size_t)x * no + orow[m]];
     };
     // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
-    auto step = [&](int xs, const double2& yv, const double2 (&av)[4]) {
+    auto step = [&](int xs, const double2 (&yv)[NT], const double2 (&an)[4]) {
       const bool ok = xs + kk < xb;
-      const double2 y = (ok && col.ok) ? yv : make_double2(0.0, 0.0);
-      const double b0 = col.c ? y.y : y.x;
-      const double b1 = col.c ? -y.x : y.y;
+      double2 av[4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0, acc[m], 0, 0, 0);
+      for (int m = 0; m < 4; ++m) av[m] = ok ? an[m] : make_double2(0.0, 0.0);
+      double b0[NT], b1[NT];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1, acc[m], 0, 0, 0);
+      for (int n = 0; n < NT; ++n) {
+        b0[n] = col[n].c ? yv[n].y : yv[n].x;
+        b1[n] = col[n].c ? -yv[n].x : yv[n].y;
+      }
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0[n], acc[n][m], 0, 0, 0);
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[n][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1[n], acc[n][m], 0, 0, 0);
     };
     load(xa, yA, aA);
     for (int xs = xa; xs < xb; xs += 8) {
@@ -175,25 +194,30 @@ __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, 
       step(xs + 4, yB, aB);
     }
   }
-  if (w > 0) {
+  // every wave parks its tiles in LDS; wave 0 adds the four quarters in order
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) part[w - 1][4 * m + r][lane] = acc[m][r];
-  }
+      for (int r = 0; r < 4; ++r) part[w][16 * n + 4 * m + r][lane] = acc[n][m][r];
   __syncthreads();
-  if (w != 0 || !col.ok) return;
-  double* __restrict__ out = reinterpret_cast<double*>(b + (size_t)col.k * ldb + col.a) + col.c;
+  if (w != 0) return;
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int n = 0; n < NT; ++n) {
+    if (!col[n].ok) continue;
+    double* __restrict__ out = reinterpret_cast<double*>(b + (size_t)col[n].k * ldb + col[n].a) + col[n].c;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int o = o0 + 16 * m + kk + 4 * r;
-      double v = acc[m][r];
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) v += part[j][4 * m + r][lane];
-      if (o < no) out[(size_t)o * NA * 2] = v * scale;
-    }
+      for (int r = 0; r < 4; ++r) {
+        const int o = o0 + 16 * m + kk + 4 * r;
+        double v = part[0][16 * n + 4 * m + r][lane];
+#pragma unroll
+        for (int j = 1; j < 4; ++j) v += part[j][16 * n + 4 * m + r][lane];
+        if (o < no) out[(size_t)o * NA * 2] = v * scale;
+      }
+  }
 }
 
 // obf_phs(k, x) = exp(i (phsx + phsy + phsz)),
