@@ -164,16 +164,14 @@ __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, 
       for (int m = 0; m < 4; ++m) an[m] = obfT[(size_t)x * no + orow[m]];
     };
     // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
-    auto step = [&](int xs, const double2 (&yv)[NT], const double2 (&an)[4]) {
-      const bool ok = xs + kk < xb;
-      double2 av[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) av[m] = ok ? an[m] : make_double2(0.0, 0.0);
+    auto step = [&](int xs, const double2 (&yv)[NT], const double2 (&av)[4]) {
+      const bool ok = xs + kk < xb;  // rows past the quarter contribute zero
       double b0[NT], b1[NT];
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
-        b0[n] = col[n].c ? yv[n].y : yv[n].x;
-        b1[n] = col[n].c ? -yv[n].x : yv[n].y;
+        const double2 y = ok ? yv[n] : make_double2(0.0, 0.0);
+        b0[n] = col[n].c ? y.y : y.x;
+        b1[n] = col[n].c ? -y.x : y.y;
       }
 #pragma unroll
       for (int n = 0; n < NT; ++n)
