@@ -63,6 +63,7 @@ _SIGS = {
     "ocean_gpu_build_w": (C.c_int, [_H, _DP, _DP, _DP, C.c_int32, C.c_double, C.c_double]),
     "ocean_gpu_load_mult": (C.c_int, [_H, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int32]),
     "ocean_gpu_load_energies": (C.c_int, [_H, _DP, _DP, _DP]),
+    "ocean_gpu_mult_setup": (C.c_int, [_H, _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP, _IP]),
     "ocean_gpu_xact": (C.c_int, [_H, _DP, _DP, _DP, _DP, C.c_int32, _IP]),
     "ocean_gpu_vec_new": (C.c_int, [_H, _IP]),
     "ocean_gpu_vec_free": (C.c_int, [_H, C.c_int32]),
@@ -332,6 +333,38 @@ class OceanGPU:
         self.nptab = len(rt)
         _chk(_lib().ocean_gpu_build_w(self.h, _p(rd), _p(rt), _p(pt), len(rt), epsilon0, scale),
              "build_w")
+
+    def mult_setup(self, slater, scfac, xi_eV):
+        """Interaction blocks on the device (ocean_gpu_mult_setup).  slater: per valence
+        l = lvl..lvh a pair (fk, gk) of dicts k -> (nproj, nproj) matrices already
+        multiplied by their scale factors.  Returns (X list of (mham, mham) complex
+        blocks in Hartree, somelr, someli, have_so)."""
+        d = self.desc
+        lc = d.lc
+        flat, sizes = [], []
+        for i, l in enumerate(range(d.lvl, d.lvh + 1)):
+            npl = d.nproj[l - d.lmin] if d.lmin <= l <= d.lmax else 0
+            fk, gk = slater[i]
+            if npl:
+                ks = [("f", k) for k in range(2, min(2 * lc, 2 * l) + 1, 2)] + \
+                     [("g", k) for k in range(abs(lc - l), lc + l + 1, 2)]
+                for which, k in ks:
+                    m = (fk if which == "f" else gk)[k]
+                    flat.append(np.asfortranarray(m, np.float64).ravel(order="F"))
+            sizes.append(4 * (2 * lc + 1) * (2 * l + 1) * npl)
+        sl = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros(1))
+        ntot = max(sum(m * m for m in sizes), 1)
+        mhr, mhi = np.zeros(ntot), np.zeros(ntot)
+        na = self.na
+        sr, si = np.zeros((na, na), order="F"), np.zeros((na, na), order="F")
+        hso = C.c_int32()
+        _chk(_lib().ocean_gpu_mult_setup(self.h, _p(sl), scfac, xi_eV, _p(mhr), _p(mhi), _p(sr), _p(si),
+                                         C.byref(hso)), "mult_setup")
+        X, j = [], 0
+        for m in sizes:
+            X.append(np.asfortranarray((mhr[j:j + m * m] + 1j * mhi[j:j + m * m]).reshape(m, m, order="F")))
+            j += m * m
+        return X, sr, si, bool(hso.value)
 
     def load_mult(self, m):
         npmax = max(m.nproj)

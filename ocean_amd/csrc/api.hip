@@ -6,6 +6,7 @@
 #include "action.hpp"
 #include "common.hpp"
 #include "obf.hpp"
+#include "setup.hpp"
 #include "vec.hpp"
 
 using og::kBlock;
@@ -959,6 +960,61 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
   h->have_so = have_so ? 1 : 0;
   h->has_mult = true;
   return OCEAN_GPU_OK;
+}
+
+int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, double xi_ev, double* mhr,
+                         double* mhi, double* somelr, double* someli, int32_t* have_so) {
+  if (!h || !slater || !mhr || !mhi || !somelr || !someli || !have_so) return OCEAN_GPU_ERR_ARG;
+  const ocean_bse_desc& d = h->d;
+  const int lc = d.lc, na = h->na;
+  // per valence l: the block size and where its Slater matrices and block start
+  std::vector<int> npl_l, nsl_l;
+  std::vector<size_t> jb_l, sb_l;
+  size_t jtot = 0, stot = 0;
+  for (int l = d.lvl; l <= d.lvh; ++l) {
+    const int npl = (l >= d.lmin && l <= d.lmax) ? d.nproj[l - d.lmin] : 0;
+    const int kfh = std::min(2 * lc, 2 * l);
+    const int nF = kfh >= 2 ? kfh / 2 : 0, nG = (lc + l - std::abs(lc - l)) / 2 + 1;
+    const size_t mham = (size_t)4 * (2 * lc + 1) * (2 * l + 1) * npl;
+    npl_l.push_back(npl);
+    nsl_l.push_back((nF + nG) * npl * npl);
+    jb_l.push_back(jtot);
+    sb_l.push_back(stot);
+    jtot += mham * mham;
+    stot += (size_t)(nF + nG) * npl * npl;
+  }
+  double *dsl = nullptr, *dmh = nullptr, *dso = nullptr;
+  OG_TRY(dmalloc(&dsl, std::max<size_t>(stot, 1)));
+  int rc = dmalloc(&dmh, 2 * std::max<size_t>(jtot, 1));
+  if (rc == 0) rc = dmalloc(&dso, 2 * (size_t)na * na);
+  if (rc == 0 && stot && hipMemcpy(dsl, slater, stot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = OCEAN_GPU_ERR_HIP;
+  // nbsemhsetup2 works in eV; the module converts the blocks to Hartree (OCEAN_multiplet.f90:388-389)
+  for (size_t i = 0; rc == 0 && i < npl_l.size(); ++i) {
+    const int npl = npl_l[i];
+    if (npl == 0) continue;
+    const int l = d.lvl + (int)i;
+    const size_t mham = (size_t)4 * (2 * lc + 1) * (2 * l + 1) * npl;
+    og::k_mham<<<grid_for(mham * mham), kBlock, 0, h->stream>>>(lc, l, npl, dsl + sb_l[i], scfac, kEV2Hartree,
+                                                               dmh + jb_l[i], dmh + jtot + jb_l[i]);
+    if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  const double xi = xi_ev * kEV2Hartree;
+  if (rc == 0) {
+    og::k_soprep<<<1, kBlock, 0, h->stream>>>(lc, xi, dso, dso + (size_t)na * na);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0 && jtot &&
+      (hipMemcpy(mhr, dmh, jtot * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+       hipMemcpy(mhi, dmh + jtot, jtot * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0 && (hipMemcpy(somelr, dso, (size_t)na * na * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(someli, dso + (size_t)na * na, (size_t)na * na * sizeof(double),
+                            hipMemcpyDeviceToHost) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  *have_so = (xi >= 1.0e-12) ? 1 : 0;  // soprep's switch, OCEAN_multiplet.f90:612
+  dfree(dsl); dfree(dmh); dfree(dso);
+  return rc;
 }
 
 int ocean_gpu_load_energies(ocean_gpu_handle* h, const double* e_re, const double* e_im, const double* allow) {

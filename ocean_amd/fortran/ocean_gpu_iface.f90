@@ -96,6 +96,17 @@ module ocean_gpu_iface
       integer(c_int32_t), intent(in) :: xoff(*)
     end function
 
+    ! nbsemhsetup2 blocks and soprep formed on the device (see ocean_gpu.h)
+    integer(c_int) function ocean_gpu_mult_setup(h, slater, scfac, xi_ev, mhr, mhi, somelr, someli, have_so) &
+        bind(C, name='ocean_gpu_mult_setup')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: slater(*)
+      real(c_double), value :: scfac, xi_ev
+      real(c_double), intent(out) :: mhr(*), mhi(*), somelr(*), someli(*)
+      integer(c_int32_t), intent(out) :: have_so
+    end function
+
     integer(c_int) function ocean_gpu_load_w(h, w) bind(C, name='ocean_gpu_load_w')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

@@ -2,8 +2,9 @@
 
 Reads the files OCEAN2 reads when it sets up the core-level BSE for the first core
 run of ``runlist`` and turns them into the arrays the C-ABI loaders take; the two
-setup steps with a device form (the Bloch phase and W) run on the GPU through
-``ocean_gpu_load_bloch_raw`` and ``ocean_gpu_build_w``.  File formats and the
+setup steps with a device form run on the GPU: the Bloch (or OBF) phase and W through
+``ocean_gpu_load_bloch_raw`` / ``ocean_gpu_load_obf`` and ``ocean_gpu_build_w``, the
+multiplet interaction blocks and the spin-orbit matrix through ``ocean_gpu_mult_setup``.  File formats and the
 derived quantities follow the reference routines cited at each reader (paths
 relative to the OCEAN 3.2.1 tree).
 
@@ -18,6 +19,7 @@ legacy (sequential) cks files.
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 from dataclasses import dataclass, field
 
@@ -435,6 +437,8 @@ class RunInputs:
     obf_mode: str = "bloch"   # "bloch" (con.u2.dat), "obf" (lr_act_obf2) or "fake" (use_fake_obf)
     obf: np.ndarray | None = None
     obf2u: np.ndarray | None = None
+    slater: list | None = None  # per valence l: (fk, gk) dicts k -> scaled Slater matrix
+    scfac: float = 1.0
 
 
 def read_rundir(d):
@@ -464,16 +468,19 @@ def read_rundir(d):
     c = projector_array(cks, S, lmin, lmax, nproj, bpk)
     mpm = read_cmel(d, lmin, lmax, nproj)
     X = []
+    slater = []
+    scfac = 1.0
     for lv in range(S.lvl, S.lvh + 1):
         npl = nproj[lv - lmin]
         fk, gk, scfac = read_slater(d, S, lv, npl)
+        slater.append((fk, gk))
         X.append(np.asfortranarray(sh.exchange_matrix(S.ZNL[2], lv, npl, fk, gk, scfac) * EV2HARTREE))
     sr, si, have_so = sh.spin_orbit(S.ZNL[2], S.xi_eV)
     mult = MultipletInputs(lmin, lmax, nproj, S.lvl, S.lvh, c, mpm, X, sr, si, have_so)
     mels = read_mels(d, S, nptot)
     psi0, kpref = starting_vector(mels, cks, S, bpk)
     return RunInputs(S, u_raw, xoff, tau_c, xshift, rdist, rtab, ptab, energy, allow, mult, psi0,
-                     kpref, efermi, shift, mode, obf, obf2u)
+                     kpref, efermi, shift, mode, obf, obf2u, slater, scfac)
 
 
 def open_rundir(d, precision=None):
@@ -486,6 +493,10 @@ def open_rundir(d, precision=None):
     prec = core.FP64 if precision is None else precision
     g = core.OceanGPU(S.nb, S.kmesh, S.xmesh, S.ZNL[2], S.nspn, S.celvol, S.inter_scale, m.lmin, m.lmax,
                       tuple(m.nproj), m.lvl, m.lvh, True, S.mult, S.inter_scale >= 0.000001, prec)  # inter_min, OCEAN_system.f90:157
+    if R.slater is not None:
+        # the interaction blocks and the spin-orbit matrix formed on the device
+        X, sr, si, have_so = g.mult_setup(R.slater, R.scfac, S.xi_eV)
+        m = dataclasses.replace(m, X=X, somelr=sr, someli=si, have_spin_orbit=have_so)
     if R.obf_mode == "obf":
         g.load_obf(R.obf, R.obf2u, R.xoff)
     else:
