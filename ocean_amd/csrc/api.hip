@@ -962,8 +962,9 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
   return OCEAN_GPU_OK;
 }
 
-int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, double xi_ev, double* mhr,
-                         double* mhi, double* somelr, double* someli, int32_t* have_so) {
+int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, double xi_ev,
+                         const double* life_time_ev, double* mhr, double* mhi, double* somelr, double* someli,
+                         int32_t* have_so) {
   if (!h || !slater || !mhr || !mhi || !somelr || !someli || !have_so) return OCEAN_GPU_ERR_ARG;
   const ocean_bse_desc& d = h->d;
   const int lc = d.lc, na = h->na;
@@ -1000,8 +1001,15 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
     if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }
   const double xi = xi_ev * kEV2Hartree;
+  // core_broaden.ipt: lifetimes of the j = lc + 1/2 and lc - 1/2 levels (OCEAN_multiplet.f90:613-627)
+  double l_alpha = 0.0, l_beta = 0.0;
+  if (life_time_ev) {
+    const double t1 = life_time_ev[0] * kEV2Hartree, t2 = life_time_ev[1] * kEV2Hartree;
+    l_alpha = 2.0 / (double)(2 * lc + 1) * (t1 - t2);
+    l_beta = (double)(lc + 1) / (double)(2 * lc + 1) * t1 + (double)lc / (double)(2 * lc + 1) * t2;
+  }
   if (rc == 0) {
-    og::k_soprep<<<1, kBlock, 0, h->stream>>>(lc, xi, dso, dso + (size_t)na * na);
+    og::k_soprep<<<1, kBlock, 0, h->stream>>>(lc, xi, l_alpha, l_beta, dso, dso + (size_t)na * na);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }
   if (rc == 0 && jtot &&
@@ -1012,7 +1020,7 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
                   hipMemcpy(someli, dso + (size_t)na * na, (size_t)na * na * sizeof(double),
                             hipMemcpyDeviceToHost) != hipSuccess))
     rc = OCEAN_GPU_ERR_HIP;
-  *have_so = (xi >= 1.0e-12) ? 1 : 0;  // soprep's switch, OCEAN_multiplet.f90:612
+  *have_so = (xi >= 1.0e-12 || life_time_ev) ? 1 : 0;  // soprep's switch, OCEAN_multiplet.f90:612-615
   dfree(dsl); dfree(dmh); dfree(dso);
   return rc;
 }

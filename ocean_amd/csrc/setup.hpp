@@ -103,9 +103,12 @@ __global__ void k_mham(int lc, int lv, int npl, const double* __restrict__ slate
 }
 
 // S(ic, jc) = -xi <cm_jc, cs_jc | L.S | cm_ic, cs_ic> where the valence spins agree
-// (soprep through jimel, OCEAN_multiplet.f90:613-671; CNBSE/zcmsnmult/zf90/jimel.f90),
-// with L.S = Lz Sz + (L+ S- + L- S+) / 2; somelr/someli(ic, jc) column-major.
-__global__ void k_soprep(int lc, double xi, double* __restrict__ sr, double* __restrict__ si) {
+// (soprep through jimel, OCEAN_multiplet.f90:642-671; CNBSE/zcmsnmult/zf90/jimel.f90),
+// with L.S = Lz Sz + (L+ S- + L- S+) / 2; the spin-orbit-dependent lifetimes add
+// i (-<L.S> l_alpha - l_beta on the diagonal) (:664-668).  somelr/someli(ic, jc)
+// column-major.
+__global__ void k_soprep(int lc, double xi, double l_alpha, double l_beta, double* __restrict__ sr,
+                         double* __restrict__ si) {
   const int na = 4 * (2 * lc + 1);
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < na * na; e += gridDim.x * blockDim.x) {
     const int ic = e % na, jc = e / na;
@@ -120,7 +123,7 @@ __global__ void k_soprep(int lc, double xi, double* __restrict__ sr, double* __r
       if (m2 == m1 - 1 && s1 == -1 && s2 == 1) v += 0.5 * sqrt(ll - (double)(m1 * (m1 - 1)));  // L- S+
     }
     sr[e] = -xi * v;
-    si[e] = 0.0;
+    si[e] = -v * l_alpha - (ic == jc && v1 == v2 ? l_beta : 0.0);
   }
 }
 

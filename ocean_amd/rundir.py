@@ -13,7 +13,7 @@ formats (for this reader's tests, and as input for OCEAN2's own CPU build).
 
 Not read (a run that needs them raises ``NotImplementedError``): XES / valence /
 RIXS runs, GW corrections, doping and ``findNewFermi.ipt``, occupation types other
-than the default, ``core_broaden.ipt``, ``rpa``, ``tdlda``, ``isolated.inp``,
+than the default, ``rpa``, ``tdlda``, ``isolated.inp``,
 ``pbc.inp``, ``epsilon3D``, ``xshift_override.ipt``, ``file_brange.ipt`` and the
 legacy (sequential) cks files.
 """
@@ -131,7 +131,7 @@ def read_system(d):
     """Meshes, spins, ZNL, bands, lattice, first core run, site, mode, epsilon,
     bse.in (OCEAN2/OCEAN_system.f90:166-231, :310-335, :644-726;
     OCEAN_multiplet.f90:298-303; OCEAN_haydock.f90:2109-2116)."""
-    _refuse(d, ["force_val_ham_spin.ipt", "file_brange.ipt", "rpa", "tdlda", "core_broaden.ipt",
+    _refuse(d, ["force_val_ham_spin.ipt", "file_brange.ipt", "rpa", "tdlda",
                 "isolated.inp", "pbc.inp", "epsilon3D", "xshift_override.ipt"])
     xmesh = tuple(int(t) for t in _tokens(_p(d, "xmesh.ipt"))[:3])
     kmesh = tuple(int(t) for t in _tokens(_p(d, "kmesh.ipt"))[:3])
@@ -439,6 +439,7 @@ class RunInputs:
     obf2u: np.ndarray | None = None
     slater: list | None = None  # per valence l: (fk, gk) dicts k -> scaled Slater matrix
     scfac: float = 1.0
+    life_time: np.ndarray | None = None  # core_broaden.ipt (eV)
 
 
 def read_rundir(d):
@@ -475,12 +476,16 @@ def read_rundir(d):
         fk, gk, scfac = read_slater(d, S, lv, npl)
         slater.append((fk, gk))
         X.append(np.asfortranarray(sh.exchange_matrix(S.ZNL[2], lv, npl, fk, gk, scfac) * EV2HARTREE))
-    sr, si, have_so = sh.spin_orbit(S.ZNL[2], S.xi_eV)
+    # core_broaden.ipt: spin-orbit-dependent core-hole lifetimes (eV), OCEAN_multiplet.f90:613-627
+    life_time = None
+    if os.path.exists(_p(d, "core_broaden.ipt")):
+        life_time = np.array([_ffloat(t) for t in _tokens(_p(d, "core_broaden.ipt"))[:2]])
+    sr, si, have_so = sh.spin_orbit(S.ZNL[2], S.xi_eV, life_time)
     mult = MultipletInputs(lmin, lmax, nproj, S.lvl, S.lvh, c, mpm, X, sr, si, have_so)
     mels = read_mels(d, S, nptot)
     psi0, kpref = starting_vector(mels, cks, S, bpk)
     return RunInputs(S, u_raw, xoff, tau_c, xshift, rdist, rtab, ptab, energy, allow, mult, psi0,
-                     kpref, efermi, shift, mode, obf, obf2u, slater, scfac)
+                     kpref, efermi, shift, mode, obf, obf2u, slater, scfac, life_time)
 
 
 def open_rundir(d, precision=None):
@@ -495,7 +500,7 @@ def open_rundir(d, precision=None):
                       tuple(m.nproj), m.lvl, m.lvh, True, S.mult, S.inter_scale >= 0.000001, prec)  # inter_min, OCEAN_system.f90:157
     if R.slater is not None:
         # the interaction blocks and the spin-orbit matrix formed on the device
-        X, sr, si, have_so = g.mult_setup(R.slater, R.scfac, S.xi_eV)
+        X, sr, si, have_so = g.mult_setup(R.slater, R.scfac, S.xi_eV, R.life_time)
         m = dataclasses.replace(m, X=X, somelr=sr, someli=si, have_spin_orbit=have_so)
     if R.obf_mode == "obf":
         g.load_obf(R.obf, R.obf2u, R.xoff)
@@ -537,7 +542,7 @@ def _fortran_sequential(path, arrays):
 def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn=1, extra_bands=2, seed=0,
                  xi_eV=0.0, lmin=0, lmax=2, nproj=(2, 1, 1), inter_scale=1.0, eps=6.0,
                  efermi_eV=4.0, eshift_eV=0.7, tau=(0.13, 0.27, 0.41), niter=30, a_bohr=5.5,
-                 obf=None):
+                 obf=None, life_time_eV=None):
     """Write a synthetic core XAS run directory in the file formats OCEAN2 reads
     (seeded: Bloch functions orthonormal over x per k, sorted band energies, the
     screened-potential table exp(-r/3)/(4 max(r, 0.2)), random projector overlaps,
@@ -582,6 +587,8 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     _w(os.path.join(d, "metal"), ".false.\n")
     _w(os.path.join(d, "efermiinrydberg.ipt"), "%.17g\n" % (2.0 * efermi_eV * EV2HARTREE))
     _w(os.path.join(d, "eshift.ipt"), "%.17g\n" % eshift_eV)
+    if life_time_eV is not None:
+        _w(os.path.join(d, "core_broaden.ipt"), "%.17g %.17g\n" % tuple(life_time_eV))
     # Bloch functions: orthonormal over x per (k, spin), all bands in the file
     u = np.zeros((nspn, nk, bif, nx), np.complex128)
     for s in range(nspn):
@@ -651,7 +658,7 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
                 tau=np.array(tau), nb=nb, kmesh=kmesh, xmesh=xmesh, lc=lc, nspn=nspn, bif=bif, lmin=lmin,
                 lmax=lmax, nproj=list(nproj), xi_eV=xi_eV, inter_scale=inter_scale, eps=eps,
                 efermi=efermi_eV * EV2HARTREE, eshift=eshift_eV * EV2HARTREE, nptot=nptot, celvol=vol, niter=niter,
-                obf=obf_arrays)
+                obf=obf_arrays, life_time=None if life_time_eV is None else np.array(life_time_eV, float))
 
 
 __all__ = ["read_system", "read_u2", "read_bloch_control", "read_obf", "read_energies", "read_rpottrim", "read_prjfile", "read_parcksc",

@@ -250,18 +250,21 @@ def _lm_matrix(l):
     return Lx + 0j, Ly, Lz + 0j
 
 
-def spin_orbit(lc, xi_eV):
+def spin_orbit(lc, xi_eV, life_time_eV=None):
     """S(ic, jc) = -xi <core ic| L.S |core jc> where valence spins match (Hartree).
 
-    Returns (somelr, someli, have_spin_orbit); the imaginary part is zero without
-    spin-orbit-dependent lifetimes.
+    Returns (somelr, someli, have_spin_orbit).  life_time_eV (the two values of
+    core_broaden.ipt) adds i(-<L.S> l_alpha - l_beta delta) with l_alpha and l_beta
+    chosen so that the j = lc + 1/2 and lc - 1/2 levels get -life_time(1) and
+    -life_time(2) (OCEAN2/OCEAN_multiplet.f90:613-627, :664-668); without it the
+    imaginary part is zero.
     """
     xi = xi_eV * EV2HARTREE
     lab = alpha_table(lc)
     na = len(lab)
     L = _lm_matrix(lc)
     S = _lm_matrix_half()
-    sr = np.zeros((na, na), order="F")
+    ls = np.zeros((na, na))
     for ic, (s1, m1, v1) in enumerate(lab):
         for jc, (s2, m2, v2) in enumerate(lab):
             if v1 != v2:
@@ -271,9 +274,19 @@ def spin_orbit(lc, xi_eV):
             val = 0j
             for i in range(3):
                 val += L[i][m2 + lc, m1 + lc] * S[i][(s2 + 1) // 2, (s1 + 1) // 2]
-            sr[ic, jc] = -xi * val.real
+            ls[ic, jc] = val.real
+    sr = np.asfortranarray(-xi * ls)
     si = np.zeros((na, na), order="F")
-    return sr, si, xi >= 1e-12
+    if life_time_eV is None:
+        return sr, si, xi >= 1e-12
+    t1, t2 = (float(v) * EV2HARTREE for v in life_time_eV[:2])
+    l_alpha = 2.0 / (2 * lc + 1) * (t1 - t2)
+    l_beta = (lc + 1) / (2 * lc + 1) * t1 + lc / (2 * lc + 1) * t2
+    for ic in range(na):
+        for jc in range(na):
+            if lab[ic][2] == lab[jc][2]:
+                si[ic, jc] = -ls[ic, jc] * l_alpha - (l_beta if ic == jc else 0.0)
+    return sr, si, True
 
 
 def _lm_matrix_half():

@@ -351,9 +351,11 @@ def test_fortran_ospect_run_directory(tmp_path):
     _check_opcons(txt, ref)
 
 
-@pytest.mark.parametrize("lc,lvl,lvh,nproj,xi", [(0, 0, 3, (1, 1, 1, 1), 0.0), (1, 0, 2, (2, 1, 2), 6.0),
-                                                  (1, 1, 3, (1, 2, 1), 3.0)])
-def test_mult_setup_matches_oracle(lc, lvl, lvh, nproj, xi):
+@pytest.mark.parametrize("lc,lvl,lvh,nproj,xi,lt", [(0, 0, 3, (1, 1, 1, 1), 0.0, None),
+                                                     (1, 0, 2, (2, 1, 2), 6.0, None),
+                                                     (1, 1, 3, (1, 2, 1), 3.0, (0.3, 0.55)),
+                                                     (0, 0, 1, (2, 1), 0.0, (0.4, 0.1))])
+def test_mult_setup_matches_oracle(lc, lvl, lvh, nproj, xi, lt):
     """ocean_gpu_mult_setup (nbsemhsetup2 blocks and soprep on the device) against the
     oracle's quadrature restatement of nbsemhsetup2 and its jimel-based soprep."""
     from ocean_amd import core
@@ -371,11 +373,11 @@ def test_mult_setup_matches_oracle(lc, lvl, lvh, nproj, xi):
             gk[k], sg[k] = 0.5 * (m + m.T), rng.uniform(0.7, 0.9)
         slater.append(({k: fk[k] * sf[k] for k in fk}, {k: gk[k] * sg[k] for k in gk}))
         refs.append(orc.nbsemhsetup2(lc, l, npl, fk, sf, gk, sg, scfac=0.95) * orc.EV2HARTREE)
-    X, sr, si, have_so = g.mult_setup(slater, 0.95, xi)
+    X, sr, si, have_so = g.mult_setup(slater, 0.95, xi, lt)
     for got, ref in zip(X, refs):
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), np.abs(got - ref).max()
-    rsr, rsi, rhave = orc.soprep(lc, xi)
+    rsr, rsi, rhave = orc.soprep(lc, xi, lt)
     assert np.abs(sr - rsr).max() <= 1e-16 + 1e-14 * np.abs(rsr).max()
-    assert np.abs(si).max() == 0.0 and have_so == rhave
+    assert np.abs(si - rsi).max() <= 1e-16 + 1e-14 * np.abs(rsi).max() and have_so == rhave
     g.close()

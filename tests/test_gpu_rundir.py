@@ -13,6 +13,9 @@ SHAPES = {
     "k_edge": dict(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=1, nproj=(2, 1, 1)),
     "l23_spin": dict(nb=6, kmesh=(2, 1, 3), xmesh=(3, 2, 3), lc=1, nspn=2, seed=2, xi_eV=3.0, lmin=1, lmax=2,
                      nproj=(1, 2)),
+    # core_broaden.ipt: spin-orbit-dependent lifetimes make S complex (non-Hermitian H)
+    "l23_broaden": dict(nb=6, kmesh=(2, 2, 1), xmesh=(3, 3, 2), lc=1, seed=4, xi_eV=4.0, lmin=1, lmax=2,
+                        nproj=(1, 1), life_time_eV=(0.3, 0.55)),
 }
 
 
@@ -31,7 +34,7 @@ def oracle_problem(gen, R):
         gk = {k: gen["slater"][("gk", lv, k)][0] for k in range(abs(gen["lc"] - lv), gen["lc"] + lv + 1, 2)}
         sg = {k: gen["slater"][("gk", lv, k)][1] for k in gk}
         X.append(orc.nbsemhsetup2(gen["lc"], lv, npl, fk, sf, gk, sg) * EV2H)
-    sr, si, have_so = orc.soprep(gen["lc"], gen["xi_eV"])
+    sr, si, have_so = orc.soprep(gen["lc"], gen["xi_eV"], gen.get("life_time"))
     M = orc.Multiplet(m.lmin, m.lmax, list(m.nproj), m.lvl, m.lvh, m.c, m.mpm, X, sr, si, have_so)
     P = orc.Problem(S.nb, S.kmesh, S.xmesh, gen["lc"], S.nspn, S.celvol, S.inter_scale, u, W, energy, allow, M)
     psi = orc.psi_dotter(gen["mels"], gen["cks"], gen["lc"], gen["nb"], S.nk, gen["nspn"],

@@ -218,6 +218,18 @@ def test_xact_obf_drops_multiplet_and_is_hermitian():
     assert np.abs(H - np.conj(H.T)).max() < 1e-12 * np.abs(H).max()
 
 
+@pytest.mark.parametrize("lc", [0, 1])
+def test_soprep_lifetimes_closed_form(lc):
+    """someli = -<L.S> l_alpha - l_beta: the j = lc + 1/2 states (2lc + 2 per valence
+    spin) get -life_time(1), the j = lc - 1/2 states (2lc) get -life_time(2)."""
+    lt = (0.3, 0.55)
+    _, si, have = orc.soprep(lc, 0.0, lt)
+    assert have
+    ev = np.sort(np.linalg.eigvalsh(si)) / orc.EV2HARTREE
+    want = np.sort([-lt[1]] * (2 * (2 * lc)) + [-lt[0]] * (2 * (2 * lc + 2)))
+    assert np.abs(ev - want).max() < 1e-13
+
+
 def test_fg_combo_equals_dense_operator():
     """H_fg = conj(C) (mul K) C^T with K = -mpm (direct) + X^T block (exchange)."""
     inp = _tiny(lc=1, seed=4)

@@ -103,10 +103,10 @@ def test_w_table_and_geometry(case):
 def test_unsupported_inputs_are_refused(tmp_path):
     d = str(tmp_path / "run")
     write_rundir(d, seed=3)
-    open(os.path.join(d, "core_broaden.ipt"), "w").write("0.1\n")
+    open(os.path.join(d, "tdlda"), "w").write("\n")
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
-    os.remove(os.path.join(d, "core_broaden.ipt"))
+    os.remove(os.path.join(d, "tdlda"))
     open(os.path.join(d, "doping"), "w").write("0.1\n")
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
@@ -172,3 +172,15 @@ def test_obf_needs_one_spin(tmp_path):
     write_rundir(d, obf=dict(num=4), **dict(SHAPES["l23_spin"]))
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
+
+
+def test_core_broaden_lifetimes(tmp_path):
+    """core_broaden.ipt: the spin-orbit matrix gains the lifetime term, equal to the
+    oracle's soprep; the j = lc +- 1/2 levels get -life_time(1) and -life_time(2)."""
+    d = str(tmp_path / "broaden")
+    gen = write_rundir(d, life_time_eV=(0.3, 0.55), **SHAPES["l23_spin"])
+    R = rundir.read_rundir(d)
+    assert np.array_equal(R.life_time, gen["life_time"])
+    sr, si, have = orc.soprep(gen["lc"], gen["xi_eV"], gen["life_time"])
+    assert have and R.mult.have_spin_orbit
+    assert np.abs(R.mult.somelr - sr).max() == 0.0 and np.abs(R.mult.someli - si).max() == 0.0
