@@ -137,13 +137,15 @@ int  ocean_gpu_build_w(ocean_gpu_handle* h, const double* rdist, const double* r
  * content of core_broaden.ipt, :613-627) adds the spin-orbit-dependent lifetimes, NULL
  * when the file is absent; have_so = 0 when xi < 1e-12 Hartree and no lifetimes (:612-615).
  *   slater  per l = lvl..lvh with nproj(l) > 0: the F^k matrices, k = 2, 4, ..,
- *           min(2 lc, 2 l), then the G^k, k = |lc - l|, .., lc + l step 2; each
- *           nproj(l) x nproj(l) column-major and multiplied by its file's scale
- *           factor (fk/gk files, :2064-2114)
- *   scfac   the overall scale (scfac file, default 1); xi_ev from bse.in (eV). */
-int  ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, double xi_ev,
-                          const double* life_time_ev, double* mhr, double* mhi, double* somelr,
-                          double* someli, int32_t* have_so);
+ *           min(2 lc, 2 l) (with tdlda: the one kk matrix of k = 0, :2082-2094), then
+ *           the G^k, k = |lc - l|, .., lc + l step 2; each nproj(l) x nproj(l)
+ *           column-major and multiplied by its file's scale factor (fk/kk/gk files,
+ *           :2064-2114)
+ *   scfac   the overall scale (scfac file, default 1); tdlda the tdlda file's switch;
+ *           xi_ev from bse.in (eV). */
+int  ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, int32_t tdlda,
+                          double xi_ev, const double* life_time_ev, double* mhr, double* mhi,
+                          double* somelr, double* someli, int32_t* have_so);
 int  ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* mpci,
                          const double* mpm, const double* mhr, const double* mhi,
                          const double* somelr, const double* someli, int32_t have_so);

@@ -63,7 +63,8 @@ _SIGS = {
     "ocean_gpu_build_w": (C.c_int, [_H, _DP, _DP, _DP, C.c_int32, C.c_double, C.c_double]),
     "ocean_gpu_load_mult": (C.c_int, [_H, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int32]),
     "ocean_gpu_load_energies": (C.c_int, [_H, _DP, _DP, _DP]),
-    "ocean_gpu_mult_setup": (C.c_int, [_H, _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP, _DP, _IP]),
+    "ocean_gpu_mult_setup": (C.c_int, [_H, _DP, C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP, _DP, _DP,
+                                       _IP]),
     "ocean_gpu_xact": (C.c_int, [_H, _DP, _DP, _DP, _DP, C.c_int32, _IP]),
     "ocean_gpu_vec_new": (C.c_int, [_H, _IP]),
     "ocean_gpu_vec_free": (C.c_int, [_H, C.c_int32]),
@@ -334,12 +335,13 @@ class OceanGPU:
         _chk(_lib().ocean_gpu_build_w(self.h, _p(rd), _p(rt), _p(pt), len(rt), epsilon0, scale),
              "build_w")
 
-    def mult_setup(self, slater, scfac, xi_eV, life_time_eV=None):
+    def mult_setup(self, slater, scfac, xi_eV, life_time_eV=None, tdlda=False):
         """Interaction blocks on the device (ocean_gpu_mult_setup).  slater: per valence
         l = lvl..lvh a pair (fk, gk) of dicts k -> (nproj, nproj) matrices already
         multiplied by their scale factors; life_time_eV the two values of
-        core_broaden.ipt or None.  Returns (X list of (mham, mham) complex blocks in
-        Hartree, somelr, someli, have_so)."""
+        core_broaden.ipt or None; tdlda: fk holds the k = 0 kk matrix instead of the
+        F^k.  Returns (X list of (mham, mham) complex blocks in Hartree, somelr,
+        someli, have_so)."""
         d = self.desc
         lc = d.lc
         flat, sizes = [], []
@@ -347,7 +349,8 @@ class OceanGPU:
             npl = d.nproj[l - d.lmin] if d.lmin <= l <= d.lmax else 0
             fk, gk = slater[i]
             if npl:
-                ks = [("f", k) for k in range(2, min(2 * lc, 2 * l) + 1, 2)] + \
+                kf = [0] if tdlda else list(range(2, min(2 * lc, 2 * l) + 1, 2))
+                ks = [("f", k) for k in kf] + \
                      [("g", k) for k in range(abs(lc - l), lc + l + 1, 2)]
                 for which, k in ks:
                     m = (fk if which == "f" else gk)[k]
@@ -360,7 +363,8 @@ class OceanGPU:
         sr, si = np.zeros((na, na), order="F"), np.zeros((na, na), order="F")
         hso = C.c_int32()
         lt = None if life_time_eV is None else np.ascontiguousarray(life_time_eV, np.float64)[:2]
-        _chk(_lib().ocean_gpu_mult_setup(self.h, _p(sl), scfac, xi_eV, None if lt is None else _p(lt), _p(mhr),
+        _chk(_lib().ocean_gpu_mult_setup(self.h, _p(sl), scfac, int(bool(tdlda)), xi_eV,
+                                         None if lt is None else _p(lt), _p(mhr),
                                          _p(mhi), _p(sr), _p(si), C.byref(hso)), "mult_setup")
         X, j = [], 0
         for m in sizes:

@@ -962,7 +962,7 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
   return OCEAN_GPU_OK;
 }
 
-int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, double xi_ev,
+int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac, int32_t tdlda, double xi_ev,
                          const double* life_time_ev, double* mhr, double* mhi, double* somelr, double* someli,
                          int32_t* have_so) {
   if (!h || !slater || !mhr || !mhi || !somelr || !someli || !have_so) return OCEAN_GPU_ERR_ARG;
@@ -975,7 +975,7 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
   for (int l = d.lvl; l <= d.lvh; ++l) {
     const int npl = (l >= d.lmin && l <= d.lmax) ? d.nproj[l - d.lmin] : 0;
     const int kfh = std::min(2 * lc, 2 * l);
-    const int nF = kfh >= 2 ? kfh / 2 : 0, nG = (lc + l - std::abs(lc - l)) / 2 + 1;
+    const int nF = tdlda ? 1 : (kfh >= 2 ? kfh / 2 : 0), nG = (lc + l - std::abs(lc - l)) / 2 + 1;
     const size_t mham = (size_t)4 * (2 * lc + 1) * (2 * l + 1) * npl;
     npl_l.push_back(npl);
     nsl_l.push_back((nF + nG) * npl * npl);
@@ -997,6 +997,7 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
     const int l = d.lvl + (int)i;
     const size_t mham = (size_t)4 * (2 * lc + 1) * (2 * l + 1) * npl;
     og::k_mham<<<grid_for(mham * mham), kBlock, 0, h->stream>>>(lc, l, npl, dsl + sb_l[i], scfac, kEV2Hartree,
+                                                               tdlda ? 1 : 0,
                                                                dmh + jb_l[i], dmh + jtot + jb_l[i]);
     if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }

@@ -62,12 +62,13 @@ __device__ __forceinline__ void alpha_labels_d(int a, int lc, int& cs, int& cm, 
 //   direct  -F^k gaunt(lc, cm2, k, q, lc, cm1) gaunt*(lv, vm1, k, q, lv, vm2) 4pi/(2k+1)
 //   exchange G^k gaunt(lc, cm2, k, q, lv, vm2) gaunt*(lv, vm1, k, q, lc, cm1) 4pi/(2k+1)
 // (:2083-2160), all times scfac; the finished element is multiplied by post (the
-// module's eV -> Hartree, OCEAN_multiplet.f90:388-389).
+// module's eV -> Hartree, OCEAN_multiplet.f90:388-389).  tdlda: the direct-like sum
+// runs over k = 0 only, with the kk-file matrix in the F slot (:2082-2094).
 __global__ void k_mham(int lc, int lv, int npl, const double* __restrict__ slater, double scfac, double post,
-                       double* __restrict__ mhr, double* __restrict__ mhi) {
+                       int tdlda, double* __restrict__ mhr, double* __restrict__ mhi) {
   const int nm = 2 * lv + 1;
   const int mham = 4 * (2 * lc + 1) * nm * npl;
-  const int nF = min(2 * lc, 2 * lv) >= 2 ? min(2 * lc, 2 * lv) / 2 : 0;
+  const int nF = tdlda ? 1 : (min(2 * lc, 2 * lv) >= 2 ? min(2 * lc, 2 * lv) / 2 : 0);
   const double fourpi = 12.566370614359172953850573533118;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < mham * mham; e += gridDim.x * blockDim.x) {
     const int i1 = e % mham, i2 = e / mham;
@@ -80,7 +81,7 @@ __global__ void k_mham(int lc, int lv, int npl, const double* __restrict__ slate
     if (cs2 == cs1 && vs1 == vs2 && cm2 + vm1 == cm1 + vm2) {
       const int q = cm2 - cm1;
       for (int j = 0; j < nF; ++j) {
-        const int k = 2 * (j + 1);
+        const int k = tdlda ? 0 : 2 * (j + 1);
         if (abs(q) > k) continue;
         const double f = slater[(size_t)j * npl * npl + nu1 + (size_t)nu2 * npl];
         acc -= f * scfac * gaunt_d(lc, cm2, k, q, lc, cm1, false) * gaunt_d(lv, vm1, k, q, lv, vm2, true) *

@@ -98,12 +98,13 @@ module ocean_gpu_iface
 
     ! nbsemhsetup2 blocks and soprep formed on the device (see ocean_gpu.h)
     ! life_time: c_loc of the two core_broaden.ipt values, or c_null_ptr
-    integer(c_int) function ocean_gpu_mult_setup(h, slater, scfac, xi_ev, life_time, mhr, mhi, somelr, &
-        someli, have_so) bind(C, name='ocean_gpu_mult_setup')
+    integer(c_int) function ocean_gpu_mult_setup(h, slater, scfac, tdlda, xi_ev, life_time, mhr, mhi, &
+        somelr, someli, have_so) bind(C, name='ocean_gpu_mult_setup')
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: slater(*)
       real(c_double), value :: scfac, xi_ev
+      integer(c_int32_t), value :: tdlda
       type(c_ptr), value :: life_time
       real(c_double), intent(out) :: mhr(*), mhi(*), somelr(*), someli(*)
       integer(c_int32_t), intent(out) :: have_so

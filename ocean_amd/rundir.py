@@ -13,9 +13,8 @@ formats (for this reader's tests, and as input for OCEAN2's own CPU build).
 
 Not read (a run that needs them raises ``NotImplementedError``): XES / valence /
 RIXS runs, GW corrections, doping and ``findNewFermi.ipt``, occupation types other
-than the default, ``rpa``, ``tdlda``, ``isolated.inp``,
-``pbc.inp``, ``epsilon3D``, ``xshift_override.ipt``, ``file_brange.ipt`` and the
-legacy (sequential) cks files.
+than the default, ``isolated.inp``, ``pbc.inp``, ``epsilon3D``, ``file_brange.ipt``
+and the legacy (sequential) cks files.
 """
 from __future__ import annotations
 
@@ -109,6 +108,9 @@ class RunSystem:
     lvh: int
     xi_eV: float
     solver: dict = field(default_factory=dict)
+    rpa: bool = False             # rpa file: no long-range term, no direct multiplet term
+    tdlda: bool = False           # tdlda file: F^0 (kk files) in place of the F^k
+    xshift_override: tuple = (1, 1, 1)
 
     @property
     def nk(self):
@@ -131,8 +133,7 @@ def read_system(d):
     """Meshes, spins, ZNL, bands, lattice, first core run, site, mode, epsilon,
     bse.in (OCEAN2/OCEAN_system.f90:166-231, :310-335, :644-726;
     OCEAN_multiplet.f90:298-303; OCEAN_haydock.f90:2109-2116)."""
-    _refuse(d, ["force_val_ham_spin.ipt", "file_brange.ipt", "rpa", "tdlda",
-                "isolated.inp", "pbc.inp", "epsilon3D", "xshift_override.ipt"])
+    _refuse(d, ["force_val_ham_spin.ipt", "file_brange.ipt", "isolated.inp", "pbc.inp", "epsilon3D"])
     xmesh = tuple(int(t) for t in _tokens(_p(d, "xmesh.ipt"))[:3])
     kmesh = tuple(int(t) for t in _tokens(_p(d, "kmesh.ipt"))[:3])
     nspn = int(_tokens(_p(d, "nspin"))[0])
@@ -194,8 +195,17 @@ def read_system(d):
         v = bl[3].split()
         solver.update(niter=int(v[0]), ne=int(v[1]), el=_ffloat(v[2]), eh=_ffloat(v[3]),
                       gam=_ffloat(v[4]), ebase=_ffloat(v[5]) if len(v) > 5 else 0.0)
-    return RunSystem(xmesh, kmesh, nspn, ZNL, nb, brange, avec, celvol, run, tau, inter_scale,
-                     epsilon0, mult, lvl, lvh, xi, solver)
+    S = RunSystem(xmesh, kmesh, nspn, ZNL, nb, brange, avec, celvol, run, tau, inter_scale,
+                  epsilon0, mult, lvl, lvh, xi, solver)
+    # rpa: long_range off (OCEAN2/OCEAN_system.f90:338-341) and no cmel (OCEAN_multiplet.f90:488-489)
+    S.rpa = os.path.exists(_p(d, "rpa"))
+    # tdlda: a logical in the file (OCEAN2/OCEAN_multiplet.f90:2065-2070)
+    S.tdlda = os.path.exists(_p(d, "tdlda")) and _fbool(_tokens(_p(d, "tdlda"))[0])
+    # xshift_override.ipt: three 0/1 factors on the x shift, else all 1 (OCEAN2/OCEAN_bloch.f90:279-292)
+    if os.path.exists(_p(d, "xshift_override.ipt")):
+        xo = tuple(int(t) for t in _tokens(_p(d, "xshift_override.ipt"))[:3])
+        S.xshift_override = xo if (max(xo) <= 1 and min(xo) >= 0) else (1, 1, 1)
+    return S
 
 
 def read_u2(d, S: RunSystem):
@@ -382,7 +392,10 @@ def read_slater(d, S: RunSystem, lv, npl):
         v = np.array([_ffloat(x) for x in t[:npl * npl + 1]])
         return v[:npl * npl].reshape(npl, npl, order="F") * v[npl * npl]
 
-    fk = {k: one("fk", k) for k in range(2, min(2 * lc, 2 * lv) + 1, 2)}
+    if S.tdlda:
+        fk = {0: one("kk", 0)}  # OCEAN2/OCEAN_multiplet.f90:2082-2094
+    else:
+        fk = {k: one("fk", k) for k in range(2, min(2 * lc, 2 * lv) + 1, 2)}
     gk = {k: one("gk", k) for k in range(abs(lc - lv), lc + lv + 1, 2)}
     return fk, gk, scfac
 
@@ -459,7 +472,9 @@ def read_rundir(d):
         # con.u2.dat, or the Bloch functions OCEAN_obf_lrLOAD builds from the OBFs;
         # both are centred and phased as OCEAN_bloch_lrLOAD does
         u_raw = read_u2(d, S) if mode == "bloch" else sh.obf_fake_bloch(obf, obf2u)
-        xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau)
+        # OCEAN_obf_lrLOAD centres without the override (OCEAN2/OCEAN_obf.f90:165-186)
+        xo = S.xshift_override if mode == "bloch" else (1, 1, 1)
+        xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau, xo)
     rdist, _ = sh.w_geometry(S.avec, S.kmesh, S.xmesh, tau_c, xshift)
     rtab, ptab = read_rpottrim(d, S)
     energy, allow, efermi, shift = read_energies(d, S)
@@ -467,7 +482,8 @@ def read_rundir(d):
     nptot = sum((2 * l + 1) * nproj[i] for i, l in enumerate(range(lmin, lmax + 1)))
     cks, bpk = read_parcksc(d, S, nptot)
     c = projector_array(cks, S, lmin, lmax, nproj, bpk)
-    mpm = read_cmel(d, lmin, lmax, nproj)
+    mpm = read_cmel(d, lmin, lmax, nproj) if not S.rpa else \
+        np.zeros((max(nproj), max(nproj), lmax - lmin + 1), order="F")
     X = []
     slater = []
     scfac = 1.0
@@ -475,7 +491,7 @@ def read_rundir(d):
         npl = nproj[lv - lmin]
         fk, gk, scfac = read_slater(d, S, lv, npl)
         slater.append((fk, gk))
-        X.append(np.asfortranarray(sh.exchange_matrix(S.ZNL[2], lv, npl, fk, gk, scfac) * EV2HARTREE))
+        X.append(np.asfortranarray(sh.exchange_matrix(S.ZNL[2], lv, npl, fk, gk, scfac, S.tdlda) * EV2HARTREE))
     # core_broaden.ipt: spin-orbit-dependent core-hole lifetimes (eV), OCEAN_multiplet.f90:613-627
     life_time = None
     if os.path.exists(_p(d, "core_broaden.ipt")):
@@ -496,11 +512,12 @@ def open_rundir(d, precision=None):
     S = R.system
     m = R.mult
     prec = core.FP64 if precision is None else precision
+    long_range = S.inter_scale >= 0.000001 and not S.rpa  # inter_min, OCEAN_system.f90:157; rpa :338-341
     g = core.OceanGPU(S.nb, S.kmesh, S.xmesh, S.ZNL[2], S.nspn, S.celvol, S.inter_scale, m.lmin, m.lmax,
-                      tuple(m.nproj), m.lvl, m.lvh, True, S.mult, S.inter_scale >= 0.000001, prec)  # inter_min, OCEAN_system.f90:157
+                      tuple(m.nproj), m.lvl, m.lvh, True, S.mult, long_range, prec)
     if R.slater is not None:
         # the interaction blocks and the spin-orbit matrix formed on the device
-        X, sr, si, have_so = g.mult_setup(R.slater, R.scfac, S.xi_eV, R.life_time)
+        X, sr, si, have_so = g.mult_setup(R.slater, R.scfac, S.xi_eV, R.life_time, S.tdlda)
         m = dataclasses.replace(m, X=X, somelr=sr, someli=si, have_spin_orbit=have_so)
     if R.obf_mode == "obf":
         g.load_obf(R.obf, R.obf2u, R.xoff)
@@ -542,7 +559,7 @@ def _fortran_sequential(path, arrays):
 def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn=1, extra_bands=2, seed=0,
                  xi_eV=0.0, lmin=0, lmax=2, nproj=(2, 1, 1), inter_scale=1.0, eps=6.0,
                  efermi_eV=4.0, eshift_eV=0.7, tau=(0.13, 0.27, 0.41), niter=30, a_bohr=5.5,
-                 obf=None, life_time_eV=None):
+                 obf=None, life_time_eV=None, rpa=False, tdlda=False, xshift_override=None):
     """Write a synthetic core XAS run directory in the file formats OCEAN2 reads
     (seeded: Bloch functions orthonormal over x per k, sorted band energies, the
     screened-potential table exp(-r/3)/(4 max(r, 0.2)), random projector overlaps,
@@ -589,6 +606,12 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     _w(os.path.join(d, "eshift.ipt"), "%.17g\n" % eshift_eV)
     if life_time_eV is not None:
         _w(os.path.join(d, "core_broaden.ipt"), "%.17g %.17g\n" % tuple(life_time_eV))
+    if rpa:
+        _w(os.path.join(d, "rpa"), "\n")
+    if tdlda:
+        _w(os.path.join(d, "tdlda"), ".true.\n")
+    if xshift_override is not None:
+        _w(os.path.join(d, "xshift_override.ipt"), "%d %d %d\n" % tuple(xshift_override))
     # Bloch functions: orthonormal over x per (k, spin), all bands in the file
     u = np.zeros((nspn, nk, bif, nx), np.complex128)
     for s in range(nspn):
@@ -636,7 +659,8 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     slater = {}
     for li, lv in enumerate(range(lvl, lvh + 1)):
         npl = nproj[lv - lmin]
-        for pre, ks in (("fk", range(2, min(2 * lc, 2 * lv) + 1, 2)), ("gk", range(abs(lc - lv), lc + lv + 1, 2))):
+        fpre, fks = ("kk", [0]) if tdlda else ("fk", range(2, min(2 * lc, 2 * lv) + 1, 2))
+        for pre, ks in ((fpre, fks), ("gk", range(abs(lc - lv), lc + lv + 1, 2))):
             for k in ks:
                 m = rng.uniform(0.5, 2.0, (npl, npl))
                 m = 0.5 * (m + m.T)
@@ -658,7 +682,8 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
                 tau=np.array(tau), nb=nb, kmesh=kmesh, xmesh=xmesh, lc=lc, nspn=nspn, bif=bif, lmin=lmin,
                 lmax=lmax, nproj=list(nproj), xi_eV=xi_eV, inter_scale=inter_scale, eps=eps,
                 efermi=efermi_eV * EV2HARTREE, eshift=eshift_eV * EV2HARTREE, nptot=nptot, celvol=vol, niter=niter,
-                obf=obf_arrays, life_time=None if life_time_eV is None else np.array(life_time_eV, float))
+                obf=obf_arrays, life_time=None if life_time_eV is None else np.array(life_time_eV, float),
+                rpa=rpa, tdlda=tdlda, xshift_override=xshift_override)
 
 
 __all__ = ["read_system", "read_u2", "read_bloch_control", "read_obf", "read_energies", "read_rpottrim", "read_prjfile", "read_parcksc",

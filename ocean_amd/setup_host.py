@@ -203,17 +203,18 @@ def gaunt(l1, m1, k, q, l3, m3, conj_mid):
         * three_j(l1, k, l3, 0, 0, 0) * three_j(l1, k, l3, -m1, q, m3)
 
 
-def exchange_matrix(lc, lv, nproj, fk, gk, scfac=1.0):
+def exchange_matrix(lc, lv, nproj, fk, gk, scfac=1.0, tdlda=False):
     """Multiplet block mh_l (eV) for valence l, element [i1, i2] (nbsemhsetup2 layout).
 
     fk[k], gk[k]: (nproj, nproj) Slater integrals already multiplied by their scale
     factors.  Index order inside the block: nu fastest, then valence m, then alpha.
+    tdlda: the direct-like sum runs over k = 0 only (fk[0] from the kk file).
     """
     labels = [(s, m, v, vm, nu) for (s, m, v) in alpha_table(lc)
               for vm in range(-lv, lv + 1) for nu in range(nproj)]
     n = len(labels)
     mh = np.zeros((n, n), np.complex128, order="F")
-    kf = list(range(2, min(2 * lc, 2 * lv) + 1, 2))
+    kf = [0] if tdlda else list(range(2, min(2 * lc, 2 * lv) + 1, 2))
     kg = list(range(abs(lc - lv), lc + lv + 1, 2))
     for i1, (cs1, cm1, vs1, vm1, nu1) in enumerate(labels):
         for i2, (cs2, cm2, vs2, vm2, nu2) in enumerate(labels):

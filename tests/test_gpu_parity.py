@@ -354,26 +354,29 @@ def test_fortran_ospect_run_directory(tmp_path):
 @pytest.mark.parametrize("lc,lvl,lvh,nproj,xi,lt", [(0, 0, 3, (1, 1, 1, 1), 0.0, None),
                                                      (1, 0, 2, (2, 1, 2), 6.0, None),
                                                      (1, 1, 3, (1, 2, 1), 3.0, (0.3, 0.55)),
-                                                     (0, 0, 1, (2, 1), 0.0, (0.4, 0.1))])
+                                                     (0, 0, 1, (2, 1), 0.0, (0.4, 0.1)),
+                                                     (1, 0, 2, (1, 2, 1), 2.0, "tdlda")])
 def test_mult_setup_matches_oracle(lc, lvl, lvh, nproj, xi, lt):
     """ocean_gpu_mult_setup (nbsemhsetup2 blocks and soprep on the device) against the
     oracle's quadrature restatement of nbsemhsetup2 and its jimel-based soprep."""
     from ocean_amd import core
     g = core.OceanGPU(4, (2, 2, 2), (2, 2, 2), lc, 1, 100.0, 1.0, lvl, lvh, nproj, lvl, lvh)
+    tdlda = lt == "tdlda"
+    lt = None if tdlda else lt
     rng = np.random.default_rng(31 + lc + lvl)
     slater, refs = [], []
     for l in range(lvl, lvh + 1):
         npl = nproj[l - lvl]
         fk, gk, sf, sg = {}, {}, {}, {}
-        for k in range(2, min(2 * lc, 2 * l) + 1, 2):
+        for k in ([0] if tdlda else range(2, min(2 * lc, 2 * l) + 1, 2)):
             m = rng.uniform(0.5, 2.0, (npl, npl))
             fk[k], sf[k] = 0.5 * (m + m.T), rng.uniform(0.7, 0.9)
         for k in range(abs(lc - l), lc + l + 1, 2):
             m = rng.uniform(0.5, 2.0, (npl, npl))
             gk[k], sg[k] = 0.5 * (m + m.T), rng.uniform(0.7, 0.9)
         slater.append(({k: fk[k] * sf[k] for k in fk}, {k: gk[k] * sg[k] for k in gk}))
-        refs.append(orc.nbsemhsetup2(lc, l, npl, fk, sf, gk, sg, scfac=0.95) * orc.EV2HARTREE)
-    X, sr, si, have_so = g.mult_setup(slater, 0.95, xi, lt)
+        refs.append(orc.nbsemhsetup2(lc, l, npl, fk, sf, gk, sg, scfac=0.95, tdlda=tdlda) * orc.EV2HARTREE)
+    X, sr, si, have_so = g.mult_setup(slater, 0.95, xi, lt, tdlda)
     for got, ref in zip(X, refs):
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), np.abs(got - ref).max()

@@ -16,12 +16,18 @@ SHAPES = {
     # core_broaden.ipt: spin-orbit-dependent lifetimes make S complex (non-Hermitian H)
     "l23_broaden": dict(nb=6, kmesh=(2, 2, 1), xmesh=(3, 3, 2), lc=1, seed=4, xi_eV=4.0, lmin=1, lmax=2,
                         nproj=(1, 1), life_time_eV=(0.3, 0.55)),
+    # tdlda (F^0 from the kk files) with a partial x-shift override
+    "l23_tdlda": dict(nb=6, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=1, seed=5, xi_eV=2.0, lmin=0, lmax=2,
+                      nproj=(1, 1, 1), tdlda=True, xshift_override=(0, 1, 1)),
+    # rpa: no long-range term and no direct multiplet term
+    "k_rpa": dict(nb=6, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=6, nproj=(1, 1, 1), rpa=True),
 }
 
 
 def oracle_problem(gen, R):
     S = R.system
-    u, tau_c, xshift = orc.bloch_phase(R.u_raw, S.xmesh, S.kmesh, S.tau)
+    xo = gen.get("xshift_override") or (1, 1, 1)
+    u, tau_c, xshift = orc.bloch_phase(R.u_raw, S.xmesh, S.kmesh, S.tau, xo)
     W = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, R.rtab, R.ptab, S.epsilon0, S.inter_scale)
     energy, allow = orc.energies_core_xas(gen["E"][:gen["nb"]], gen["efermi"], gen["eshift"], gen["lc"],
                                           gen["nspn"])
@@ -29,14 +35,16 @@ def oracle_problem(gen, R):
     X = []
     for lv in range(m.lvl, m.lvh + 1):
         npl = m.nproj[lv - m.lmin]
-        fk = {k: gen["slater"][("fk", lv, k)][0] for k in range(2, min(2 * gen["lc"], 2 * lv) + 1, 2)}
-        sf = {k: gen["slater"][("fk", lv, k)][1] for k in fk}
+        fpre, fks = ("kk", [0]) if gen.get("tdlda") else ("fk", range(2, min(2 * gen["lc"], 2 * lv) + 1, 2))
+        fk = {k: gen["slater"][(fpre, lv, k)][0] for k in fks}
+        sf = {k: gen["slater"][(fpre, lv, k)][1] for k in fk}
         gk = {k: gen["slater"][("gk", lv, k)][0] for k in range(abs(gen["lc"] - lv), gen["lc"] + lv + 1, 2)}
         sg = {k: gen["slater"][("gk", lv, k)][1] for k in gk}
-        X.append(orc.nbsemhsetup2(gen["lc"], lv, npl, fk, sf, gk, sg) * EV2H)
+        X.append(orc.nbsemhsetup2(gen["lc"], lv, npl, fk, sf, gk, sg, tdlda=bool(gen.get("tdlda"))) * EV2H)
     sr, si, have_so = orc.soprep(gen["lc"], gen["xi_eV"], gen.get("life_time"))
     M = orc.Multiplet(m.lmin, m.lmax, list(m.nproj), m.lvl, m.lvh, m.c, m.mpm, X, sr, si, have_so)
-    P = orc.Problem(S.nb, S.kmesh, S.xmesh, gen["lc"], S.nspn, S.celvol, S.inter_scale, u, W, energy, allow, M)
+    P = orc.Problem(S.nb, S.kmesh, S.xmesh, gen["lc"], S.nspn, S.celvol, S.inter_scale, u, W, energy, allow, M,
+                    long_range=not gen.get("rpa"))
     psi = orc.psi_dotter(gen["mels"], gen["cks"], gen["lc"], gen["nb"], S.nk, gen["nspn"],
                          bands_in_file=gen["bif"])
     psi0, kpref = orc.core_pnorm(psi, S.nk, S.celvol)

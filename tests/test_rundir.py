@@ -103,10 +103,10 @@ def test_w_table_and_geometry(case):
 def test_unsupported_inputs_are_refused(tmp_path):
     d = str(tmp_path / "run")
     write_rundir(d, seed=3)
-    open(os.path.join(d, "tdlda"), "w").write("\n")
+    open(os.path.join(d, "isolated.inp"), "w").write("\n")
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
-    os.remove(os.path.join(d, "tdlda"))
+    os.remove(os.path.join(d, "isolated.inp"))
     open(os.path.join(d, "doping"), "w").write("0.1\n")
     with pytest.raises(NotImplementedError):
         rundir.read_rundir(d)
@@ -184,3 +184,28 @@ def test_core_broaden_lifetimes(tmp_path):
     sr, si, have = orc.soprep(gen["lc"], gen["xi_eV"], gen["life_time"])
     assert have and R.mult.have_spin_orbit
     assert np.abs(R.mult.somelr - sr).max() == 0.0 and np.abs(R.mult.someli - si).max() == 0.0
+
+
+def test_rpa_tdlda_and_xshift_override(tmp_path):
+    """rpa: no cmel (mpm zero) and no long-range term; tdlda: the blocks from the kk
+    matrices (F^0) against the oracle's quadrature nbsemhsetup2; xshift_override.ipt
+    factors on the x shift (an invalid value falls back to 1 1 1)."""
+    d = str(tmp_path / "opts")
+    gen = write_rundir(d, lc=1, xi_eV=2.0, lmin=0, lmax=2, nproj=(1, 1, 1), seed=5, tdlda=True, rpa=True,
+                       xshift_override=(0, 1, 1), tau=(0.6, 0.27, 0.41))
+    R = rundir.read_rundir(d)
+    S = R.system
+    assert S.rpa and S.tdlda and S.xshift_override == (0, 1, 1)
+    assert np.abs(R.mult.mpm).max() == 0.0
+    xo, tc, xs = sh.x_offsets(S.xmesh, S.kmesh, S.tau, (0, 1, 1))
+    assert np.array_equal(R.xoff, xo) and np.array_equal(R.xshift, xs) and xs[0] == 0
+    for i, lv in enumerate(range(S.lvl, S.lvh + 1)):
+        fk = {0: gen["slater"][("kk", lv, 0)][0]}
+        sf = {0: gen["slater"][("kk", lv, 0)][1]}
+        ks = range(abs(1 - lv), 1 + lv + 1, 2)
+        gk = {k: gen["slater"][("gk", lv, k)][0] for k in ks}
+        sg = {k: gen["slater"][("gk", lv, k)][1] for k in ks}
+        ref = orc.nbsemhsetup2(1, lv, 1, fk, sf, gk, sg, tdlda=True) * EV2H
+        assert np.abs(R.mult.X[i] - ref).max() <= 1e-14 * np.abs(ref).max()
+    open(os.path.join(d, "xshift_override.ipt"), "w").write("2 1 1\n")
+    assert rundir.read_system(d).xshift_override == (1, 1, 1)
