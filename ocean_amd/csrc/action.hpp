@@ -48,6 +48,7 @@ struct ActArgs {
   int K1, K2, K3;
   int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
   const T2* phs;              // OBF form: supercell phase [x][k] applied around the transform
+  const double2* lrbuf;       // [a][k][n]: sum_x conj(u) phi' from k_adj_lr (else the adjoint streams u)
 };
 
 // Problem p of a batched launch owns workgroups [off[p], off[p+1]).
@@ -850,7 +851,8 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   const int t = threadIdx.x, bd = blockDim.x;
   const int n = t % nb, xs = t / nb;
   const bool active = xs < XS;
-  const bool lr = (flags & F_LR) != 0, fg = (flags & F_FG) != 0;
+  // with lrbuf the long-range sums come from the matrix-core kernel k_adj_lr
+  const bool lr = (flags & F_LR) != 0 && A.lrbuf == nullptr, fg = (flags & F_FG) != 0;
   const int nel = na * nb;
 
   // epilogue operands of the first kEP elements owned by this thread
@@ -936,6 +938,15 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     double2 part[NA];
 #pragma unroll
     for (int j = 0; j < NA; ++j) part[j] = make_double2(-(double)acc[j].x, -(double)acc[j].y);
+    if (A.lrbuf != nullptr && (flags & F_LR) && xs == 0) {
+#pragma unroll
+      for (int j = 0; j < NA; ++j)
+        if (j < acnt) {
+          const double2 v = A.lrbuf[((size_t)(s + j * astep) * nk + k) * nb + n];
+          part[j].x -= v.x;
+          part[j].y -= v.y;
+        }
+    }
     if (fg && active) {
 #pragma unroll
       for (int i = 0; i < kCP; ++i) {

@@ -5,6 +5,7 @@
 
 #include "action.hpp"
 #include "common.hpp"
+#include "gemm64.hpp"
 #include "obf.hpp"
 #include "setup.hpp"
 #include "vec.hpp"
@@ -267,6 +268,16 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     OG_CHECK(hipGetLastError());
   }
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
+  // many alphas in FP64: the long-range sums on the matrix cores, the rest of the
+  // adjoint (multiplet back-projection, epilogue) in k_adj
+  if constexpr (std::is_same<T2, double2>::value && NA >= 6) {
+    if ((flags & og::F_LR) && h->lrbuf && h->ns == 1 && h->na == NA) {
+      a.lrbuf = h->lrbuf;
+      og::k_adj_lr<NA><<<dim3((h->nb + 63) / 64, h->nk), 256, 0, h->stream>>>(
+          uT, h->ldT, phi, h->ldphi, h->lrbuf, h->nb, h->nk, h->nx);
+      OG_CHECK(hipGetLastError());
+    }
+  }
   og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
@@ -652,6 +663,7 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   if (rc == 0) rc = dmalloc(&h->red, (size_t)kRedBlocks * 8 + 64);
   if (rc == 0) rc = dmalloc(&h->tmp_in, nv);
   if (rc == 0) rc = dmalloc(&h->tmp_out, nv);
+  if (rc == 0 && h->na >= 6 && h->ns == 1) rc = dmalloc(&h->lrbuf, nv);
   if (rc == 0) rc = ensure_precision_buffers(h);
   if (rc != 0) {
     ocean_gpu_destroy(h);
@@ -670,6 +682,7 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
   dfree(h->amp); dfree(h->mcount);
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
+  dfree(h->lrbuf);
   for (auto& v : h->vecs) dfree(v);
   if (h->host_pin) (void)hipHostFree(h->host_pin);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1868,9 +1881,9 @@ using B = ocean_gpu_batch;
 // Device tables of one group for one set of (in, out) vectors.
 struct Tables {
   void* args = nullptr;  // og::ActArgs<T2>[np]
-  int* off = nullptr;    // [3][np + 1] first workgroup of each problem: proj, fft, adj
+  int* off = nullptr;    // [4][np + 1] first workgroup of each problem: proj, fft, adj, matrix-core lr
   int np = 0;
-  int nblk[3] = {0, 0, 0};
+  int nblk[4] = {0, 0, 0, 0};
   size_t sm[3] = {0, 0, 0};
   int threads[3] = {256, kBlock, 512};
   int lmax = 8;
@@ -1899,7 +1912,7 @@ int build_tables(const B* b, const B::Group& g, const std::vector<const double2*
                  const std::vector<double2*>& out, const Tiles& t, Tables& tb) {
   const int np = (int)g.idx.size();
   std::vector<og::ActArgs<T2>> host((size_t)np);
-  std::vector<int> off((size_t)3 * (np + 1), 0);
+  std::vector<int> off((size_t)4 * (np + 1), 0);
   tb = Tables{};
   tb.np = np;
   tb.lmax = g.lmax;
@@ -1914,11 +1927,18 @@ int build_tables(const B* b, const B::Group& g, const std::vector<const double2*
     off[q + 1] = off[q] + ge.nblk_proj;
     off[(np + 1) + q + 1] = off[(np + 1) + q] + ge.nblk_fft;
     off[2 * (np + 1) + q + 1] = off[2 * (np + 1) + q] + ge.nblk_adj;
+    // many alphas in FP64: the long-range sums on the matrix cores, as in xact_t
+    int nlr = 0;
+    if (std::is_same<T2, double2>::value && NA >= 6 && h->lrbuf && h->ns == 1 && h->na == NA) {
+      host[q].lrbuf = h->lrbuf;
+      nlr = ((h->nb + 63) / 64) * h->nk;
+    }
+    off[3 * (np + 1) + q + 1] = off[3 * (np + 1) + q] + nlr;
     tb.sm[0] = std::max(tb.sm[0], ge.sm_proj);
     tb.sm[1] = std::max(tb.sm[1], ge.sm_fft);
     tb.sm[2] = std::max(tb.sm[2], ge.sm_adj);
   }
-  for (int i = 0; i < 3; ++i) tb.nblk[i] = off[(size_t)i * (np + 1) + np];
+  for (int i = 0; i < 4; ++i) tb.nblk[i] = off[(size_t)i * (np + 1) + np];
   tb.threads[0] = t.proj;
   tb.threads[2] = t.adj;
   OG_CHECK(hipMalloc(&tb.args, sizeof(og::ActArgs<T2>) * (size_t)np));
@@ -1944,8 +1964,13 @@ int launch_tables(const Tables& tb, hipStream_t st, int stages) {
     else
       og::k_fft_w_batch<T2, 16><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
   }
-  if (stages & 4)
+  if (stages & 4) {
+    if constexpr (std::is_same<T2, double2>::value && NA >= 6) {
+      if (tb.nblk[3] > 0)
+        og::k_adj_lr_batch<NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
+    }
     og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+  }
   OG_CHECK(hipGetLastError());
   return OCEAN_GPU_OK;
 }

@@ -9,36 +9,13 @@
 // transform (k_fft_w with PHS).
 #pragma once
 #include "common.hpp"
+#include "gemm64.hpp"
 
 namespace og {
 
 // The two x-side products are GEMMs (M = x or o, N = the nk nalpha columns, K = o
-// or x) and run on the FP64 matrix cores: v_mfma_f64_16x16x4f64, where lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[l / 16 + 4 r]
-// [l % 16], r = 0..3 (measured on gfx950; the FP32 16x16x4 form returns rows
-// 4 (l / 16) + r instead).  The complex product is a real GEMM with the complex parts
-// unrolled along K: for every group of four K indices (kk = l / 16) the first
-// instruction takes the real parts of A and the second the imaginary parts, with
-// B[(kk, re)][(col, c)] and B[(kk, im)][(col, c)] formed from the lane's own complex
-// B element, so each lane loads one complex A element per M tile and one complex B
-// element per group.  Output columns are flattened as G = 2 (k nalpha + a) + c
-// (c = real/imaginary part); an N tile is 16 consecutive G.
-typedef double __attribute__((ext_vector_type(4))) og_d4;
-
-struct ObfCol {
-  int k, a, c;
-  bool ok;
-};
-__device__ __forceinline__ ObfCol obf_col(int G, int na, int nk) {
-  ObfCol r;
-  r.k = G / (2 * na);
-  r.a = (G >> 1) % na;
-  r.c = G & 1;
-  r.ok = r.k < nk;
-  r.k = min(r.k, nk - 1);
-  return r;
-}
-
+// or x) on the FP64 matrix cores; the fragment layout and the complex unrolling
+// are described in gemm64.hpp.
 // phi(k, x, a) = sum_o beta(k, o, a) obf(o, x)              (long_range.f90:373-382)
 // beta [k][o][a] (k stride ldb), obf [o][x], phi [k][x][a] (k stride ldphi).
 // Workgroup: 64 x points (four M tiles) by four N tiles, one per wave; the waves
@@ -52,11 +29,11 @@ __global__ __launch_bounds__(256) void k_obf_fwd(const double2* __restrict__ bet
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = lane & 15, kk = lane >> 4;
   const int x0 = blockIdx.x * 64;
-  ObfCol col[NT];
+  MfCol col[NT];
   const double2* bp[NT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    col[n] = obf_col(((blockIdx.y * 4 + w) * NT + n) * 16 + i, NA, nk);
+    col[n] = mf_col(((blockIdx.y * 4 + w) * NT + n) * 16 + i, NA, nk);
     bp[n] = beta + (size_t)col[n].k * ldb + col[n].a;
   }
   int xr[4];
@@ -136,11 +113,11 @@ __global__ __launch_bounds__(256) void k_obf_bwd(const double2* __restrict__ Y, 
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform x quarter
   const int i = lane & 15, kk = lane >> 4;
   const int o0 = blockIdx.x * 64;
-  ObfCol col[NT];
+  MfCol col[NT];
   const double2* yp[NT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    col[n] = obf_col((blockIdx.y * NT + n) * 16 + i, NA, nk);
+    col[n] = mf_col((blockIdx.y * NT + n) * 16 + i, NA, nk);
     yp[n] = Y + (size_t)col[n].k * ldphi + col[n].a;
   }
   int orow[4];
