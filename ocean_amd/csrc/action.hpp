@@ -851,8 +851,12 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   const int t = threadIdx.x, bd = blockDim.x;
   const int n = t % nb, xs = t / nb;
   const bool active = xs < XS;
-  // with lrbuf the long-range sums come from the matrix-core kernel k_adj_lr
-  const bool lr = (flags & F_LR) != 0 && A.lrbuf == nullptr, fg = (flags & F_FG) != 0;
+  // with lrbuf (many alphas only) the long-range sums come from the matrix-core
+  // kernel k_adj_lr; the fewer-alpha instances keep their register budget
+  bool lr = (flags & F_LR) != 0;
+  constexpr bool kLrBuf = NA >= 6 && sizeof(T2) == 16;  // FP64 many-alpha instances only
+  if constexpr (kLrBuf) lr = lr && A.lrbuf == nullptr;
+  const bool fg = (flags & F_FG) != 0;
   const int nel = na * nb;
 
   // epilogue operands of the first kEP elements owned by this thread
@@ -938,14 +942,16 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     double2 part[NA];
 #pragma unroll
     for (int j = 0; j < NA; ++j) part[j] = make_double2(-(double)acc[j].x, -(double)acc[j].y);
-    if (A.lrbuf != nullptr && (flags & F_LR) && xs == 0) {
+    if constexpr (kLrBuf) {
+      if (A.lrbuf != nullptr && (flags & F_LR) && xs == 0) {
 #pragma unroll
-      for (int j = 0; j < NA; ++j)
-        if (j < acnt) {
-          const double2 v = A.lrbuf[((size_t)(s + j * astep) * nk + k) * nb + n];
-          part[j].x -= v.x;
-          part[j].y -= v.y;
-        }
+        for (int j = 0; j < NA; ++j)
+          if (j < acnt) {
+            const double2 v = A.lrbuf[((size_t)(s + j * astep) * nk + k) * nb + n];
+            part[j].x -= v.x;
+            part[j].y -= v.y;
+          }
+      }
     }
     if (fg && active) {
 #pragma unroll

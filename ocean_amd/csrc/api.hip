@@ -269,11 +269,12 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
   }
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
   // many alphas in FP64: the long-range sums on the matrix cores, the rest of the
-  // adjoint (multiplet back-projection, epilogue) in k_adj
-  if constexpr (std::is_same<T2, double2>::value && NA >= 6) {
+  // adjoint (multiplet back-projection, epilogue) in k_adj.  (The FP32 instance of
+  // k_adj_lr is correct but measured no faster than the FP32 VALU adjoint.)
+  if constexpr (NA >= 6 && std::is_same<T2, double2>::value) {
     if ((flags & og::F_LR) && h->lrbuf && h->ns == 1 && h->na == NA) {
       a.lrbuf = h->lrbuf;
-      og::k_adj_lr<NA><<<dim3((h->nb + 63) / 64, h->nk), 256, 0, h->stream>>>(
+      og::k_adj_lr<T2, NA><<<dim3((h->nb + 63) / 64, h->nk), 256, 0, h->stream>>>(
           uT, h->ldT, phi, h->ldphi, h->lrbuf, h->nb, h->nk, h->nx);
       OG_CHECK(hipGetLastError());
     }
@@ -1965,9 +1966,9 @@ int launch_tables(const Tables& tb, hipStream_t st, int stages) {
       og::k_fft_w_batch<T2, 16><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
   }
   if (stages & 4) {
-    if constexpr (std::is_same<T2, double2>::value && NA >= 6) {
+    if constexpr (NA >= 6 && std::is_same<T2, double2>::value) {
       if (tb.nblk[3] > 0)
-        og::k_adj_lr_batch<NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
+        og::k_adj_lr_batch<T2, NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
     }
     og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
   }

@@ -237,7 +237,7 @@ struct ocean_gpu_handle {
   size_t host_pin_n = 0;
   double2* tmp_in = nullptr;   // host-plane xact staging
   double2* tmp_out = nullptr;
-  double2* lrbuf = nullptr;    // [a][k][n] long-range sums of the matrix-core adjoint (many alphas, FP64)
+  double2* lrbuf = nullptr;    // [a][k][n] long-range sums of the matrix-core adjoint (many alphas)
 
   std::vector<double2*> vecs;  // device vectors (OCEAN_vector analogue)
 

@@ -33,28 +33,52 @@ __device__ __forceinline__ MfCol mf_col(int G, int na, int nk) {
 }
 
 
+typedef float __attribute__((ext_vector_type(4))) og_f4;
+
+// One 16x16x4 matrix-core step in either precision, and the output row of a lane's
+// r-th accumulator: the f64 form returns rows l/16 + 4r, the f32 form 4(l/16) + r
+// (both measured on gfx950).
+template <typename T> struct Mf;
+template <> struct Mf<double> {
+  using V = og_d4;
+  static __device__ __forceinline__ V step(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int kk, int r) { return kk + 4 * r; }
+};
+template <> struct Mf<float> {
+  using V = og_f4;
+  static __device__ __forceinline__ V step(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int kk, int r) { return 4 * kk + r; }
+};
+
 // lr(n, k, a) = sum_x conj(uT(k, x, n)) phi'(k, x, a)          (long_range.f90:943-961)
-// for the many-alpha edges, written to lrbuf [a][k][n] for the adjoint's epilogue.
-// The VALU adjoint reads each phi'(k, x, :) as nalpha LDS broadcasts per band row,
-// which bounds it at twelve alphas; here it is the B operand of a GEMM per k
-// (M = bands, N = the 2 nalpha real columns, K = x).  Workgroup: 64 bands (four M
-// tiles) by all N tiles of one k; the four waves take quarters of the x range and
+// for the many-alpha edges, written to lrbuf [a][k][n] (FP64) for the adjoint's
+// epilogue.  The VALU adjoint reads each phi'(k, x, :) as nalpha LDS broadcasts per
+// band row, which bounds it at twelve alphas; here it is the B operand of a GEMM
+// per k (M = bands, N = the 2 nalpha real columns, K = x), in the precision of the
+// stored u (FP64, or FP32 for the mixed form, which accumulates in FP32 as its VALU
+// adjoint does).  Workgroup: 64 bands (four M tiles) by all N tiles of one k, so
+// every uT element is read once; the four waves take quarters of the x range and
 // their tiles are added in LDS in a fixed order.  nspn = 1.
-template <int NA>
-__device__ __forceinline__ void adj_lr_body(const double2* __restrict__ uT, int ldT,
-                                            const double2* __restrict__ phi, int ldphi,
-                                            double2* __restrict__ lrbuf, int nb, int nk, int nx, int bx,
+template <typename T2, int NA>
+__device__ __forceinline__ void adj_lr_body(const T2* __restrict__ uT, int ldT, const T2* __restrict__ phi,
+                                            int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx, int bx,
                                             int k) {
-  // all 2 NA real columns in one wave (NT N tiles), so each uT element is read once
+  using T = typename Scalar<T2>::T;
+  using M = Mf<T>;
+  using V = typename M::V;
   constexpr int NT = (2 * NA + 15) / 16;
-  __shared__ double part[4][NT * 16][64];
+  __shared__ T part[4][NT * 16][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = lane & 15, kk = lane >> 4;
   const int n0 = bx * 64;
   int ca[NT], cc[NT];
   bool cok[NT];
-  const double2* yp[NT];
+  const T2* yp[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int G = t * 16 + i;
@@ -63,19 +87,19 @@ __device__ __forceinline__ void adj_lr_body(const double2* __restrict__ uT, int 
     cok[t] = ca[t] < NA;
     yp[t] = phi + (size_t)k * ldphi + min(ca[t], NA - 1);
   }
-  const double2* __restrict__ ap = uT + (size_t)k * ldT;
+  const T2* __restrict__ ap = uT + (size_t)k * ldT;
   int nrow[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) nrow[m] = min(n0 + 16 * m + i, nb - 1);
   const int xa = (nx * w) >> 2, xb = (nx * (w + 1)) >> 2;
-  og_d4 acc[NT][4];
+  V acc[NT][4];
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int m = 0; m < 4; ++m) acc[t][m] = og_d4{0.0, 0.0, 0.0, 0.0};
+    for (int m = 0; m < 4; ++m) acc[t][m] = V{0, 0, 0, 0};
   {
-    double2 yA[NT], aA[4], yB[NT], aB[4];
-    auto load = [&](int xs, double2 (&yn)[NT], double2 (&an)[4]) {
+    T2 yA[NT], aA[4], yB[NT], aB[4];
+    auto load = [&](int xs, T2 (&yn)[NT], T2 (&an)[4]) {
       const int x = max(min(xs + kk, xb - 1), 0);
 #pragma unroll
       for (int t = 0; t < NT; ++t) yn[t] = yp[t][(size_t)x * NA];
@@ -83,23 +107,23 @@ __device__ __forceinline__ void adj_lr_body(const double2* __restrict__ uT, int 
       for (int m = 0; m < 4; ++m) an[m] = ld_stream(ap + (size_t)x * nb + nrow[m]);
     };
     // C = conj(A) Y: re: ar yr + ai yi, im: ar yi - ai yr
-    auto step = [&](int xs, const double2 (&yv)[NT], const double2 (&av)[4]) {
+    auto step = [&](int xs, const T2 (&yv)[NT], const T2 (&av)[4]) {
       const bool ok = xs + kk < xb;
-      double b0[NT], b1[NT];
+      T b0[NT], b1[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const double2 y = ok ? yv[t] : make_double2(0.0, 0.0);
+        const T2 y = ok ? yv[t] : cmake<T2>(0, 0);
         b0[t] = cc[t] ? y.y : y.x;
         b1[t] = cc[t] ? -y.x : y.y;
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[t][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0[t], acc[t][m], 0, 0, 0);
+        for (int m = 0; m < 4; ++m) acc[t][m] = M::step(av[m].x, b0[t], acc[t][m]);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[t][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1[t], acc[t][m], 0, 0, 0);
+        for (int m = 0; m < 4; ++m) acc[t][m] = M::step(av[m].y, b1[t], acc[t][m]);
     };
     load(xa, yA, aA);
     for (int xs = xa; xs < xb; xs += 8) {
@@ -127,33 +151,32 @@ __device__ __forceinline__ void adj_lr_body(const double2* __restrict__ uT, int 
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + 16 * m + kk + 4 * r;
-        double v = part[0][16 * t + 4 * m + r][lane];
+        const int n = n0 + 16 * m + M::row(kk, r);
+        T v = part[0][16 * t + 4 * m + r][lane];
 #pragma unroll
         for (int j = 1; j < 4; ++j) v += part[j][16 * t + 4 * m + r][lane];
-        if (n < nb) out[(size_t)n * 2] = v;
+        if (n < nb) out[(size_t)n * 2] = (double)v;
       }
   }
 }
 
-template <int NA>
-__global__ __launch_bounds__(256) void k_adj_lr(const double2* __restrict__ uT, int ldT,
-                                                const double2* __restrict__ phi, int ldphi,
-                                                double2* __restrict__ lrbuf, int nb, int nk, int nx) {
-  adj_lr_body<NA>(uT, ldT, phi, ldphi, lrbuf, nb, nk, nx, blockIdx.x, blockIdx.y);
+template <typename T2, int NA>
+__global__ __launch_bounds__(256) void k_adj_lr(const T2* __restrict__ uT, int ldT, const T2* __restrict__ phi,
+                                                int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx) {
+  adj_lr_body<T2, NA>(uT, ldT, phi, ldphi, lrbuf, nb, nk, nx, blockIdx.x, blockIdx.y);
 }
 
 // Batched form: problem p owns workgroups [off[p], off[p+1]), numbered band block
 // fastest, then k; problems without lrbuf own none.
-template <int NA>
-__global__ __launch_bounds__(256) void k_adj_lr_batch(const ActArgs<double2>* __restrict__ As,
+template <typename T2, int NA>
+__global__ __launch_bounds__(256) void k_adj_lr_batch(const ActArgs<T2>* __restrict__ As,
                                                       const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
-  const ActArgs<double2>& A = As[p];
+  const ActArgs<T2>& A = As[p];
   const int nbb = (A.nb + 63) / 64;
   const int loc = blockIdx.x - off[p];
-  adj_lr_body<NA>(A.uT, A.ldT, A.phi, A.ldphi, const_cast<double2*>(A.lrbuf), A.nb, A.nk, A.nx, loc % nbb,
-                  loc / nbb);
+  adj_lr_body<T2, NA>(A.uT, A.ldT, A.phi, A.ldphi, const_cast<double2*>(A.lrbuf), A.nb, A.nk, A.nx, loc % nbb,
+                      loc / nbb);
 }
 
 }  // namespace og
