@@ -440,6 +440,99 @@ __device__ __forceinline__ void dft_pass(const T2* __restrict__ tw, int G, int n
   }
 }
 
+// In-register radix-2 DFT of L (a power of two) values given in bit-reversed order;
+// the result is in natural order.
+template <int SIGN, int L, typename T2>
+__device__ __forceinline__ void fft_regs(T2 (&v)[L]) {
+#pragma unroll
+  for (int len = 2; len <= L; len <<= 1) {
+    const int half = len / 2, step = L / len;
+#pragma unroll
+    for (int i = 0; i < L; i += len) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        const T2 t = tw_mul<SIGN, L>(j * step, v[i + j + half]);
+        const T2 u = v[i + j];
+        v[i + j] = cmake<T2>(u.x + t.x, u.y + t.y);
+        v[i + j + half] = cmake<T2>(u.x - t.x, u.y - t.y);
+      }
+    }
+  }
+}
+
+// The last forward pass, the W product and the first backward pass fused along one
+// k dimension: each thread keeps its pencil in registers through F_+, W and F_-, so
+// the W product costs no LDS round trip (the 3-D transform is separable: the
+// backward passes may run in the reverse dimension order).
+template <typename T2, int L, typename Ld, typename St, typename Wf>
+__device__ __forceinline__ void dft_pass_w(const T2* __restrict__ twp, const T2* __restrict__ twm, int G, int nk,
+                                           int S, Ld ld, St st, Wf wf) {
+  using T = typename Scalar<T2>::T;
+  const int npen = (G * nk) / L;
+  for (int q = threadIdx.x; q < npen; q += blockDim.x) {
+    const int g = q % G, r = q / G;
+    const int base = (r / S) * (S * L) + (r % S);
+    T2 v[L], u[L];
+    if constexpr ((L & (L - 1)) == 0) {
+#pragma unroll
+      for (int j = 0; j < L; ++j) v[bitrev<L>(j)] = ld(base + j * S, g);
+      fft_regs<1, L>(v);
+#pragma unroll
+      for (int m = 0; m < L; ++m) {
+        const T w = wf(base + m * S, g);
+        u[bitrev<L>(m)] = cmake<T2>(v[m].x * w, v[m].y * w);
+      }
+      fft_regs<-1, L>(u);
+#pragma unroll
+      for (int m = 0; m < L; ++m) st(base + m * S, g, u[m]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < L; ++j) v[j] = ld(base + j * S, g);
+#pragma unroll
+      for (int m = 0; m < L; ++m) {
+        T2 acc = cmake<T2>(0, 0);
+#pragma unroll
+        for (int j = 0; j < L; ++j) cfma(acc, v[j], twp[(j * m) % L]);
+        const T w = wf(base + m * S, g);
+        u[m] = cmake<T2>(acc.x * w, acc.y * w);
+      }
+#pragma unroll
+      for (int m = 0; m < L; ++m) {
+        T2 acc = cmake<T2>(0, 0);
+#pragma unroll
+        for (int j = 0; j < L; ++j) cfma(acc, u[j], twm[(j * m) % L]);
+        st(base + m * S, g, acc);
+      }
+    }
+  }
+}
+
+template <typename T2, int LMAX, typename Ld, typename St, typename Wf>
+__device__ __forceinline__ void dft_run_w(const T2* twp, const T2* twm, int G, int nk, int L, int S, Ld ld, St st,
+                                          Wf wf) {
+  switch (L) {
+    case 1: dft_pass_w<T2, 1>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 2: dft_pass_w<T2, 2>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 3: dft_pass_w<T2, 3>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 4: dft_pass_w<T2, 4>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 5: dft_pass_w<T2, 5>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 6: dft_pass_w<T2, 6>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 7: dft_pass_w<T2, 7>(twp, twm, G, nk, S, ld, st, wf); break;
+    case 8: dft_pass_w<T2, 8>(twp, twm, G, nk, S, ld, st, wf); break;
+    default:
+      if constexpr (LMAX > 8) {
+        switch (L) {
+          case 9: dft_pass_w<T2, 9>(twp, twm, G, nk, S, ld, st, wf); break;
+          case 10: dft_pass_w<T2, 10>(twp, twm, G, nk, S, ld, st, wf); break;
+          case 12: dft_pass_w<T2, 12>(twp, twm, G, nk, S, ld, st, wf); break;
+          case 16: dft_pass_w<T2, 16>(twp, twm, G, nk, S, ld, st, wf); break;
+          default: break;  // rejected on the host
+        }
+      }
+      break;
+  }
+}
+
 template <typename T2, int LMAX, int SIGN, typename Ld, typename St>
 __device__ __forceinline__ void dft_run(const T2* tw, int G, int nk, int L, int S, Ld ld, St st) {
   switch (L) {
@@ -471,8 +564,9 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
 
 // Strip of G consecutive phi columns (c = x*na + a) per workgroup, all nk rows:
 //   pass 1 (K3, F_+): HBM -> registers -> LDS, with W(x, :) staged beside it
-//   passes 2, 3 (K2, K1, F_+), the last one scaling by W on its way back to LDS
-//   passes 4-6 (K3, K2, K1, F_-), the last one scaling by 1/N_k into HBM.
+//   pass 2 (K2, F_+)
+//   pass 3 (K1): F_+, the W product and F_- on each pencil in registers
+//   passes 4, 5 (K2, K3, F_-), the last one scaling by 1/N_k into HBM.
 // nred > 0 prepends that many workgroups running the multiplet reduction, which
 // is independent of phi and overlaps the transform.
 // PHS (the OBF form, long_range.f90:388-406): phi is multiplied by phs(k, x) as it
@@ -534,11 +628,7 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   };
   auto ld_lds = [&](int k, int g) -> T2 { return lds[lidx(k * G + g)]; };
   auto st_lds = [&](int k, int g, T2 v) { lds[lidx(k * G + g)] = v; };
-  auto st_w = [&](int k, int g, T2 v) {
-    const T w = wl[((c0 + min(g, gcnt - 1)) / na - xlo) * nk + k];
-    v.x *= w; v.y *= w;
-    lds[lidx(k * G + g)] = v;
-  };
+  auto w_at = [&](int k, int g) -> T { return wl[((c0 + min(g, gcnt - 1)) / na - xlo) * nk + k]; };
   const T inv = (T)1 / (T)nk;
   auto st_hbm = [&](int k, int g, T2 v) {
     if (g < gcnt) {
@@ -560,13 +650,12 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   __syncthreads();
   dft_run<T2, LMAX, 1>(twl + 2 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX, 1>(twl + 4 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_w);
-  __syncthreads();
-  dft_run<T2, LMAX, -1>(twl + 1 * LMAX, G, nk, K3, 1, ld_lds, st_lds);
+  // K1: F_+, W, F_- in registers
+  dft_run_w<T2, LMAX>(twl + 4 * LMAX, twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_lds, w_at);
   __syncthreads();
   dft_run<T2, LMAX, -1>(twl + 3 * LMAX, G, nk, K2, K3, ld_lds, st_lds);
   __syncthreads();
-  dft_run<T2, LMAX, -1>(twl + 5 * LMAX, G, nk, K1, K2 * K3, ld_lds, st_hbm);
+  dft_run<T2, LMAX, -1>(twl + 1 * LMAX, G, nk, K3, 1, ld_lds, st_hbm);
 }
 
 template <typename T2, int LMAX, bool PHS = false>
