@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libocean_gpu.so")
 SRC = [os.path.join(HERE, "csrc", "api.hip")]
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("common.hpp", "action.hpp", "vec.hpp", "obf.hpp")] + \
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("common.hpp", "action.hpp", "vec.hpp", "obf.hpp", "gemm64.hpp", "setup.hpp")] + \
     [os.path.join(ROOT, "include", "ocean_gpu.h")]
 
 HIPCC_FLAGS = [

@@ -317,21 +317,119 @@ __device__ __forceinline__ void proj_body(const ActArgs<T2>& A, int bx, int k, i
   }
 }
 
+// M tiles (16 x points each) per wave of the FP64 matrix-core projection
+constexpr int kProjMT64 = 4;
+
+// FP64 projection for many alphas on the matrix cores: per k the GEMM
+// phi[x][(a, c)] = sum_n u(n, k, x) (A psi)(n, k, a) with the complex parts unrolled
+// along K (v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16], B[l / 16][l % 16]
+// and receives D[l / 16 + 4 r][l % 16]).  Each wave covers 16 MT x points (MT M tiles)
+// and all 2 nalpha real columns of its spin, so every u element is read once; A psi
+// is formed per lane from psi and allow (L1/L2 hits), and the next group of four
+// bands is loaded while the current one multiplies.
+template <int NA, int MT>
+__device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx, int k, int s) {
+  constexpr int NT = (2 * NA + 15) / 16;
+  const int nb = A.nb, nk = A.nk, nx = A.nx, na = A.na, ns = A.ns;
+  const int astep = (ns == 2) ? 2 : 1;
+  const int acnt = (ns == 2) ? na / 2 : na;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  const int x0 = bx * (int)blockDim.x * MT / 4 + w * 16 * MT;
+  int ja[NT], jc[NT];
+  bool jok[NT];
+  size_t boff[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int G = t * 16 + i;
+    jc[t] = G & 1;
+    jok[t] = (G >> 1) < acnt;
+    ja[t] = s + min(G >> 1, acnt - 1) * astep;
+    boff[t] = ((size_t)ja[t] * nk + k) * nb;
+  }
+  const double2* __restrict__ up = A.u + ((size_t)s * nk + k) * nb * A.ldu;
+  int xr[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) xr[m] = min(x0 + 16 * m + i, nx - 1);
+  og_d4 acc[NT][MT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[t][m] = og_d4{0.0, 0.0, 0.0, 0.0};
+  double2 bA[NT], aA[MT], bB[NT], aB[MT];
+  auto load = [&](int n0, double2 (&bn)[NT], double2 (&an)[MT]) {
+    const int n = min(n0 + kk, nb - 1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double2 pv = A.psi[boff[t] + n];
+      const double m = A.allow[boff[t] + n];
+      bn[t] = make_double2(pv.x * m, pv.y * m);
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) an[m] = ld_stream(up + (size_t)n * A.ldu + xr[m]);
+  };
+  // C = A B: re: ar br - ai bi, im: ar bi + ai br
+  auto step = [&](int n0, const double2 (&bv)[NT], const double2 (&an)[MT]) {
+    const bool ok = n0 + kk < nb;
+    double2 av[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) av[m] = ok ? an[m] : make_double2(0.0, 0.0);
+    double b0[NT], b1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      b0[t] = jc[t] ? bv[t].y : bv[t].x;
+      b1[t] = jc[t] ? bv[t].x : -bv[t].y;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[t][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].x, b0[t], acc[t][m], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[t][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m].y, b1[t], acc[t][m], 0, 0, 0);
+  };
+  for (int n0 = 0; n0 < nb; n0 += 4) {
+    load(n0, bA, aA);
+    step(n0, bA, aA);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (!jok[t]) continue;
+    double* __restrict__ out = reinterpret_cast<double*>(A.phi + (size_t)k * A.ldphi + ja[t]) + jc[t];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 16 * m + kk + 4 * r;
+        if (x < nx) out[(size_t)x * na * 2] = acc[t][m][r];
+      }
+  }
+}
+
 // Workgroup b of a projection launch: slab tile bx of (k, s).  With the multiplet
 // on, each tile then forms its share (chunks bx, bx + px, ...) of that k's multiplet
 // partials (16 KB of c per k), after its slab so that the stream starts at once.
 template <typename T2, int NA>
 __device__ __forceinline__ void proj_dispatch(const ActArgs<T2>& A, int b) {
   const int r = b / A.px, bx = b - r * A.px, k = r % A.nk, s = r / A.nk;
-  proj_body<T2, NA>(A, bx, k, s);
+  if constexpr (sizeof(T2) == 16 && NA >= 6)
+    proj_mf64_body<NA, kProjMT64>(A, bx, k, s);
+  else
+    proj_body<T2, NA>(A, bx, k, s);
   if (A.flags & F_FG) mproj_k(A.cproj, A.psi, A.allow, A.apart, A.nb, A.nk, A.na, A.ns, A.P, k, s, bx, A.px);
 }
 
 // One problem: a 1-D grid of px * nk * ns slab tiles.  Workgroups are dispatched
 // round-robin over the 8 XCDs; the remap gives each XCD a contiguous range of tiles,
 // so the px tiles of one k (which stage the same A psi) share an L2.
+// The FP64 matrix-core form keeps four waves per SIMD (at most 128 VGPRs).
 template <typename T2, int NA>
-__global__ __launch_bounds__(512) void k_proj(const ActArgs<T2> A) {
+constexpr int kProjWaves = (sizeof(T2) == 16 && NA >= 6) ? 4 : 1;
+
+template <typename T2, int NA>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<T2, NA>))) void k_proj(
+    const ActArgs<T2> A) {
   int b = blockIdx.x;
   const int n = gridDim.x;
   if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
@@ -340,8 +438,8 @@ __global__ __launch_bounds__(512) void k_proj(const ActArgs<T2> A) {
 
 // Many problems: workgroup b belongs to the problem whose range holds it.
 template <typename T2, int NA>
-__global__ __launch_bounds__(512) void k_proj_batch(const ActArgs<T2>* __restrict__ As,
-                                                    const int* __restrict__ off, int np) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<T2, NA>))) void k_proj_batch(
+    const ActArgs<T2>* __restrict__ As, const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2> A = As[p];
   const int local = blockIdx.x - off[p];

@@ -173,9 +173,13 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   // FP32 streaming workgroups cover two x points per thread; the FP32 matrix-core
   // form (many alphas) covers 32 per wave and stages B as nb4 rows of kBRow floats
   const bool mf = sizeof(T) == 4 && NA >= 6;
-  const int xper = mf ? 32 * (t.proj / 64) : t.proj * (sizeof(T) == 4 ? 2 : 1);
+  const bool mf64 = sizeof(T) == 8 && NA >= 6;
+  const int xper = mf ? 32 * (t.proj / 64) : mf64 ? 16 * og::kProjMT64 * (t.proj / 64)
+                                                  : t.proj * (sizeof(T) == 4 ? 2 : 1);
   a.px = (nx + xper - 1) / xper;
-  g.sm_proj = mf ? (size_t)((nb + 3) & ~3) * og::kBRow * sizeof(float) : (size_t)acnt * nb * sizeof(T2);
+  // the FP64 matrix-core form (many alphas) forms A psi per lane and stages nothing
+  g.sm_proj = mf ? (size_t)((nb + 3) & ~3) * og::kBRow * sizeof(float)
+              : mf64 ? 0 : (size_t)acnt * nb * sizeof(T2);
   g.nblk_proj = a.px * nk * ns;
   // transform
   a.ncol = nx * na;

@@ -41,6 +41,9 @@ struct Tiles {
   int G = 0;       // transform columns per strip, 0 = the largest that fits
 };
 
+// accumulator of one 16x16x4 FP64 matrix-core instruction (four rows per lane)
+typedef double __attribute__((ext_vector_type(4))) og_d4;
+
 // complex helpers (double2 / float2 as interleaved complex)
 template <typename T2> struct Scalar;
 template <> struct Scalar<double2> { using T = double; };

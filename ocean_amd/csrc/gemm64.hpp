@@ -16,7 +16,6 @@ namespace og {
 // B element, so each lane loads one complex A element per M tile and one complex B
 // element per group.  Output columns are flattened as G = 2 (k nalpha + a) + c
 // (c = real/imaginary part); an N tile is 16 consecutive G.
-typedef double __attribute__((ext_vector_type(4))) og_d4;
 
 struct MfCol {
   int k, a, c;
