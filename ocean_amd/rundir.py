@@ -200,7 +200,13 @@ def read_system(d):
     # rpa: long_range off (OCEAN2/OCEAN_system.f90:338-341) and no cmel (OCEAN_multiplet.f90:488-489)
     S.rpa = os.path.exists(_p(d, "rpa"))
     # tdlda: a logical in the file (OCEAN2/OCEAN_multiplet.f90:2065-2070)
-    S.tdlda = os.path.exists(_p(d, "tdlda")) and _fbool(_tokens(_p(d, "tdlda"))[0])
+    # (an empty file ends that read with an end-of-file error)
+    S.tdlda = False
+    if os.path.exists(_p(d, "tdlda")):
+        tt = _tokens(_p(d, "tdlda"))
+        if not tt:
+            raise ValueError(f"{_p(d, 'tdlda')}: empty, a Fortran logical is expected")
+        S.tdlda = _fbool(tt[0])
     # xshift_override.ipt: three 0/1 factors on the x shift, else all 1 (OCEAN2/OCEAN_bloch.f90:279-292)
     if os.path.exists(_p(d, "xshift_override.ipt")):
         xo = tuple(int(t) for t in _tokens(_p(d, "xshift_override.ipt"))[:3])

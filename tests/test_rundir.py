@@ -209,3 +209,9 @@ def test_rpa_tdlda_and_xshift_override(tmp_path):
         assert np.abs(R.mult.X[i] - ref).max() <= 1e-14 * np.abs(ref).max()
     open(os.path.join(d, "xshift_override.ipt"), "w").write("2 1 1\n")
     assert rundir.read_system(d).xshift_override == (1, 1, 1)
+    # tdlda holding .false. is the plain F^k form; an empty file is an input error
+    open(os.path.join(d, "tdlda"), "w").write(".false.\n")
+    assert not rundir.read_system(d).tdlda
+    open(os.path.join(d, "tdlda"), "w").write("")
+    with pytest.raises(ValueError, match="tdlda"):
+        rundir.read_system(d)
