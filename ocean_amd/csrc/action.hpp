@@ -325,8 +325,9 @@ constexpr int kProjMT64 = 4;
 // along K (v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16], B[l / 16][l % 16]
 // and receives D[l / 16 + 4 r][l % 16]).  Each wave covers 16 MT x points (MT M tiles)
 // and all 2 nalpha real columns of its spin, so every u element is read once; A psi
-// is formed per lane from psi and allow (L1/L2 hits), and the next group of four
-// bands is loaded while the current one multiplies.
+// is formed per lane from psi and allow (L1/L2 hits).  Single-buffered fragments
+// keep four waves per SIMD, which hide the u stream's latency; double buffering
+// needed 166 VGPRs (three waves) and was slower.
 template <int NA, int MT>
 __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx, int k, int s) {
   constexpr int NT = (2 * NA + 15) / 16;
