@@ -1205,31 +1205,30 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   };
   if constexpr (kLrBuf) {
     // One spin, many alphas: every element e = t + i bd of this thread has band
-    // n = t % nb, so psi(b, k, n) for all b is loaded once, and each element's
-    // S row is fetched in one batch (same products in the same order as finish).
-    if ((flags & F_SO) && ns == 1 && na <= NA && bd % nb == 0) {
+    // n = t % nb, so psi(b, k, n) for all b is loaded once; S (conjugated for the
+    // backward action) is staged in the reduction area, free after the last
+    // barrier, so the element loop waits on no global load beyond its own operands
+    // (same products in the same order as finish).
+    if ((flags & F_SO) && ns == 1 && na <= NA && bd % nb == 0 && na * na <= max(XC * NA, JR * XS * nb)) {
       double2 pb[NA];
 #pragma unroll
       for (int b = 0; b < NA; ++b) pb[b] = psi[((size_t)min(b, na - 1) * nk + k) * nb + n];
-      for (int e = t; e < nel; e += bd) {
+      double2* Ss = red;
+      for (int i = t; i < na * na; i += bd) {
+        double2 v = S[i];
+        if (back) v.y = -v.y;
+        Ss[i] = v;
+      }
+      __syncthreads();
+      auto finish_so = [&](int e, double2 pv, double2 ev, double am) {
         const int a = e / nb;
         const size_t off = ((size_t)a * nk + k) * nb + n;
-        double2 pv = psi[off], ev = energy[off];
-        const double am = allow[off];
-        double2 sr[NA];
-#pragma unroll
-        for (int b = 0; b < NA; ++b) sr[b] = S[a * na + min(b, na - 1)];
         double2 inter = res[e];
         inter.x *= am;
         inter.y *= am;
 #pragma unroll
-        for (int b = 0; b < NA; ++b) {
-          if (b < na) {
-            double2 sab = sr[b];
-            if (back) sab.y = -sab.y;
-            cfma(inter, sab, pb[b]);
-          }
-        }
+        for (int b = 0; b < NA; ++b)
+          if (b < na) cfma(inter, Ss[a * na + b], pb[b]);
         double2 r = inter;
         if (flags & F_E0) {
           if (back) ev.y = -ev.y;
@@ -1237,6 +1236,13 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
           r.y = ev.x * pv.y + ev.y * pv.x + inter.y;
         }
         out[off] = r;
+      };
+#pragma unroll
+      for (int i = 0; i < kEP; ++i)
+        if (t + i * bd < nel) finish_so(t + i * bd, e_psi[i], e_en[i], e_m[i]);
+      for (int e = t + kEP * bd; e < nel; e += bd) {
+        const size_t off = ((size_t)(e / nb) * nk + k) * nb + n;
+        finish_so(e, psi[off], energy[off], allow[off]);
       }
       return;
     }
