@@ -357,7 +357,7 @@ __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx
   for (int t = 0; t < NT; ++t)
 #pragma unroll
     for (int m = 0; m < MT; ++m) acc[t][m] = og_d4{0.0, 0.0, 0.0, 0.0};
-  double2 bA[NT], aA[MT], bB[NT], aB[MT];
+  double2 bA[NT], aA[MT];
   auto load = [&](int n0, double2 (&bn)[NT], double2 (&an)[MT]) {
     const int n = min(n0 + kk, nb - 1);
 #pragma unroll
