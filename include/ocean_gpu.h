@@ -217,7 +217,8 @@ int  ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const d
  * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
  * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,
  * rel_error, complete_iter, global_iter) as written to absspct (:864-869).
- * x_re/x_im (nb, nk, nalpha, nE) may be NULL.  depth <= 48. */
+ * x_re/x_im (nb, nk, nalpha, nE) may be NULL.  1 <= depth <= 256 (OCEAN_GPU_ERR_ARG
+ * otherwise; the reference default cnbse.gmres.nloop is 80, Common/oparse.json:16). */
 int  ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay_im,
                      int32_t nE, const double* energies_eV, double gres_eV, double gprc_eV,
                      double ffff, int32_t depth, double kpref, double* out,

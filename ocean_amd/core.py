@@ -136,9 +136,18 @@ def _p(a):
     return a.ctypes.data_as(_DP)
 
 
-def _planes(z):
+def _planes(z, shape=None, name="array"):
     z = np.asarray(z)
+    if shape is not None:
+        _need(z, shape, name)
     return (np.asfortranarray(z.real, dtype=np.float64), np.asfortranarray(z.imag, dtype=np.float64))
+
+
+def _need(a, shape, name):
+    """The library reads and writes exactly prod(shape) elements through raw pointers:
+    refuse anything else before the call."""
+    if tuple(np.shape(a)) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(np.shape(a))}, expected {tuple(shape)}")
 
 
 def _chk(rc, where):
@@ -241,6 +250,7 @@ class OceanGPU:
         self.desc = d
         self.nb, self.nk, self.nx = nb, int(np.prod(kmesh)), int(np.prod(xmesh))
         self.na = 4 * (2 * lc + 1)
+        self.nspn = nspn
         self.shape = (nb, self.nk, self.na)
         h = _H()
         _chk(lib.ocean_gpu_create(C.byref(d), C.byref(h)), "ocean_gpu_create")
@@ -286,11 +296,13 @@ class OceanGPU:
         _chk(_lib().ocean_gpu_set_precision(self.h, precision), "set_precision")
 
     def load_bloch(self, u):
-        re, im = _planes(u)
+        re, im = _planes(u, (self.nb, self.nk, self.nx, self.nspn), "u")
         _chk(_lib().ocean_gpu_load_bloch(self.h, _p(re), _p(im)), "load_bloch")
 
     def load_bloch_planes(self, re, im):
         """u as two F-ordered float64 planes (nb, nk, nx, nspn)."""
+        for a, nm in ((re, "u_re"), (im, "u_im")):
+            _need(a, (self.nb, self.nk, self.nx, self.nspn), nm)
         _chk(_lib().ocean_gpu_load_bloch(self.h, _p(re), _p(im)), "load_bloch")
 
     def load_bloch_dev(self, re_ptr, im_ptr):
@@ -299,8 +311,9 @@ class OceanGPU:
              "load_bloch_dev")
 
     def load_bloch_raw(self, u_raw, xoff):
-        re, im = _planes(u_raw)
+        re, im = _planes(u_raw, (self.nb, self.nk, self.nx, self.nspn), "u_raw")
         xo = np.ascontiguousarray(xoff, np.int32)
+        _need(xo, (self.nx, 3), "xoff")
         _chk(_lib().ocean_gpu_load_bloch_raw(self.h, _p(re), _p(im), xo.ctypes.data_as(_IP)),
              "load_bloch_raw")
 
@@ -324,13 +337,17 @@ class OceanGPU:
                                        xo.ctypes.data_as(_IP)), "load_obf")
 
     def load_w(self, W):
+        _need(W, (self.nk, self.nx), "W")
         Wf = np.asfortranarray(W, np.float64)
         _chk(_lib().ocean_gpu_load_w(self.h, _p(Wf)), "load_w")
 
     def build_w(self, rdist, rtab, ptab, epsilon0, scale=1.0):
+        _need(rdist, (self.nk, self.nx), "rdist")
         rd = np.asfortranarray(rdist, np.float64)
         rt = np.ascontiguousarray(rtab, np.float64)
         pt = np.ascontiguousarray(ptab, np.float64)
+        if rt.ndim != 1 or pt.shape != rt.shape:
+            raise ValueError("rtab and ptab must be 1-D of the same length")
         self.nptab = len(rt)
         _chk(_lib().ocean_gpu_build_w(self.h, _p(rd), _p(rt), _p(pt), len(rt), epsilon0, scale),
              "build_w")
@@ -362,7 +379,10 @@ class OceanGPU:
         na = self.na
         sr, si = np.zeros((na, na), order="F"), np.zeros((na, na), order="F")
         hso = C.c_int32()
-        lt = None if life_time_eV is None else np.ascontiguousarray(life_time_eV, np.float64)[:2]
+        lt = None if life_time_eV is None else np.ascontiguousarray(life_time_eV, np.float64).ravel()
+        if lt is not None and lt.size < 2:
+            raise ValueError("life_time_eV needs the two values of core_broaden.ipt")
+        lt = None if lt is None else np.ascontiguousarray(lt[:2])
         _chk(_lib().ocean_gpu_mult_setup(self.h, _p(sl), scfac, int(bool(tdlda)), xi_eV,
                                          None if lt is None else _p(lt), _p(mhr),
                                          _p(mhi), _p(sr), _p(si), C.byref(hso)), "mult_setup")
@@ -376,8 +396,18 @@ class OceanGPU:
         npmax = max(m.nproj)
         nl = m.lmax - m.lmin + 1
         c = m.c
-        if c.shape[2] != npmax or c.shape[3] != 2 * m.lmax + 1 or c.shape[4] != nl:
-            raise ValueError("projector array has the wrong shape")
+        _need(c, (self.nb, self.nk, npmax, 2 * m.lmax + 1, nl, self.nspn), "projector array c")
+        _need(m.mpm, (npmax, npmax, nl), "mpm")
+        _need(m.somelr, (self.na, self.na), "somelr")
+        _need(m.someli, (self.na, self.na), "someli")
+        d = self.desc
+        if (m.lmin, m.lmax, m.lvl, m.lvh) != (d.lmin, d.lmax, d.lvl, d.lvh) or \
+                list(m.nproj) != list(d.nproj[:nl]):
+            raise ValueError("multiplet l ranges / projector counts differ from the handle's")
+        for l, Xl in zip(range(m.lvl, m.lvh + 1), m.X):
+            npl = m.nproj[l - m.lmin] if m.lmin <= l <= m.lmax else 0
+            mham = 4 * (2 * d.lc + 1) * (2 * l + 1) * npl
+            _need(Xl, (mham, mham), f"X[l={l}]")
         cr, ci = _planes(c)
         mpm = np.asfortranarray(m.mpm, np.float64)
         mhr, mhi, _, _ = m.packed()
@@ -387,16 +417,19 @@ class OceanGPU:
                                         _p(si), int(bool(m.have_spin_orbit))), "load_mult")
 
     def load_energies(self, energy, allow):
-        er, ei = _planes(energy)
+        er, ei = _planes(energy, self.shape, "energy")
+        _need(allow, self.shape, "allow")
         al = np.asfortranarray(allow, np.float64)
         _chk(_lib().ocean_gpu_load_energies(self.h, _p(er), _p(ei), _p(al)), "load_energies")
 
     # ---- the action ----
     def xact(self, psi, backwards=False, hflag=None):
-        pr, pi = _planes(psi)
+        pr, pi = _planes(psi, self.shape, "psi")
         hr = np.zeros(self.shape, order="F")
         hi = np.zeros(self.shape, order="F")
         hf = None if hflag is None else np.ascontiguousarray(hflag, np.int32)
+        if hf is not None:
+            _need(hf, (6,), "hflag")
         _chk(_lib().ocean_gpu_xact(self.h, _p(pr), _p(pi), _p(hr), _p(hi), int(bool(backwards)),
                                    None if hf is None else hf.ctypes.data_as(_IP)), "xact")
         return hr + 1j * hi
@@ -410,7 +443,7 @@ class OceanGPU:
         return i.value
 
     def vec_put(self, vid, z):
-        re, im = _planes(z)
+        re, im = _planes(z, self.shape, "vector")
         _chk(_lib().ocean_gpu_vec_put(self.h, vid, _p(re), _p(im)), "vec_put")
 
     def vec_get(self, vid):
@@ -454,6 +487,8 @@ class OceanGPU:
 
     def xact_vec(self, vin, vout, backwards=False, hflag=None):
         hf = None if hflag is None else np.ascontiguousarray(hflag, np.int32)
+        if hf is not None:
+            _need(hf, (6,), "hflag")
         _chk(_lib().ocean_gpu_xact_vec(self.h, vin, vout, int(bool(backwards)),
                                        None if hf is None else hf.ctypes.data_as(_IP)), "xact_vec")
 
@@ -478,15 +513,20 @@ class OceanGPU:
         return out
 
     # ---- solvers ----
-    def haydock(self, psi0, niter):
-        """Returns (real_a, real_b, imag_a) with niter+1 entries each (index 0..niter)."""
-        pr, pi = _planes(psi0)
+    def haydock(self, psi0, niter, return_status=False):
+        """Returns (real_a, real_b, imag_a) with niter+1 entries each (index 0..niter).
+        With return_status the library's code is not raised: (a, b, ai, niter_done, rc)
+        (rc = -1 and niter_done = the failing iteration on a NaN, as the reference's
+        OCEAN_hay_ab_twoterm, OCEAN2/OCEAN_haydock.f90:912-920)."""
+        pr, pi = _planes(psi0, self.shape, "psi0")
         a = np.zeros(niter + 1)
         b = np.zeros(niter + 1)
         ai = np.zeros(niter + 1)
         done = C.c_int32()
         rc = _lib().ocean_gpu_haydock(self.h, _p(pr), _p(pi), niter, _p(a), _p(b), _p(ai),
                                       C.byref(done))
+        if return_status:
+            return a, b, ai, done.value, rc
         _chk(rc, "haydock")
         return a, b, ai
 
@@ -495,9 +535,11 @@ class OceanGPU:
         """Misfit L = sum_E (sigma(E) - target)^2 of the GMRES spectrum and its gradient
         (ocean_gpu_spectral_grad): dict(loss, sigma, g_ptab, g_scale, g_gamma_eV, g_scissor_eV)."""
         kpref = self.kpref if kpref is None else kpref
-        pr, pi = _planes(hay_vec)
+        pr, pi = _planes(hay_vec, self.shape, "hay_vec")
         E = np.ascontiguousarray(energies_eV, np.float64)
         tg = np.ascontiguousarray(target, np.float64)
+        if E.ndim != 1 or tg.shape != E.shape:
+            raise ValueError("energies and target must be 1-D of the same length")
         sig = np.zeros(len(E))
         loss = C.c_double()
         gp = np.zeros(getattr(self, "nptab", 0)) if ptab else None
@@ -513,7 +555,7 @@ class OceanGPU:
     def haydock_early(self, psi0, niter, spacing, threshold, gam, el, eh, ne, kpref=None):
         """Haydock with the reference's early exit; returns (a, b, ai, niter_done)."""
         kpref = self.kpref if kpref is None else kpref
-        pr, pi = _planes(psi0)
+        pr, pi = _planes(psi0, self.shape, "psi0")
         a, b, ai = np.zeros(niter + 1), np.zeros(niter + 1), np.zeros(niter + 1)
         done = C.c_int32()
         _chk(_lib().ocean_gpu_haydock_early(self.h, _p(pr), _p(pi), niter, spacing, threshold, kpref,
@@ -522,11 +564,15 @@ class OceanGPU:
         return a, b, ai, done.value
 
     def gmres(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref=None,
-              return_x=False):
-        """Rows (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref, rel_err, iters, global_iters)."""
+              return_x=False, return_status=False):
+        """Rows (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref, rel_err, iters, global_iters).
+        With return_status the library's code is returned beside the rows instead of
+        raised."""
         kpref = self.kpref if kpref is None else kpref
-        pr, pi = _planes(hay_vec)
+        pr, pi = _planes(hay_vec, self.shape, "hay_vec")
         E = np.ascontiguousarray(energies_eV, np.float64)
+        if E.ndim != 1:
+            raise ValueError("energies must be 1-D")
         out = np.zeros((len(E), 6))
         xr = xi = None
         if return_x:
@@ -534,6 +580,8 @@ class OceanGPU:
             xi = np.zeros(self.shape + (len(E),), order="F")
         rc = _lib().ocean_gpu_gmres(self.h, _p(pr), _p(pi), len(E), _p(E), gres_eV, gprc_eV, ffff,
                                     depth, kpref, out.ctypes.data_as(_DP), _p(xr), _p(xi))
+        if return_status:
+            return (out, xr + 1j * xi, rc) if return_x else (out, rc)
         _chk(rc, "gmres")
         if return_x:
             return out, xr + 1j * xi
@@ -584,6 +632,8 @@ class OceanBatch:
         """nrep batched actions; returns ms per batched action (all problems)."""
         ii = np.ascontiguousarray(in_ids, np.int32)
         oo = np.ascontiguousarray(out_ids, np.int32)
+        _need(ii, (len(self.problems),), "in_ids")
+        _need(oo, (len(self.problems),), "out_ids")
         _chk(_lib().ocean_gpu_batch_xact(self.b, ii.ctypes.data_as(_IP), oo.ctypes.data_as(_IP), nrep),
              "batch_xact")
         ms = C.c_double()
@@ -593,7 +643,9 @@ class OceanBatch:
     def haydock(self, psi0s, niter):
         """Per problem (real_a, real_b, imag_a); also returns the recursion's ms."""
         n = len(self.problems)
-        planes = [_planes(p) for p in psi0s]
+        if len(psi0s) != n:
+            raise ValueError("one starting vector per problem")
+        planes = [_planes(p, g.shape, "psi0") for p, g in zip(psi0s, self.problems)]
         pr = (_DP * n)(*[_p(x[0]) for x in planes])
         pi = (_DP * n)(*[_p(x[1]) for x in planes])
         A = [np.zeros(niter + 1) for _ in range(n)]

@@ -494,59 +494,133 @@ __global__ void k_multiaxpy(const double2* const* __restrict__ Wl, const double2
 //   L(j,k) = (C(j,k) - sum_{m<k} L(j,m) conj(L(k,m)) D(m)) / D(k),  k < j = cur-1
 //   D(j)   =  C(j,j) - sum_{k<j} L(j,k) conj(L(j,k)) D(k)
 // with L and D kept in global memory between calls; then coef = -C^-1 <au|g> by the
-// two triangular solves.  One thread: cur is at most kMaxGmresDepth and the work is
-// O(cur^2).  A zero pivot leaves its 1-based index in *info (zhptrf's info) and the
-// coefficients at 0.
-constexpr int kMaxGmresDepth = 48;
+// two triangular solves.  One wave: lane l owns indices l, l + 64, .. (kLsSlots of
+// them), and every triangular sweep is right-looking, so each of its cur steps is
+// one broadcast of the finished value plus one FMA per owned index; the strictly
+// lower triangle of L is staged in LDS (packed rows) when it fits, otherwise read
+// from global memory.  A zero pivot leaves its 1-based index in *info (zhptrf's
+// info) and the coefficients at 0.
+constexpr int kLsSlots = 4;
+constexpr int kMaxGmresDepth = 64 * kLsSlots;
+constexpr int kLsPacked = 3916;  // LDS-staged rows 1 .. cur-2: (cur-2)(cur-1)/2 entries, cur <= 90
 
-__global__ void k_ls_update(const double* __restrict__ fin, double2* __restrict__ L, double* __restrict__ D,
-                            double2* __restrict__ coef, int cur, int depth, int* __restrict__ info) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ double2 ls_bcast(const double2 (&v)[kLsSlots], int k) {
+  const int q = k >> 6, src = k & 63;
+  double2 s = v[0];
+#pragma unroll
+  for (int i = 1; i < kLsSlots; ++i)
+    if (q == i) s = v[i];
+  return make_double2(__shfl(s.x, src, 64), __shfl(s.y, src, 64));
+}
+
+__global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin, double2* __restrict__ L,
+                                                  double* __restrict__ D, double2* __restrict__ coef, int cur,
+                                                  int depth, int* __restrict__ info) {
+  __shared__ double2 Ls[kLsPacked];
+  __shared__ double Dl[kMaxGmresDepth];
+  const int lane = threadIdx.x;
   const int j = cur - 1;
-  double2 r[kMaxGmresDepth];
+  const bool in_lds = j < 2 || (size_t)(j - 1) * j / 2 <= (size_t)kLsPacked;
+  // rows 1 .. j - 1 of L and D(0 .. j-1); row j is formed here and kept in registers
+  if (in_lds)
+    for (int i = 1; i < j; ++i)
+      for (int k = lane; k < i; k += 64) Ls[i * (i - 1) / 2 + k] = L[(size_t)i * depth + k];
+  for (int k = lane; k < j; k += 64) Dl[k] = D[k];
+  __syncthreads();
+  auto Lat = [&](int i, int k) -> double2 {  // L(i, k), k < i < j
+    return in_lds ? Ls[i * (i - 1) / 2 + k] : L[(size_t)i * depth + k];
+  };
   // C(j, k) = conj(<au_k|au_j>) for k < j (the lower triangle of the Hermitian
   // matrix whose upper column j holds <au_k|au_j>); C(j, j) = |au_j|^2
+  double2 t[kLsSlots], lj[kLsSlots];
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int i = lane + 64 * q;
+    t[q] = i < j ? make_double2(fin[4 * i], -fin[4 * i + 1]) : make_double2(0.0, 0.0);
+    lj[q] = make_double2(0.0, 0.0);
+  }
+  // row j: after step k, t(k) is final and L(j, k) = t(k) / D(k) updates every t(kk), kk > k
+  double dsum = 0.0;
   for (int k = 0; k < j; ++k) {
-    double2 t = make_double2(fin[4 * k], -fin[4 * k + 1]);
-    for (int m = 0; m < k; ++m) {
-      const double2 a = L[j * depth + m], b = L[k * depth + m];
-      const double d = D[m];
-      // a * conj(b) * d
-      t.x -= (a.x * b.x + a.y * b.y) * d;
-      t.y -= (a.y * b.x - a.x * b.y) * d;
+    const double2 tk = ls_bcast(t, k);
+    const double dk = Dl[k];
+    const double2 a = make_double2(tk.x / dk, tk.y / dk);
+    if (lane == 0) dsum -= (a.x * a.x + a.y * a.y) * dk;
+#pragma unroll
+    for (int q = 0; q < kLsSlots; ++q) {
+      const int kk = lane + 64 * q;
+      if (kk == k) lj[q] = a;
+      if (kk > k && kk < j) {
+        const double2 b = Lat(kk, k);
+        // a * conj(b) * d
+        t[q].x -= (a.x * b.x + a.y * b.y) * dk;
+        t[q].y -= (a.y * b.x - a.x * b.y) * dk;
+      }
     }
-    L[j * depth + k] = make_double2(t.x / D[k], t.y / D[k]);
   }
-  double dj = fin[4 * j];
-  for (int k = 0; k < j; ++k) {
-    const double2 a = L[j * depth + k];
-    dj -= (a.x * a.x + a.y * a.y) * D[k];
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int k = lane + 64 * q;
+    if (k < j) L[(size_t)j * depth + k] = lj[q];
   }
-  D[j] = dj;
-  L[j * depth + j] = make_double2(1.0, 0.0);
+  if (lane == 0) {
+    const double dj = fin[4 * j] + dsum;
+    D[j] = dj;
+    Dl[j] = dj;
+    L[(size_t)j * depth + j] = make_double2(1.0, 0.0);
+  }
+  __syncthreads();
   int bad = 0;
   for (int k = 0; k <= j; ++k)
-    if (D[k] == 0.0) { bad = k + 1; break; }
+    if (Dl[k] == 0.0) { bad = k + 1; break; }
   if (bad) {
-    *info = bad;
-    for (int i = 0; i < cur; ++i) coef[i] = make_double2(0.0, 0.0);
+    if (lane == 0) *info = bad;
+    for (int i = lane; i < cur; i += 64) coef[i] = make_double2(0.0, 0.0);
     return;
   }
-  for (int i = 0; i < cur; ++i) r[i] = make_double2(-fin[4 * i + 2], -fin[4 * i + 3]);
-  for (int i = 0; i < cur; ++i)
-    for (int k = 0; k < i; ++k) {
-      const double2 l = L[i * depth + k], v = r[k];
-      r[i].x -= l.x * v.x - l.y * v.y;
-      r[i].y -= l.x * v.y + l.y * v.x;
+  // coef = -C^-1 rvec: L y = -rvec, then D, then L^H coef = y (row j of L from the
+  // registers of its owners)
+  double2 r[kLsSlots];
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int i = lane + 64 * q;
+    r[q] = i < cur ? make_double2(-fin[4 * i + 2], -fin[4 * i + 3]) : make_double2(0.0, 0.0);
+  }
+  for (int k = 0; k < cur; ++k) {
+    const double2 v = ls_bcast(r, k);
+    const double2 ljk = ls_bcast(lj, k);
+#pragma unroll
+    for (int q = 0; q < kLsSlots; ++q) {
+      const int i = lane + 64 * q;
+      if (i > k && i < cur) {
+        const double2 l = i == j ? ljk : Lat(i, k);
+        r[q].x -= l.x * v.x - l.y * v.y;
+        r[q].y -= l.x * v.y + l.y * v.x;
+      }
     }
-  for (int i = 0; i < cur; ++i) { r[i].x /= D[i]; r[i].y /= D[i]; }
-  for (int i = cur - 1; i >= 0; --i)
-    for (int k = i + 1; k < cur; ++k) {
-      const double2 l = L[k * depth + i], v = r[k];  // conj(l) * v
-      r[i].x -= l.x * v.x + l.y * v.y;
-      r[i].y -= l.x * v.y - l.y * v.x;
+  }
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int i = lane + 64 * q;
+    if (i < cur) { r[q].x /= Dl[i]; r[q].y /= Dl[i]; }
+  }
+  for (int k = cur - 1; k > 0; --k) {
+    const double2 v = ls_bcast(r, k);
+#pragma unroll
+    for (int q = 0; q < kLsSlots; ++q) {
+      const int i = lane + 64 * q;
+      if (i < k) {
+        const double2 l = k == j ? lj[q] : Lat(k, i);  // conj(l) * v
+        r[q].x -= l.x * v.x + l.y * v.y;
+        r[q].y -= l.x * v.y - l.y * v.x;
+      }
     }
-  for (int i = 0; i < cur; ++i) coef[i] = r[i];
+  }
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int i = lane + 64 * q;
+    if (i < cur) coef[i] = r[q];
+  }
 }
 
 // y = (er + i ei) x - y  (OCEAN_psi_axmy, OCEAN_psi.f90:1468-1562)

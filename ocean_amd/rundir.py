@@ -372,6 +372,75 @@ def projector_array(cks, S: RunSystem, lmin, lmax, nproj, bands_per_k):
     return c
 
 
+def _fortran_e(x, w, dgt):
+    """x in the Fortran edit descriptor Ew.d (0.ddd..E+ee, right-justified)."""
+    x = float(x)
+    if x == 0.0:
+        body = "0." + "0" * dgt + "E+00"
+    else:
+        m, e = ("%.*E" % (dgt - 1, abs(x))).split("E")
+        digits = m.replace(".", "")
+        ex = int(e) + 1
+        es = ("E%+03d" % ex) if abs(ex) < 100 else ("%+04d" % ex)
+        body = ("-" if x < 0 else "") + "0." + digits + es
+    return body.rjust(w)
+
+
+def projector_files_present(d, S: RunSystem):
+    """The inputs nbsemkcmel reads (prjfile, radfile, ae<l>, rpot) are in the directory."""
+    r = S.run
+    add04 = "z%03d" % S.ZNL[0]
+    add14 = "%2s%04d_n%02dl%02d" % (r.elname, r.indx, S.ZNL[1], S.ZNL[2])
+    return all(os.path.exists(_p(d, n)) for n in ("prjfile" + add04, "radfile" + add04, "rpot.z" + add14))
+
+
+def make_cmel(d, S: RunSystem):
+    """What ocean.x does at set-up through nbsemkcmel (OCEAN2/nbsemkcmel.f90:8-209,
+    called by OCEAN_create_central, OCEAN_multiplet.f90:376): read prjfile<z>,
+    radfile<z> (rmax, nrmax, nr), rpot.z<run> ((V, r) rows) and ae<l><z> (r,
+    phi_1..phi_nproj rows), split the potential at rmax, integrate the projector
+    matrices, and write rpottrim.z<run> ('#', count; E24.16 pairs), rpotdiff.z<run>
+    (2f10.5) and cmel<l> (cmel then nmel, 4(1x,e15.8)) in the reference's formats, so
+    the readers below see the values ocean.x reads back (eight significant digits
+    in cmel)."""
+    r = S.run
+    add04 = "z%03d" % S.ZNL[0]
+    add14 = "%2s%04d_n%02dl%02d" % (r.elname, r.indx, S.ZNL[1], S.ZNL[2])
+    t = _tokens(_p(d, "prjfile" + add04))
+    lmin, lmax = int(t[0]), int(t[1])
+    nnu = [int(v) for v in t[2:2 + lmax - lmin + 1]]
+    t = _tokens(_p(d, "radfile" + add04))
+    rmax, nr = _ffloat(t[0]), int(t[2])
+    if rmax > 9.9:
+        raise ValueError("radfile: bad rmax (nbsemkcmel.f90:43 stops above 9.9)")
+    rows = []
+    with open(_p(d, "rpot.z" + add14)) as f:
+        for ln in f:
+            p = ln.split()
+            if len(p) < 2:
+                break
+            try:
+                rows.append((_ffloat(p[0]), _ffloat(p[1])))
+            except ValueError:
+                break  # the reference's read loop ends at the first unreadable record
+    v = np.array([a for a, _ in rows])
+    rv = np.array([b for _, b in rows])
+    vtrim, vdiff = sh.trim_core_hole_potential(v, rv, rmax)
+    _w(_p(d, "rpottrim.z" + add14), "#%8d\n" % v.size +
+       "".join("%s %s\n" % (_fortran_e(a, 24, 16), _fortran_e(b, 24, 16)) for a, b in zip(vtrim, rv)))
+    _w(_p(d, "rpotdiff.z" + add14), "".join("%10.5f%10.5f\n" % (a, b) for a, b in zip(vdiff, rv)))
+    for li, l in enumerate(range(lmin, lmax + 1)):
+        vals = np.array([_ffloat(x) for x in _tokens(_p(d, "ae%d%s" % (l, add04)))[:nr * (1 + nnu[li])]])
+        tab = vals.reshape(nr, 1 + nnu[li])
+        cmel, nmel = sh.projector_cmel(tab[:, 0], tab[:, 1:], rv, vdiff, rmax)
+        out = []
+        for m in (cmel, nmel):
+            flat = m.ravel(order="F")
+            for i0 in range(0, flat.size, 4):
+                out.append("".join(" " + _fortran_e(x, 15, 8) for x in flat[i0:i0 + 4]) + "\n")
+        _w(_p(d, "cmel%d" % l), "".join(out))
+
+
 def read_cmel(d, lmin, lmax, nproj):
     """cmel<l>: nproj(l)^2 values (eV) in column order -> mpm (Hartree)
     (OCEAN2/OCEAN_multiplet.f90:486-504)."""
@@ -482,6 +551,8 @@ def read_rundir(d):
         xo = S.xshift_override if mode == "bloch" else (1, 1, 1)
         xoff, tau_c, xshift = sh.x_offsets(S.xmesh, S.kmesh, S.tau, xo)
     rdist, _ = sh.w_geometry(S.avec, S.kmesh, S.xmesh, tau_c, xshift)
+    if projector_files_present(d, S):
+        make_cmel(d, S)  # ocean.x writes rpottrim and cmel<l> at set-up
     rtab, ptab = read_rpottrim(d, S)
     energy, allow, efermi, shift = read_energies(d, S)
     lmin, lmax, nproj = read_prjfile(d, S)
@@ -568,15 +639,17 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
                  obf=None, life_time_eV=None, rpa=False, tdlda=False, xshift_override=None):
     """Write a synthetic core XAS run directory in the file formats OCEAN2 reads
     (seeded: Bloch functions orthonormal over x per k, sorted band energies, the
-    screened-potential table exp(-r/3)/(4 max(r, 0.2)), random projector overlaps,
-    dipole elements, cmel and Slater integrals).  Returns the arrays the files hold.
+    screened potential exp(-r/3)/(4 max(r, 0.2)), projector radial functions,
+    random projector overlaps, dipole elements and Slater integrals).  Returns the
+    arrays the files hold.  Like a CNBSE directory before ocean.x has run, it holds
+    no rpottrim or cmel: the reader forms them as ocean.x does (make_cmel).
 
     Formats: OCEAN2/OCEAN_system.f90:166-231, :310-335, :644-726 (meshes, nspin, ZNL,
     nbuse, brange, mode, cks.normal, runlist, xyz.wyck); bse.in
     (OCEAN_multiplet.f90:298-303, OCEAN_haydock.f90:2109-2116); con.u2.dat
-    (OCEAN_bloch.f90:998-1115); wvfcninfo (OCEAN_energies.f90:649-687); rpottrim
-    (nbsemkcmel.f90:134-141 writes it); prjfile, cmel, fk/gk
-    (OCEAN_multiplet.f90:309-316, :486-504, :2064-2114); parcksc
+    (OCEAN_bloch.f90:998-1115); wvfcninfo (OCEAN_energies.f90:649-687); prjfile,
+    radfile, ae<l>, rpot (nbsemkcmel.f90:31-87, :145-150); fk/gk
+    (OCEAN_multiplet.f90:309-316, :2064-2114); parcksc
     (OCEAN_multiplet.f90:426-451); mels (OCEAN_psi.f90:5964-5975).
 
     obf = dict(num=..., fake=False) writes the OBF form instead of con.u2.dat:
@@ -645,22 +718,29 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     E = np.sort(rng.uniform(0.0, 30.0, size=(bif, nk, nspn)), axis=0) * EV2HARTREE
     hdr = np.array([bif, nk] + ([nspn] if nspn == 2 else []), np.int32)
     _fortran_sequential(os.path.join(d, "wvfcninfo"), [hdr, np.asfortranarray(E)])
-    # screened potential table (ptab, rtab rows)
-    rtab = 12.0 * np.arange(1, 201) / 200.0
-    ptab = np.exp(-rtab / 3.0) / (4.0 * np.maximum(rtab, 0.2))
+    # the inputs of nbsemkcmel, which ocean.x runs at set-up: the screened
+    # potential as (V, r) rows from r = 0 to 12 bohr, the projector radius and
+    # logarithmic radial grid (rmax on the grid, 60 Bode panels inside it), the
+    # projector radial functions per l
     add14 = "%2s%04d_n%02dl%02d" % (el, indx, n, lc)
-    _w(os.path.join(d, "rpottrim.z" + add14),
-       "#%8d\n" % rtab.size + "".join("%24.16E %24.16E\n" % (p, r) for p, r in zip(ptab, rtab)))
-    # projectors
+    rv = 12.0 * np.arange(0, 201) / 200.0
+    v = np.exp(-rv / 3.0) / (4.0 * np.maximum(rv, 0.2))
+    _w(os.path.join(d, "rpot.z" + add14), "".join("%.17g %.17g\n" % (a, b) for a, b in zip(v, rv)))
+    rmax, dl, nr = 2.0, 0.025, 321
+    rad = rmax * np.exp(dl * (np.arange(nr) - 240))
+    _w(os.path.join(d, "radfilez%03d" % Z), "%.17g %d %d\n" % (rmax, nr, nr))
     nl = lmax - lmin + 1
     assert len(nproj) == nl
     _w(os.path.join(d, "prjfilez%03d" % Z), "%d %d\n%s\n" % (lmin, lmax, " ".join(str(v) for v in nproj)))
-    cmel = {}
+    ae = {}
     for li, l in enumerate(range(lmin, lmax + 1)):
-        m = rng.uniform(-1.0, 1.0, (nproj[li], nproj[li]))
-        m = 0.5 * (m + m.T)
-        cmel[l] = m
-        _w(os.path.join(d, "cmel%d" % l), " ".join("%.17g" % v for v in m.ravel(order="F")) + "\n")
+        ph = np.stack([rad ** l * np.exp(-0.5 * (rad / (0.8 * (1.0 + 0.3 * nu))) ** 2) for nu in range(nproj[li])],
+                      axis=1)
+        ph = ph / np.sqrt(np.trapezoid(ph * ph * (rad ** 2)[:, None], rad, axis=0))
+        ph = ph * rng.uniform(0.8, 1.2, nproj[li])
+        ae[l] = ph
+        _w(os.path.join(d, "ae%dz%03d" % (l, Z)),
+           "".join(" ".join("%.17g" % t for t in (rad[i],) + tuple(ph[i])) + "\n" for i in range(nr)))
     add10 = "z%03dn%02dl%02d" % (Z, n, lc)
     slater = {}
     for li, lv in enumerate(range(lvl, lvh + 1)):
@@ -684,7 +764,7 @@ def write_synthetic_rundir(d, nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, nspn
     mels = rng.standard_normal((nptot, 2 * lc + 1)) + 1j * rng.standard_normal((nptot, 2 * lc + 1))
     _w(os.path.join(d, "mels.z%03dn%02dl%02dp%02d" % (Z, n, lc, photon)),
        "".join("%.17g %.17g\n" % (mels[p, m].real, mels[p, m].imag) for m in range(2 * lc + 1) for p in range(nptot)))
-    return dict(u=u, E=E, rtab=rtab, ptab=ptab, cmel=cmel, slater=slater, cks=cks, mels=mels, avec=avec,
+    return dict(u=u, E=E, rpot=(v, rv), rmax=rmax, rad=rad, ae=ae, slater=slater, cks=cks, mels=mels, avec=avec,
                 tau=np.array(tau), nb=nb, kmesh=kmesh, xmesh=xmesh, lc=lc, nspn=nspn, bif=bif, lmin=lmin,
                 lmax=lmax, nproj=list(nproj), xi_eV=xi_eV, inter_scale=inter_scale, eps=eps,
                 efermi=efermi_eV * EV2HARTREE, eshift=eshift_eV * EV2HARTREE, nptot=nptot, celvol=vol, niter=niter,

@@ -297,6 +297,69 @@ def _lm_matrix_half():
     return sx, sy, sz
 
 
+def trim_core_hole_potential(v, rv, rmax):
+    """Split the screened core-hole potential at the projector radius, as
+    nbsemkcmel does before the multiplet integrals (OCEAN2/nbsemkcmel.f90:98-127):
+    j is where its bisection over rv for rmax ends (1-based, moved past rv(j) <
+    rmax); inside, the long-range table keeps the flat value v(j) and the
+    difference v - v(j) goes to the projector matrix.  Returns (vtrim, vdiff)."""
+    v = np.asarray(v, np.float64)
+    rv = np.asarray(rv, np.float64)
+    n = v.size
+    lo, hi = 1, n
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if rv[mid - 1] < rmax:
+            lo = mid + 1
+        elif rv[mid - 1] > rmax:
+            hi = mid - 1
+        else:
+            break
+    if rv[lo - 1] < rmax:
+        lo += 1
+    if lo > n:
+        raise ValueError("the core-hole potential table ends inside the projector radius")
+    inside = np.arange(n) < lo - 1
+    vtrim = np.where(inside, v[lo - 1], v)
+    vdiff = np.where(inside, v - v[lo - 1], 0.0)
+    return vtrim, vdiff
+
+
+def projector_cmel(rad, phi, rv, vdiff, rmax):
+    """The direct core-hole matrix of the projectors and their overlap,
+    cmel(mu, nu) = int phi_mu phi_nu vdiff r^2 dr (eV) and nmel (no potential), by
+    nbsemkcmel's Bode rule on the logarithmic grid up to rmax
+    (OCEAN2/nbsemkcmel.f90:145-204): four-point panels ending at rmax with weights
+    (28, 64, 24, 64)/45, half end steps of the log measure, vdiff interpolated
+    linearly (intval with 'err': the grid must lie inside the table).
+    rad[nr], phi[nr, nnu]; returns (cmel_eV, nmel) of shape (nnu, nnu)."""
+    rad = np.asarray(rad, np.float64)
+    phi = np.asarray(phi, np.float64).reshape(rad.size, -1)
+    nr = rad.size
+    dl = math.log(rad[-1] / rad[0]) / (nr - 1)
+    ix = 1 + int(math.floor(math.log(rmax / rad[0]) / dl + 0.5))  # nint of a positive value
+    if ix < 5 or ix > nr:
+        raise ValueError("rmax does not lie on the radial grid")
+    dr = dl * rad[:ix]
+    dr[0] *= 0.5
+    dr[ix - 1] *= 0.5
+    rv = np.asarray(rv, np.float64)
+    r = rad[:ix]
+    if r.min() < rv[0] or r.max() > rv[-1]:
+        raise ValueError("the radial grid leaves the core-hole potential table (intval 'err')")
+    val = interp_cap(rv, vdiff, r)
+    # panel weights: panel ending at i (1-based, i = ix, ix-4, .., >= 5) gives point
+    # i - j the weight b(j)
+    w = np.zeros(ix)
+    bw = np.array([28.0, 64.0, 24.0, 64.0]) / 45.0
+    for i in range(ix, 4, -4):
+        w[i - 4:i] += bw[::-1]
+    q = w * dr * r * r
+    cmel = (phi[:ix].T * (q * val)) @ phi[:ix] * HARTREE2EV
+    nmel = (phi[:ix].T * q) @ phi[:ix]
+    return cmel, nmel
+
+
 def radial_cmel(rgrid, phis, vdiff):
     """Direct core-projector matrix int phi_mu phi_nu vdiff r^2 dr (eV), trapezoid rule."""
     f = phis[:, :, None] * phis[:, None, :] * (vdiff * rgrid ** 2)[:, None, None]

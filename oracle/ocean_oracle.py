@@ -1482,8 +1482,13 @@ def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, d
     h1 = act(one)
     x = np.zeros_like(rhs)
     ax = np.zeros_like(rhs)
-    u_mat = [None] * (depth + 1)
-    au_mat = [None] * (depth + 1)
+    # u_matrix / au_matrix (rows 1..depth, F-ordered vectors): the Gram column, the
+    # right-hand side <au_i|g> and the x / ax updates are matrix-vector products
+    shape, nvec = rhs.shape, rhs.size
+    U = np.zeros((depth + 1, nvec), np.complex128)
+    AU = np.zeros((depth + 1, nvec), np.complex128)
+    flat = lambda v: np.ravel(v, order="F")  # noqa: E731
+    unflat = lambda v: np.asfortranarray(v.reshape(shape, order="F"))  # noqa: E731
     C = np.zeros((depth, depth), np.complex128)  # upper triangle of the packed Gram matrix
     rows = []
     xs = []
@@ -1495,18 +1500,17 @@ def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, d
     rval = ival = 0.0
 
     def update(cur, g):
-        # C(i, cur) = conj(<au_cur|au_i>) for i < cur; C(cur,cur) = |au_cur|^2
+        # C(i, cur) = conj(<au_cur|au_i>) = <au_i|au_cur> for i < cur; C(cur,cur) = |au_cur|^2
         nonlocal x, ax
-        for i in range(cur):
-            r, im = psi_dot(au_mat[cur], au_mat[i + 1]) if i < cur - 1 else (psi_nrm(au_mat[cur]), 0.0)
-            C[i, cur - 1] = r - 1j * im
-        rvec = np.array([complex(*psi_dot(au_mat[i + 1], g)) for i in range(cur)])
+        if cur > 1:
+            C[:cur - 1, cur - 1] = np.conj(AU[1:cur]) @ AU[cur]
+        C[cur - 1, cur - 1] = np.vdot(AU[cur], AU[cur]).real
+        rvec = np.conj(AU[1:cur + 1]) @ flat(g)
         A = np.triu(C[:cur, :cur])
         A = A + np.conj(np.triu(A, 1)).T
         coeff = np.linalg.solve(A, -rvec)
-        for i in range(cur):
-            x = x + coeff[i] * u_mat[i + 1]
-            ax = ax + coeff[i] * au_mat[i + 1]
+        x = x + unflat(coeff @ U[1:cur + 1])
+        ax = ax + unflat(coeff @ AU[1:cur + 1])
 
     energies = list(energies_eV)
     for step, E_eV in enumerate(energies, start=1):
@@ -1525,8 +1529,7 @@ def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, d
             g = ax - rhs
         else:
             delta = ener - energies[step - 2] * EV2HARTREE
-            for i in range(1, prev_iter + 1):
-                au_mat[i] = au_mat[i] + delta * u_mat[i]
+            AU[1:prev_iter + 1] += delta * U[1:prev_iter + 1]
             ax = ener * x + 1j * gres * x - act(x)
             g = ax - rhs
             for it in range(1, prev_iter + 1):
@@ -1554,9 +1557,9 @@ def gmres_do_recycle(P: Problem, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, d
                 complete_iter += 1
                 global_iter += 1
                 pg = g * pcdiv
-                u_mat[it] = pg
+                U[it] = flat(pg)
                 apg = ener * pg + 1j * gres * pg - act(pg)
-                au_mat[it] = apg
+                AU[it] = flat(apg)
                 update(it, g)
                 if it == depth:
                     ax = ener * x + 1j * gres * x - act(x)

@@ -5,7 +5,7 @@ import pytest
 
 import ocean_oracle as orc
 from helpers import relerr
-from rundir_files import EV2H, write_rundir
+from rundir_files import EV2H, expected_tables, write_rundir
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +28,12 @@ def oracle_problem(gen, R):
     S = R.system
     xo = gen.get("xshift_override") or (1, 1, 1)
     u, tau_c, xshift = orc.bloch_phase(R.u_raw, S.xmesh, S.kmesh, S.tau, xo)
-    W = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, R.rtab, R.ptab, S.epsilon0, S.inter_scale)
+    # the W table and the direct projector matrix from the oracle's nbsemkcmel on the
+    # directory's rpot / radfile / ae files (the reader forms them itself)
+    rtab, ptab, mpm = expected_tables(gen)
+    if gen.get("rpa"):
+        mpm = np.zeros_like(mpm)
+    W = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, rtab, ptab, S.epsilon0, S.inter_scale)
     energy, allow = orc.energies_core_xas(gen["E"][:gen["nb"]], gen["efermi"], gen["eshift"], gen["lc"],
                                           gen["nspn"])
     m = R.mult
@@ -42,7 +47,7 @@ def oracle_problem(gen, R):
         sg = {k: gen["slater"][("gk", lv, k)][1] for k in gk}
         X.append(orc.nbsemhsetup2(gen["lc"], lv, npl, fk, sf, gk, sg, tdlda=bool(gen.get("tdlda"))) * EV2H)
     sr, si, have_so = orc.soprep(gen["lc"], gen["xi_eV"], gen.get("life_time"))
-    M = orc.Multiplet(m.lmin, m.lmax, list(m.nproj), m.lvl, m.lvh, m.c, m.mpm, X, sr, si, have_so)
+    M = orc.Multiplet(m.lmin, m.lmax, list(m.nproj), m.lvl, m.lvh, m.c, mpm, X, sr, si, have_so)
     P = orc.Problem(S.nb, S.kmesh, S.xmesh, gen["lc"], S.nspn, S.celvol, S.inter_scale, u, W, energy, allow, M,
                     long_range=not gen.get("rpa"))
     psi = orc.psi_dotter(gen["mels"], gen["cks"], gen["lc"], gen["nb"], S.nk, gen["nspn"],

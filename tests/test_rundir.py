@@ -9,7 +9,7 @@ import pytest
 import ocean_oracle as orc
 from ocean_amd import rundir
 from ocean_amd import setup_host as sh
-from rundir_files import EV2H, write_rundir
+from rundir_files import EV2H, expected_tables, write_rundir
 
 SHAPES = {
     "k_edge": dict(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=0, seed=1, nproj=(2, 1, 1)),
@@ -62,9 +62,9 @@ def test_multiplet_files(case):
     d, gen, R = case
     m = R.mult
     lmin, lmax, nproj = gen["lmin"], gen["lmax"], gen["nproj"]
-    for li, l in enumerate(range(lmin, lmax + 1)):
-        npl = nproj[li]
-        assert np.array_equal(m.mpm[:npl, :npl, li], gen["cmel"][l] * EV2H)
+    # mpm from the cmel files the reader formed (nbsemkcmel), as ocean.x reads them back
+    _, _, mpm = expected_tables(gen)
+    assert np.abs(m.mpm - mpm).max() <= 1e-15 * np.abs(mpm).max()
     # projector order (l, m, nu) of the cks rows, bands of each k past nb skipped
     S = R.system
     ip = 0
@@ -91,7 +91,8 @@ def test_multiplet_files(case):
 def test_w_table_and_geometry(case):
     d, gen, R = case
     S = R.system
-    assert np.array_equal(R.rtab, gen["rtab"]) and np.array_equal(R.ptab, gen["ptab"])
+    rtab, ptab, _ = expected_tables(gen)
+    assert np.array_equal(R.rtab, rtab) and np.array_equal(R.ptab, ptab)
     _, tau_c, xshift = orc.bloch_phase(np.zeros((1, S.nk, S.nx, 1), complex, order="F"), S.xmesh, S.kmesh,
                                        S.tau)
     Wref = orc.populate_W(S.avec, S.kmesh, S.xmesh, tau_c, xshift, R.rtab, R.ptab, S.epsilon0,
@@ -215,3 +216,55 @@ def test_rpa_tdlda_and_xshift_override(tmp_path):
     open(os.path.join(d, "tdlda"), "w").write("")
     with pytest.raises(ValueError, match="tdlda"):
         rundir.read_system(d)
+
+
+def test_nbsemkcmel_outputs_are_formed_like_ocean_x(tmp_path):
+    """A fresh directory holds only nbsemkcmel's inputs; reading it writes rpottrim,
+    rpotdiff and cmel<l> in the reference's formats (OCEAN2/nbsemkcmel.f90:134-202),
+    whose values the reader then takes, and a second read reproduces them."""
+    import re
+    d = str(tmp_path / "fresh")
+    gen = write_rundir(d, seed=4, nproj=(2, 1, 1))
+    names = os.listdir(d)
+    assert not any(n.startswith(("cmel", "rpottrim", "rpotdiff")) for n in names)
+    R1 = rundir.read_rundir(d)
+    add14 = "Fe0001_n01l00"
+    lines = open(os.path.join(d, "rpottrim.z" + add14)).read().splitlines()
+    assert re.fullmatch(r"#\s*\d+", lines[0]) and int(lines[0][1:]) == gen["rpot"][0].size
+    # (E24.16, 1X, E24.16): right-justified 0.d..dE+ee fields
+    for ln in lines[1:]:
+        assert len(ln) == 49 and ln[24] == " "
+        for f in (ln[:24], ln[25:]):
+            assert re.fullmatch(r" *-?0\.\d{16}E[+-]\d\d", f), f
+    cm = open(os.path.join(d, "cmel0")).read().splitlines()
+    # 2 x 2 cmel then 2 x 2 nmel, four (1X, E15.8) fields per line
+    assert len(cm) == 2
+    for ln in cm:
+        assert len(ln) == 64
+        for i in range(4):
+            f = ln[16 * i:16 * i + 16]
+            assert f[0] == " " and re.fullmatch(r" *-?0\.\d{8}E[+-]\d\d", f[1:]), f
+    assert os.path.exists(os.path.join(d, "rpotdiff.z" + add14))
+    R2 = rundir.read_rundir(d)
+    assert np.array_equal(R1.mult.mpm, R2.mult.mpm) and np.array_equal(R1.ptab, R2.ptab)
+    # the flat part of the table inside the projector radius and the tail outside it
+    v, rv = gen["rpot"]
+    inside = rv < gen["rmax"]
+    assert np.all(R1.ptab[inside] == R1.ptab[inside][0])
+    assert np.allclose(R1.ptab[~inside], v[~inside], rtol=1e-15, atol=0)
+
+
+def test_projector_cmel_bode_rule_closed_form():
+    """The product's Bode-rule quadrature on the logarithmic grid against a closed
+    form: phi = r exp(-r^2/2) (one projector), vdiff = 1 - r/rmax inside rmax."""
+    rmax, dl = 2.0, 0.01
+    rad = rmax * np.exp(dl * (np.arange(1201) - 1000))
+    phi = (rad * np.exp(-0.5 * rad ** 2))[:, None]
+    rv = np.linspace(0.0, 10.0, 20001)
+    vdiff = np.where(rv < rmax, 1.0 - rv / rmax, 0.0)
+    cmel, nmel = sh.projector_cmel(rad, phi, rv, vdiff, rmax)
+    from scipy.integrate import quad
+    ref_n = quad(lambda r: r ** 4 * np.exp(-r * r), 0.0, rmax)[0]
+    ref_c = quad(lambda r: r ** 4 * np.exp(-r * r) * (1.0 - r / rmax), 0.0, rmax)[0]
+    assert abs(nmel[0, 0] - ref_n) < 1e-7 * ref_n
+    assert abs(cmel[0, 0] / sh.HARTREE2EV - ref_c) < 1e-6 * ref_c
