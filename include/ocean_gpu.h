@@ -256,6 +256,17 @@ int  ocean_gpu_spectrum(const double* real_a, const double* real_b, const double
                         int32_t niter, double kpref, double gam, double el, double eh,
                         int32_t ne, int32_t literal_cmplx, double* out);
 
+/* ocean_gpu_spectrum without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and output as
+ * ocean_gpu_spectrum; the coefficients and the output are staged in a fixed device
+ * array of 2^19 doubles (4 MiB) per GPU, held until the process ends, so
+ * 3 (niter + 1) + 5 (ne + 1) must not exceed 2^19 (OCEAN_GPU_ERR_ARG otherwise).
+ * Every call on one GPU uses that GPU's array: calls that overlap on one GPU race. */
+int  ocean_gpu_spectrum_noalloc(const double* real_a, const double* real_b, const double* imag_a,
+                                int32_t niter, double kpref, double gam, double el, double eh,
+                                int32_t ne, int32_t literal_cmplx, double* out);
+
 /* ---- batches (no reference counterpart: OCEAN2 runs one site per process) ----
  * A batch runs many independent problems -- each a handle loaded as above, of
  * any size (nb, k-mesh, x-mesh, edge, precision) -- together: one launch per

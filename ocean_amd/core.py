@@ -89,6 +89,8 @@ _SIGS = {
                                   C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
+    "ocean_gpu_spectrum_noalloc": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_write_core": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP]),
     "ocean_gpu_tridiag_eigs": (C.c_int, [_DP, _DP, C.c_int32, _DP]),
     "ocean_gpu_ospect": (C.c_int, [_DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP, _IP]),

@@ -1,6 +1,7 @@
 // C-ABI of the gfx950 core-level BSE library (declared in include/ocean_gpu.h).
 #include <algorithm>
 #include <complex>
+#include <mutex>
 #include <type_traits>
 
 #include "action.hpp"
@@ -1562,6 +1563,66 @@ int ocean_gpu_spectrum(const double* real_a, const double* real_b, const double*
   }
   dfree(d);
   return rc;
+}
+
+}  // extern "C"
+
+// ---- the continued fraction without allocation ----
+// ocean_gpu_spectrum stages a, b, imag_a and the output in a buffer it allocates per
+// call.  The allocation-free form stages them in this file-scope array instead (each
+// GPU holds its own copy until the process ends).  Host code never passes the
+// array's name to a kernel or copy: it is a host-side stand-in, so every use goes
+// through the device address that hipGetSymbolAddress returns for the current GPU.
+namespace {
+
+constexpr size_t kNoAllocDoubles = (size_t)1 << 19;  // 4 MiB per GPU
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of the current GPU's scratch array, looked up once per GPU.
+int scratch_for_device(double** out) {
+  int dev = 0;
+  OG_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return OCEAN_GPU_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    OG_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return OCEAN_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocean_gpu_spectrum_noalloc(const double* real_a, const double* real_b, const double* imag_a, int32_t niter,
+                               double kpref, double gam, double el, double eh, int32_t ne, int32_t literal_cmplx,
+                               double* out) {
+  if (!real_a || !real_b || !out || niter <= 0 || ne <= 0) return OCEAN_GPU_ERR_ARG;
+  const size_t n1 = (size_t)niter + 1;
+  if (3 * n1 + 5 * ((size_t)ne + 1) > kNoAllocDoubles) return OCEAN_GPU_ERR_ARG;
+  double* d = nullptr;
+  OG_TRY(scratch_for_device(&d));
+  OG_CHECK(hipMemcpy(d, real_a, n1 * sizeof(double), hipMemcpyHostToDevice));
+  OG_CHECK(hipMemcpy(d + n1, real_b, n1 * sizeof(double), hipMemcpyHostToDevice));
+  if (imag_a) {
+    OG_CHECK(hipMemcpy(d + 2 * n1, imag_a, n1 * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    // the array keeps the previous call's contents: a missing imag_a is zero
+    og::k_fill_d<<<(int)((n1 + kBlock - 1) / kBlock), kBlock>>>(d + 2 * n1, 0.0, n1);
+    OG_CHECK(hipGetLastError());
+  }
+  og::k_cfrac<<<(ne + 1 + 127) / 128, 128>>>(d, d + n1, d + 2 * n1, niter, kpref, gam, el, eh, ne,
+                                             literal_cmplx, kHartree2eV, d + 3 * n1);
+  OG_CHECK(hipGetLastError());
+  OG_CHECK(hipMemcpy(out, d + 3 * n1, 5 * ((size_t)ne + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  return OCEAN_GPU_OK;
 }
 
 // ---- GMRES ----
