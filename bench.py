@@ -62,22 +62,50 @@ def measured_traffic(kname):
         return None
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and the logical CPUs this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return name, avail, os.cpu_count() or 1
+
+
 def cpu_baseline(inp, budget_s):
+    """The C/OpenMP restatement of the reference's CPU path (oracle/ocean_oracle.c) on
+    the host cores given to this GPU: lr_kernel's projection / transform / adjoint
+    per (alpha, x) with OpenMP over x points (the reference's OpenMP runs over alpha,
+    long_range.f90:816-868, i.e. at most 4 threads at the K edge; its x points are
+    what the reference spreads over MPI ranks, so the x split is the layout that can
+    use a whole host), multiplet and energies as OCEAN_fg_combo / OCEAN_energies_act."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import c_oracle  # the checker/baseline, never the measured GPU product
 
-    nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    model, avail, total = cpu_model()
+    # the host's CPU share for this GPU, as the environment states it (OMP_NUM_THREADS),
+    # else every CPU this process may run on
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
     cs = c_oracle.CSystem(inp)
     t0 = time.perf_counter()
     cs.xact(inp.psi0, nthreads=nthr)
     one = time.perf_counter() - t0
-    nrep = max(1, min(200, int(budget_s / max(one, 1e-3))))
+    nrep = max(1, min(400, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     cs.xact(inp.psi0, nthreads=nthr, nrep=nrep)
     dt = time.perf_counter() - t0
     return {"value": nrep / dt, "unit": "H·ψ/s", "cores": nthr, "kind": "port",
+            "cpu_model": model, "cpus_visible": total, "cpus_allowed": avail,
             "sample": f"{nrep} FP64 H.psi applications of the same workload "
-                      f"(C/OpenMP restatement, {dt:.1f} s)"}
+                      f"(C/OpenMP restatement, OpenMP over x points, {nthr} threads, {dt:.1f} s)"}
 
 
 def batch_workload(n, niter, reps, log_every=8):

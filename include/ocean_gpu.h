@@ -297,6 +297,17 @@ int  ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, co
                              int32_t niter, double* const* real_a, double* const* real_b,
                              double* const* imag_a, int32_t* niter_done);
 
+/* GMRES of all problems at once (as ocean_gpu_gmres per problem; energies, starting
+ * vectors and kpref per problem, the solver settings shared): each problem's
+ * recycled solve runs on its own handle's stream, driven by a pool of host threads,
+ * so the solves overlap on the device.  rcs[n] (nullable) receives each problem's
+ * code; the return value is the first nonzero one in problem order.  The wall time
+ * of the whole call is read back with ocean_gpu_batch_last_ms. */
+int  ocean_gpu_batch_gmres(ocean_gpu_batch* b, const double* const* hay_re, const double* const* hay_im,
+                           const int32_t* nE, const double* const* energies_eV, double gres_eV,
+                           double gprc_eV, double ffff, int32_t depth, const double* kpref,
+                           double* const* out, int32_t* rcs);
+
 /* write_core as OCEAN 3.2.1 writes absspct (OCEAN2/OCEAN_haydock.f90:1801-1894) and,
  * when gam_gauss > gam0/10, write_core_Voigt (:1896-1989).  Energies in Hartree except
  * ebase (eV, added to the printed energy as the reference does). */

@@ -108,6 +108,8 @@ _SIGS = {
     "ocean_gpu_batch_last_ms": (C.c_int, [_H, _DP]),
     "ocean_gpu_batch_haydock": (C.c_int, [_H, C.POINTER(_DP), C.POINTER(_DP), C.c_int32,
                                           C.POINTER(_DP), C.POINTER(_DP), C.POINTER(_DP), _IP]),
+    "ocean_gpu_batch_gmres": (C.c_int, [_H, C.POINTER(_DP), C.POINTER(_DP), _IP, C.POINTER(_DP), C.c_double,
+                                        C.c_double, C.c_double, C.c_int32, _DP, C.POINTER(_DP), _IP]),
 }
 
 
@@ -662,3 +664,31 @@ class OceanBatch:
         ms = C.c_double()
         _chk(_lib().ocean_gpu_batch_last_ms(self.b, C.byref(ms)), "batch_last_ms")
         return [(A[q], Bv[q], AI[q]) for q in range(n)], ms.value
+
+    def gmres(self, hay_vecs, energies_eV, gres_eV, gprc_eV, ffff, depth, kprefs=None, return_status=False):
+        """GMRES of every problem at once (ocean_gpu_batch_gmres).  energies_eV: one
+        list per problem.  Returns (rows per problem, wall ms) -- and the per-problem
+        codes with return_status (no exception then)."""
+        n = len(self.problems)
+        if len(hay_vecs) != n or len(energies_eV) != n:
+            raise ValueError("one starting vector and one energy list per problem")
+        planes = [_planes(v, g.shape, "hay_vec") for v, g in zip(hay_vecs, self.problems)]
+        Es = [np.ascontiguousarray(e, np.float64) for e in energies_eV]
+        kp = np.ascontiguousarray([g.kpref if kprefs is None else kprefs[q] for q, g in enumerate(self.problems)],
+                                  np.float64)
+        outs = [np.zeros((len(e), 6)) for e in Es]
+        nE = np.ascontiguousarray([len(e) for e in Es], np.int32)
+        pr = (_DP * n)(*[_p(x[0]) for x in planes])
+        pi = (_DP * n)(*[_p(x[1]) for x in planes])
+        pe = (_DP * n)(*[e.ctypes.data_as(_DP) for e in Es])
+        po = (_DP * n)(*[o.ctypes.data_as(_DP) for o in outs])
+        rcs = np.zeros(n, np.int32)
+        rc = _lib().ocean_gpu_batch_gmres(self.b, pr, pi, nE.ctypes.data_as(_IP), pe, gres_eV, gprc_eV, ffff,
+                                          depth, kp.ctypes.data_as(_DP), po, rcs.ctypes.data_as(_IP))
+        ms = C.c_double()
+        _chk(_lib().ocean_gpu_batch_last_ms(self.b, C.byref(ms)), "batch_last_ms")
+        if return_status:
+            return outs, ms.value, rcs
+        _chk(rc, "batch_gmres")
+        return outs, ms.value
+

@@ -49,7 +49,15 @@ struct ActArgs {
   int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
   const T2* phs;              // OBF form: supercell phase [x][k] applied around the transform
   const double2* lrbuf;       // [a][k][n]: sum_x conj(u) phi' from k_adj_lr (else the adjoint streams u)
+  const int* skip;            // nullable: when *skip != 0 every kernel of the action returns at once
+                              // (a solver that has converged on the device drops the work it
+                              // enqueued ahead)
 };
+
+template <typename T2>
+__device__ __forceinline__ bool skipped(const ActArgs<T2>& A) {
+  return A.skip != nullptr && *A.skip != 0;
+}
 
 // Problem p of a batched launch owns workgroups [off[p], off[p+1]).
 __device__ __forceinline__ int find_problem(const int* __restrict__ off, int np, int b) {
@@ -431,6 +439,7 @@ constexpr int kProjWaves = (sizeof(T2) == 16 && NA >= 6) ? 4 : 1;
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<T2, NA>))) void k_proj(
     const ActArgs<T2> A) {
+  if (skipped(A)) return;
   int b = blockIdx.x;
   const int n = gridDim.x;
   if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
@@ -443,6 +452,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<
     const ActArgs<T2>* __restrict__ As, const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2> A = As[p];
+  if (skipped(A)) return;
   const int local = blockIdx.x - off[p];
   proj_dispatch<T2, NA>(A, local);
 }
@@ -451,7 +461,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<
 // Batched 3-D DFT over the k-mesh, staged in LDS, fused with the W multiply:
 //   phi' = (1/N_k) F_- [ W(:,x) . F_+ phi ]    long_range.f90:920-939
 // F_+ = FFTW_BACKWARD (exp +2 pi i), F_- = FFTW_FORWARD, k3 fastest
-// (Modules/FFT_wrapper.f90:67-72, :185-215).  One pencil per thread, L <= 16.
+// (Modules/FFT_wrapper.f90:67-72, :185-215).  One pencil per thread; edge classes
+// LMAX = 8, 16 and 24 (radix 2 for powers of two up to 16, direct sums otherwise).
 // ---------------------------------------------------------------------------
 template <int L>
 __device__ __forceinline__ constexpr int bitrev(int i) {
@@ -625,7 +636,17 @@ __device__ __forceinline__ void dft_run_w(const T2* twp, const T2* twm, int G, i
           case 10: dft_pass_w<T2, 10>(twp, twm, G, nk, S, ld, st, wf); break;
           case 12: dft_pass_w<T2, 12>(twp, twm, G, nk, S, ld, st, wf); break;
           case 16: dft_pass_w<T2, 16>(twp, twm, G, nk, S, ld, st, wf); break;
-          default: break;  // rejected on the host
+          default:
+            if constexpr (LMAX > 16) {
+              switch (L) {
+                case 15: dft_pass_w<T2, 15>(twp, twm, G, nk, S, ld, st, wf); break;
+                case 18: dft_pass_w<T2, 18>(twp, twm, G, nk, S, ld, st, wf); break;
+                case 20: dft_pass_w<T2, 20>(twp, twm, G, nk, S, ld, st, wf); break;
+                case 24: dft_pass_w<T2, 24>(twp, twm, G, nk, S, ld, st, wf); break;
+                default: break;  // rejected on the host
+              }
+            }
+            break;
         }
       }
       break;
@@ -650,7 +671,17 @@ __device__ __forceinline__ void dft_run(const T2* tw, int G, int nk, int L, int 
           case 10: dft_pass<T2, 10, SIGN>(tw, G, nk, S, ld, st); break;
           case 12: dft_pass<T2, 12, SIGN>(tw, G, nk, S, ld, st); break;
           case 16: dft_pass<T2, 16, SIGN>(tw, G, nk, S, ld, st); break;
-          default: break;  // rejected on the host
+          default:
+            if constexpr (LMAX > 16) {
+              switch (L) {
+                case 15: dft_pass<T2, 15, SIGN>(tw, G, nk, S, ld, st); break;
+                case 18: dft_pass<T2, 18, SIGN>(tw, G, nk, S, ld, st); break;
+                case 20: dft_pass<T2, 20, SIGN>(tw, G, nk, S, ld, st); break;
+                case 24: dft_pass<T2, 24, SIGN>(tw, G, nk, S, ld, st); break;
+                default: break;  // rejected on the host
+              }
+            }
+            break;
         }
       }
       break;
@@ -759,6 +790,7 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
 
 template <typename T2, int LMAX, bool PHS = false>
 __global__ __launch_bounds__(256) void k_fft_w(const ActArgs<T2> A) {
+  if (skipped(A)) return;
   fft_body<T2, LMAX, PHS>(A, blockIdx.x);
 }
 
@@ -767,6 +799,7 @@ __global__ __launch_bounds__(256) void k_fft_w_batch(const ActArgs<T2>* __restri
                                                      const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2> A = As[p];
+  if (skipped(A)) return;
   fft_body<T2, LMAX>(A, blockIdx.x - off[p]);
 }
 
@@ -1047,6 +1080,9 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   const bool fg = (flags & F_FG) != 0;
   const int nel = na * nb;
 
+  // the many-alpha FP64 epilogue stages psi(:, k, :) in LDS instead (see below)
+  const bool so_stage = kLrBuf && (flags & F_SO) && ns == 1 && na <= NA && bd % nb == 0 &&
+                        na * na + na * nb <= max(XC * NA, JR * XS * nb);
   // epilogue operands of the first kEP elements owned by this thread
   constexpr int kEP = 2;
   double2 e_psi[kEP], e_en[kEP];
@@ -1056,7 +1092,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     const int e = min(t + i * bd, nel - 1);
     const int a = e / nb, nn = e - a * nb;
     const size_t off = ((size_t)a * nk + k) * nb + nn;
-    e_psi[i] = psi[off];
+    e_psi[i] = so_stage ? make_double2(0.0, 0.0) : psi[off];
     e_en[i] = energy[off];
     e_m[i] = allow[off];
   }
@@ -1205,30 +1241,35 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   };
   if constexpr (kLrBuf) {
     // One spin, many alphas: every element e = t + i bd of this thread has band
-    // n = t % nb, so psi(b, k, n) for all b is loaded once; S (conjugated for the
-    // backward action) is staged in the reduction area, free after the last
-    // barrier, so the element loop waits on no global load beyond its own operands
-    // (same products in the same order as finish).
-    if ((flags & F_SO) && ns == 1 && na <= NA && bd % nb == 0 && na * na <= max(XC * NA, JR * XS * nb)) {
-      double2 pb[NA];
-#pragma unroll
-      for (int b = 0; b < NA; ++b) pb[b] = psi[((size_t)min(b, na - 1) * nk + k) * nb + n];
+    // n = t % nb, and the XS threads of a band all need psi(:, k, n) for the
+    // spin-orbit sum.  psi(:, k, :) (contiguous band rows of each alpha) and S
+    // (conjugated for the backward action) are staged once per workgroup in the
+    // reduction area, free after the last barrier, so every psi element is read from
+    // HBM once per k instead of once per x-split thread (same products in the same
+    // order as finish).
+    if (so_stage) {
       double2* Ss = red;
+      double2* Ps = red + na * na;  // psi(b, k, n) at Ps[b * nb + n]
       for (int i = t; i < na * na; i += bd) {
         double2 v = S[i];
         if (back) v.y = -v.y;
         Ss[i] = v;
       }
+      for (int i = t; i < na * nb; i += bd) {
+        const int b = i / nb, nn = i - b * nb;
+        Ps[i] = psi[((size_t)b * nk + k) * nb + nn];
+      }
       __syncthreads();
-      auto finish_so = [&](int e, double2 pv, double2 ev, double am) {
+      auto finish_so = [&](int e, double2 ev, double am) {
         const int a = e / nb;
         const size_t off = ((size_t)a * nk + k) * nb + n;
+        const double2 pv = Ps[a * nb + n];
         double2 inter = res[e];
         inter.x *= am;
         inter.y *= am;
 #pragma unroll
         for (int b = 0; b < NA; ++b)
-          if (b < na) cfma(inter, Ss[a * na + b], pb[b]);
+          if (b < na) cfma(inter, Ss[a * na + b], Ps[b * nb + n]);
         double2 r = inter;
         if (flags & F_E0) {
           if (back) ev.y = -ev.y;
@@ -1239,10 +1280,10 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
       };
 #pragma unroll
       for (int i = 0; i < kEP; ++i)
-        if (t + i * bd < nel) finish_so(t + i * bd, e_psi[i], e_en[i], e_m[i]);
+        if (t + i * bd < nel) finish_so(t + i * bd, e_en[i], e_m[i]);
       for (int e = t + kEP * bd; e < nel; e += bd) {
         const size_t off = ((size_t)(e / nb) * nk + k) * nb + n;
-        finish_so(e, psi[off], energy[off], allow[off]);
+        finish_so(e, energy[off], allow[off]);
       }
       return;
     }
@@ -1260,6 +1301,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
 // One problem: one workgroup per k.
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) void k_adj(const ActArgs<T2> A) {
+  if (skipped(A)) return;
   adj_body<T2, NA>(A, blockIdx.x);
 }
 
@@ -1268,6 +1310,7 @@ __global__ __launch_bounds__(512) void k_adj_batch(const ActArgs<T2>* __restrict
                                                    const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2> A = As[p];
+  if (skipped(A)) return;
   adj_body<T2, NA>(A, blockIdx.x - off[p]);
 }
 

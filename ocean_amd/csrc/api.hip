@@ -1,7 +1,10 @@
 // C-ABI of the gfx950 core-level BSE library (declared in include/ocean_gpu.h).
 #include <algorithm>
 #include <complex>
+#include <atomic>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 
 #include "action.hpp"
@@ -100,9 +103,27 @@ int na_tmpl(int acnt) {
   return 12;
 }
 
+// k-mesh edges of the LDS transform: 1..10, 12 and 16 as before, and the 2^a 3^b 5^c
+// edges 15, 18, 20 and 24 that the reference's bundled FFT also takes
+// (Modules/jlmfft.f:64-74) by direct sums in the 24-edge class.
 bool dim_ok(int L) {
-  return L == 1 || L == 2 || L == 3 || L == 4 || L == 5 || L == 6 || L == 7 || L == 8 || L == 9 ||
-         L == 10 || L == 12 || L == 16;
+  return (L >= 1 && L <= 10) || L == 12 || L == 15 || L == 16 || L == 18 || L == 20 || L == 24;
+}
+
+// Transform instantiation class of a k mesh: largest edge <= 8, <= 16 or <= 24.
+int edge_class(const int km[3]) {
+  const int m = std::max(km[0], std::max(km[1], km[2]));
+  return m <= 8 ? 8 : (m <= 16 ? 16 : 24);
+}
+
+// Launches with more than 64 KB of dynamic LDS (large k meshes, many bands) state
+// their size to the runtime first; the device limit is checked when the shape is set.
+template <typename K>
+int lds_opt_in(K* kernel, size_t bytes) {
+  if (bytes <= 65536) return OCEAN_GPU_OK;
+  OG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+  return OCEAN_GPU_OK;
 }
 
 int flags_of(const H* h, int backwards, const int32_t* hflag) {
@@ -191,19 +212,24 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   a.nfft = (a.ncol + G - 1) / G;
   a.nred = (lr && fg) ? (a.nj + og::kRedPerBlock - 1) / og::kRedPerBlock : 0;
   a.K1 = h->km[0]; a.K2 = h->km[1]; a.K3 = h->km[2];
-  g.lmax = std::max(h->km[0], std::max(h->km[1], h->km[2])) <= 8 ? 8 : 16;
+  g.lmax = edge_class(h->km);
   const size_t tile = (size_t)nk * G;
   g.sm_fft = (6 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2) +
              (size_t)((G - 1) / na + 2) * nk * sizeof(T);
   g.sm_fft = std::max(g.sm_fft, (size_t)a.nj * sizeof(double2));
+  if (g.sm_fft > h->lds_max || g.sm_proj > h->lds_max) return OCEAN_GPU_ERR_ARG;  // k mesh or bands too large
   g.nblk_fft = a.nred + a.nfft;
   // adjoint: x-splits, phi' chunk and reduction rounds sharing one LDS area
   if (nb > t.adj) return OCEAN_GPU_ERR_ARG;
   const int XS = std::max(1, t.adj / nb);
   g.adj_threads = nb * XS;
   const size_t resb = (size_t)na * nb * sizeof(double2);
-  if (resb + (size_t)XS * nb * sizeof(double2) > kLdsBudget) return OCEAN_GPU_ERR_ARG;
-  const size_t budget = kLdsBudget - resb;
+  // two adjoint workgroups per CU while the result rows fit half the LDS; past that
+  // the device's whole per-workgroup LDS (many bands at the L2,3 edge)
+  size_t lds_total = kLdsBudget;
+  if (resb + (size_t)XS * nb * sizeof(double2) > lds_total) lds_total = h->lds_max;
+  if (resb + (size_t)XS * nb * sizeof(double2) > lds_total) return OCEAN_GPU_ERR_ARG;
+  const size_t budget = lds_total - resb;
   const size_t per_j = (size_t)XS * nb * sizeof(double2);
   a.JR = (int)std::max<size_t>(1, std::min<size_t>(acnt, budget / per_j));
   int XC = nx;
@@ -245,20 +271,27 @@ Tiles tiles_for(const H* h) {
 
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
-           const typename og::Scalar<T2>::T* W, hipEvent_t* ev) {
+           const typename og::Scalar<T2>::T* W, hipEvent_t* ev, const int* skip) {
   og::ActArgs<T2> a;
   Geom<T2> g;
   const Tiles tl = tiles_for<T2, NA>(h);
   OG_TRY((make_args<T2, NA>(h, in, out, flags, tl, u, uT, phi, W, a, g)));
+  a.skip = skip;
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
+    OG_TRY(lds_opt_in(og::k_proj<T2, NA>, g.sm_proj));
     og::k_proj<T2, NA><<<g.nblk_proj, tl.proj, g.sm_proj, h->stream>>>(a);
     OG_CHECK(hipGetLastError());
     if (ev) OG_CHECK(hipEventRecord(ev[1], h->stream));
-    if (g.lmax == 8)
+    if (g.lmax == 8) {
       og::k_fft_w<T2, 8><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
-    else
+    } else if (g.lmax == 16) {
+      OG_TRY(lds_opt_in(og::k_fft_w<T2, 16>, g.sm_fft));
       og::k_fft_w<T2, 16><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+    } else {
+      OG_TRY(lds_opt_in(og::k_fft_w<T2, 24>, g.sm_fft));
+      og::k_fft_w<T2, 24><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+    }
     OG_CHECK(hipGetLastError());
     if (ev) OG_CHECK(hipEventRecord(ev[2], h->stream));
   } else if (ev) {
@@ -280,10 +313,11 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     if ((flags & og::F_LR) && h->lrbuf && h->ns == 1 && h->na == NA) {
       a.lrbuf = h->lrbuf;
       og::k_adj_lr<T2, NA><<<dim3((h->nb + 63) / 64, h->nk), 256, 0, h->stream>>>(
-          uT, h->ldT, phi, h->ldphi, h->lrbuf, h->nb, h->nk, h->nx);
+          uT, h->ldT, phi, h->ldphi, h->lrbuf, h->nb, h->nk, h->nx, skip);
       OG_CHECK(hipGetLastError());
     }
   }
+  OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
   og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
@@ -302,7 +336,7 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
 // accumulated into (:445-446).  Timing events: proj = band-side projection, fft =
 // the x side (both obf products and the transform), adj = the adjoint.
 template <int NA>
-int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev) {
+int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev, const int* skip) {
   flags &= ~og::F_FG;
   og::ActArgs<double2> a, ab;
   Geom<double2> g, gb;
@@ -312,6 +346,8 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   const BandSide bs{h->no, h->ldu_o, h->ldT_o, h->ldb};
   OG_TRY((make_args<double2, NA>(h, in, out, flags, tl, h->obf_u, h->obf_uT, h->obf_beta, nullptr, ab, gb,
                                  &bs)));
+  a.skip = skip;
+  ab.skip = skip;
   const int nk = h->nk, nx = h->nx, no = h->no;
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   og::k_proj<double2, NA><<<gb.nblk_proj, tl.proj, gb.sm_proj, h->stream>>>(ab);
@@ -322,10 +358,15 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
   og::k_obf_fwd<NA, 2><<<dim3((nx + 63) / 64, (ntiles + 7) / 8), 256, 0, h->stream>>>(
       h->obf_beta, h->ldb, h->obf_f, h->phi64, h->ldphi, nk, nx, no);
   OG_CHECK(hipGetLastError());
-  if (g.lmax == 8)
+  if (g.lmax == 8) {
     og::k_fft_w<double2, 8, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
-  else
+  } else if (g.lmax == 16) {
+    OG_TRY(lds_opt_in(og::k_fft_w<double2, 16, true>, g.sm_fft));
     og::k_fft_w<double2, 16, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+  } else {
+    OG_TRY(lds_opt_in(og::k_fft_w<double2, 24, true>, g.sm_fft));
+    og::k_fft_w<double2, 24, true><<<g.nblk_fft, kBlock, g.sm_fft, h->stream>>>(a);
+  }
   OG_CHECK(hipGetLastError());
   og::k_obf_bwd<NA, 1><<<dim3((no + 63) / 64, ntiles), 256, 0, h->stream>>>(
       h->phi64, h->ldphi, h->obf_fT, h->obf_beta, h->ldb, nk, nx, no, -(double)nk);
@@ -334,26 +375,28 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev)
     OG_CHECK(hipEventRecord(ev[2], h->stream));
     OG_CHECK(hipEventRecord(ev[3], h->stream));
   }
+  OG_TRY(lds_opt_in(og::k_adj<double2, NA>, gb.sm_adj));
   og::k_adj<double2, NA><<<gb.nblk_adj, gb.adj_threads, gb.sm_adj, h->stream>>>(ab);
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
   return OCEAN_GPU_OK;
 }
 
-int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr) {
+int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr,
+             const int* skip = nullptr) {
   const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
   const int NA = na_tmpl(acnt);
   const bool mixed = (h->precision == OCEAN_GPU_MIXED);
   if (h->obf_on && (flags & og::F_LR)) {
     if (mixed || h->ns != 1) return OCEAN_GPU_ERR_STATE;
-    if (NA == 4 && h->na == 4) return xact_obf_t<4>(h, in, out, flags, ev);
-    if (NA == 12 && h->na == 12) return xact_obf_t<12>(h, in, out, flags, ev);
+    if (NA == 4 && h->na == 4) return xact_obf_t<4>(h, in, out, flags, ev, skip);
+    if (NA == 12 && h->na == 12) return xact_obf_t<12>(h, in, out, flags, ev, skip);
     return OCEAN_GPU_ERR_ARG;
   }
 #define OG_DISPATCH(NAV)                                                              \
   if (NA == NAV)                                                                      \
-    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev) \
-                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev);
+    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev, skip) \
+                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev, skip);
   OG_DISPATCH(2)
   OG_DISPATCH(4)
   OG_DISPATCH(6)
@@ -417,10 +460,12 @@ int dev_nrm(H* h, const double2* p, double* v) {
 // <W_i|v1>, <W_i|v2> for i < m: partials [blk][i][4].  Eight W rows per pass with
 // all their loads in flight; the 32 sums of a pass reduce by one butterfly per
 // wave and one LDS round across the waves.
-__global__ __launch_bounds__(kBlock) void k_multidot2(const double2* const* __restrict__ Wl, int m,
+__global__ __launch_bounds__(kBlock) void k_multidot2(const int* __restrict__ skip,
+                                                      const double2* const* __restrict__ Wl, int m,
                                                       const double2* __restrict__ v1,
                                                       const double2* __restrict__ v2, size_t n,
                                                       double* __restrict__ part) {
+  if (*skip) return;
   constexpr int R = 8;
   __shared__ double lds[(kBlock / 64) * 4 * R];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -457,7 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_multidot2(const double2* const* __re
 
 // out[j] = sum_b part[b*nv + j]: one wave per value, lane-strided partial sums
 // then a butterfly (fixed order, so every run gives the same bits).
-__global__ void k_multifinish(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
+__global__ void k_multifinish(const int* __restrict__ skip, const double* __restrict__ part, int nblk, int nv,
+                              double* __restrict__ out) {
+  if (*skip) return;
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= nv) return;
@@ -467,8 +514,9 @@ __global__ void k_multifinish(const double* __restrict__ part, int nblk, int nv,
 
 // y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341),
 // eight W rows per step with their loads in flight, summed in i order.
-__global__ void k_multiaxpy(const double2* const* __restrict__ Wl, const double2* __restrict__ c, int m,
-                            double2* __restrict__ y, size_t n) {
+__global__ void k_multiaxpy(const int* __restrict__ skip, const double2* const* __restrict__ Wl,
+                            const double2* __restrict__ c, int m, double2* __restrict__ y, size_t n) {
+  if (*skip) return;
   constexpr int R = 8;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     double2 yv = y[e];
@@ -514,9 +562,11 @@ __device__ __forceinline__ double2 ls_bcast(const double2 (&v)[kLsSlots], int k)
   return make_double2(__shfl(s.x, src, 64), __shfl(s.y, src, 64));
 }
 
-__global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin, double2* __restrict__ L,
-                                                  double* __restrict__ D, double2* __restrict__ coef, int cur,
-                                                  int depth, int* __restrict__ info) {
+__global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const double* __restrict__ fin,
+                                                  double2* __restrict__ L, double* __restrict__ D,
+                                                  double2* __restrict__ coef, int cur, int depth,
+                                                  int* __restrict__ info) {
+  if (*skip) return;
   __shared__ double2 Ls[kLsPacked];
   __shared__ double Dl[kMaxGmresDepth];
   const int lane = threadIdx.x;
@@ -575,7 +625,10 @@ __global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin
   for (int k = 0; k <= j; ++k)
     if (Dl[k] == 0.0) { bad = k + 1; break; }
   if (bad) {
-    if (lane == 0) *info = bad;
+    if (lane == 0) {
+      *info = bad;
+      *skip = 2;  // the solve stops here: later enqueued work is dropped
+    }
     for (int i = lane; i < cur; i += 64) coef[i] = make_double2(0.0, 0.0);
     return;
   }
@@ -625,7 +678,9 @@ __global__ __launch_bounds__(64) void k_ls_update(const double* __restrict__ fin
 }
 
 // y = (er + i ei) x - y  (OCEAN_psi_axmy, OCEAN_psi.f90:1468-1562)
-__global__ void k_axmy(const double2* __restrict__ x, double2* __restrict__ y, double er, double ei, size_t n) {
+__global__ void k_axmy(const int* __restrict__ skip, const double2* __restrict__ x, double2* __restrict__ y,
+                       double er, double ei, size_t n) {
+  if (*skip) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     const double2 xv = x[e];
     double2 yv = y[e];
@@ -638,15 +693,18 @@ __global__ void k_axmy(const double2* __restrict__ x, double2* __restrict__ y, d
 }
 
 // z = x .* y (complex)  (OCEAN_psi_3element_mult)
-__global__ void k_cmul3(double2* __restrict__ z, const double2* __restrict__ x, const double2* __restrict__ y, size_t n) {
+__global__ void k_cmul3(const int* __restrict__ skip, double2* __restrict__ z, const double2* __restrict__ x,
+                        const double2* __restrict__ y, size_t n) {
+  if (*skip) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
     z[e] = og::cmul(x[e], y[e]);
 }
 
 // g = ax - rhs ; partials of |g|^2 and <rhs|x>
-__global__ __launch_bounds__(kBlock) void k_resid(const double2* __restrict__ ax, const double2* __restrict__ rhs,
-                                                  const double2* __restrict__ x, double2* __restrict__ g, size_t n,
-                                                  double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_resid(const int* __restrict__ skip, const double2* __restrict__ ax,
+                                                  const double2* __restrict__ rhs, const double2* __restrict__ x,
+                                                  double2* __restrict__ g, size_t n, double* __restrict__ part) {
+  if (*skip) return;
   __shared__ double lds[4 * 3];
   double v[3] = {0, 0, 0};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -664,7 +722,9 @@ __global__ __launch_bounds__(kBlock) void k_resid(const double2* __restrict__ ax
 }
 
 // pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2)  (OCEAN_psi_min_set_prec, :2907-2994)
-__global__ void k_set_prec(const double2* __restrict__ h1, double2* __restrict__ pc, double E, double gprc, size_t n) {
+__global__ void k_set_prec(const int* __restrict__ skip, const double2* __restrict__ h1, double2* __restrict__ pc,
+                           double E, double gprc, size_t n) {
+  if (*skip) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     const double2 hv = h1[e];
     const double dr = E - hv.x, di = -hv.y;
@@ -680,6 +740,64 @@ __global__ void k_mask(double2* __restrict__ y, const double2* __restrict__ x, c
     double2 v = x[e];
     v.x *= m[e]; v.y *= m[e];
     y[e] = v;
+  }
+}
+
+// GMRES control state on the device: the solve of one energy enqueues several
+// iterations ahead and every kernel of them returns at once when `done` is set, so
+// the host reads the state back once per block of iterations, not per iteration.
+struct GmCtl {
+  int done;     // 0 running, 1 converged, 2 least-squares pivot failure
+  int conv_it;  // the iteration (1..depth of its cycle) at which it converged
+  int info;     // zhptrf-style pivot index of a failure
+  int pad;
+  double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
+};
+
+__global__ void k_gm_reset(GmCtl* __restrict__ c) {
+  if (threadIdx.x == 0) {
+    c->done = 0;
+    c->conv_it = 0;
+  }
+}
+
+// y += (ar + i ai) x unless the solve has stopped (recycling step of make_recycle)
+__global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai, const double2* __restrict__ x,
+                             double2* __restrict__ y, size_t n) {
+  if (*skip) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 xv = x[i];
+    double2 yv = y[i];
+    yv.x += ar * xv.x; yv.y += ar * xv.y;
+    yv.x += -ai * xv.y; yv.y += ai * xv.x;
+    y[i] = yv;
+  }
+}
+
+// The three residual sums (same order as k_finish) and, with check, the
+// reference's stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and
+// (-|g|^2 / Im<rhs|x> < 0.001 or |g|^2 < ffff / 10).
+__global__ __launch_bounds__(256) void k_resid_fin(const double* __restrict__ part, int nblk, GmCtl* __restrict__ c,
+                                                   double ffff, int it, int check) {
+  __shared__ double sv[3];
+  if (c->done) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (w < 3) {
+    const double v = og::wave_sum_parts(part, nblk, 3, w, lane);
+    if (lane == 0) sv[w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c->gval = sv[0];
+    c->rval = sv[1];
+    c->ival = sv[2];
+    if (check && sv[0] < ffff) {
+      const double rel = -sv[0] / sv[2];
+      if (rel < 0.001 || sv[0] < ffff / 10.0) {
+        c->done = 1;
+        c->conv_it = it;
+      }
+    }
   }
 }
 
@@ -722,9 +840,18 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   h->ldu = (h->nx + 33) & ~1;  // even, so FP32 rows pair up into 16-byte loads
   h->ldphi = h->nx * h->na + 32;
   h->ldT = h->nx * h->nb + 32;
-  // the adjoint keeps one k's result (nalpha x nb complex) in LDS
-  if (h->nb > 512 || (size_t)h->na * h->nb * sizeof(double2) > 48 * 1024) { delete h; return OCEAN_GPU_ERR_ARG; }
   h->precision = d->precision;
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+      h->lds_max = (size_t)lds;
+  }
+  // the adjoint keeps one k's result (nalpha x nb complex) and one x-split row in LDS
+  if (h->nb > 512 || (size_t)(h->na + 1) * h->nb * sizeof(double2) > h->lds_max) {
+    delete h;
+    return OCEAN_GPU_ERR_ARG;
+  }
   int P = 0;
   for (int l = d->lmin; l <= d->lmax; ++l) {
     if (d->nproj[l - d->lmin] < 0) { delete h; return OCEAN_GPU_ERR_ARG; }
@@ -1641,6 +1768,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   const size_t nv = h->nvec();
   const int grid = grid_for(nv);
   const double gres = gres_eV * kEV2Hartree, gprc = gprc_eV * kEV2Hartree;
+  hipStream_t st = h->stream;
   // vectors: rhs, h1, x, ax, g, pcdiv, u[1..depth], au[1..depth]
   const int nfix = 6;
   std::vector<double2*> V(nfix + 2 * depth, nullptr);
@@ -1655,100 +1783,107 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 16);
   double2* dL = nullptr;
   double* dD = nullptr;
-  int* dinfo = nullptr;
-  double* hpin = nullptr;
+  GmCtl* ctl = nullptr;
+  GmCtl* hctl = nullptr;
   if (rc == 0) rc = dmalloc(&dL, (size_t)depth * depth);
   if (rc == 0) rc = dmalloc(&dD, (size_t)depth);
-  if (rc == 0) rc = dmalloc(&dinfo, (size_t)2);
-  if (rc == 0 && hipHostMalloc((void**)&hpin, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+  if (rc == 0) rc = dmalloc(&ctl, (size_t)1);
+  if (rc == 0 && hipHostMalloc((void**)&hctl, sizeof(GmCtl), hipHostMallocDefault) != hipSuccess)
     rc = OCEAN_GPU_ERR_NOMEM;
-  if (rc == 0 && hipMemsetAsync(dinfo, 0, 2 * sizeof(int), h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   auto cleanup = [&]() {
     for (auto& p : V) dfree(p);
-    dfree(dlist); dfree(dcoef); dfree(part); dfree(dL); dfree(dD); dfree(dinfo);
-    if (hpin) (void)hipHostFree(hpin);
+    dfree(dlist); dfree(dcoef); dfree(part); dfree(dL); dfree(dD); dfree(ctl);
+    if (hctl) (void)hipHostFree(hctl);
   };
   if (rc) { cleanup(); return rc; }
+  int* skip = &ctl->done;
+  int* info = &ctl->info;
   double2 *rhs = V[0], *h1 = V[1], *x = V[2], *ax = V[3], *g = V[4], *pc = V[5];
   double2** U = &V[nfix];            // U[i], i = 0..depth-1  (reference u_matrix(i+1))
   double2** AU = &V[nfix + depth];   // AU[i]
   std::vector<double2*> hlist(2 * depth);
   for (int i = 0; i < depth; ++i) { hlist[i] = AU[i]; hlist[depth + i] = U[i]; }
 
-  auto sync_check = [&]() -> int {
+  auto launched = [&]() -> int {
     if (hipGetLastError() != hipSuccess) return OCEAN_GPU_ERR_HIP;
     return OCEAN_GPU_OK;
   };
-  OG_CHECK(hipMemcpyAsync(rhs, rhs_dev, nv * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-  og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(1.0, 0.0), nv);
-  rc = sync_check();
+  OG_CHECK(hipMemcpyAsync(rhs, rhs_dev, nv * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  k_gm_reset<<<1, 64, 0, st>>>(ctl);
+  og::k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double2*>(ctl), make_double2(0.0, 0.0), sizeof(GmCtl) / 16);
+  og::k_fill<<<grid, kBlock, 0, st>>>(x, make_double2(1.0, 0.0), nv);
+  rc = launched();
   if (rc == 0) rc = xact_dev(h, x, h1, f);  // h1 = H . 1
   if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
                         ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
 
-  double gval = 0.0, rval = 0.0, ival = 0.0;
+  // outv = (E + i gres) in - H in, dropped once the solve has stopped
   auto axmy_xact = [&](double2* in, double2* outv, double E) -> int {
-    // outv = (E + i gres) in - H in
-    OG_TRY(xact_dev(h, in, outv, f));
-    k_axmy<<<grid, kBlock, 0, h->stream>>>(in, outv, E, gres, nv);
-    return sync_check();
+    OG_TRY(xact_dev(h, in, outv, f, nullptr, skip));
+    k_axmy<<<grid, kBlock, 0, st>>>(skip, in, outv, E, gres, nv);
+    return launched();
   };
-  // The only host synchronisation per iteration: the residual norm, <rhs|x> and
-  // the least-squares pivot flag of the updates since the last one.
-  auto resid = [&]() -> int {  // g = ax - rhs ; gval ; (rval, ival) = <rhs|x>
-    double* fin = part + (size_t)kRedBlocks * 4 * depth;
-    k_resid<<<kRedBlocks, kBlock, 0, h->stream>>>(ax, rhs, x, g, nv, part);
-    og::k_finish<<<1, 64, 0, h->stream>>>(part, kRedBlocks, 3, fin);
-    OG_TRY(sync_check());
-    OG_CHECK(hipMemcpyAsync(hpin, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    OG_CHECK(hipMemcpyAsync(hpin + 3, dinfo, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    OG_CHECK(hipStreamSynchronize(h->stream));
-    gval = hpin[0]; rval = hpin[1]; ival = hpin[2];
-    int info = 0;
-    std::memcpy(&info, hpin + 3, sizeof(int));
-    return info;  // zhptrf's info of a failed least-squares step, else 0
+  // g = ax - rhs and its sums; with check, the stopping test of iteration it
+  auto resid = [&](int it, int check) -> int {
+    k_resid<<<kRedBlocks, kBlock, 0, st>>>(skip, ax, rhs, x, g, nv, part);
+    k_resid_fin<<<1, 256, 0, st>>>(part, kRedBlocks, ctl, ffff, it, check);
+    return launched();
   };
-  // update_gmres(cur): Gram column cur, LS solve and x/ax updates, all on the
-  // device (OCEAN_gmres.f90:241-344)
+  // update_gmres(cur): Gram column cur, least-squares solve and x / ax updates
+  // (OCEAN_gmres.f90:241-344)
   auto update = [&](int cur) -> int {
-    k_multidot2<<<kRedBlocks, kBlock, 0, h->stream>>>((const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
+    k_multidot2<<<kRedBlocks, kBlock, 0, st>>>(skip, (const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
-    k_multifinish<<<(4 * cur + 3) / 4, 256, 0, h->stream>>>(part, kRedBlocks, 4 * cur, fin);
-    k_ls_update<<<1, 64, 0, h->stream>>>(fin, dL, dD, dcoef, cur, depth, dinfo);
-    k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)(dlist + depth), dcoef, cur, x, nv);
-    k_multiaxpy<<<grid, kBlock, 0, h->stream>>>((const double2* const*)dlist, dcoef, cur, ax, nv);
-    return sync_check();
+    k_multifinish<<<(4 * cur + 3) / 4, 256, 0, st>>>(skip, part, kRedBlocks, 4 * cur, fin);
+    k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, cur, depth, info);
+    k_multiaxpy<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), dcoef, cur, x, nv);
+    k_multiaxpy<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)dlist, dcoef, cur, ax, nv);
+    return launched();
   };
+  auto read_ctl = [&]() -> int {
+    OG_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st));
+    OG_CHECK(hipStreamSynchronize(st));
+    return hctl->done == 2 ? (hctl->info > 0 ? hctl->info : OCEAN_GPU_ERR_HIP) : OCEAN_GPU_OK;
+  };
+  auto set_prec = [&](double E) -> int {
+    k_set_prec<<<grid, kBlock, 0, st>>>(skip, h1, pc, E, gprc, nv);
+    return launched();
+  };
+  // iterations enqueued ahead of each read-back of the device state
+  constexpr int kAhead = 8;
 
   int global_iter = 0, prev_iter = 0;
   double prev_energy = -1e300;
   for (int step = 0; step < nE && rc == 0; ++step) {
     int complete_iter = 0;
     const double E = energies_eV[step] * kEV2Hartree;
+    k_gm_reset<<<1, 64, 0, st>>>(ctl);
+    if ((rc = launched())) break;
     if (prev_iter == 0) {
       prev_energy = E;
-      k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
-      if ((rc = sync_check())) break;
+      if ((rc = set_prec(E))) break;
     }
     bool converged = false;
     if (prev_iter == 0) {
       if (step > 0) {
         if ((rc = axmy_xact(x, ax, E))) break;
       } else {
-        og::k_fill<<<grid, kBlock, 0, h->stream>>>(x, make_double2(0.0, 0.0), nv);
-        og::k_fill<<<grid, kBlock, 0, h->stream>>>(ax, make_double2(0.0, 0.0), nv);
+        og::k_fill<<<grid, kBlock, 0, st>>>(x, make_double2(0.0, 0.0), nv);
+        og::k_fill<<<grid, kBlock, 0, st>>>(ax, make_double2(0.0, 0.0), nv);
       }
-      if ((rc = resid())) break;
+      if ((rc = resid(0, 0))) break;
     } else {
+      // make_recycle: au += (E - E_prev) u, then the updates over the kept basis
       const double delta = E - energies_eV[step - 1] * kEV2Hartree;
-      for (int i = 0; i < prev_iter; ++i) og::k_axpy<<<grid, kBlock, 0, h->stream>>>(delta, 0.0, U[i], AU[i], nv);
+      for (int i = 0; i < prev_iter; ++i) k_axpy_gated<<<grid, kBlock, 0, st>>>(skip, delta, 0.0, U[i], AU[i], nv);
       if ((rc = axmy_xact(x, ax, E))) break;
-      if ((rc = resid())) break;
+      if ((rc = resid(0, 0))) break;
       for (int it = 1; it <= prev_iter && rc == 0; ++it) {
         if ((rc = update(it))) break;
-        rc = resid();
+        rc = resid(0, 0);
       }
-      if (rc) break;
+      if (rc || (rc = read_ctl())) break;
+      const double gval = hctl->gval, ival = hctl->ival;
       if (gval < ffff) {
         const double rel = -gval / ival;
         if (rel < 0.001 || gval < ffff / 10.0) converged = true;
@@ -1756,8 +1891,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
       if (!converged && std::fabs(prev_energy - E) > std::fabs(gprc) / 4.0) {
         prev_iter = 0;
         prev_energy = E;
-        k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
-        if ((rc = sync_check())) break;
+        if ((rc = set_prec(E))) break;
       }
     }
     while (!converged && rc == 0) {
@@ -1765,33 +1899,40 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         rc = OCEAN_GPU_ERR_NOCONV;
         break;
       }
-      for (int it = prev_iter + 1; it <= depth; ++it) {
-        ++complete_iter;
-        ++global_iter;
-        k_cmul3<<<grid, kBlock, 0, h->stream>>>(U[it - 1], g, pc, nv);
-        if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
-        if ((rc = update(it))) break;
-        if (it == depth) {
-          if ((rc = axmy_xact(x, ax, E))) break;
-          if (E != prev_energy) {
-            k_set_prec<<<grid, kBlock, 0, h->stream>>>(h1, pc, E, gprc, nv);
-            prev_energy = E;
+      const int cycle_start = prev_iter;
+      const bool new_prec_at_end = (E != prev_energy);
+      int it = prev_iter + 1;
+      while (it <= depth && rc == 0) {
+        const int iend = std::min(depth, it + kAhead - 1);
+        for (; it <= iend && rc == 0; ++it) {
+          k_cmul3<<<grid, kBlock, 0, st>>>(skip, U[it - 1], g, pc, nv);
+          if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
+          if ((rc = update(it))) break;
+          if (it == depth) {
+            if ((rc = axmy_xact(x, ax, E))) break;
+            if (new_prec_at_end && (rc = set_prec(E))) break;
           }
-          prev_iter = 0;
+          rc = resid(it, 1);
         }
-        if ((rc = resid())) break;
-        if (gval < ffff) {
-          const double rel = -gval / ival;
-          if (rel < 0.001 || gval < ffff / 10.0) {
-            prev_iter = (it == depth) ? 0 : it;
-            converged = true;
-            break;
-          }
+        if (rc || (rc = read_ctl())) break;
+        if (hctl->done == 1) {
+          const int c = hctl->conv_it;
+          complete_iter += c - cycle_start;
+          global_iter += c - cycle_start;
+          if (c == depth) prev_energy = E;
+          prev_iter = (c == depth) ? 0 : c;
+          converged = true;
+          break;
         }
       }
-      if (!converged) prev_iter = 0;
+      if (rc || converged) break;
+      complete_iter += depth - cycle_start;
+      global_iter += depth - cycle_start;
+      prev_energy = E;
+      prev_iter = 0;
     }
     if (rc) break;
+    const double gval = hctl->gval, rval = hctl->rval, ival = hctl->ival;
     double* o = out + 6 * (size_t)step;
     o[0] = energies_eV[step];
     o[1] = 1.0 - rval * kpref;
@@ -1801,10 +1942,10 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     o[5] = global_iter;
     if (x_re && x_im) rc = get_planes(h, x, x_re + step * nv, x_im + step * nv, nv);
     if (xs_dev && rc == 0)
-      rc = hipMemcpyAsync(xs_dev + step * nv, x, nv * sizeof(double2), hipMemcpyDeviceToDevice, h->stream) == hipSuccess
+      rc = hipMemcpyAsync(xs_dev + step * nv, x, nv * sizeof(double2), hipMemcpyDeviceToDevice, st) == hipSuccess
                ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
   }
-  if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0 && hipStreamSynchronize(st) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   cleanup();
   return rc;
 }
@@ -1875,10 +2016,15 @@ int grad_t(H* h, int nE, const double2* xs, const double2* ys, const std::vector
         og::k_proj<T2, NA><<<g.nblk_proj, tl.proj, g.sm_proj, h->stream>>>(a);
         const size_t tile = (size_t)h->nk * a.G;
         const size_t smf = (3 * (size_t)g.lmax + tile + tile / 8 + 1) * sizeof(T2);
-        if (g.lmax == 8)
+        if (g.lmax == 8) {
           og::k_fft_fwd<T2, 8><<<a.nfft, kBlock, smf, h->stream>>>(a);
-        else
+        } else if (g.lmax == 16) {
+          if ((rc = lds_opt_in(og::k_fft_fwd<T2, 16>, smf))) break;
           og::k_fft_fwd<T2, 16><<<a.nfft, kBlock, smf, h->stream>>>(a);
+        } else {
+          if ((rc = lds_opt_in(og::k_fft_fwd<T2, 24>, smf))) break;
+          og::k_fft_fwd<T2, 24><<<a.nfft, kBlock, smf, h->stream>>>(a);
+        }
         if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
       }
       if (rc) break;
@@ -2096,19 +2242,27 @@ template <typename T2, int NA>
 int launch_tables(const Tables& tb, hipStream_t st, int stages) {
   const auto* A = static_cast<const og::ActArgs<T2>*>(tb.args);
   const int np = tb.np;
-  if (stages & 1)
+  if (stages & 1) {
+    OG_TRY(lds_opt_in(og::k_proj_batch<T2, NA>, tb.sm[0]));
     og::k_proj_batch<T2, NA><<<tb.nblk[0], tb.threads[0], tb.sm[0], st>>>(A, tb.off, np);
+  }
   if (stages & 2) {
-    if (tb.lmax == 8)
+    if (tb.lmax == 8) {
       og::k_fft_w_batch<T2, 8><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
-    else
+    } else if (tb.lmax == 16) {
+      OG_TRY(lds_opt_in(og::k_fft_w_batch<T2, 16>, tb.sm[1]));
       og::k_fft_w_batch<T2, 16><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
+    } else {
+      OG_TRY(lds_opt_in(og::k_fft_w_batch<T2, 24>, tb.sm[1]));
+      og::k_fft_w_batch<T2, 24><<<tb.nblk[1], kBlock, tb.sm[1], st>>>(A, tb.off + (np + 1), np);
+    }
   }
   if (stages & 4) {
     if constexpr (NA >= 6 && std::is_same<T2, double2>::value) {
       if (tb.nblk[3] > 0)
         og::k_adj_lr_batch<T2, NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
     }
+    OG_TRY(lds_opt_in(og::k_adj_batch<T2, NA>, tb.sm[2]));
     og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
   }
   OG_CHECK(hipGetLastError());
@@ -2191,7 +2345,7 @@ int ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_bat
     B::Group key;
     key.prec = h->precision;
     key.NA = na_tmpl(acnt);
-    key.lmax = std::max(h->km[0], std::max(h->km[1], h->km[2])) <= 8 ? 8 : 16;
+    key.lmax = edge_class(h->km);
     auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const B::Group& g) {
       return g.prec == key.prec && g.NA == key.NA && g.lmax == key.lmax;
     });
@@ -2457,3 +2611,46 @@ int ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, con
   for (auto& s : Sd) dfree(s);
   return rc ? rc : rc_nan;
 }
+
+// GMRES of every problem of the batch at once: each problem's recycled solve runs on
+// its own handle's stream, driven by one of up to kGmresWorkers host threads, so the
+// solves of small problems overlap on the device and each solve's host read-backs
+// (one per block of iterations) overlap the others' kernels.
+int ocean_gpu_batch_gmres(ocean_gpu_batch* b, const double* const* hay_re, const double* const* hay_im,
+                          const int32_t* nE, const double* const* energies_eV, double gres_eV, double gprc_eV,
+                          double ffff, int32_t depth, const double* kpref, double* const* out, int32_t* rcs) {
+  if (!b || !hay_re || !hay_im || !nE || !energies_eV || !kpref || !out || depth <= 0) return OCEAN_GPU_ERR_ARG;
+  const int n = (int)b->hs.size();
+  for (int q = 0; q < n; ++q)
+    if (!hay_re[q] || !hay_im[q] || nE[q] <= 0 || !energies_eV[q] || !out[q]) return OCEAN_GPU_ERR_ARG;
+  constexpr int kGmresWorkers = 16;
+  std::vector<int> rc(n, OCEAN_GPU_OK);
+  std::atomic<int> next{0};
+  const int dev = b->hs[0]->device;
+  auto worker = [&]() {
+    if (hipSetDevice(dev) != hipSuccess) return;
+    for (int q = next++; q < n; q = next++) {
+      H* h = b->hs[q];
+      double2* rhs = nullptr;
+      int r = gmres_rhs(h, hay_re[q], hay_im[q], &rhs);
+      if (r == 0) r = gmres_run(h, rhs, nE[q], energies_eV[q], gres_eV, gprc_eV, ffff, depth, kpref[q], out[q],
+                                nullptr, nullptr, nullptr);
+      dfree(rhs);
+      rc[q] = r;
+    }
+  };
+  for (int q = 0; q < n; ++q) rc[q] = OCEAN_GPU_ERR_HIP;  // until its worker reports
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  const int nw = std::min(n, kGmresWorkers);
+  for (int w = 0; w < nw; ++w) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  b->t_last = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  int first = OCEAN_GPU_OK;
+  for (int q = 0; q < n; ++q) {
+    if (rcs) rcs[q] = rc[q];
+    if (first == OCEAN_GPU_OK && rc[q] != OCEAN_GPU_OK) first = rc[q];
+  }
+  return first;
+}
+

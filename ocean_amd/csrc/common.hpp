@@ -32,7 +32,7 @@ namespace og {
 constexpr int kBlock = 256;      // threads per workgroup for streaming kernels
 constexpr int kRedBlocks = 256;  // fixed partial count: deterministic reductions
 constexpr int kMaxAlpha = 12;    // nalpha <= 4 (2 lc + 1) with lc <= 1
-constexpr int kMaxDim = 16;      // largest k-mesh edge handled by the LDS DFT
+constexpr int kMaxDim = 24;      // largest k-mesh edge handled by the LDS DFT
 
 // Tile shapes of the action kernels (defaults; the autotuner picks per batch group).
 struct Tiles {
@@ -185,6 +185,7 @@ struct ocean_gpu_handle {
   int ldT = 0;      // uT k-slab stride (elements), >= nx * nb
   int precision = OCEAN_GPU_FP64;
   int device = 0;
+  size_t lds_max = 65536;  // the device's LDS bytes per workgroup (160 KB on gfx950)
   hipStream_t stream = nullptr;
 
   // long-range state

@@ -161,7 +161,9 @@ __device__ __forceinline__ void adj_lr_body(const T2* __restrict__ uT, int ldT, 
 
 template <typename T2, int NA>
 __global__ __launch_bounds__(256) void k_adj_lr(const T2* __restrict__ uT, int ldT, const T2* __restrict__ phi,
-                                                int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx) {
+                                                int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx,
+                                                const int* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
   adj_lr_body<T2, NA>(uT, ldT, phi, ldphi, lrbuf, nb, nk, nx, blockIdx.x, blockIdx.y);
 }
 
@@ -172,6 +174,7 @@ __global__ __launch_bounds__(256) void k_adj_lr_batch(const ActArgs<T2>* __restr
                                                       const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
   const ActArgs<T2>& A = As[p];
+  if (skipped(A)) return;
   const int nbb = (A.nb + 63) / 64;
   const int loc = blockIdx.x - off[p];
   adj_lr_body<T2, NA>(A.uT, A.ldT, A.phi, A.ldphi, const_cast<double2*>(A.lrbuf), A.nb, A.nk, A.nx, loc % nbb,

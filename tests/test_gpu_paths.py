@@ -5,7 +5,9 @@ Against the numpy oracle on seeded inputs:
     multiplet partials split over tiles), with nb > 64 (two 64-band blocks of the
     matrix-core adjoint at nalpha 12) and nb = 128 (the FP32 matrix-core B staging);
   * k-mesh edges above 8 (the 16-edge transform class: radix 2 at 16, direct sums
-    at 9, 10 and 12);
+    at 9, 10 and 12; the 24-edge class: direct sums at 15, 18, 20 and 24) and a 16^3
+    mesh whose transform strip needs more than 64 KB of LDS;
+  * 400 bands at the L2,3 edge (adjoint rows and FP32 B staging past 64 KB of LDS);
   * GMRES at the reference's default depth of 80 (cnbse.gmres.nloop,
     Common/oparse.json:16) and at 100, with restarts inside one energy.
 Failure returns (each ends normally on the host): NaN in the starting vector -> -1
@@ -28,6 +30,9 @@ WIDE = {
     "l23_x512_nb72": dict(nb=72, kmesh=(2, 2, 2), xmesh=(8, 8, 8), lc=1, seed=42, xi_eV=5.0),
     "l23_x512_nb128": dict(nb=128, kmesh=(2, 1, 2), xmesh=(8, 8, 8), lc=1, seed=43, xi_eV=5.0),
     "k_x1000_nb40": dict(nb=40, kmesh=(2, 2, 1), xmesh=(10, 10, 10), lc=0, seed=44),
+    # 400 bands at the L2,3 edge: the adjoint's result rows and the FP32 matrix-core
+    # B staging both need more than 64 KB of LDS
+    "l23_nb400": dict(nb=400, kmesh=(2, 1, 1), xmesh=(8, 8, 7), lc=1, seed=45, xi_eV=3.0),
 }
 
 EDGES = {
@@ -35,6 +40,13 @@ EDGES = {
     "k12": dict(nb=8, kmesh=(2, 12, 3), xmesh=(3, 2, 3), lc=0, seed=52),
     "k9_l23": dict(nb=8, kmesh=(9, 1, 2), xmesh=(3, 3, 2), lc=1, seed=53, xi_eV=2.0),
     "k10": dict(nb=8, kmesh=(2, 1, 10), xmesh=(3, 3, 3), lc=0, seed=54),
+    # the 2^a 3^b 5^c edges of the 24-edge class (direct sums)
+    "k24": dict(nb=6, kmesh=(24, 2, 1), xmesh=(3, 2, 2), lc=0, seed=55),
+    "k20_l23": dict(nb=6, kmesh=(1, 20, 2), xmesh=(2, 3, 2), lc=1, seed=56, xi_eV=2.0),
+    "k15_18": dict(nb=4, kmesh=(15, 1, 18), xmesh=(2, 2, 2), lc=0, seed=57),
+    # a 16^3 mesh: one transform column is 64 KB of FP64, so the strip needs the
+    # device's larger LDS
+    "k16cube": dict(nb=2, kmesh=(16, 16, 16), xmesh=(2, 2, 2), lc=0, seed=58),
 }
 
 
@@ -64,7 +76,7 @@ def test_action_matches_oracle(name):
             assert err < 1e-5, (name, back, err)
 
 
-@pytest.mark.parametrize("name", ["k16", "l23_x512_nb72"])
+@pytest.mark.parametrize("name", ["k16", "l23_x512_nb72", "k24"])
 def test_haydock_matches_oracle(name):
     inp = synth.make_inputs(**dict(WIDE, **EDGES)[name])
     P = to_problem(inp)
