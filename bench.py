@@ -313,6 +313,10 @@ def main():
     ap.add_argument("--no-obf", action="store_true", help="skip the OBF long-range form")
     ap.add_argument("--obf-only", action="store_true",
                     help="only the OBF form's H.psi rate and kernel times (for profiling)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only the mixed form's H.psi rate and kernel times at the 64-band K edge (for profiling)")
+    ap.add_argument("--gmres-only", action="store_true",
+                    help="only GMRES at 64 energies at the 64-band K edge (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -335,6 +339,21 @@ def main():
         return
     if args.obf_only:
         print(json.dumps(obf_workload(reps=args.steps)), flush=True)
+        return
+    if args.mixed_only or args.gmres_only:
+        from ocean_amd import core, synth
+        inp = synth.workload("k_nb64")
+        g = core.OceanGPU.from_inputs(inp, precision=core.MIXED if args.mixed_only else core.FP64)
+        if args.gmres_only:
+            out = gmres_workload(inp, g)
+        else:
+            vin, vout = g.vec_new(inp.psi0), g.vec_new()
+            g.xact_repeat(vin, vout, max(args.warmup, 1))
+            ms = g.xact_repeat(vin, vout, args.steps)
+            out = {"hpsi_per_s_mixed": round(1e3 / ms, 1),
+                   "per_kernel_ms_mixed": {k: round(v, 5) for k, v in g.profile_xact(vin, vout, args.steps).items()}}
+        g.close()
+        print(json.dumps(out), flush=True)
         return
 
     from ocean_amd import core, synth

@@ -353,6 +353,15 @@ int  ocean_gpu_ospect(const double* real_a, const double* real_b, int32_t iter, 
  * abslanc (OCEAN2/OCEAN_haydock.f90:2273-2316, :2354-2356).  w has n+1 entries. */
 int  ocean_gpu_tridiag_eigs(const double* real_a, const double* real_b, int32_t n, double* w);
 
+/* Tile shapes of one problem's action (no reference counterpart): the batch
+ * autotuner run on that problem alone (ocean_gpu_batch_autotune, reps launches per
+ * candidate) and its picks kept by the handle; tiles[3] = (projection workgroup,
+ * adjoint thread budget, transform strip width, 0 = widest) as the handle stores
+ * them (the FP32 projection runs half as many threads per x at four alphas or fewer
+ * and twice as many at more). */
+int  ocean_gpu_autotune(ocean_gpu_handle* h, int32_t reps);
+int  ocean_gpu_get_tiles(ocean_gpu_handle* h, int32_t* tiles);
+
 /* Average durations (ms) measured with HIP events on the library's stream:
  * "total" from the last ocean_gpu_xact_vec_repeat, "proj", "fft", "mult", "adj"
  * from the last ocean_gpu_profile_xact. */

@@ -95,6 +95,8 @@ _SIGS = {
     "ocean_gpu_tridiag_eigs": (C.c_int, [_DP, _DP, C.c_int32, _DP]),
     "ocean_gpu_ospect": (C.c_int, [_DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP, _IP]),
     "ocean_gpu_last_timing": (C.c_int, [_H, C.c_char_p, _DP]),
+    "ocean_gpu_autotune": (C.c_int, [_H, C.c_int32]),
+    "ocean_gpu_get_tiles": (C.c_int, [_H, _IP]),
     "ocean_gpu_load_bloch_dev": (C.c_int, [_H, C.c_void_p, C.c_void_p]),
     "ocean_gpu_spectral_grad": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, C.c_double, C.c_double,
                                           C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP, _DP,
@@ -500,6 +502,16 @@ class OceanGPU:
         """nrep back-to-back actions; returns the event-timed ms per action."""
         _chk(_lib().ocean_gpu_xact_vec_repeat(self.h, vin, vout, nrep), "xact_vec_repeat")
         return self.timing("total")
+
+    def autotune(self, reps=5):
+        """Tile shapes of this problem's action picked by the autotuner; returns them."""
+        _chk(_lib().ocean_gpu_autotune(self.h, reps), "autotune")
+        return self.tiles()
+
+    def tiles(self):
+        t = np.zeros(3, np.int32)
+        _chk(_lib().ocean_gpu_get_tiles(self.h, t.ctypes.data_as(_IP)), "get_tiles")
+        return tuple(int(v) for v in t)
 
     def timing(self, name):
         v = C.c_double()

@@ -52,7 +52,17 @@ struct ActArgs {
   const int* skip;            // nullable: when *skip != 0 every kernel of the action returns at once
                               // (a solver that has converged on the device drops the work it
                               // enqueued ahead)
+  int axmy;                   // 1: the adjoint writes c psi - H psi instead of H psi
+  double2 axmy_c;             // c = E + i gamma of a GMRES step (OCEAN_psi_axmy)
 };
+
+// out = c psi - r in OCEAN_psi_axmy's operation order (OCEAN2/OCEAN_psi.f90:1468-1562)
+template <typename T2>
+__device__ __forceinline__ double2 axmy_epilogue(const ActArgs<T2>& A, double2 pv, double2 r) {
+  if (!A.axmy) return r;
+  const double br = A.axmy_c.x * pv.x - r.x, bi = A.axmy_c.x * pv.y - r.y;
+  return make_double2(br - A.axmy_c.y * pv.y, bi + A.axmy_c.y * pv.x);
+}
 
 template <typename T2>
 __device__ __forceinline__ bool skipped(const ActArgs<T2>& A) {
@@ -1237,7 +1247,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
       r.x = ev.x * pv.x - ev.y * pv.y + inter.x;
       r.y = ev.x * pv.y + ev.y * pv.x + inter.y;
     }
-    out[off] = r;
+    out[off] = axmy_epilogue(A, pv, r);
   };
   if constexpr (kLrBuf) {
     // One spin, many alphas: every element e = t + i bd of this thread has band
@@ -1276,7 +1286,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
           r.x = ev.x * pv.x - ev.y * pv.y + inter.x;
           r.y = ev.x * pv.y + ev.y * pv.x + inter.y;
         }
-        out[off] = r;
+        out[off] = axmy_epilogue(A, pv, r);
       };
 #pragma unroll
       for (int i = 0; i < kEP; ++i)

@@ -271,12 +271,16 @@ Tiles tiles_for(const H* h) {
 
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
-           const typename og::Scalar<T2>::T* W, hipEvent_t* ev, const int* skip) {
+           const typename og::Scalar<T2>::T* W, hipEvent_t* ev, const int* skip, const double2* axmy) {
   og::ActArgs<T2> a;
   Geom<T2> g;
   const Tiles tl = tiles_for<T2, NA>(h);
   OG_TRY((make_args<T2, NA>(h, in, out, flags, tl, u, uT, phi, W, a, g)));
   a.skip = skip;
+  if (axmy) {
+    a.axmy = 1;
+    a.axmy_c = *axmy;
+  }
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
     OG_TRY(lds_opt_in(og::k_proj<T2, NA>, g.sm_proj));
@@ -336,7 +340,8 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
 // accumulated into (:445-446).  Timing events: proj = band-side projection, fft =
 // the x side (both obf products and the transform), adj = the adjoint.
 template <int NA>
-int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev, const int* skip) {
+int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev, const int* skip,
+               const double2* axmy) {
   flags &= ~og::F_FG;
   og::ActArgs<double2> a, ab;
   Geom<double2> g, gb;
@@ -348,6 +353,10 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev,
                                  &bs)));
   a.skip = skip;
   ab.skip = skip;
+  if (axmy) {
+    ab.axmy = 1;
+    ab.axmy_c = *axmy;
+  }
   const int nk = h->nk, nx = h->nx, no = h->no;
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   og::k_proj<double2, NA><<<gb.nblk_proj, tl.proj, gb.sm_proj, h->stream>>>(ab);
@@ -382,21 +391,23 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev,
   return OCEAN_GPU_OK;
 }
 
+// skip (nullable): every kernel returns at once while *skip != 0; axmy (nullable):
+// out = c in - H in with c = *axmy, formed in the adjoint's epilogue.
 int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr,
-             const int* skip = nullptr) {
+             const int* skip = nullptr, const double2* axmy = nullptr) {
   const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
   const int NA = na_tmpl(acnt);
   const bool mixed = (h->precision == OCEAN_GPU_MIXED);
   if (h->obf_on && (flags & og::F_LR)) {
     if (mixed || h->ns != 1) return OCEAN_GPU_ERR_STATE;
-    if (NA == 4 && h->na == 4) return xact_obf_t<4>(h, in, out, flags, ev, skip);
-    if (NA == 12 && h->na == 12) return xact_obf_t<12>(h, in, out, flags, ev, skip);
+    if (NA == 4 && h->na == 4) return xact_obf_t<4>(h, in, out, flags, ev, skip, axmy);
+    if (NA == 12 && h->na == 12) return xact_obf_t<12>(h, in, out, flags, ev, skip, axmy);
     return OCEAN_GPU_ERR_ARG;
   }
 #define OG_DISPATCH(NAV)                                                              \
   if (NA == NAV)                                                                      \
-    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev, skip) \
-                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev, skip);
+    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev, skip, axmy) \
+                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev, skip, axmy);
   OG_DISPATCH(2)
   OG_DISPATCH(4)
   OG_DISPATCH(6)
@@ -512,28 +523,36 @@ __global__ void k_multifinish(const int* __restrict__ skip, const double* __rest
   if (lane == 0) out[j] = s;
 }
 
-// y += sum_i c_i W_i (the x and ax updates of update_gmres, OCEAN_gmres.f90:336-341),
-// eight W rows per step with their loads in flight, summed in i order.
-__global__ void k_multiaxpy(const int* __restrict__ skip, const double2* const* __restrict__ Wl,
-                            const double2* __restrict__ c, int m, double2* __restrict__ y, size_t n) {
+// x += sum_i c_i U_i and ax += sum_i c_i AU_i together (the updates of update_gmres,
+// OCEAN_gmres.f90:336-341), eight rows of each per step with their loads in flight,
+// each sum in i order.
+__global__ void k_multiaxpy2(const int* __restrict__ skip, const double2* const* __restrict__ Ul,
+                             const double2* const* __restrict__ AUl, const double2* __restrict__ c, int m,
+                             double2* __restrict__ x, double2* __restrict__ ax, size_t n) {
   if (*skip) return;
   constexpr int R = 8;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double2 yv = y[e];
+    double2 xv = x[e], av = ax[e];
     for (int i0 = 0; i0 < m; i0 += R) {
-      double2 w[R];
+      double2 w[R], a[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) w[q] = Wl[min(i0 + q, m - 1)][e];
+      for (int q = 0; q < R; ++q) {
+        w[q] = Ul[min(i0 + q, m - 1)][e];
+        a[q] = AUl[min(i0 + q, m - 1)][e];
+      }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         if (i0 + q < m) {
           const double2 ci = c[i0 + q];
-          yv.x += ci.x * w[q].x; yv.y += ci.x * w[q].y;
-          yv.x += -ci.y * w[q].y; yv.y += ci.y * w[q].x;
+          xv.x += ci.x * w[q].x; xv.y += ci.x * w[q].y;
+          xv.x += -ci.y * w[q].y; xv.y += ci.y * w[q].x;
+          av.x += ci.x * a[q].x; av.y += ci.x * a[q].y;
+          av.x += -ci.y * a[q].y; av.y += ci.y * a[q].x;
         }
       }
     }
-    y[e] = yv;
+    x[e] = xv;
+    ax[e] = av;
   }
 }
 
@@ -542,8 +561,9 @@ __global__ void k_multiaxpy(const int* __restrict__ skip, const double2* const* 
 // restart or recycled energy), so its LDL^H factor (no pivoting) grows by one row:
 //   L(j,k) = (C(j,k) - sum_{m<k} L(j,m) conj(L(k,m)) D(m)) / D(k),  k < j = cur-1
 //   D(j)   =  C(j,j) - sum_{k<j} L(j,k) conj(L(j,k)) D(k)
-// with L and D kept in global memory between calls; then coef = -C^-1 <au|g> by the
-// two triangular solves.  One wave: lane l owns indices l, l + 64, .. (kLsSlots of
+// with D and the strictly lower triangle of L (packed row by row: L(i, k) at
+// i (i - 1) / 2 + k) kept in global memory between calls; then coef = -C^-1 <au|g> by
+// the two triangular solves.  One wave: lane l owns indices l, l + 64, .. (kLsSlots of
 // them), and every triangular sweep is right-looking, so each of its cur steps is
 // one broadcast of the finished value plus one FMA per owned index; the strictly
 // lower triangle of L is staged in LDS (packed rows) when it fits, otherwise read
@@ -571,15 +591,24 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
   __shared__ double Dl[kMaxGmresDepth];
   const int lane = threadIdx.x;
   const int j = cur - 1;
-  const bool in_lds = j < 2 || (size_t)(j - 1) * j / 2 <= (size_t)kLsPacked;
-  // rows 1 .. j - 1 of L and D(0 .. j-1); row j is formed here and kept in registers
-  if (in_lds)
-    for (int i = 1; i < j; ++i)
-      for (int k = lane; k < i; k += 64) Ls[i * (i - 1) / 2 + k] = L[(size_t)i * depth + k];
+  const int npk = j * (j - 1) / 2;  // strictly lower rows 1 .. j - 1, packed row by row
+  const bool in_lds = npk <= kLsPacked;
+  // rows 1 .. j - 1 of L (contiguous in the packed layout: four loads in flight per
+  // lane) and D(0 .. j-1); row j is formed here and kept in registers
+  if (in_lds) {
+    for (int e0 = lane; e0 < npk; e0 += 4 * 64) {
+      double2 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = L[min(e0 + 64 * q, npk - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (e0 + 64 * q < npk) Ls[e0 + 64 * q] = v[q];
+    }
+  }
   for (int k = lane; k < j; k += 64) Dl[k] = D[k];
   __syncthreads();
   auto Lat = [&](int i, int k) -> double2 {  // L(i, k), k < i < j
-    return in_lds ? Ls[i * (i - 1) / 2 + k] : L[(size_t)i * depth + k];
+    return in_lds ? Ls[i * (i - 1) / 2 + k] : L[(size_t)i * (i - 1) / 2 + k];
   };
   // C(j, k) = conj(<au_k|au_j>) for k < j (the lower triangle of the Hermitian
   // matrix whose upper column j holds <au_k|au_j>); C(j, j) = |au_j|^2
@@ -612,13 +641,12 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
 #pragma unroll
   for (int q = 0; q < kLsSlots; ++q) {
     const int k = lane + 64 * q;
-    if (k < j) L[(size_t)j * depth + k] = lj[q];
+    if (k < j) L[(size_t)npk + k] = lj[q];  // row j at j (j - 1) / 2
   }
   if (lane == 0) {
     const double dj = fin[4 * j] + dsum;
     D[j] = dj;
     Dl[j] = dj;
-    L[(size_t)j * depth + j] = make_double2(1.0, 0.0);
   }
   __syncthreads();
   int bad = 0;
@@ -677,48 +705,12 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
   }
 }
 
-// y = (er + i ei) x - y  (OCEAN_psi_axmy, OCEAN_psi.f90:1468-1562)
-__global__ void k_axmy(const int* __restrict__ skip, const double2* __restrict__ x, double2* __restrict__ y,
-                       double er, double ei, size_t n) {
-  if (*skip) return;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const double2 xv = x[e];
-    double2 yv = y[e];
-    double br = er * xv.x - yv.x;
-    double bi = er * xv.y - yv.y;
-    yv.x = br - ei * xv.y;
-    yv.y = bi + ei * xv.x;
-    y[e] = yv;
-  }
-}
-
 // z = x .* y (complex)  (OCEAN_psi_3element_mult)
 __global__ void k_cmul3(const int* __restrict__ skip, double2* __restrict__ z, const double2* __restrict__ x,
                         const double2* __restrict__ y, size_t n) {
   if (*skip) return;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
     z[e] = og::cmul(x[e], y[e]);
-}
-
-// g = ax - rhs ; partials of |g|^2 and <rhs|x>
-__global__ __launch_bounds__(kBlock) void k_resid(const int* __restrict__ skip, const double2* __restrict__ ax,
-                                                  const double2* __restrict__ rhs, const double2* __restrict__ x,
-                                                  double2* __restrict__ g, size_t n, double* __restrict__ part) {
-  if (*skip) return;
-  __shared__ double lds[4 * 3];
-  double v[3] = {0, 0, 0};
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const double2 a = ax[e], b = rhs[e], xv = x[e];
-    const double2 gv = make_double2(a.x - b.x, a.y - b.y);
-    g[e] = gv;
-    v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
-    v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
-    v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
-  }
-  og::block_sum<3>(v, lds);
-  if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = v[0]; part[3 * blockIdx.x + 1] = v[1]; part[3 * blockIdx.x + 2] = v[2];
-  }
 }
 
 // pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2)  (OCEAN_psi_min_set_prec, :2907-2994)
@@ -750,7 +742,7 @@ struct GmCtl {
   int done;     // 0 running, 1 converged, 2 least-squares pivot failure
   int conv_it;  // the iteration (1..depth of its cycle) at which it converged
   int info;     // zhptrf-style pivot index of a failure
-  int pad;
+  int count;    // arrival counter of the residual workgroups (rests at 0)
   double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
 };
 
@@ -774,17 +766,48 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
   }
 }
 
-// The three residual sums (same order as k_finish) and, with check, the
-// reference's stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and
+// One residual step: g = ax - rhs, the next basis vector unext = g . pc when given
+// (the first product of the next iteration, OCEAN_gmres.f90:760-766), the partial
+// sums of |g|^2 and <rhs|x>, and in the last workgroup to arrive (device-scope
+// counter) the three sums in a fixed order and, with check, the reference's
+// stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and
 // (-|g|^2 / Im<rhs|x> < 0.001 or |g|^2 < ffff / 10).
-__global__ __launch_bounds__(256) void k_resid_fin(const double* __restrict__ part, int nblk, GmCtl* __restrict__ c,
-                                                   double ffff, int it, int check) {
-  __shared__ double sv[3];
+__global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, const double2* __restrict__ ax,
+                                                       const double2* __restrict__ rhs, const double2* __restrict__ x,
+                                                       double2* __restrict__ g, const double2* __restrict__ pc,
+                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
+                                                       double ffff, int it, int check) {
   if (c->done) return;
+  __shared__ double lds[4 * 3];
+  __shared__ double sv[3];
+  __shared__ int is_last;
+  double v[3] = {0, 0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = ax[e], b = rhs[e], xv = x[e];
+    const double2 gv = make_double2(a.x - b.x, a.y - b.y);
+    g[e] = gv;
+    if (unext) unext[e] = og::cmul(gv, pc[e]);
+    v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
+    v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
+    v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
+  }
+  og::block_sum<3>(v, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = v[0]; part[3 * blockIdx.x + 1] = v[1]; part[3 * blockIdx.x + 2] = v[2];
+    __threadfence();
+    is_last = atomicAdd(&c->count, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (w < 3) {
-    const double v = og::wave_sum_parts(part, nblk, 3, w, lane);
-    if (lane == 0) sv[w] = v;
+    // other workgroups (possibly on other XCDs) wrote the partials: device-scope loads
+    double acc = 0.0;
+    for (int b = lane; b < (int)gridDim.x; b += 64)
+      acc += __hip_atomic_load(&part[3 * b + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = og::wave_sum(acc);
+    if (lane == 0) sv[w] = acc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -798,6 +821,7 @@ __global__ __launch_bounds__(256) void k_resid_fin(const double* __restrict__ pa
         c->conv_it = it;
       }
     }
+    atomicExch(&c->count, 0);
   }
 }
 
@@ -1780,12 +1804,12 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   double* part = nullptr;
   if (rc == 0) rc = dmalloc(&dlist, 2 * (size_t)depth + 2);
   if (rc == 0) rc = dmalloc(&dcoef, (size_t)depth);
-  if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 16);
+  if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 3 * (size_t)kRedBlocks + 16);
   double2* dL = nullptr;
   double* dD = nullptr;
   GmCtl* ctl = nullptr;
   GmCtl* hctl = nullptr;
-  if (rc == 0) rc = dmalloc(&dL, (size_t)depth * depth);
+  if (rc == 0) rc = dmalloc(&dL, (size_t)depth * (depth - 1) / 2 + 1);
   if (rc == 0) rc = dmalloc(&dD, (size_t)depth);
   if (rc == 0) rc = dmalloc(&ctl, (size_t)1);
   if (rc == 0 && hipHostMalloc((void**)&hctl, sizeof(GmCtl), hipHostMallocDefault) != hipSuccess)
@@ -1817,16 +1841,18 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
                         ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
 
-  // outv = (E + i gres) in - H in, dropped once the solve has stopped
+  // outv = (E + i gres) in - H in (formed in the action's epilogue), dropped once
+  // the solve has stopped
   auto axmy_xact = [&](double2* in, double2* outv, double E) -> int {
-    OG_TRY(xact_dev(h, in, outv, f, nullptr, skip));
-    k_axmy<<<grid, kBlock, 0, st>>>(skip, in, outv, E, gres, nv);
+    const double2 c = make_double2(E, gres);
+    OG_TRY(xact_dev(h, in, outv, f, nullptr, skip, &c));
     return launched();
   };
-  // g = ax - rhs and its sums; with check, the stopping test of iteration it
-  auto resid = [&](int it, int check) -> int {
-    k_resid<<<kRedBlocks, kBlock, 0, st>>>(skip, ax, rhs, x, g, nv, part);
-    k_resid_fin<<<1, 256, 0, st>>>(part, kRedBlocks, ctl, ffff, it, check);
+  // g = ax - rhs, its sums and (with check) the stopping test of iteration it; unext
+  // (nullable) receives the next basis vector g . pc
+  double* rpart = part + (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth;
+  auto resid = [&](int it, int check, double2* unext) -> int {
+    k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, pc, unext, nv, rpart, ffff, it, check);
     return launched();
   };
   // update_gmres(cur): Gram column cur, least-squares solve and x / ax updates
@@ -1836,8 +1862,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
     k_multifinish<<<(4 * cur + 3) / 4, 256, 0, st>>>(skip, part, kRedBlocks, 4 * cur, fin);
     k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, cur, depth, info);
-    k_multiaxpy<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), dcoef, cur, x, nv);
-    k_multiaxpy<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)dlist, dcoef, cur, ax, nv);
+    k_multiaxpy2<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), (const double2* const*)dlist,
+                                          dcoef, cur, x, ax, nv);
     return launched();
   };
   auto read_ctl = [&]() -> int {
@@ -1871,16 +1897,16 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         og::k_fill<<<grid, kBlock, 0, st>>>(x, make_double2(0.0, 0.0), nv);
         og::k_fill<<<grid, kBlock, 0, st>>>(ax, make_double2(0.0, 0.0), nv);
       }
-      if ((rc = resid(0, 0))) break;
+      if ((rc = resid(0, 0, nullptr))) break;
     } else {
       // make_recycle: au += (E - E_prev) u, then the updates over the kept basis
       const double delta = E - energies_eV[step - 1] * kEV2Hartree;
       for (int i = 0; i < prev_iter; ++i) k_axpy_gated<<<grid, kBlock, 0, st>>>(skip, delta, 0.0, U[i], AU[i], nv);
       if ((rc = axmy_xact(x, ax, E))) break;
-      if ((rc = resid(0, 0))) break;
+      if ((rc = resid(0, 0, nullptr))) break;
       for (int it = 1; it <= prev_iter && rc == 0; ++it) {
         if ((rc = update(it))) break;
-        rc = resid(0, 0);
+        rc = resid(0, 0, nullptr);
       }
       if (rc || (rc = read_ctl())) break;
       const double gval = hctl->gval, ival = hctl->ival;
@@ -1905,14 +1931,16 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
       while (it <= depth && rc == 0) {
         const int iend = std::min(depth, it + kAhead - 1);
         for (; it <= iend && rc == 0; ++it) {
-          k_cmul3<<<grid, kBlock, 0, st>>>(skip, U[it - 1], g, pc, nv);
+          // the first basis vector of a cycle from g and the current preconditioner;
+          // later ones were formed by the previous iteration's residual step
+          if (it == cycle_start + 1) k_cmul3<<<grid, kBlock, 0, st>>>(skip, U[it - 1], g, pc, nv);
           if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
           if ((rc = update(it))) break;
           if (it == depth) {
             if ((rc = axmy_xact(x, ax, E))) break;
             if (new_prec_at_end && (rc = set_prec(E))) break;
           }
-          rc = resid(it, 1);
+          rc = resid(it, 1, it < depth ? U[it] : nullptr);
         }
         if (rc || (rc = read_ctl())) break;
         if (hctl->done == 1) {
@@ -2445,6 +2473,33 @@ int ocean_gpu_batch_autotune(ocean_gpu_batch* b, int32_t reps) {
     g.tiles = best;
     for (int i = 0; i < 3; ++i) g.tune_ms[i] = bms[i];
   }
+  return OCEAN_GPU_OK;
+}
+
+// One problem's tiles by the batch autotuner on a batch of that problem alone (the
+// batched kernels run the single-problem bodies).  The single-problem launch derives
+// its FP32 projection workgroup from the handle's tile (tiles_for), so the pick is
+// stored in the handle's convention.
+int ocean_gpu_autotune(ocean_gpu_handle* h, int32_t reps) {
+  if (!h || reps <= 0) return OCEAN_GPU_ERR_ARG;
+  ocean_gpu_batch* b = nullptr;
+  OG_TRY(ocean_gpu_batch_create(&h, 1, &b));
+  int rc = ocean_gpu_batch_autotune(b, reps);
+  if (rc == 0 && b->groups.size() == 1) {
+    const B::Group& gr = b->groups[0];
+    Tiles t = gr.tiles;
+    if (gr.prec == OCEAN_GPU_MIXED) t.proj = gr.NA < 6 ? 2 * t.proj : std::max(64, t.proj / 2);
+    h->tiles = t;
+  }
+  ocean_gpu_batch_destroy(b);
+  return rc;
+}
+
+int ocean_gpu_get_tiles(ocean_gpu_handle* h, int32_t* tiles) {
+  if (!h || !tiles) return OCEAN_GPU_ERR_ARG;
+  tiles[0] = h->tiles.proj;
+  tiles[1] = h->tiles.adj;
+  tiles[2] = h->tiles.G;
   return OCEAN_GPU_OK;
 }
 

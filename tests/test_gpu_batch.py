@@ -130,3 +130,18 @@ def test_batch_gmres_matches_single_solves_and_oracle(members):
     bt.close()
     for g in gs:
         g.close()
+
+
+def test_single_problem_autotune_keeps_parity(members):
+    """ocean_gpu_autotune picks one problem's tiles with the batch autotuner; the
+    action with the picked tiles still matches the oracle (FP64 and mixed)."""
+    for q in (0, 2):
+        i = members[q]
+        P = to_problem(i)
+        for prec, tol in ((core.FP64, 1e-12), (core.MIXED, 1e-5)):
+            with core.OceanGPU.from_inputs(i, precision=prec) as g:
+                before = g.tiles()
+                after = g.autotune(reps=2)
+                print("member", q, "precision", prec, "tiles", before, "->", after, flush=True)
+                assert after[1] >= i.nb and after[0] in (64, 128, 256, 512, 1024)
+                assert relerr(g.xact(i.psi0), orc.xact(P, i.psi0)) < tol
