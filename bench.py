@@ -154,7 +154,7 @@ def batch_workload(n, niter, reps, log_every=8):
         bt.xact(vin, vout, 1)
         ms = bt.xact(vin, vout, reps)
         out[f"batch_{name}_hpsi_per_s"] = round(n / (ms * 1e-3), 1)
-        out[f"batch_{name}_tiles"] = [list(gr[:6]) for gr in groups]
+        out[f"batch_{name}_tiles"] = [list(gr[:7]) for gr in groups]
         log(f"batch {name}: {ms:.3f} ms per batched H.psi; groups {groups}")
         coeffs, ms_h = bt.haydock([i.psi0 for i in inps], niter)
         out[f"batch_{name}_spectra_per_s"] = round(n / (ms_h * 1e-3), 2)
@@ -284,6 +284,45 @@ def obf_workload(inp=None, num_obf=128, reps=50):
             "obf_xside_gflop": round(gf, 3)}
 
 
+def multi_rhs_workload(inp, nrhs=3, reps=50, niter=None):
+    """Three polarisations of the 64-band K-edge site sharing its Hamiltonian
+    (ocean_gpu_create_multi): every u stream of the action serves all three right-hand
+    sides.  H.psi/s counts right-hand sides (one stacked action = nrhs H.psi); spectra/s
+    counts the nrhs recursions of one ocean_gpu_haydock_multi run plus their continued
+    fractions.  The mixed spectra are compared with the FP64 ones of the same vectors."""
+    from ocean_amd import core, setup_host as sh
+
+    p = inp.params
+    n = p["niter"] if niter is None else niter
+    E2H = core.EV2HARTREE
+    rng = np.random.default_rng(77)
+    psi0s = [inp.psi0]
+    for _ in range(nrhs - 1):
+        z = np.asfortranarray(rng.standard_normal(inp.psi0.shape) + 1j * rng.standard_normal(inp.psi0.shape))
+        psi0s.append(sh.normalise_psi0(z, inp.nk, inp.celvol)[0])
+    stacked = np.concatenate(psi0s, axis=2)
+    out, spec = {"multi_rhs": nrhs}, {}
+    g = core.OceanGPU.from_inputs(inp, nrhs=nrhs)
+    vin, vout = g.vec_new(stacked), g.vec_new()
+    for prec, name in ((core.FP64, "fp64"), (core.MIXED, "mixed")):
+        g.set_precision(prec)
+        g.xact_repeat(vin, vout, 3)
+        ms = g.xact_repeat(vin, vout, reps)
+        out[f"multi_hpsi_per_s_{name}"] = round(nrhs * 1e3 / ms, 1)
+        out[f"multi_per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in g.profile_xact(vin, vout, reps).items()}
+        log(f"multi-rhs {name}: {ms:.4f} ms per stacked H.psi ({nrhs} right-hand sides)")
+        t0 = time.perf_counter()
+        a, b, _ = g.haydock_multi(psi0s, n)
+        sp = [core.spectrum(a[r], b[r], n, inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])[:, 2]
+              for r in range(nrhs)]
+        out[f"multi_spectra_per_s_{name}"] = round(nrhs / (time.perf_counter() - t0), 3)
+        spec[name] = np.array(sp)
+    g.close()
+    pk = np.abs(spec["fp64"]).max(axis=1)
+    out["multi_mixed_spectrum_err_of_peak"] = float((np.abs(spec["mixed"] - spec["fp64"]).max(axis=1) / pk).max())
+    return out
+
+
 def gmres_workload(inp, g, n=64):
     """GMRES at the 64-band K-edge workload: n energies over [-5, 40] eV to |g|^2 < 1e-16 (residual 1e-8)."""
     E = np.linspace(-5.0, 40.0, n)
@@ -315,6 +354,9 @@ def main():
                     help="only the OBF form's H.psi rate and kernel times (for profiling)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the mixed form's H.psi rate and kernel times at the 64-band K edge (for profiling)")
+    ap.add_argument("--multi-only", action="store_true",
+                    help="only the three-polarisation shared-Hamiltonian rates at the 64-band K edge (for profiling)")
+    ap.add_argument("--no-multi", action="store_true", help="skip the shared-Hamiltonian right-hand sides")
     ap.add_argument("--gmres-only", action="store_true",
                     help="only GMRES at 64 energies at the 64-band K edge (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -339,6 +381,10 @@ def main():
         return
     if args.obf_only:
         print(json.dumps(obf_workload(reps=args.steps)), flush=True)
+        return
+    if args.multi_only:
+        from ocean_amd import synth
+        print(json.dumps(multi_rhs_workload(synth.workload("k_nb64"), reps=args.steps, niter=40)), flush=True)
         return
     if args.mixed_only or args.gmres_only:
         from ocean_amd import core, synth
@@ -426,6 +472,10 @@ def main():
         extras["mixed_speedup_hpsi"] = round(extras["hpsi_per_s_mixed"] / hpsi_fp64, 3)
         g.set_precision(core.FP64)
 
+    if not args.no_extras and not args.no_multi:
+        extras.update(multi_rhs_workload(inp, reps=max(20, K // 4)))
+        extras["multi_mixed_speedup_vs_plain_fp64"] = round(extras["multi_hpsi_per_s_mixed"] / hpsi_fp64, 3)
+        extras["multi_fp64_speedup_vs_plain_fp64"] = round(extras["multi_hpsi_per_s_fp64"] / hpsi_fp64, 3)
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})

@@ -77,6 +77,22 @@ typedef struct ocean_bse_desc {
 typedef struct ocean_gpu_handle ocean_gpu_handle;
 
 int  ocean_gpu_create(const ocean_bse_desc* desc, ocean_gpu_handle** out);
+/* A handle whose vectors carry nrhs right-hand sides that share one Hamiltonian: the
+ * photon runs of one site and edge, which the reference solves one after another
+ * over the same loaded state (run list OCEAN2/OCEAN_system.f90:879-913, loop
+ * OCEAN2/OCEAN.f90:52-84).  Every vector is (nb, nk, nrhs * nalpha): right-hand side
+ * r holds alphas r * nalpha .. (r + 1) * nalpha - 1, and each u stream of the action
+ * serves all of them.  ocean_gpu_load_energies, ocean_gpu_load_mult and
+ * ocean_gpu_mult_setup take one right-hand side's arrays (nalpha alphas), which
+ * every right-hand side shares.  The alphas per spin, nrhs * nalpha / nspn, must be
+ * 2, 4, 6 or 12 (K edge: nrhs 1 or 3, or 6 with nspn 2; L2,3: nrhs 2 with nspn 2);
+ * OCEAN_GPU_ERR_UNSUPPORTED otherwise.  Such a handle runs xact, the vector
+ * operations, ocean_gpu_autotune and ocean_gpu_haydock_multi, and batched actions;
+ * the single-right-hand-side solvers (Haydock, GMRES, gradient, batched Haydock and
+ * GMRES) and the OBF form refuse it (OCEAN_GPU_ERR_UNSUPPORTED).  No reference
+ * counterpart: it replaces nrhs separate runs. */
+int  ocean_gpu_create_multi(const ocean_bse_desc* desc, int32_t nrhs, ocean_gpu_handle** out);
+int  ocean_gpu_nrhs(ocean_gpu_handle* h, int32_t* nrhs);
 int  ocean_gpu_destroy(ocean_gpu_handle* h);
 const char* ocean_gpu_strerror(int code);
 int  ocean_gpu_set_precision(ocean_gpu_handle* h, int32_t precision);
@@ -213,6 +229,17 @@ int  ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const d
                              double gam, double el, double eh, int32_t ne, double* real_a,
                              double* real_b, double* imag_a, int32_t* niter_done);
 
+/* ocean_gpu_haydock for every right-hand side of an ocean_gpu_create_multi handle
+ * at once: psi0 is the stacked starting vector (nb, nk, nrhs * nalpha), each
+ * right-hand side normalised on its own; real_a, real_b, imag_a hold nrhs runs of
+ * niter+1 entries (right-hand side r at r * (niter + 1)); niter_done[nrhs].  Each
+ * right-hand side's coefficients are those of ocean_gpu_haydock on its own vector
+ * (OCEAN2/OCEAN_haydock.f90:190-320, :824-938), to rounding.  Returns -1 if any
+ * recursion met a NaN. */
+int  ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im,
+                             int32_t niter, double* real_a, double* real_b, double* imag_a,
+                             int32_t* niter_done);
+
 /* Recycled preconditioned GMRES over energies (OCEAN2/OCEAN_gmres.f90:508-908).
  * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
  * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,
@@ -277,9 +304,10 @@ typedef struct ocean_gpu_batch ocean_gpu_batch;
 int  ocean_gpu_batch_create(ocean_gpu_handle* const* hs, int32_t n, ocean_gpu_batch** out);
 int  ocean_gpu_batch_destroy(ocean_gpu_batch* b);
 int  ocean_gpu_batch_ngroups(ocean_gpu_batch* b, int32_t* ngroups);
-/* info[6] = (problems, precision, alpha template, projection workgroup, adjoint
- * thread budget, transform strip width (0 = widest)); tune_ms[3] (nullable) = the
- * autotuner's best time of each stage (ms per launch). */
+/* info[7] = (problems, precision, alpha template, projection workgroup, adjoint
+ * thread budget, transform strip width (0 = widest), adjoint form (1 matrix cores,
+ * 0 VALU, -1 the default, matrix cores)); tune_ms[3] (nullable) = the autotuner's best
+ * time of each stage (ms per launch). */
 int  ocean_gpu_batch_group_info(ocean_gpu_batch* b, int32_t group, int32_t* info, double* tune_ms);
 /* Times every candidate tile shape of each stage on the batch's own problems
  * (reps launches each) and keeps the fastest per group. */
@@ -355,12 +383,18 @@ int  ocean_gpu_tridiag_eigs(const double* real_a, const double* real_b, int32_t 
 
 /* Tile shapes of one problem's action (no reference counterpart): the batch
  * autotuner run on that problem alone (ocean_gpu_batch_autotune, reps launches per
- * candidate) and its picks kept by the handle; tiles[3] = (projection workgroup,
- * adjoint thread budget, transform strip width, 0 = widest) as the handle stores
+ * candidate) and its picks kept by the handle; tiles[4] = (projection workgroup,
+ * adjoint thread budget, transform strip width (0 = widest), adjoint form (1 the
+ * matrix-core adjoint with its fused epilogue, 0 the VALU adjoint, -1 the default,
+ * the matrix cores; it applies to many alphas on one spin)) as the handle stores
  * them (the FP32 projection runs half as many threads per x at four alphas or fewer
  * and twice as many at more). */
 int  ocean_gpu_autotune(ocean_gpu_handle* h, int32_t reps);
 int  ocean_gpu_get_tiles(ocean_gpu_handle* h, int32_t* tiles);
+/* Sets the handle's tiles[4] as ocean_gpu_get_tiles reports them (OCEAN_GPU_ERR_ARG
+ * for values outside projection / adjoint 64..1024 in steps of 64, strip 0..64,
+ * form -1..1); a shape the problem cannot take is refused by the next action. */
+int  ocean_gpu_set_tiles(ocean_gpu_handle* h, const int32_t* tiles);
 
 /* Average durations (ms) measured with HIP events on the library's stream:
  * "total" from the last ocean_gpu_xact_vec_repeat, "proj", "fft", "mult", "adj"

@@ -53,6 +53,10 @@ _H = C.c_void_p
 
 _SIGS = {
     "ocean_gpu_create": (C.c_int, [C.POINTER(Desc), C.POINTER(_H)]),
+    "ocean_gpu_create_multi": (C.c_int, [C.POINTER(Desc), C.c_int32, C.POINTER(_H)]),
+    "ocean_gpu_nrhs": (C.c_int, [_H, _IP]),
+    "ocean_gpu_haydock_multi": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, _DP, _DP, _IP]),
+    "ocean_gpu_set_tiles": (C.c_int, [_H, _IP]),
     "ocean_gpu_destroy": (C.c_int, [_H]),
     "ocean_gpu_strerror": (C.c_char_p, [C.c_int]),
     "ocean_gpu_set_precision": (C.c_int, [_H, C.c_int32]),
@@ -236,7 +240,10 @@ class OceanGPU:
 
     def __init__(self, nb, kmesh, xmesh, lc, nspn, celvol, inter_scale=1.0, lmin=0, lmax=3,
                  nproj=(1, 1, 1, 1), lvl=None, lvh=None, e0=True, mult=True, long_range=True,
-                 precision=FP64):
+                 precision=FP64, nrhs=1):
+        """nrhs > 1: a handle whose vectors stack nrhs right-hand sides sharing this
+        Hamiltonian (ocean_gpu_create_multi); energies and multiplet inputs are one
+        right-hand side's (shape0), vectors are (nb, nk, nrhs * nalpha)."""
         lib = _lib()
         d = Desc()
         d.num_bands = nb
@@ -255,20 +262,26 @@ class OceanGPU:
         d.precision = precision
         self.desc = d
         self.nb, self.nk, self.nx = nb, int(np.prod(kmesh)), int(np.prod(xmesh))
-        self.na = 4 * (2 * lc + 1)
+        self.na0 = 4 * (2 * lc + 1)
+        self.nrhs = int(nrhs)
+        self.na = self.nrhs * self.na0
         self.nspn = nspn
         self.shape = (nb, self.nk, self.na)
+        self.shape0 = (nb, self.nk, self.na0)
         h = _H()
-        _chk(lib.ocean_gpu_create(C.byref(d), C.byref(h)), "ocean_gpu_create")
+        if self.nrhs == 1:
+            _chk(lib.ocean_gpu_create(C.byref(d), C.byref(h)), "ocean_gpu_create")
+        else:
+            _chk(lib.ocean_gpu_create_multi(C.byref(d), self.nrhs, C.byref(h)), "ocean_gpu_create_multi")
         self.h = h
         self.kpref = None
 
     @classmethod
-    def from_inputs(cls, inp, precision=FP64, e0=True, mult=True, long_range=True):
+    def from_inputs(cls, inp, precision=FP64, e0=True, mult=True, long_range=True, nrhs=1):
         """Create a handle and load every piece of state from synthetic inputs."""
         m = inp.mult
         g = cls(inp.nb, inp.kmesh, inp.xmesh, inp.lc, inp.nspn, inp.celvol, inp.inter_scale,
-                m.lmin, m.lmax, tuple(m.nproj), m.lvl, m.lvh, e0, mult, long_range, precision)
+                m.lmin, m.lmax, tuple(m.nproj), m.lvl, m.lvh, e0, mult, long_range, precision, nrhs)
         g.load_inputs(inp)
         return g
 
@@ -382,7 +395,7 @@ class OceanGPU:
         sl = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros(1))
         ntot = max(sum(m * m for m in sizes), 1)
         mhr, mhi = np.zeros(ntot), np.zeros(ntot)
-        na = self.na
+        na = self.na0
         sr, si = np.zeros((na, na), order="F"), np.zeros((na, na), order="F")
         hso = C.c_int32()
         lt = None if life_time_eV is None else np.ascontiguousarray(life_time_eV, np.float64).ravel()
@@ -404,8 +417,8 @@ class OceanGPU:
         c = m.c
         _need(c, (self.nb, self.nk, npmax, 2 * m.lmax + 1, nl, self.nspn), "projector array c")
         _need(m.mpm, (npmax, npmax, nl), "mpm")
-        _need(m.somelr, (self.na, self.na), "somelr")
-        _need(m.someli, (self.na, self.na), "someli")
+        _need(m.somelr, (self.na0, self.na0), "somelr")
+        _need(m.someli, (self.na0, self.na0), "someli")
         d = self.desc
         if (m.lmin, m.lmax, m.lvl, m.lvh) != (d.lmin, d.lmax, d.lvl, d.lvh) or \
                 list(m.nproj) != list(d.nproj[:nl]):
@@ -423,8 +436,9 @@ class OceanGPU:
                                         _p(si), int(bool(m.have_spin_orbit))), "load_mult")
 
     def load_energies(self, energy, allow):
-        er, ei = _planes(energy, self.shape, "energy")
-        _need(allow, self.shape, "allow")
+        """One right-hand side's energies and allow mask (shape0), shared by all."""
+        er, ei = _planes(energy, self.shape0, "energy")
+        _need(allow, self.shape0, "allow")
         al = np.asfortranarray(allow, np.float64)
         _chk(_lib().ocean_gpu_load_energies(self.h, _p(er), _p(ei), _p(al)), "load_energies")
 
@@ -508,8 +522,12 @@ class OceanGPU:
         _chk(_lib().ocean_gpu_autotune(self.h, reps), "autotune")
         return self.tiles()
 
+    def set_tiles(self, proj, adj, G=0, adj_mf=-1):
+        t = np.array([proj, adj, G, adj_mf], np.int32)
+        _chk(_lib().ocean_gpu_set_tiles(self.h, t.ctypes.data_as(_IP)), "set_tiles")
+
     def tiles(self):
-        t = np.zeros(3, np.int32)
+        t = np.zeros(4, np.int32)
         _chk(_lib().ocean_gpu_get_tiles(self.h, t.ctypes.data_as(_IP)), "get_tiles")
         return tuple(int(v) for v in t)
 
@@ -544,6 +562,26 @@ class OceanGPU:
         if return_status:
             return a, b, ai, done.value, rc
         _chk(rc, "haydock")
+        return a, b, ai
+
+    def haydock_multi(self, psi0s, niter, return_status=False):
+        """The recursions of all nrhs right-hand sides at once (ocean_gpu_haydock_multi).
+        psi0s: (nrhs, nb, nk, nalpha) starting vectors, or the stacked (nb, nk, nrhs *
+        nalpha) vector.  Returns (a, b, ai), each (nrhs, niter + 1); with return_status
+        (a, b, ai, niter_done[nrhs], rc)."""
+        z = np.asarray(psi0s)
+        if z.shape == (self.nrhs,) + self.shape0:
+            z = np.concatenate(list(z), axis=2)
+        pr, pi = _planes(z, self.shape, "psi0s")
+        R, n1 = self.nrhs, niter + 1
+        a, b, ai = np.zeros(R * n1), np.zeros(R * n1), np.zeros(R * n1)
+        done = np.zeros(R, np.int32)
+        rc = _lib().ocean_gpu_haydock_multi(self.h, _p(pr), _p(pi), niter, _p(a), _p(b), _p(ai),
+                                            done.ctypes.data_as(_IP))
+        a, b, ai = a.reshape(R, n1), b.reshape(R, n1), ai.reshape(R, n1)
+        if return_status:
+            return a, b, ai, done, rc
+        _chk(rc, "haydock_multi")
         return a, b, ai
 
     def spectral_grad(self, hay_vec, energies_eV, gres_eV, gprc_eV, ffff, depth, target, kpref=None,
@@ -628,12 +666,12 @@ class OceanBatch:
             pass
 
     def groups(self):
-        """[(problems, precision, alpha template, proj tile, adj tile, G, tune_ms[3]), ...]"""
+        """[(problems, precision, alpha template, proj tile, adj tile, G, adjoint form, tune_ms[3]), ...]"""
         ng = C.c_int32()
         _chk(_lib().ocean_gpu_batch_ngroups(self.b, C.byref(ng)), "batch_ngroups")
         out = []
         for g in range(ng.value):
-            info = np.zeros(6, np.int32)
+            info = np.zeros(7, np.int32)
             ms = np.zeros(3)
             _chk(_lib().ocean_gpu_batch_group_info(self.b, g, info.ctypes.data_as(_IP), _p(ms)),
                  "batch_group_info")

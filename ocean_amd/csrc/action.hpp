@@ -38,9 +38,11 @@ struct ActArgs {
   const double2* Kmat;        // (na P)^2
   double2* hmult;             // [a*P+p]
   int* mcount;                // reduction arrival counter (rests at 0)
-  const double2* S;           // [a][b]
+  const double2* S;           // [a][b] over the na0 alphas of one right-hand side
   double mul;                 // inter_scale / (N_k celvol)
   int nb, nk, nx, na, ns, P, nj;
+  int na0;                    // alphas per right-hand side: alpha a of a vector belongs to
+                              // right-hand side a / na0 (na = na0 for one right-hand side)
   int ldu, ldT, ldphi;
   int flags;
   int px;                     // projection x-tiles (blockDim.x x-points each)
@@ -48,7 +50,6 @@ struct ActArgs {
   int K1, K2, K3;
   int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
   const T2* phs;              // OBF form: supercell phase [x][k] applied around the transform
-  const double2* lrbuf;       // [a][k][n]: sum_x conj(u) phi' from k_adj_lr (else the adjoint streams u)
   const int* skip;            // nullable: when *skip != 0 every kernel of the action returns at once
                               // (a solver that has converged on the device drops the work it
                               // enqueued ahead)
@@ -700,7 +701,7 @@ __device__ __forceinline__ void dft_run(const T2* tw, int G, int nk, int L, int 
 
 __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict__ amp,
                           const double2* __restrict__ K, double2* __restrict__ hmult, int* counter,
-                          int nk, int nj, double mul, int r, int nred, double2* lds);
+                          int nk, int nj, int nj0, double mul, int r, int nred, double2* lds);
 
 // Strip of G consecutive phi columns (c = x*na + a) per workgroup, all nk rows:
 //   pass 1 (K3, F_+): HBM -> registers -> LDS, with W(x, :) staged beside it
@@ -718,7 +719,7 @@ __device__ __forceinline__ void fft_body(const ActArgs<T2>& A, int blk) {
   // the reduction workgroups come first so that they are dispatched first
   const int nred = A.nred, nfft = A.nfft;
   if (blk < nred) {
-    mred_tail(A.apart, A.amp, A.Kmat, A.hmult, A.mcount, A.nk, A.nj, A.mul, blk, nred,
+    mred_tail(A.apart, A.amp, A.Kmat, A.hmult, A.mcount, A.nk, A.nj, A.na0 * A.P, A.mul, blk, nred,
               reinterpret_cast<double2*>(smem));
     return;
   }
@@ -958,13 +959,16 @@ __device__ __forceinline__ void mred_sum(const double2* __restrict__ apart, doub
 }
 
 // h(i) = mul * sum_j K(i, j) amp(j): direct -mpm a plus exchange X a
-// (OCEAN_multiplet.f90:1812-1936); one wave per row, coalesced rows of K.
+// (OCEAN_multiplet.f90:1812-1936); one wave per row, coalesced rows of K.  With
+// several right-hand sides K is block diagonal over blocks of nj0 amplitudes and a
+// row sums its own block only (so a NaN in one right-hand side stays there).
 __device__ __forceinline__ void mred_matvec(const double2* __restrict__ K, const double2* amp,
-                                            double2* __restrict__ hmult, int nj, double mul) {
+                                            double2* __restrict__ hmult, int nj, int nj0, double mul) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int i = wid; i < nj; i += nw) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int j = lane; j < nj; j += 64) cfma(acc, K[(size_t)i * nj + j], amp[j]);
+    const int j0 = i - i % nj0;
+    for (int j = j0 + lane; j < j0 + nj0; j += 64) cfma(acc, K[(size_t)i * nj + j], amp[j]);
     acc.x = wave_sum(acc.x);
     acc.y = wave_sum(acc.y);
     if (lane == 0) hmult[i] = make_double2(acc.x * mul, acc.y * mul);
@@ -974,11 +978,11 @@ __device__ __forceinline__ void mred_matvec(const double2* __restrict__ K, const
 // Stand-alone reduction (used when the long-range term is off): one workgroup.
 __global__ __launch_bounds__(1024) void k_mred(
     const double2* __restrict__ apart, const double2* __restrict__ K, double2* __restrict__ hmult,
-    int nk, int nj, double mul) {
+    int nk, int nj, int nj0, double mul) {
   __shared__ double2 amp[kMaxAlpha * 64];
   mred_sum(apart, amp, nk, 0, nj);
   __syncthreads();
-  mred_matvec(K, amp, hmult, nj, mul);
+  mred_matvec(K, amp, hmult, nj, nj0, mul);
 }
 
 // Reduction run by nred extra workgroups of another kernel's grid, one wave per
@@ -991,7 +995,7 @@ constexpr int kRedPerBlock = 4;
 
 __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict__ amp,
                           const double2* __restrict__ K, double2* __restrict__ hmult, int* counter,
-                          int nk, int nj, double mul, int r, int nred, double2* lds) {
+                          int nk, int nj, int nj0, double mul, int r, int nred, double2* lds) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   mred_sum(apart, amp, nk, r * kRedPerBlock, min(nj, (r + 1) * kRedPerBlock));
   __threadfence();
@@ -1008,21 +1012,29 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
     lds[j] = make_double2(ax, ay);
   }
   __syncthreads();
+  // K is block diagonal over right-hand sides (blocks of nj0): each row sums its own
+  // block, and a pass of eight rows runs over the columns of the blocks they touch
   constexpr int R = 8;
   for (int i0 = wid * R; i0 < nj; i0 += nw * R) {
     double v[2 * R];
 #pragma unroll
     for (int q = 0; q < 2 * R; ++q) v[q] = 0.0;
-    for (int c0 = 0; c0 < nj; c0 += 64) {
+    const int iend = min(i0 + R, nj) - 1;
+    const int c_lo = i0 - i0 % nj0, c_hi = iend - iend % nj0 + nj0;
+    int rb[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) rb[q] = (min(i0 + q, nj - 1)) / nj0;
+    for (int c0 = c_lo; c0 < c_hi; c0 += 64) {
       const int j = c0 + lane;
       const int jc = min(j, nj - 1);
       double2 kv[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) kv[q] = K[(size_t)min(i0 + q, nj - 1) * nj + jc];
       const double2 a = lds[jc];
-      if (j < nj) {
+      if (j < c_hi) {
 #pragma unroll
         for (int q = 0; q < R; ++q) {
+          if (j / nj0 != rb[q]) continue;
           v[2 * q] = fma(kv[q].x, a.x, v[2 * q]); v[2 * q] = fma(-kv[q].y, a.y, v[2 * q]);
           v[2 * q + 1] = fma(kv[q].x, a.y, v[2 * q + 1]); v[2 * q + 1] = fma(kv[q].y, a.x, v[2 * q + 1]);
         }
@@ -1082,17 +1094,10 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
   const int t = threadIdx.x, bd = blockDim.x;
   const int n = t % nb, xs = t / nb;
   const bool active = xs < XS;
-  // with lrbuf (many alphas only) the long-range sums come from the matrix-core
-  // kernel k_adj_lr; the fewer-alpha instances keep their register budget
-  bool lr = (flags & F_LR) != 0;
-  constexpr bool kLrBuf = NA >= 6 && sizeof(T2) == 16;  // FP64 many-alpha instances only
-  if constexpr (kLrBuf) lr = lr && A.lrbuf == nullptr;
+  const bool lr = (flags & F_LR) != 0;
   const bool fg = (flags & F_FG) != 0;
   const int nel = na * nb;
-
-  // the many-alpha FP64 epilogue stages psi(:, k, :) in LDS instead (see below)
-  const bool so_stage = kLrBuf && (flags & F_SO) && ns == 1 && na <= NA && bd % nb == 0 &&
-                        na * na + na * nb <= max(XC * NA, JR * XS * nb);
+  const int na0 = A.na0;
   // epilogue operands of the first kEP elements owned by this thread
   constexpr int kEP = 2;
   double2 e_psi[kEP], e_en[kEP];
@@ -1102,7 +1107,7 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     const int e = min(t + i * bd, nel - 1);
     const int a = e / nb, nn = e - a * nb;
     const size_t off = ((size_t)a * nk + k) * nb + nn;
-    e_psi[i] = so_stage ? make_double2(0.0, 0.0) : psi[off];
+    e_psi[i] = psi[off];
     e_en[i] = energy[off];
     e_m[i] = allow[off];
   }
@@ -1176,17 +1181,6 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     double2 part[NA];
 #pragma unroll
     for (int j = 0; j < NA; ++j) part[j] = make_double2(-(double)acc[j].x, -(double)acc[j].y);
-    if constexpr (kLrBuf) {
-      if (A.lrbuf != nullptr && (flags & F_LR) && xs == 0) {
-#pragma unroll
-        for (int j = 0; j < NA; ++j)
-          if (j < acnt) {
-            const double2 v = A.lrbuf[((size_t)(s + j * astep) * nk + k) * nb + n];
-            part[j].x -= v.x;
-            part[j].y -= v.y;
-          }
-      }
-    }
     if (fg && active) {
 #pragma unroll
       for (int i = 0; i < kCP; ++i) {
@@ -1235,10 +1229,12 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     inter.x *= am;
     inter.y *= am;
     if (flags & F_SO) {
-      for (int b = 0; b < na; ++b) {
-        double2 sab = S[a * na + b];
+      // S mixes the alphas of one right-hand side: rows a0 .. a0 + na0 - 1
+      const int ar = a % na0, a0 = a - ar;
+      for (int b = 0; b < na0; ++b) {
+        double2 sab = S[ar * na0 + b];
         if (back) sab.y = -sab.y;
-        cfma(inter, sab, psi[((size_t)b * nk + k) * nb + nn]);
+        cfma(inter, sab, psi[((size_t)(a0 + b) * nk + k) * nb + nn]);
       }
     }
     double2 r = inter;
@@ -1249,55 +1245,6 @@ __device__ __forceinline__ void adj_body(const ActArgs<T2>& A, int blk) {
     }
     out[off] = axmy_epilogue(A, pv, r);
   };
-  if constexpr (kLrBuf) {
-    // One spin, many alphas: every element e = t + i bd of this thread has band
-    // n = t % nb, and the XS threads of a band all need psi(:, k, n) for the
-    // spin-orbit sum.  psi(:, k, :) (contiguous band rows of each alpha) and S
-    // (conjugated for the backward action) are staged once per workgroup in the
-    // reduction area, free after the last barrier, so every psi element is read from
-    // HBM once per k instead of once per x-split thread (same products in the same
-    // order as finish).
-    if (so_stage) {
-      double2* Ss = red;
-      double2* Ps = red + na * na;  // psi(b, k, n) at Ps[b * nb + n]
-      for (int i = t; i < na * na; i += bd) {
-        double2 v = S[i];
-        if (back) v.y = -v.y;
-        Ss[i] = v;
-      }
-      for (int i = t; i < na * nb; i += bd) {
-        const int b = i / nb, nn = i - b * nb;
-        Ps[i] = psi[((size_t)b * nk + k) * nb + nn];
-      }
-      __syncthreads();
-      auto finish_so = [&](int e, double2 ev, double am) {
-        const int a = e / nb;
-        const size_t off = ((size_t)a * nk + k) * nb + n;
-        const double2 pv = Ps[a * nb + n];
-        double2 inter = res[e];
-        inter.x *= am;
-        inter.y *= am;
-#pragma unroll
-        for (int b = 0; b < NA; ++b)
-          if (b < na) cfma(inter, Ss[a * na + b], Ps[b * nb + n]);
-        double2 r = inter;
-        if (flags & F_E0) {
-          if (back) ev.y = -ev.y;
-          r.x = ev.x * pv.x - ev.y * pv.y + inter.x;
-          r.y = ev.x * pv.y + ev.y * pv.x + inter.y;
-        }
-        out[off] = axmy_epilogue(A, pv, r);
-      };
-#pragma unroll
-      for (int i = 0; i < kEP; ++i)
-        if (t + i * bd < nel) finish_so(t + i * bd, e_en[i], e_m[i]);
-      for (int e = t + kEP * bd; e < nel; e += bd) {
-        const size_t off = ((size_t)(e / nb) * nk + k) * nb + n;
-        finish_so(e, energy[off], allow[off]);
-      }
-      return;
-    }
-  }
 #pragma unroll
   for (int i = 0; i < kEP; ++i)
     if (t + i * bd < nel) finish(t + i * bd, e_psi[i], e_en[i], e_m[i]);

@@ -184,7 +184,7 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   a.cproj = h->cproj; a.apart = h->apart; a.amp = h->amp; a.Kmat = h->Kmat; a.hmult = h->hmult;
   a.mcount = h->mcount; a.S = h->Smat;
   a.mul = h->d.inter_scale / ((double)nk * h->d.celvol);
-  a.nb = nb; a.nk = nk; a.nx = nx; a.na = na; a.ns = ns; a.P = h->P; a.nj = na * h->P;
+  a.nb = nb; a.nk = nk; a.nx = nx; a.na = na; a.ns = ns; a.P = h->P; a.nj = na * h->P; a.na0 = h->na0;
   a.ldu = bs ? bs->ldu : h->ldu;
   a.ldT = bs ? bs->ldT : h->ldT;
   a.ldphi = bs ? bs->ldphi : h->ldphi;
@@ -239,6 +239,16 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   g.sm_adj = resb + std::max((size_t)XC * NA * sizeof(T2), (size_t)a.JR * per_j);
   g.nblk_adj = nk;
   return OCEAN_GPU_OK;
+}
+
+// Many alphas on one spin: the adjoint as a GEMM per k on the matrix cores with the
+// whole epilogue in the same kernel (k_adj_lr), where the VALU adjoint is bound by
+// its nalpha LDS broadcasts of phi' per band row (measured faster in both precisions
+// at 12 alphas).  The tile can force either form.
+template <typename T2, int NA>
+bool adj_mf(const H* h, int flags, const Tiles& t) {
+  if (NA < 6 || !(flags & og::F_LR) || h->ns != 1 || h->na != NA) return false;
+  return t.adj_mf != 0;
 }
 
 // The device state of one precision form.
@@ -306,23 +316,19 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     og::k_mproj<<<dim3(h->nk, h->ns), kBlock, 0, h->stream>>>(h->cproj, in, h->allow, h->apart, h->nb,
                                                              h->nk, h->na, h->ns, h->P);
     OG_CHECK(hipGetLastError());
-    og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, a.nj, a.mul);
+    og::k_mred<<<1, 1024, 0, h->stream>>>(h->apart, h->Kmat, h->hmult, h->nk, a.nj, h->na0 * h->P, a.mul);
     OG_CHECK(hipGetLastError());
   }
   if (ev) OG_CHECK(hipEventRecord(ev[3], h->stream));
-  // many alphas in FP64: the long-range sums on the matrix cores, the rest of the
-  // adjoint (multiplet back-projection, epilogue) in k_adj.  (The FP32 instance of
-  // k_adj_lr is correct but measured no faster than the FP32 VALU adjoint.)
-  if constexpr (NA >= 6 && std::is_same<T2, double2>::value) {
-    if ((flags & og::F_LR) && h->lrbuf && h->ns == 1 && h->na == NA) {
-      a.lrbuf = h->lrbuf;
-      og::k_adj_lr<T2, NA><<<dim3((h->nb + 63) / 64, h->nk), 256, 0, h->stream>>>(
-          uT, h->ldT, phi, h->ldphi, h->lrbuf, h->nb, h->nk, h->nx, skip);
-      OG_CHECK(hipGetLastError());
-    }
+  bool mf = false;
+  if constexpr (NA >= 6) {
+    mf = adj_mf<T2, NA>(h, flags, tl);
+    if (mf) og::k_adj_lr<T2, NA><<<((h->nb + 63) / 64) * h->nk, 256, 0, h->stream>>>(a);
   }
-  OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
-  og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
+  if (!mf) {
+    OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
+    og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
+  }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
   return OCEAN_GPU_OK;
@@ -844,9 +850,14 @@ const char* ocean_gpu_strerror(int code) {
   }
 }
 
-int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
+}  // extern "C"
+
+namespace {
+
+int create_impl(const ocean_bse_desc* d, int nrhs, ocean_gpu_handle** out) {
   if (!d || !out) return OCEAN_GPU_ERR_ARG;
   *out = nullptr;
+  if (nrhs < 1) return OCEAN_GPU_ERR_ARG;
   if (d->num_bands <= 0 || d->nspn < 1 || d->nspn > 2 || d->lc < 0 || d->lc > 1) return OCEAN_GPU_ERR_ARG;
   for (int i = 0; i < 3; ++i)
     if (d->kmesh[i] <= 0 || d->xmesh[i] <= 0 || !dim_ok(d->kmesh[i])) return OCEAN_GPU_ERR_ARG;
@@ -859,8 +870,19 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   h->km[0] = d->kmesh[0]; h->km[1] = d->kmesh[1]; h->km[2] = d->kmesh[2];
   h->nk = d->kmesh[0] * d->kmesh[1] * d->kmesh[2];
   h->nx = d->xmesh[0] * d->xmesh[1] * d->xmesh[2];
-  h->na = 4 * (2 * d->lc + 1);
+  h->na0 = 4 * (2 * d->lc + 1);
+  h->nrhs = nrhs;
+  h->na = nrhs * h->na0;
   h->ns = d->nspn;
+  // the right-hand sides run as na alphas of the action kernels, whose alpha counts
+  // per spin are 2, 4, 6 and 12
+  {
+    const int acnt = h->ns == 2 ? h->na / 2 : h->na;
+    if (nrhs > 1 && (acnt > 12 || na_tmpl(acnt) != acnt)) {
+      delete h;
+      return OCEAN_GPU_ERR_UNSUPPORTED;
+    }
+  }
   h->ldu = (h->nx + 33) & ~1;  // even, so FP32 rows pair up into 16-byte loads
   h->ldphi = h->nx * h->na + 32;
   h->ldT = h->nx * h->nb + 32;
@@ -894,13 +916,28 @@ int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) {
   if (rc == 0) rc = dmalloc(&h->red, (size_t)kRedBlocks * 8 + 64);
   if (rc == 0) rc = dmalloc(&h->tmp_in, nv);
   if (rc == 0) rc = dmalloc(&h->tmp_out, nv);
-  if (rc == 0 && h->na >= 6 && h->ns == 1) rc = dmalloc(&h->lrbuf, nv);
   if (rc == 0) rc = ensure_precision_buffers(h);
   if (rc != 0) {
     ocean_gpu_destroy(h);
     return rc;
   }
   *out = h;
+  return OCEAN_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocean_gpu_create(const ocean_bse_desc* d, ocean_gpu_handle** out) { return create_impl(d, 1, out); }
+
+int ocean_gpu_create_multi(const ocean_bse_desc* d, int32_t nrhs, ocean_gpu_handle** out) {
+  return create_impl(d, nrhs, out);
+}
+
+int ocean_gpu_nrhs(ocean_gpu_handle* h, int32_t* nrhs) {
+  if (!h || !nrhs) return OCEAN_GPU_ERR_ARG;
+  *nrhs = h->nrhs;
   return OCEAN_GPU_OK;
 }
 
@@ -913,7 +950,6 @@ int ocean_gpu_destroy(ocean_gpu_handle* h) {
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
   dfree(h->amp); dfree(h->mcount);
   dfree(h->energy); dfree(h->allow); dfree(h->red); dfree(h->scal); dfree(h->tmp_in); dfree(h->tmp_out);
-  dfree(h->lrbuf);
   for (auto& v : h->vecs) dfree(v);
   if (h->host_pin) (void)hipHostFree(h->host_pin);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1025,6 +1061,7 @@ int ocean_gpu_load_obf(ocean_gpu_handle* h, int32_t num_obf, const double* obf_r
   if (!h || num_obf <= 0 || !obf_re || !obf_im || !obf2u_re || !obf2u_im || !xoff) return OCEAN_GPU_ERR_ARG;
   // OCEAN_obf_load handles nspn = 1 only (OCEAN_obf.f90:669-673); the form is FP64
   if (h->ns != 1 || h->precision != OCEAN_GPU_FP64) return OCEAN_GPU_ERR_STATE;
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   const int nb = h->nb, nk = h->nk, nx = h->nx, no = num_obf;
   h->obf_on = false;
   dfree(h->obf_u); dfree(h->obf_uT); dfree(h->obf_f); dfree(h->obf_fT); dfree(h->obf_phs); dfree(h->obf_beta);
@@ -1130,7 +1167,9 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
                         int32_t have_so) {
   if (!h || !mpcr || !mpci || !mpm || !mhr || !mhi || !somelr || !someli) return OCEAN_GPU_ERR_ARG;
   const ocean_bse_desc& d = h->d;
-  const int nb = h->nb, nk = h->nk, ns = h->ns, na = h->na, P = h->P, lc = d.lc;
+  // the blocks of one right-hand side (na0 alphas); with several right-hand sides K
+  // is block diagonal over them and S is applied per right-hand side
+  const int nb = h->nb, nk = h->nk, ns = h->ns, na = h->na0, P = h->P, lc = d.lc;
   const int lmin = d.lmin, lmax = d.lmax, nl = lmax - lmin + 1;
   int npmax = 0;
   for (int l = lmin; l <= lmax; ++l) npmax = std::max(npmax, d.nproj[l - lmin]);
@@ -1183,20 +1222,25 @@ int ocean_gpu_load_mult(ocean_gpu_handle* h, const double* mpcr, const double* m
     }
     jb += (size_t)mham * mham;
   }
-  std::vector<double2> Kd((size_t)nj * nj);
-  for (size_t i = 0; i < Kd.size(); ++i) Kd[i] = make_double2(K[i].real(), K[i].imag());
+  const int njt = h->na * P;  // = nrhs nj
+  std::vector<double2> Kd((size_t)njt * njt, make_double2(0.0, 0.0));
+  for (int r = 0; r < h->nrhs; ++r)
+    for (int i = 0; i < nj; ++i)
+      for (int j = 0; j < nj; ++j)
+        Kd[(size_t)(r * nj + i) * njt + r * nj + j] = make_double2(K[(size_t)i * nj + j].real(),
+                                                                   K[(size_t)i * nj + j].imag());
   std::vector<double2> S((size_t)na * na);
   for (int a = 0; a < na; ++a)
     for (int b = 0; b < na; ++b) S[a * na + b] = make_double2(somelr[(size_t)b * na + a], someli[(size_t)b * na + a]);
   dfree(h->cproj); dfree(h->Kmat); dfree(h->apart); dfree(h->hmult); dfree(h->Smat);
   dfree(h->amp); dfree(h->mcount);
   OG_TRY(dmalloc(&h->cproj, c.size()));
-  OG_TRY(dmalloc(&h->amp, (size_t)nj));
+  OG_TRY(dmalloc(&h->amp, (size_t)njt));
   OG_TRY(dmalloc(&h->mcount, (size_t)1));
   OG_CHECK(hipMemset(h->mcount, 0, sizeof(int)));
   OG_TRY(dmalloc(&h->Kmat, Kd.size()));
-  OG_TRY(dmalloc(&h->apart, (size_t)nk * nj));
-  OG_TRY(dmalloc(&h->hmult, (size_t)nj));
+  OG_TRY(dmalloc(&h->apart, (size_t)nk * njt));
+  OG_TRY(dmalloc(&h->hmult, (size_t)njt));
   OG_TRY(dmalloc(&h->Smat, S.size()));
   OG_CHECK(hipMemcpy(h->cproj, c.data(), c.size() * sizeof(double2), hipMemcpyHostToDevice));
   OG_CHECK(hipMemcpy(h->Kmat, Kd.data(), Kd.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -1211,7 +1255,7 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
                          int32_t* have_so) {
   if (!h || !slater || !mhr || !mhi || !somelr || !someli || !have_so) return OCEAN_GPU_ERR_ARG;
   const ocean_bse_desc& d = h->d;
-  const int lc = d.lc, na = h->na;
+  const int lc = d.lc, na = h->na0;
   // per valence l: the block size and where its Slater matrices and block start
   std::vector<int> npl_l, nsl_l;
   std::vector<size_t> jb_l, sb_l;
@@ -1272,9 +1316,14 @@ int ocean_gpu_mult_setup(ocean_gpu_handle* h, const double* slater, double scfac
 
 int ocean_gpu_load_energies(ocean_gpu_handle* h, const double* e_re, const double* e_im, const double* allow) {
   if (!h || !e_re || !allow) return OCEAN_GPU_ERR_ARG;
-  const size_t nv = h->nvec();
-  OG_TRY(put_planes(h, e_re, e_im, h->energy, nv));
-  OG_CHECK(hipMemcpy(h->allow, allow, nv * sizeof(double), hipMemcpyHostToDevice));
+  // one right-hand side's arrays; every right-hand side of the handle sees the same
+  const size_t nv0 = h->nvec0();
+  OG_TRY(put_planes(h, e_re, e_im, h->energy, nv0));
+  OG_CHECK(hipMemcpy(h->allow, allow, nv0 * sizeof(double), hipMemcpyHostToDevice));
+  for (int r = 1; r < h->nrhs; ++r) {
+    OG_CHECK(hipMemcpy(h->energy + r * nv0, h->energy, nv0 * sizeof(double2), hipMemcpyDeviceToDevice));
+    OG_CHECK(hipMemcpy(h->allow + r * nv0, h->allow, nv0 * sizeof(double), hipMemcpyDeviceToDevice));
+  }
   h->has_e = true;
   return OCEAN_GPU_OK;
 }
@@ -1458,6 +1507,7 @@ struct EarlyExit {
 
 int haydock_run(H* h, const double* psi0_re, const double* psi0_im, int niter, const EarlyExit* ex,
                 double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;  // ocean_gpu_haydock_multi
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
   const size_t nv = h->nvec();
@@ -1568,6 +1618,88 @@ int ocean_gpu_haydock_early(ocean_gpu_handle* h, const double* psi0_re, const do
   ex.eh = eh;
   ex.ne = ne;
   return haydock_run(h, psi0_re, psi0_im, niter, &ex, real_a, real_b, imag_a, niter_done);
+}
+
+// The recursions of every right-hand side of the handle at once: one action over the
+// stacked vector per iteration (each u stream serves all nrhs right-hand sides), then
+// the three-term steps of OCEAN_hay_ab_twoterm (OCEAN2/OCEAN_haydock.f90:824-938) per
+// right-hand side in one launch each, with the fixed-order reductions of the
+// one-right-hand-side recursion.
+int ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
+                            double* real_a, double* real_b, double* imag_a, int32_t* niter_done) {
+  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  const int R = h->nrhs;
+  const size_t nv = h->nvec(), nv0 = h->nvec0();
+  const size_t n1 = (size_t)niter + 1;
+  const size_t nscal = 3 * n1 + 3 * (size_t)kRedBlocks;  // a, ai, b, part1 (2 kRedBlocks), part2
+  double2* V[3] = {nullptr, nullptr, nullptr};
+  double* S = nullptr;
+  int* dcount = nullptr;  // [0] iteration number, [1] arrival counter
+  og::HayArgs* dhay[3] = {nullptr, nullptr, nullptr};
+  int rc = OCEAN_GPU_OK;
+  for (int i = 0; i < 3 && rc == 0; ++i) rc = dmalloc(&V[i], nv);
+  if (rc == 0) rc = dmalloc(&S, (size_t)R * nscal);
+  if (rc == 0) rc = dmalloc(&dcount, (size_t)2);
+  for (int i = 0; i < 3 && rc == 0; ++i) rc = dmalloc(&dhay[i], (size_t)R);
+  if (rc == 0) rc = put_planes(h, psi0_re, psi0_im, V[0], nv);
+  // rotation r of an iteration: (psi, old, hv) = (V0, V1, V2), (V2, V0, V1), (V1, V2, V0)
+  const int rot[3][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}};
+  for (int r = 0; r < 3 && rc == 0; ++r) {
+    std::vector<og::HayArgs> ha((size_t)R);
+    for (int q = 0; q < R; ++q) {
+      double* sq = S + (size_t)q * nscal;
+      ha[q] = og::HayArgs{V[rot[r][2]] + q * nv0, V[rot[r][1]] + q * nv0, V[rot[r][0]] + q * nv0,
+                          sq + 3 * n1, sq + 3 * n1 + 2 * kRedBlocks, sq, sq + n1, sq + 2 * n1, nv0};
+    }
+    if (hipMemcpy(dhay[r], ha.data(), sizeof(og::HayArgs) * R, hipMemcpyHostToDevice) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  // psi_old = 0, coefficients 0 (b_0 = 0), iteration 1
+  if (rc == 0) {
+    og::k_fill<<<grid_for(nv), kBlock, 0, h->stream>>>(V[1], make_double2(0.0, 0.0), nv);
+    og::k_fill_d<<<grid_for((size_t)R * nscal), kBlock, 0, h->stream>>>(S, 0.0, (size_t)R * nscal);
+    const int init[2] = {1, 0};
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(dcount, init, sizeof(init), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  for (int it = 1; it <= niter && rc == 0; ++it) {
+    const int r = (it - 1) % 3;
+    rc = xact_dev(h, V[rot[r][0]], V[rot[r][2]], f);
+    if (rc) break;
+    og::k_hay1_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount);
+    og::k_hay2_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount);
+    og::k_hay3_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount, dcount + 1);
+    if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  // coefficients back to the host, NaN check per right-hand side as the single recursion
+  int rc_nan = OCEAN_GPU_OK;
+  std::vector<double> hb(3 * n1);
+  for (int q = 0; q < R && rc == 0; ++q) {
+    if (hipMemcpy(hb.data(), S + (size_t)q * nscal, hb.size() * sizeof(double), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      rc = OCEAN_GPU_ERR_HIP;
+      break;
+    }
+    int done = niter;
+    for (int i = 0; i < niter; ++i)
+      if (hb[i] != hb[i]) { done = i + 1; rc_nan = OCEAN_GPU_ERR_NAN; break; }
+    for (size_t i = 0; i < n1; ++i) {
+      real_a[q * n1 + i] = (int)i < done ? hb[i] : 0.0;
+      real_b[q * n1 + i] = (int)i <= done ? hb[2 * n1 + i] : 0.0;
+      if (imag_a) imag_a[q * n1 + i] = (int)i < done ? hb[n1 + i] : 0.0;
+    }
+    if (niter_done) niter_done[q] = done;
+  }
+  for (auto& v : V) dfree(v);
+  for (auto& d : dhay) dfree(d);
+  dfree(S);
+  dfree(dcount);
+  return rc ? rc : rc_nan;
 }
 
 // ---- spectrum, OCEAN 3.2.1 write_core (nine columns / Voigt) ----
@@ -1787,6 +1919,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
               double gprc_eV, double ffff, int depth, double kpref, double* out, double2* xs_dev,
               double* x_re, double* x_im) {
   if (depth > kMaxGmresDepth) return OCEAN_GPU_ERR_ARG;
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   const int f = flags_of(h, 0, nullptr);
   OG_TRY(check_ready(h, f));
   const size_t nv = h->nvec();
@@ -2080,6 +2213,7 @@ int ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const dou
   if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !target || !loss)
     return OCEAN_GPU_ERR_ARG;
   if (h->obf_on) return OCEAN_GPU_ERR_UNSUPPORTED;  // the W derivative is taken on the Bloch form
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   if (g_ptab && (h->nptab <= 0 || !h->w_i0)) return OCEAN_GPU_ERR_STATE;  // W must come from build_w
   const size_t nv = h->nvec();
   double2* rhs = nullptr;
@@ -2159,6 +2293,7 @@ int ocean_gpu_gmres(ocean_gpu_handle* h, const double* hay_re, const double* hay
                     const double* energies_eV, double gres_eV, double gprc_eV, double ffff, int32_t depth,
                     double kpref, double* out, double* x_re, double* x_im) {
   if (!h || !hay_re || !hay_im || nE <= 0 || !energies_eV || depth <= 0 || !out) return OCEAN_GPU_ERR_ARG;
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   double2* rhs = nullptr;
   OG_TRY(gmres_rhs(h, hay_re, hay_im, &rhs));
   const int rc = gmres_run(h, rhs, nE, energies_eV, gres_eV, gprc_eV, ffff, depth, kpref, out, nullptr, x_re, x_im);
@@ -2241,11 +2376,11 @@ int build_tables(const B* b, const B::Group& g, const std::vector<const double2*
     off[q + 1] = off[q] + ge.nblk_proj;
     off[(np + 1) + q + 1] = off[(np + 1) + q] + ge.nblk_fft;
     off[2 * (np + 1) + q + 1] = off[2 * (np + 1) + q] + ge.nblk_adj;
-    // many alphas in FP64: the long-range sums on the matrix cores, as in xact_t
+    // many alphas on one spin: the matrix-core adjoint, as in xact_t
     int nlr = 0;
-    if (std::is_same<T2, double2>::value && NA >= 6 && h->lrbuf && h->ns == 1 && h->na == NA) {
-      host[q].lrbuf = h->lrbuf;
+    if (adj_mf<T2, NA>(h, f, t)) {
       nlr = ((h->nb + 63) / 64) * h->nk;
+      off[2 * (np + 1) + q + 1] = off[2 * (np + 1) + q];
     }
     off[3 * (np + 1) + q + 1] = off[3 * (np + 1) + q] + nlr;
     tb.sm[0] = std::max(tb.sm[0], ge.sm_proj);
@@ -2286,12 +2421,14 @@ int launch_tables(const Tables& tb, hipStream_t st, int stages) {
     }
   }
   if (stages & 4) {
-    if constexpr (NA >= 6 && std::is_same<T2, double2>::value) {
+    if constexpr (NA >= 6) {
       if (tb.nblk[3] > 0)
         og::k_adj_lr_batch<T2, NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
     }
-    OG_TRY(lds_opt_in(og::k_adj_batch<T2, NA>, tb.sm[2]));
-    og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+    if (tb.nblk[2] > 0) {
+      OG_TRY(lds_opt_in(og::k_adj_batch<T2, NA>, tb.sm[2]));
+      og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+    }
   }
   OG_CHECK(hipGetLastError());
   return OCEAN_GPU_OK;
@@ -2323,16 +2460,23 @@ void group_vectors(const B::Group& g, const std::vector<const double2*>& in_all,
 }
 
 // Time `stages` of group g over reps launches (ms per launch), after one warm-up.
+// Average ms per launch of the given stages: one untimed launch, then reps timed
+// ones, repeated with more launches when reps of them take under 2 ms (a few
+// launches of a short stage measure clock ramps and launch gaps, not the kernel).
 int time_group(B* b, const B::Group& g, const Tables& tb, int stages, int reps, float* ms) {
   hipEvent_t e0, e1;
   OG_CHECK(hipEventCreate(&e0));
   OG_CHECK(hipEventCreate(&e1));
   int rc = group_launch(g, tb, b->stream, stages);
-  if (rc == 0) rc = hipEventRecord(e0, b->stream) == hipSuccess ? 0 : OCEAN_GPU_ERR_HIP;
-  for (int r = 0; r < reps && rc == 0; ++r) rc = group_launch(g, tb, b->stream, stages);
-  if (rc == 0 && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                  hipEventElapsedTime(ms, e0, e1) != hipSuccess))
-    rc = OCEAN_GPU_ERR_HIP;
+  for (int pass = 0; pass < 2 && rc == 0; ++pass) {
+    if (rc == 0) rc = hipEventRecord(e0, b->stream) == hipSuccess ? 0 : OCEAN_GPU_ERR_HIP;
+    for (int r = 0; r < reps && rc == 0; ++r) rc = group_launch(g, tb, b->stream, stages);
+    if (rc == 0 && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+      rc = OCEAN_GPU_ERR_HIP;
+    if (rc != 0 || *ms >= 2.0f) break;
+    reps = std::min(1000, (int)(2.0f * reps / std::max(*ms, 1e-3f)) + 1);
+  }
   if (rc == 0) *ms /= (float)reps;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -2416,6 +2560,7 @@ int ocean_gpu_batch_group_info(ocean_gpu_batch* b, int32_t group, int32_t* info,
   info[3] = g.tiles.proj;
   info[4] = g.tiles.adj;
   info[5] = g.tiles.G;
+  info[6] = g.tiles.adj_mf;
   if (tune_ms)
     for (int i = 0; i < 3; ++i) tune_ms[i] = g.tune_ms[i];
   return OCEAN_GPU_OK;
@@ -2444,17 +2589,26 @@ int ocean_gpu_batch_autotune(ocean_gpu_batch* b, int32_t reps) {
     const int proj_c[3] = {128, 256, 512};
     const int adj_c[2] = {256, 512};
     const int G_c[5] = {0, 1, 2, 4, 8};
+    // the adjoint's form: VALU at either thread budget, or the matrix-core form
+    // where the group holds problems it applies to (many alphas on one spin)
+    bool mf_ok = false;
+    for (int q : g.idx) mf_ok = mf_ok || (g.NA >= 6 && b->hs[q]->ns == 1 && b->hs[q]->na == g.NA);
     Tiles best = g.tiles;
     float bms[3] = {1e30f, 1e30f, 1e30f};
     for (int stage = 0; stage < 3; ++stage) {
-      const int nc = stage == 0 ? 3 : (stage == 1 ? 5 : 2);
+      const int nc = stage == 0 ? 3 : (stage == 1 ? 5 : (mf_ok ? 3 : 2));
       for (int c = 0; c < nc; ++c) {
         Tiles t = best;
         if (stage == 0) t.proj = proj_c[c];
         if (stage == 1) t.G = G_c[c];
         if (stage == 2) {
-          if (adj_c[c] < nbmax) continue;
-          t.adj = adj_c[c];
+          if (c == 2) {
+            t.adj_mf = 1;
+          } else {
+            if (adj_c[c] < nbmax) continue;
+            t.adj = adj_c[c];
+            if (mf_ok) t.adj_mf = 0;
+          }
         }
         Tables tb;
         if (group_build(b, g, in, out, t, tb) != OCEAN_GPU_OK) continue;  // shape not possible
@@ -2466,7 +2620,10 @@ int ocean_gpu_batch_autotune(ocean_gpu_batch* b, int32_t reps) {
           bms[stage] = ms;
           if (stage == 0) best.proj = t.proj;
           if (stage == 1) best.G = t.G;
-          if (stage == 2) best.adj = t.adj;
+          if (stage == 2) {
+            best.adj = t.adj;
+            best.adj_mf = t.adj_mf;
+          }
         }
       }
     }
@@ -2495,11 +2652,25 @@ int ocean_gpu_autotune(ocean_gpu_handle* h, int32_t reps) {
   return rc;
 }
 
+int ocean_gpu_set_tiles(ocean_gpu_handle* h, const int32_t* tiles) {
+  if (!h || !tiles) return OCEAN_GPU_ERR_ARG;
+  const int proj = tiles[0], adj = tiles[1], G = tiles[2], mf = tiles[3];
+  if (proj < 64 || proj > 1024 || (proj & 63) || adj < 64 || adj > 1024 || (adj & 63) || G < 0 || G > 64 ||
+      mf < -1 || mf > 1)
+    return OCEAN_GPU_ERR_ARG;
+  h->tiles.proj = proj;
+  h->tiles.adj = adj;
+  h->tiles.G = G;
+  h->tiles.adj_mf = mf;
+  return OCEAN_GPU_OK;
+}
+
 int ocean_gpu_get_tiles(ocean_gpu_handle* h, int32_t* tiles) {
   if (!h || !tiles) return OCEAN_GPU_ERR_ARG;
   tiles[0] = h->tiles.proj;
   tiles[1] = h->tiles.adj;
   tiles[2] = h->tiles.G;
+  tiles[3] = h->tiles.adj_mf;
   return OCEAN_GPU_OK;
 }
 
@@ -2551,6 +2722,10 @@ int ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, con
                             double* const* imag_a, int32_t* niter_done) {
   if (!b || !psi_re || !psi_im || niter <= 0 || !real_a || !real_b) return OCEAN_GPU_ERR_ARG;
   const int n = (int)b->hs.size();
+  // one recursion per problem: a problem with several right-hand sides runs
+  // ocean_gpu_haydock_multi on its own
+  for (H* h : b->hs)
+    if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   const size_t n1 = (size_t)niter + 1;
   const size_t nscal = 3 * n1 + 3 * (size_t)kRedBlocks;
   std::vector<double2*> V((size_t)3 * n, nullptr);
@@ -2678,6 +2853,8 @@ int ocean_gpu_batch_gmres(ocean_gpu_batch* b, const double* const* hay_re, const
   const int n = (int)b->hs.size();
   for (int q = 0; q < n; ++q)
     if (!hay_re[q] || !hay_im[q] || nE[q] <= 0 || !energies_eV[q] || !out[q]) return OCEAN_GPU_ERR_ARG;
+  for (H* h : b->hs)
+    if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
   constexpr int kGmresWorkers = 16;
   std::vector<int> rc(n, OCEAN_GPU_OK);
   std::atomic<int> next{0};

@@ -39,6 +39,8 @@ struct Tiles {
   int proj = 256;  // projection workgroup size (x-points per workgroup)
   int adj = 512;   // adjoint thread budget (x-splits = adj / nb)
   int G = 0;       // transform columns per strip, 0 = the largest that fits
+  int adj_mf = -1; // many alphas on one spin: 1 the matrix-core adjoint with its fused
+                   // epilogue, 0 the VALU adjoint, -1 the default (matrix cores)
 };
 
 // accumulator of one 16x16x4 FP64 matrix-core instruction (four rows per lane)
@@ -175,6 +177,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= 4*N
 struct ocean_gpu_handle {
   ocean_bse_desc d{};
   int nb = 0, nk = 0, nx = 0, na = 0, ns = 0;
+  int nrhs = 1;     // right-hand sides sharing this Hamiltonian (a site's polarisations)
+  int na0 = 0;      // alphas of one right-hand side, 4 (2 lc + 1); na = nrhs na0
   int km[3] = {0, 0, 0};
   int P = 0;        // compact projector count sum_l (2l+1) nproj(l)
   // Padded leading dimensions: row strides that are large powers of two put the
@@ -241,7 +245,6 @@ struct ocean_gpu_handle {
   size_t host_pin_n = 0;
   double2* tmp_in = nullptr;   // host-plane xact staging
   double2* tmp_out = nullptr;
-  double2* lrbuf = nullptr;    // [a][k][n] long-range sums of the matrix-core adjoint (many alphas)
 
   std::vector<double2*> vecs;  // device vectors (OCEAN_vector analogue)
 
@@ -251,6 +254,7 @@ struct ocean_gpu_handle {
   double t_proj = 0, t_fft = 0, t_adj = 0, t_mult = 0, t_total = 0;
 
   size_t nvec() const { return (size_t)nb * nk * na; }
+  size_t nvec0() const { return (size_t)nb * nk * na0; }  // one right-hand side
   size_t nu() const { return (size_t)ns * nk * nb * ldu; }
   size_t nphi() const { return (size_t)nk * ldphi; }
   size_t nuT() const { return (size_t)ns * nk * ldT; }

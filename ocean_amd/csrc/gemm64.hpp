@@ -44,6 +44,8 @@ template <> struct Mf<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int kk, int r) { return kk + 4 * r; }
+  static __device__ __forceinline__ int kk_of(int row) { return row & 3; }
+  static __device__ __forceinline__ int r_of(int row) { return row >> 2; }
 };
 template <> struct Mf<float> {
   using V = og_f4;
@@ -51,40 +53,47 @@ template <> struct Mf<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int kk, int r) { return 4 * kk + r; }
+  static __device__ __forceinline__ int kk_of(int row) { return row >> 2; }
+  static __device__ __forceinline__ int r_of(int row) { return row & 3; }
 };
 
-// lr(n, k, a) = sum_x conj(uT(k, x, n)) phi'(k, x, a)          (long_range.f90:943-961)
-// for the many-alpha edges, written to lrbuf [a][k][n] (FP64) for the adjoint's
-// epilogue.  The VALU adjoint reads each phi'(k, x, :) as nalpha LDS broadcasts per
-// band row, which bounds it at twelve alphas; here it is the B operand of a GEMM
-// per k (M = bands, N = the 2 nalpha real columns, K = x), in the precision of the
-// stored u (FP64, or FP32 for the mixed form, which accumulates in FP32 as its VALU
-// adjoint does).  Workgroup: 64 bands (four M tiles) by all N tiles of one k, so
-// every uT element is read once; the four waves take quarters of the x range and
-// their tiles are added in LDS in a fixed order.  nspn = 1.
+// The adjoint for the many-alpha edges (one spin), on the matrix cores:
+//   lr(n, k, a) = sum_x conj(uT(k, x, n)) phi'(k, x, a)          (long_range.f90:943-961)
+// as a GEMM per k (M = bands, N = the 2 nalpha real columns, K = x) in the precision
+// of the stored u (FP64, or FP32 accumulating in FP32 for the mixed form), followed
+// in the same workgroup by the rest of OCEAN_xact for its 64 bands:
+//   fg(n, k, a)  = sum_p conj(c(n, k, p)) h(p, a)                  OCEAN_multiplet.f90:1943-1998
+//   out          = E psi + A (fg - lr) + S psi                     OCEAN_action.f90:337-372, :596-608
+// (and c psi - H psi for a GMRES step).  The VALU adjoint reads each phi'(k, x, :) as
+// nalpha LDS broadcasts per band row, which bounds it at twelve alphas; here
+// phi'(k, x, :) is the B operand.  Workgroup: 64 bands (four M tiles) by all N tiles
+// of one k, so every uT element is read once; the four waves take quarters of the x
+// range and their tiles are added in LDS in a fixed order.  The epilogue then gives
+// wave w the alphas w, w + 4, ... of the block's 64 bands (lane = band): rows of c
+// and the element's own psi, E and allow are coalesced band rows, h(p, a) is uniform
+// across the wave, and the spin-orbit term reads psi(:, k, n) and S staged in the LDS
+// area the tiles leave free.
 template <typename T2, int NA>
-__device__ __forceinline__ void adj_lr_body(const T2* __restrict__ uT, int ldT, const T2* __restrict__ phi,
-                                            int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx, int bx,
-                                            int k) {
+__device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k) {
   using T = typename Scalar<T2>::T;
   using M = Mf<T>;
   using V = typename M::V;
   constexpr int NT = (2 * NA + 15) / 16;
-  __shared__ T part[4][NT * 16][64];
+  __shared__ __align__(16) T part[4][NT * 16][64];  // also holds double2 staging after the tiles
+  const T2* __restrict__ uT = A.uT;
+  const T2* __restrict__ phi = A.phi;
+  const int nb = A.nb, nk = A.nk, nx = A.nx, ldT = A.ldT, ldphi = A.ldphi;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = lane & 15, kk = lane >> 4;
   const int n0 = bx * 64;
-  int ca[NT], cc[NT];
-  bool cok[NT];
+  int cc[NT];
   const T2* yp[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int G = t * 16 + i;
-    ca[t] = G >> 1;
     cc[t] = G & 1;
-    cok[t] = ca[t] < NA;
-    yp[t] = phi + (size_t)k * ldphi + min(ca[t], NA - 1);
+    yp[t] = phi + (size_t)k * ldphi + min(G >> 1, NA - 1);
   }
   const T2* __restrict__ ap = uT + (size_t)k * ldT;
   int nrow[4];
@@ -141,44 +150,121 @@ __device__ __forceinline__ void adj_lr_body(const T2* __restrict__ uT, int ldT, 
 #pragma unroll
       for (int r = 0; r < 4; ++r) part[w][16 * t + 4 * m + r][lane] = acc[t][m][r];
   __syncthreads();
-  if (w != 0) return;
+
+  // ---- epilogue: wave w, band n0 + lane, alphas w + 4 j ----
+  constexpr int APW = (NA + 3) / 4;
+  const int nl = lane, n = n0 + nl, nn = min(n, nb - 1);
+  // lr of this lane's band: column G = 2 a + c of tile t = G / 16 sits at lane group
+  // i = G % 16; band row nl = 16 m + (the instruction's output row of (kk, r))
+  const int mrow = nl >> 4, row = nl & 15;
+  const int lkk = M::kk_of(row), lr_ = M::r_of(row);
+  double2 res[APW];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    if (!cok[t]) continue;
-    double* __restrict__ out = reinterpret_cast<double*>(lrbuf + (size_t)ca[t] * nk * nb + (size_t)k * nb) + cc[t];
+  for (int j = 0; j < APW; ++j) {
+    const int a = min(w + 4 * j, NA - 1);
+    double v[2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int c = 0; c < 2; ++c) {
+      const int G = 2 * a + c, t = G >> 4, gi = G & 15;
+      const int rr = 16 * t + 4 * mrow + lr_, cl = lkk * 16 + gi;
+      T s = part[0][rr][cl];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = n0 + 16 * m + M::row(kk, r);
-        T v = part[0][16 * t + 4 * m + r][lane];
+      for (int q = 1; q < 4; ++q) s += part[q][rr][cl];
+      v[c] = (double)s;
+    }
+    res[j] = make_double2(-v[0], -v[1]);  // fg - lr starts at -lr
+  }
+  const int flags = A.flags;
+  const bool back = (flags & F_BACK) != 0;
+  const bool so = (flags & F_SO) != 0;
+  const int na0 = A.na0;
+  __syncthreads();  // every lane has its lr: the tile area is free
+  double2* Ps = reinterpret_cast<double2*>(&part[0][0][0]);  // psi(a, k, n0 + q) at Ps[a * 64 + q]
+  double2* Ss = Ps + NA * 64;                                 // S (conjugated backwards), na0 x na0
+  static_assert(sizeof(part) >= sizeof(double2) * (NA * 64 + NA * NA), "spin-orbit staging exceeds the tile area");
+  if (so) {
+    for (int e = threadIdx.x; e < NA * 64; e += blockDim.x) {
+      const int a = e >> 6, q = e & 63;
+      Ps[e] = A.psi[((size_t)a * nk + k) * nb + min(n0 + q, nb - 1)];
+    }
+    for (int e = threadIdx.x; e < na0 * na0; e += blockDim.x) {
+      double2 v = A.S[e];
+      if (back) v.y = -v.y;
+      Ss[e] = v;
+    }
+  }
+  // the element's own operands, and the projector rows (one spin: c[0][k][p][n])
+  double2 pv[APW], ev[APW];
+  double am[APW];
 #pragma unroll
-        for (int j = 1; j < 4; ++j) v += part[j][16 * t + 4 * m + r][lane];
-        if (n < nb) out[(size_t)n * 2] = (double)v;
-      }
+  for (int j = 0; j < APW; ++j) {
+    const int a = min(w + 4 * j, NA - 1);
+    const size_t off = ((size_t)a * nk + k) * nb + nn;
+    pv[j] = A.psi[off];
+    ev[j] = A.energy[off];
+    am[j] = A.allow[off];
+  }
+  if (flags & F_FG) {
+    const int P = A.P;
+    const double2* __restrict__ cb = A.cproj + (size_t)k * P * nb + nn;
+    const double2* __restrict__ hm = A.hmult;
+    constexpr int kC = 4;
+    for (int p0 = 0; p0 < P; p0 += kC) {
+      double2 cv[kC];
+#pragma unroll
+      for (int q = 0; q < kC; ++q) cv[q] = cb[(size_t)min(p0 + q, P - 1) * nb];
+#pragma unroll
+      for (int q = 0; q < kC; ++q)
+        if (p0 + q < P) {
+#pragma unroll
+          for (int j = 0; j < APW; ++j)
+            if (w + 4 * j < NA) cfmac(res[j], cv[q], hm[(w + 4 * j) * P + p0 + q]);
+        }
+    }
+  }
+  if (so) __syncthreads();
+  if (n >= nb) return;
+#pragma unroll
+  for (int j = 0; j < APW; ++j) {
+    const int a = w + 4 * j;
+    if (a >= NA) break;
+    double2 inter = res[j];
+    inter.x *= am[j];
+    inter.y *= am[j];
+    if (so) {
+      const int ar = a % na0, a0 = a - ar;
+      for (int b = 0; b < na0; ++b) cfma(inter, Ss[ar * na0 + b], Ps[(a0 + b) * 64 + nl]);
+    }
+    double2 r = inter;
+    if (flags & F_E0) {
+      double2 e = ev[j];
+      if (back) e.y = -e.y;
+      r.x = e.x * pv[j].x - e.y * pv[j].y + inter.x;
+      r.y = e.x * pv[j].y + e.y * pv[j].x + inter.y;
+    }
+    A.out[((size_t)a * nk + k) * nb + n] = axmy_epilogue(A, pv[j], r);
   }
 }
 
 template <typename T2, int NA>
-__global__ __launch_bounds__(256) void k_adj_lr(const T2* __restrict__ uT, int ldT, const T2* __restrict__ phi,
-                                                int ldphi, double2* __restrict__ lrbuf, int nb, int nk, int nx,
-                                                const int* __restrict__ skip) {
-  if (skip != nullptr && *skip != 0) return;
-  adj_lr_body<T2, NA>(uT, ldT, phi, ldphi, lrbuf, nb, nk, nx, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(256) void k_adj_lr(const ActArgs<T2> A) {
+  if (skipped(A)) return;
+  // consecutive workgroups take consecutive k of one band block
+  const int nbb = (A.nb + 63) / 64;
+  adj_lr_body<T2, NA>(A, blockIdx.x % nbb, blockIdx.x / nbb);
 }
 
 // Batched form: problem p owns workgroups [off[p], off[p+1]), numbered band block
-// fastest, then k; problems without lrbuf own none.
+// fastest, then k.
 template <typename T2, int NA>
 __global__ __launch_bounds__(256) void k_adj_lr_batch(const ActArgs<T2>* __restrict__ As,
                                                       const int* __restrict__ off, int np) {
   const int p = find_problem(off, np, blockIdx.x);
-  const ActArgs<T2>& A = As[p];
+  const ActArgs<T2> A = As[p];
   if (skipped(A)) return;
   const int nbb = (A.nb + 63) / 64;
   const int loc = blockIdx.x - off[p];
-  adj_lr_body<T2, NA>(A.uT, A.ldT, A.phi, A.ldphi, const_cast<double2*>(A.lrbuf), A.nb, A.nk, A.nx, loc % nbb,
-                      loc / nbb);
+  adj_lr_body<T2, NA>(A, loc % nbb, loc / nbb);
 }
 
 }  // namespace og
