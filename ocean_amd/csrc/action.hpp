@@ -112,6 +112,24 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
   const int astep = (ns == 2) ? 2 : 1;
   const int acnt = (ns == 2) ? na / 2 : na;
   const int nb4 = (nb + 3) & ~3;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int xb = (bx * (int)(blockDim.x >> 6) + wave) * 32;
+  const bool xon = xb < A.nx;  // waves past nx stay for the multiplet partials after the slab
+  const int m16 = lane & 15, kq = lane >> 4;
+  // x + 1 <= xb + 31 < nx + 31 < ldu: the pair stays inside the padded row
+  const float2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + min(xb, A.nx - 1) + 2 * m16;
+  // batches of kS four-row steps, double-buffered: the loads of batch b + 1 are
+  // issued before the products of batch b, so each wave keeps HBM busy while its
+  // matrix-core work runs.  The first batch is issued before the B staging below,
+  // so the slab stream starts while psi and allow are read.
+  constexpr int kS = 8;
+  const int nfull = nb / (4 * kS);
+  float4 qa[kS], qb[kS];
+  auto load_batch = [&](float4* q, int b) {
+#pragma unroll
+    for (int i = 0; i < kS; ++i) q[i] = ld_stream4(ub + (size_t)(b * 4 * kS + 4 * i + kq) * A.ldu);
+  };
+  if (xon && nfull > 0) load_batch(qa, 0);
   // each (n, alpha) element is read once (kF loads in flight per thread) and fills
   // its four B entries; columns past 2 acnt and rows past nb are zero
   const int nel = nb4 * acnt;
@@ -152,12 +170,7 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
     bt[n * kBRow + 32 + c] = 0.f;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int xb = (bx * (int)(blockDim.x >> 6) + wave) * 32;
-  if (xb < A.nx) {  // waves past nx stay for the multiplet partials after the slab
-    const int m16 = lane & 15, kq = lane >> 4;
-    // x + 1 <= xb + 31 < nx + 31 < ldu: the pair stays inside the padded row
-    const float2* ub = A.u + ((size_t)(s * nk + k) * nb) * A.ldu + xb + 2 * m16;
+  if (xon) {
     constexpr int NT = NA > 8 ? 2 : 1;  // column tiles (2 acnt <= 16 fits one)
     f32x4 ce[NT], co[NT];
 #pragma unroll
@@ -179,21 +192,10 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
         co[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w, c[t], co[t], 0, 0, 0);
       }
     };
-    // batches of kS four-row steps, double-buffered: the loads of batch b + 1 are
-    // issued before the products of batch b, so each wave keeps HBM busy while its
-    // matrix-core work runs
-    constexpr int kS = 8;
-    const int nfull = nb / (4 * kS);
-    float4 qa[kS], qb[kS];
-    auto load_batch = [&](float4* q, int b) {
-#pragma unroll
-      for (int i = 0; i < kS; ++i) q[i] = ld_stream4(ub + (size_t)(b * 4 * kS + 4 * i + kq) * A.ldu);
-    };
     auto use_batch = [&](const float4* q, int b) {
 #pragma unroll
       for (int i = 0; i < kS; ++i) step(q[i], bt + (b * 4 * kS + 4 * i + kq) * kBRow + m16);
     };
-    if (nfull > 0) load_batch(qa, 0);
     for (int b = 0; b < nfull; b += 2) {
       if (b + 1 < nfull) load_batch(qb, b + 1);
       __builtin_amdgcn_sched_barrier(0);
@@ -987,7 +989,7 @@ __global__ __launch_bounds__(1024) void k_mred(
 
 // Reduction run by nred extra workgroups of another kernel's grid, one wave per
 // amplitude: workgroup r sums amplitudes [r*kRedPerBlock, ...) over k into amp
-// (global); the last workgroup to finish (device-scope counter) applies K, eight
+// (global); the last workgroup to finish (device-scope counter) applies K, four
 // rows per wave at a time with all their loads in flight.  The counter returns
 // to zero for the next launch.  Loads are batched throughout: the whole tail is
 // a few memory latencies, so it hides under the kernel it rides on.
@@ -1013,8 +1015,10 @@ __device__ void mred_tail(const double2* __restrict__ apart, double2* __restrict
   }
   __syncthreads();
   // K is block diagonal over right-hand sides (blocks of nj0): each row sums its own
-  // block, and a pass of eight rows runs over the columns of the blocks they touch
-  constexpr int R = 8;
+  // block, and a pass of R rows runs over the columns of the blocks they touch
+  // four rows per pass: eight took the transform kernel this rides on to 198 VGPRs
+  // (two waves per SIMD) from 79
+  constexpr int R = 4;
   for (int i0 = wid * R; i0 < nj; i0 += nw * R) {
     double v[2 * R];
 #pragma unroll

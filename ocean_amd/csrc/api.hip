@@ -323,7 +323,14 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
   bool mf = false;
   if constexpr (NA >= 6) {
     mf = adj_mf<T2, NA>(h, flags, tl);
-    if (mf) og::k_adj_lr<T2, NA><<<((h->nb + 63) / 64) * h->nk, 256, 0, h->stream>>>(a);
+    if (mf) {
+      // the deep load ring where two workgroups per CU hold the whole grid
+      const int nwg = ((h->nb + 63) / 64) * h->nk;
+      if (nwg <= 2 * h->ncu)
+        og::k_adj_lr<T2, NA, true><<<nwg, 256, 0, h->stream>>>(a);
+      else
+        og::k_adj_lr<T2, NA, false><<<nwg, 256, 0, h->stream>>>(a);
+    }
   }
   if (!mf) {
     OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
@@ -778,25 +785,16 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
 // counter) the three sums in a fixed order and, with check, the reference's
 // stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and
 // (-|g|^2 / Im<rhs|x> < 0.001 or |g|^2 < ffff / 10).
-__global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, const double2* __restrict__ ax,
-                                                       const double2* __restrict__ rhs, const double2* __restrict__ x,
-                                                       double2* __restrict__ g, const double2* __restrict__ pc,
-                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
-                                                       double ffff, int it, int check) {
-  if (c->done) return;
+// The sums of |g|^2, <rhs|x> of one workgroup (v[3], block-summed in thread 0) into
+// its partials and, in the last workgroup to arrive (device-scope counter), the three
+// totals in a fixed order and, with check, the reference's stopping test
+// (OCEAN_gmres.f90:845-852): |g|^2 < ffff and (-|g|^2 / Im<rhs|x> < 0.001 or
+// |g|^2 < ffff / 10).  The counter returns to zero.
+__device__ __forceinline__ void resid_finish(GmCtl* __restrict__ c, double (&v)[3], double* __restrict__ part,
+                                             double ffff, int it, int check) {
   __shared__ double lds[4 * 3];
   __shared__ double sv[3];
   __shared__ int is_last;
-  double v[3] = {0, 0, 0};
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const double2 a = ax[e], b = rhs[e], xv = x[e];
-    const double2 gv = make_double2(a.x - b.x, a.y - b.y);
-    g[e] = gv;
-    if (unext) unext[e] = og::cmul(gv, pc[e]);
-    v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
-    v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
-    v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
-  }
   og::block_sum<3>(v, lds);
   if (threadIdx.x == 0) {
     part[3 * blockIdx.x] = v[0]; part[3 * blockIdx.x + 1] = v[1]; part[3 * blockIdx.x + 2] = v[2];
@@ -829,6 +827,75 @@ __global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, co
     }
     atomicExch(&c->count, 0);
   }
+}
+
+// One element's residual step: g = ax - rhs, the next basis vector g . pc
+// (OCEAN_gmres.f90:760-766), and its terms of |g|^2 and <rhs|x>.
+__device__ __forceinline__ void resid_elem(size_t e, double2 a, double2 xv, const double2* __restrict__ rhs,
+                                           double2* __restrict__ g, const double2* __restrict__ pc,
+                                           double2* __restrict__ unext, double (&v)[3]) {
+  const double2 b = rhs[e];
+  const double2 gv = make_double2(a.x - b.x, a.y - b.y);
+  g[e] = gv;
+  if (unext) unext[e] = og::cmul(gv, pc[e]);
+  v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
+  v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
+  v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
+}
+
+// The residual step of iteration it on its own (after an action has recomputed ax).
+__global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, const double2* __restrict__ ax,
+                                                       const double2* __restrict__ rhs, const double2* __restrict__ x,
+                                                       double2* __restrict__ g, const double2* __restrict__ pc,
+                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
+                                                       double ffff, int it, int check) {
+  if (c->done) return;
+  double v[3] = {0, 0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    resid_elem(e, ax[e], x[e], rhs, g, pc, unext, v);
+  resid_finish(c, v, part, ffff, it, check);
+}
+
+// x += sum_i c_i U_i and ax += sum_i c_i AU_i (update_gmres, OCEAN_gmres.f90:336-341;
+// eight rows of each per step with their loads in flight, each sum in i order),
+// followed in the same pass by the residual step of iteration it: the updated x and
+// ax are not read back.
+__global__ __launch_bounds__(kBlock) void k_axpy_resid(GmCtl* __restrict__ c, const double2* const* __restrict__ Ul,
+                                                       const double2* const* __restrict__ AUl,
+                                                       const double2* __restrict__ coef, int m,
+                                                       double2* __restrict__ x, double2* __restrict__ ax,
+                                                       const double2* __restrict__ rhs, double2* __restrict__ g,
+                                                       const double2* __restrict__ pc, double2* __restrict__ unext,
+                                                       size_t n, double* __restrict__ part, double ffff, int it,
+                                                       int check) {
+  if (c->done) return;
+  constexpr int R = 8;
+  double v[3] = {0, 0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double2 xv = x[e], av = ax[e];
+    for (int i0 = 0; i0 < m; i0 += R) {
+      double2 w[R], a[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        w[q] = Ul[min(i0 + q, m - 1)][e];
+        a[q] = AUl[min(i0 + q, m - 1)][e];
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (i0 + q < m) {
+          const double2 ci = coef[i0 + q];
+          xv.x += ci.x * w[q].x; xv.y += ci.x * w[q].y;
+          xv.x += -ci.y * w[q].y; xv.y += ci.y * w[q].x;
+          av.x += ci.x * a[q].x; av.y += ci.x * a[q].y;
+          av.x += -ci.y * a[q].y; av.y += ci.y * a[q].x;
+        }
+      }
+    }
+    x[e] = xv;
+    ax[e] = av;
+    resid_elem(e, av, xv, rhs, g, pc, unext, v);
+  }
+  resid_finish(c, v, part, ffff, it, check);
 }
 
 }  // namespace
@@ -888,10 +955,12 @@ int create_impl(const ocean_bse_desc* d, int nrhs, ocean_gpu_handle** out) {
   h->ldT = h->nx * h->nb + 32;
   h->precision = d->precision;
   {
-    int dev = 0, lds = 0;
+    int dev = 0, lds = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
       h->lds_max = (size_t)lds;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      h->ncu = ncu;
   }
   // the adjoint keeps one k's result (nalpha x nb complex) and one x-split row in LDS
   if (h->nb > 512 || (size_t)(h->na + 1) * h->nb * sizeof(double2) > h->lds_max) {
@@ -1989,14 +2058,20 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     return launched();
   };
   // update_gmres(cur): Gram column cur, least-squares solve and x / ax updates
-  // (OCEAN_gmres.f90:241-344)
-  auto update = [&](int cur) -> int {
+  // (OCEAN_gmres.f90:241-344); with rit >= 0 the updates carry the residual step of
+  // iteration rit (as resid(rit, check, unext))
+  auto update = [&](int cur, int rit = -1, int check = 0, double2* unext = nullptr) -> int {
     k_multidot2<<<kRedBlocks, kBlock, 0, st>>>(skip, (const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
     k_multifinish<<<(4 * cur + 3) / 4, 256, 0, st>>>(skip, part, kRedBlocks, 4 * cur, fin);
     k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, cur, depth, info);
-    k_multiaxpy2<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), (const double2* const*)dlist,
-                                          dcoef, cur, x, ax, nv);
+    if (rit >= 0)
+      k_axpy_resid<<<kRedBlocks, kBlock, 0, st>>>(ctl, (const double2* const*)(dlist + depth),
+                                                  (const double2* const*)dlist, dcoef, cur, x, ax, rhs, g, pc, unext,
+                                                  nv, rpart, ffff, rit, check);
+    else
+      k_multiaxpy2<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), (const double2* const*)dlist,
+                                            dcoef, cur, x, ax, nv);
     return launched();
   };
   auto read_ctl = [&]() -> int {
@@ -2037,10 +2112,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
       for (int i = 0; i < prev_iter; ++i) k_axpy_gated<<<grid, kBlock, 0, st>>>(skip, delta, 0.0, U[i], AU[i], nv);
       if ((rc = axmy_xact(x, ax, E))) break;
       if ((rc = resid(0, 0, nullptr))) break;
-      for (int it = 1; it <= prev_iter && rc == 0; ++it) {
-        if ((rc = update(it))) break;
-        rc = resid(0, 0, nullptr);
-      }
+      for (int it = 1; it <= prev_iter && rc == 0; ++it) rc = update(it, 0, 0, nullptr);
       if (rc || (rc = read_ctl())) break;
       const double gval = hctl->gval, ival = hctl->ival;
       if (gval < ffff) {
@@ -2068,12 +2140,15 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
           // later ones were formed by the previous iteration's residual step
           if (it == cycle_start + 1) k_cmul3<<<grid, kBlock, 0, st>>>(skip, U[it - 1], g, pc, nv);
           if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
-          if ((rc = update(it))) break;
-          if (it == depth) {
+          if (it < depth) {
+            rc = update(it, it, 1, U[it]);
+          } else {
+            // the end of a cycle recomputes ax from x before the residual
+            if ((rc = update(it))) break;
             if ((rc = axmy_xact(x, ax, E))) break;
             if (new_prec_at_end && (rc = set_prec(E))) break;
+            rc = resid(it, 1, nullptr);
           }
-          rc = resid(it, 1, it < depth ? U[it] : nullptr);
         }
         if (rc || (rc = read_ctl())) break;
         if (hctl->done == 1) {

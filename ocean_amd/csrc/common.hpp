@@ -190,6 +190,7 @@ struct ocean_gpu_handle {
   int precision = OCEAN_GPU_FP64;
   int device = 0;
   size_t lds_max = 65536;  // the device's LDS bytes per workgroup (160 KB on gfx950)
+  int ncu = 256;           // compute units of the device
   hipStream_t stream = nullptr;
 
   // long-range state

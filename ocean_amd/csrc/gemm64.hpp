@@ -73,7 +73,7 @@ template <> struct Mf<float> {
 // and the element's own psi, E and allow are coalesced band rows, h(p, a) is uniform
 // across the wave, and the spin-orbit term reads psi(:, k, n) and S staged in the LDS
 // area the tiles leave free.
-template <typename T2, int NA>
+template <typename T2, int NA, bool DEEP = false>
 __device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k) {
   using T = typename Scalar<T2>::T;
   using M = Mf<T>;
@@ -106,7 +106,12 @@ __device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k)
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[t][m] = V{0, 0, 0, 0};
   {
-    T2 yA[NT], aA[4], yB[NT], aB[4];
+    // D groups of four x in flight per wave (a ring of register buffers): the
+    // loads of group g + D - 1 are issued before the products of group g.  The deep
+    // ring (four groups in FP32, three in FP64) costs one wave of occupancy per SIMD,
+    // so it pays only when the whole grid is resident at once anyway.
+    constexpr int D = DEEP ? (sizeof(T) == 4 ? 4 : 3) : 2;
+    T2 yv[D][NT], av[D][4];
     auto load = [&](int xs, T2 (&yn)[NT], T2 (&an)[4]) {
       const int x = max(min(xs + kk, xb - 1), 0);
 #pragma unroll
@@ -133,14 +138,15 @@ __device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k)
 #pragma unroll
         for (int m = 0; m < 4; ++m) acc[t][m] = M::step(av[m].y, b1[t], acc[t][m]);
     };
-    load(xa, yA, aA);
-    for (int xs = xa; xs < xb; xs += 8) {
-      load(xs + 4, yB, aB);
-      asm volatile("" ::: "memory");  // keep the next group's loads ahead of this group's products
-      step(xs, yA, aA);
-      load(xs + 8, yA, aA);
-      asm volatile("" ::: "memory");
-      step(xs + 4, yB, aB);
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) load(xa + 4 * d, yv[d], av[d]);
+    for (int xs = xa; xs < xb; xs += 4 * D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        load(xs + 4 * (d + D - 1), yv[(d + D - 1) % D], av[(d + D - 1) % D]);
+        asm volatile("" ::: "memory");  // keep the later groups' loads ahead of this group's products
+        step(xs + 4 * d, yv[d], av[d]);
+      }
     }
   }
 #pragma unroll
@@ -246,12 +252,12 @@ __device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k)
   }
 }
 
-template <typename T2, int NA>
+template <typename T2, int NA, bool DEEP>
 __global__ __launch_bounds__(256) void k_adj_lr(const ActArgs<T2> A) {
   if (skipped(A)) return;
-  // consecutive workgroups take consecutive k of one band block
+  // consecutive workgroups take the band blocks of one k, then the next k
   const int nbb = (A.nb + 63) / 64;
-  adj_lr_body<T2, NA>(A, blockIdx.x % nbb, blockIdx.x / nbb);
+  adj_lr_body<T2, NA, DEEP>(A, blockIdx.x % nbb, blockIdx.x / nbb);
 }
 
 // Batched form: problem p owns workgroups [off[p], off[p+1]), numbered band block
