@@ -285,11 +285,12 @@ def obf_workload(inp=None, num_obf=128, reps=50):
 
 
 def multi_rhs_workload(inp, nrhs=3, reps=50, niter=None):
-    """Three polarisations of the 64-band K-edge site sharing its Hamiltonian
-    (ocean_gpu_create_multi): every u stream of the action serves all three right-hand
-    sides.  H.psi/s counts right-hand sides (one stacked action = nrhs H.psi); spectra/s
-    counts the nrhs recursions of one ocean_gpu_haydock_multi run plus their continued
-    fractions.  The mixed spectra are compared with the FP64 ones of the same vectors."""
+    """nrhs polarisations of the 64-band K-edge site sharing its Hamiltonian
+    (ocean_gpu_create_multi): every u stream of the action serves all of them.
+    H.psi/s counts right-hand sides (one stacked action = nrhs H.psi); spectra/s
+    counts the nrhs recursions of one ocean_gpu_haydock_multi run plus their
+    continued fractions.  The mixed spectra are compared with the FP64 ones of the
+    same vectors."""
     from ocean_amd import core, setup_host as sh
 
     p = inp.params
@@ -301,25 +302,26 @@ def multi_rhs_workload(inp, nrhs=3, reps=50, niter=None):
         z = np.asfortranarray(rng.standard_normal(inp.psi0.shape) + 1j * rng.standard_normal(inp.psi0.shape))
         psi0s.append(sh.normalise_psi0(z, inp.nk, inp.celvol)[0])
     stacked = np.concatenate(psi0s, axis=2)
-    out, spec = {"multi_rhs": nrhs}, {}
+    pre = f"multi{nrhs}_"
+    out, spec = {}, {}
     g = core.OceanGPU.from_inputs(inp, nrhs=nrhs)
     vin, vout = g.vec_new(stacked), g.vec_new()
     for prec, name in ((core.FP64, "fp64"), (core.MIXED, "mixed")):
         g.set_precision(prec)
         g.xact_repeat(vin, vout, 3)
         ms = g.xact_repeat(vin, vout, reps)
-        out[f"multi_hpsi_per_s_{name}"] = round(nrhs * 1e3 / ms, 1)
-        out[f"multi_per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in g.profile_xact(vin, vout, reps).items()}
+        out[f"{pre}hpsi_per_s_{name}"] = round(nrhs * 1e3 / ms, 1)
+        out[f"{pre}per_kernel_ms_{name}"] = {k: round(v, 5) for k, v in g.profile_xact(vin, vout, reps).items()}
         log(f"multi-rhs {name}: {ms:.4f} ms per stacked H.psi ({nrhs} right-hand sides)")
         t0 = time.perf_counter()
         a, b, _ = g.haydock_multi(psi0s, n)
         sp = [core.spectrum(a[r], b[r], n, inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])[:, 2]
               for r in range(nrhs)]
-        out[f"multi_spectra_per_s_{name}"] = round(nrhs / (time.perf_counter() - t0), 3)
+        out[f"{pre}spectra_per_s_{name}"] = round(nrhs / (time.perf_counter() - t0), 3)
         spec[name] = np.array(sp)
     g.close()
     pk = np.abs(spec["fp64"]).max(axis=1)
-    out["multi_mixed_spectrum_err_of_peak"] = float((np.abs(spec["mixed"] - spec["fp64"]).max(axis=1) / pk).max())
+    out[f"{pre}mixed_spectrum_err_of_peak"] = float((np.abs(spec["mixed"] - spec["fp64"]).max(axis=1) / pk).max())
     return out
 
 
@@ -384,7 +386,11 @@ def main():
         return
     if args.multi_only:
         from ocean_amd import synth
-        print(json.dumps(multi_rhs_workload(synth.workload("k_nb64"), reps=args.steps, niter=40)), flush=True)
+        inp = synth.workload("k_nb64")
+        out = {}
+        for r in (3, 6):
+            out.update(multi_rhs_workload(inp, nrhs=r, reps=args.steps, niter=40))
+        print(json.dumps(out), flush=True)
         return
     if args.mixed_only or args.gmres_only:
         from ocean_amd import core, synth
@@ -473,9 +479,11 @@ def main():
         g.set_precision(core.FP64)
 
     if not args.no_extras and not args.no_multi:
-        extras.update(multi_rhs_workload(inp, reps=max(20, K // 4)))
-        extras["multi_mixed_speedup_vs_plain_fp64"] = round(extras["multi_hpsi_per_s_mixed"] / hpsi_fp64, 3)
-        extras["multi_fp64_speedup_vs_plain_fp64"] = round(extras["multi_hpsi_per_s_fp64"] / hpsi_fp64, 3)
+        for r in (3, 6):
+            extras.update(multi_rhs_workload(inp, nrhs=r, reps=max(20, K // 4)))
+            for name in ("mixed", "fp64"):
+                extras[f"multi{r}_{name}_speedup_vs_plain_fp64"] = round(
+                    extras[f"multi{r}_hpsi_per_s_{name}"] / hpsi_fp64, 3)
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})

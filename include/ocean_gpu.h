@@ -85,8 +85,8 @@ int  ocean_gpu_create(const ocean_bse_desc* desc, ocean_gpu_handle** out);
  * serves all of them.  ocean_gpu_load_energies, ocean_gpu_load_mult and
  * ocean_gpu_mult_setup take one right-hand side's arrays (nalpha alphas), which
  * every right-hand side shares.  The alphas per spin, nrhs * nalpha / nspn, must be
- * 2, 4, 6 or 12 (K edge: nrhs 1 or 3, or 6 with nspn 2; L2,3: nrhs 2 with nspn 2);
- * OCEAN_GPU_ERR_UNSUPPORTED otherwise.  Such a handle runs xact, the vector
+ * 2, 4, 6 or 12, or 24 with nspn 1 (K edge: nrhs 1, 3 or 6, or 6 with nspn 2; L2,3:
+ * nrhs 2); OCEAN_GPU_ERR_UNSUPPORTED otherwise.  Such a handle runs xact, the vector
  * operations, ocean_gpu_autotune and ocean_gpu_haydock_multi, and batched actions;
  * the single-right-hand-side solvers (Haydock, GMRES, gradient, batched Haydock and
  * GMRES) and the OBF form refuse it (OCEAN_GPU_ERR_UNSUPPORTED).  No reference

@@ -101,12 +101,17 @@ __device__ __forceinline__ void mproj_k(const double2* __restrict__ c, const dou
 // U[x(l % 16)][k = l / 16] and B[k = l / 16][c = l % 16] and receives D[4 (l / 16) + r]
 // [l % 16] in register r.  Each lane loads 16 bytes, u(n, x) and u(n, x + 1), which
 // feed two tiles (even and odd x) of 16 x points; one wave covers 32 x points, four
-// band rows per step, and two column tiles (acnt <= 12, 2 acnt <= 32).
+// band rows per step, and all column tiles (two for acnt <= 12, three for 24).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int kBRow = 80;  // floats per B row: re part 32, im part 32, 16 pad (no bank conflicts)
+// floats per half B row (the part multiplying re u, then the part multiplying im u)
+// and per row: halves of 32 (48 for three column tiles) and 16 floats of pad, so
+// the four band rows one instruction reads start 16 or 48 banks apart (no conflicts)
+template <int NA> constexpr int proj_bhalf() { return NA > 12 ? 48 : 32; }
+template <int NA> constexpr int proj_brow() { return 2 * proj_bhalf<NA>() + 16; }
 template <int NA>
 __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx, int k, int s) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kBRow = proj_brow<NA>(), HW = proj_bhalf<NA>();
   float* bt = reinterpret_cast<float*>(smem);  // [nb4][kBRow]: B rows (n, re) then (n, im)
   const int nb = A.nb, nk = A.nk, na = A.na, ns = A.ns;
   const int astep = (ns == 2) ? 2 : 1;
@@ -157,21 +162,21 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
         float* row = bt + n * kBRow;
         const float pr = (float)(pv[f].x * mv[f]), pim = (float)(pv[f].y * mv[f]);
         row[c] = pr;
-        row[32 + c] = -pim;
+        row[HW + c] = -pim;
         row[acnt + c] = pim;
-        row[32 + acnt + c] = pr;
+        row[HW + acnt + c] = pr;
       }
     }
   }
-  const int nz = 32 - 2 * acnt;
+  const int nz = HW - 2 * acnt;
   for (int i = threadIdx.x; i < nb4 * nz; i += blockDim.x) {
     const int n = i / nz, c = 2 * acnt + (i - n * nz);
     bt[n * kBRow + c] = 0.f;
-    bt[n * kBRow + 32 + c] = 0.f;
+    bt[n * kBRow + HW + c] = 0.f;
   }
   __syncthreads();
   if (xon) {
-    constexpr int NT = NA > 8 ? 2 : 1;  // column tiles (2 acnt <= 16 fits one)
+    constexpr int NT = (2 * NA + 15) / 16;  // column tiles (2 acnt <= 16 fits one)
     f32x4 ce[NT], co[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { ce[t] = f32x4{0, 0, 0, 0}; co[t] = f32x4{0, 0, 0, 0}; }
@@ -180,7 +185,7 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
     auto step = [&](const float4& q, const float* br) {
       float b[NT], c[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) { b[t] = br[16 * t]; c[t] = br[32 + 16 * t]; }
+      for (int t = 0; t < NT; ++t) { b[t] = br[16 * t]; c[t] = br[HW + 16 * t]; }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         ce[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x, b[t], ce[t], 0, 0, 0);

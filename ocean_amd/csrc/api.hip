@@ -100,7 +100,8 @@ int na_tmpl(int acnt) {
   if (acnt <= 2) return 2;
   if (acnt <= 4) return 4;
   if (acnt <= 6) return 6;
-  return 12;
+  if (acnt <= 12) return 12;
+  return 24;
 }
 
 // k-mesh edges of the LDS transform: 1..10, 12 and 16 as before, and the 2^a 3^b 5^c
@@ -193,14 +194,14 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
   // projection
   if (t.proj < 64 || t.proj > 512 || (t.proj & 63)) return OCEAN_GPU_ERR_ARG;
   // FP32 streaming workgroups cover two x points per thread; the FP32 matrix-core
-  // form (many alphas) covers 32 per wave and stages B as nb4 rows of kBRow floats
+  // form (many alphas) covers 32 per wave and stages B as nb4 rows of proj_brow floats
   const bool mf = sizeof(T) == 4 && NA >= 6;
   const bool mf64 = sizeof(T) == 8 && NA >= 6;
   const int xper = mf ? 32 * (t.proj / 64) : mf64 ? 16 * og::kProjMT64 * (t.proj / 64)
                                                   : t.proj * (sizeof(T) == 4 ? 2 : 1);
   a.px = (nx + xper - 1) / xper;
   // the FP64 matrix-core form (many alphas) forms A psi per lane and stages nothing
-  g.sm_proj = mf ? (size_t)((nb + 3) & ~3) * og::kBRow * sizeof(float)
+  g.sm_proj = mf ? (size_t)((nb + 3) & ~3) * og::proj_brow<NA>() * sizeof(float)
               : mf64 ? 0 : (size_t)acnt * nb * sizeof(T2);
   g.nblk_proj = a.px * nk * ns;
   // transform
@@ -247,7 +248,9 @@ int make_args(const H* h, const double2* in, double2* out, int flags, const Tile
 // at 12 alphas).  The tile can force either form.
 template <typename T2, int NA>
 bool adj_mf(const H* h, int flags, const Tiles& t) {
-  if (NA < 6 || !(flags & og::F_LR) || h->ns != 1 || h->na != NA) return false;
+  if (NA < 6 || h->ns != 1 || h->na != NA) return false;
+  if (NA > 12) return true;  // 24 alphas: the only adjoint (with lr off it runs the epilogue alone)
+  if (!(flags & og::F_LR)) return false;
   return t.adj_mf != 0;
 }
 
@@ -323,6 +326,8 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
   bool mf = false;
   if constexpr (NA >= 6) {
     mf = adj_mf<T2, NA>(h, flags, tl);
+    // 24 alphas (several right-hand sides on one spin) have the matrix-core adjoint only
+    if (NA > 12 && !mf) return OCEAN_GPU_ERR_UNSUPPORTED;
     if (mf) {
       // the deep load ring where two workgroups per CU hold the whole grid
       const int nwg = ((h->nb + 63) / 64) * h->nk;
@@ -332,9 +337,11 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
         og::k_adj_lr<T2, NA, false><<<nwg, 256, 0, h->stream>>>(a);
     }
   }
-  if (!mf) {
-    OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
-    og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
+  if constexpr (NA <= 12) {
+    if (!mf) {
+      OG_TRY(lds_opt_in(og::k_adj<T2, NA>, g.sm_adj));
+      og::k_adj<T2, NA><<<g.nblk_adj, g.adj_threads, g.sm_adj, h->stream>>>(a);
+    }
   }
   OG_CHECK(hipGetLastError());
   if (ev) OG_CHECK(hipEventRecord(ev[4], h->stream));
@@ -425,6 +432,7 @@ int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = 
   OG_DISPATCH(4)
   OG_DISPATCH(6)
   OG_DISPATCH(12)
+  OG_DISPATCH(24)
 #undef OG_DISPATCH
   return OCEAN_GPU_ERR_ARG;
 }
@@ -942,10 +950,10 @@ int create_impl(const ocean_bse_desc* d, int nrhs, ocean_gpu_handle** out) {
   h->na = nrhs * h->na0;
   h->ns = d->nspn;
   // the right-hand sides run as na alphas of the action kernels, whose alpha counts
-  // per spin are 2, 4, 6 and 12
+  // per spin are 2, 4, 6 and 12, and 24 on one spin (matrix-core adjoint only)
   {
     const int acnt = h->ns == 2 ? h->na / 2 : h->na;
-    if (nrhs > 1 && (acnt > 12 || na_tmpl(acnt) != acnt)) {
+    if (nrhs > 1 && (acnt > 24 || na_tmpl(acnt) != acnt || (acnt == 24 && h->ns != 1))) {
       delete h;
       return OCEAN_GPU_ERR_UNSUPPORTED;
     }
@@ -2427,6 +2435,7 @@ int group_call(const B::Group& g, F&& f) {
     case 4: return mixed ? f(std::integral_constant<int, 4>{}, float2{}) : f(std::integral_constant<int, 4>{}, double2{});
     case 6: return mixed ? f(std::integral_constant<int, 6>{}, float2{}) : f(std::integral_constant<int, 6>{}, double2{});
     case 12: return mixed ? f(std::integral_constant<int, 12>{}, float2{}) : f(std::integral_constant<int, 12>{}, double2{});
+    case 24: return mixed ? f(std::integral_constant<int, 24>{}, float2{}) : f(std::integral_constant<int, 24>{}, double2{});
     default: return OCEAN_GPU_ERR_ARG;
   }
 }
@@ -2500,9 +2509,13 @@ int launch_tables(const Tables& tb, hipStream_t st, int stages) {
       if (tb.nblk[3] > 0)
         og::k_adj_lr_batch<T2, NA><<<tb.nblk[3], 256, 0, st>>>(A, tb.off + 3 * (np + 1), np);
     }
-    if (tb.nblk[2] > 0) {
-      OG_TRY(lds_opt_in(og::k_adj_batch<T2, NA>, tb.sm[2]));
-      og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+    if constexpr (NA <= 12) {
+      if (tb.nblk[2] > 0) {
+        OG_TRY(lds_opt_in(og::k_adj_batch<T2, NA>, tb.sm[2]));
+        og::k_adj_batch<T2, NA><<<tb.nblk[2], tb.threads[2], tb.sm[2], st>>>(A, tb.off + 2 * (np + 1), np);
+      }
+    } else if (tb.nblk[2] > 0) {
+      return OCEAN_GPU_ERR_UNSUPPORTED;  // 24 alphas: the matrix-core adjoint only
     }
   }
   OG_CHECK(hipGetLastError());

@@ -105,7 +105,8 @@ __device__ __forceinline__ void adj_lr_body(const ActArgs<T2>& A, int bx, int k)
   for (int t = 0; t < NT; ++t)
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[t][m] = V{0, 0, 0, 0};
-  {
+  // without the long-range term (nhflag) only the epilogue runs: lr = 0
+  if (A.flags & F_LR) {
     // D groups of four x in flight per wave (a ring of register buffers): the
     // loads of group g + D - 1 are issued before the products of group g.  The deep
     // ring (four groups in FP32, three in FP64) costs one wave of occupancy per SIMD,

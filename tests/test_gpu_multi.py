@@ -33,6 +33,11 @@ CASES = {
     "k_nspn2_r6": (dict(nb=8, kmesh=(2, 2, 4), xmesh=(3, 3, 3), lc=0, nspn=2, seed=25), 6, False),
     # L2,3 with its own spin-orbit term, two polarisations on two spins
     "l23_nspn2_r2": (dict(nb=8, kmesh=(2, 2, 4), xmesh=(3, 3, 3), lc=1, nspn=2, seed=26, xi_eV=3.0), 2, False),
+    # 24 alphas on one spin: six K-edge polarisations, two L2,3 ones (three column
+    # tiles in both matrix-core products)
+    "k_r6": (dict(nb=16, kmesh=(4, 4, 4), xmesh=(4, 4, 4), lc=0, seed=27), 6, True),
+    "k_r6_wide": (dict(nb=72, kmesh=(2, 2, 2), xmesh=(8, 8, 8), lc=0, seed=28), 6, True),
+    "l23_r2": (dict(nb=13, kmesh=(3, 2, 2), xmesh=(3, 3, 5), lc=1, seed=29, xi_eV=4.0), 2, False),
 }
 
 
@@ -90,8 +95,9 @@ def _psi0s(inp, R, seed):
     return out
 
 
-def test_multi_haydock_matches_oracle_and_single_recursions():
-    inp, R = _inputs("k_r3")
+@pytest.mark.parametrize("name", ["k_r3", "k_r6"])
+def test_multi_haydock_matches_oracle_and_single_recursions(name):
+    inp, R = _inputs(name)
     P = to_problem(inp)
     n = 50
     psi0s = _psi0s(inp, R, 9)
@@ -153,10 +159,12 @@ def test_multi_haydock_nan_in_one_right_hand_side():
 def test_multi_refusals():
     kw = dict(nb=4, kmesh=(2, 2, 2), xmesh=(2, 2, 2), lc=0, seed=3)
     inp = synth.make_inputs(**kw)
-    # 8 alphas on one spin: no kernel instance
-    with pytest.raises(core.OceanGPUError) as e:
-        core.OceanGPU.from_inputs(inp, nrhs=2)
-    assert e.value.code == 9963
+    # 8 and 16 alphas on one spin, 24 per spin on two spins: no kernel instance
+    for nrhs, nspn in ((2, 1), (4, 1), (12, 2)):
+        inp_s = inp if nspn == 1 else synth.make_inputs(**dict(kw, nspn=2))
+        with pytest.raises(core.OceanGPUError) as e:
+            core.OceanGPU.from_inputs(inp_s, nrhs=nrhs)
+        assert e.value.code == 9963
     with core.OceanGPU.from_inputs(inp, nrhs=3) as g:
         rc = g.haydock(np.concatenate([inp.psi0] * 3, axis=2), 4, return_status=True)[-1]
         assert rc == 9963
