@@ -1713,7 +1713,7 @@ int ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const do
   const size_t nscal = 3 * n1 + 3 * (size_t)kRedBlocks;  // a, ai, b, part1 (2 kRedBlocks), part2
   double2* V[3] = {nullptr, nullptr, nullptr};
   double* S = nullptr;
-  int* dcount = nullptr;  // [0] iteration number, [1] arrival counter
+  int* dcount = nullptr;  // [0] iteration number ([1] spare)
   og::HayArgs* dhay[3] = {nullptr, nullptr, nullptr};
   int rc = OCEAN_GPU_OK;
   for (int i = 0; i < 3 && rc == 0; ++i) rc = dmalloc(&V[i], nv);
@@ -1749,7 +1749,8 @@ int ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const do
     if (rc) break;
     og::k_hay1_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount);
     og::k_hay2_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount);
-    og::k_hay3_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount, dcount + 1);
+    og::k_hay3_batch<<<R * kRedBlocks, kBlock, 0, h->stream>>>(dhay[r], dcount);
+    og::k_hay_advance<<<1, 64, 0, h->stream>>>(dcount);
     if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
   }
   if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
@@ -2402,7 +2403,7 @@ struct ocean_gpu_batch {
   std::vector<H*> hs;
   std::vector<Group> groups;
   hipStream_t stream = nullptr;
-  int* dcount = nullptr;  // [0] Haydock iteration number, [1] arrival counter
+  int* dcount = nullptr;  // [0] Haydock iteration number ([1] spare)
   double t_last = 0.0;    // ms per batched action (all problems) of the last timed call
 };
 
@@ -2871,7 +2872,8 @@ int ocean_gpu_batch_haydock(ocean_gpu_batch* b, const double* const* psi_re, con
     for (size_t gi = 0; gi < b->groups.size(); ++gi) OG_TRY(group_launch(b->groups[gi], tabs[r][gi], b->stream, 7));
     og::k_hay1_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount);
     og::k_hay2_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount);
-    og::k_hay3_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount, b->dcount + 1);
+    og::k_hay3_batch<<<n * kRedBlocks, kBlock, 0, b->stream>>>(dhay[r], b->dcount);
+    og::k_hay_advance<<<1, 64, 0, b->stream>>>(b->dcount);
     OG_CHECK(hipGetLastError());
     return OCEAN_GPU_OK;
   };
