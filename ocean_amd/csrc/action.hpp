@@ -46,6 +46,7 @@ struct ActArgs {
   int ldu, ldT, ldphi;
   int flags;
   int px;                     // projection x-tiles (blockDim.x x-points each)
+  int bstage;                 // FP64 matrix-core projection: A psi of the k staged in LDS
   int G, ncol, nfft, nred;    // transform: columns per strip, strips, reduction workgroups
   int K1, K2, K3;
   int XC, JR;                 // adjoint: phi' chunk, alphas per reduction round
@@ -136,7 +137,9 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
   };
   if (xon && nfull > 0) load_batch(qa, 0);
   // each (n, alpha) element is read once (kF loads in flight per thread) and fills
-  // its four B entries; columns past 2 acnt and rows past nb are zero
+  // its four B entries; columns past 2 acnt and rows past nb are zero.  (Bands
+  // fastest over the threads, which makes the reads contiguous, measured slower:
+  // its LDS writes land four banks apart, 52.5 -> 61.4 us at nb 64 and 12 alphas.)
   const int nel = nb4 * acnt;
   constexpr int kF = 8;
   for (int i0 = threadIdx.x; i0 < nel; i0 += kF * blockDim.x) {
@@ -363,7 +366,7 @@ __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = lane & 15, kk = lane >> 4;
   const int x0 = bx * (int)blockDim.x * MT / 4 + w * 16 * MT;
-  int ja[NT], jc[NT];
+  int ja[NT], jc[NT], jl[NT];
   bool jok[NT];
   size_t boff[NT];
 #pragma unroll
@@ -371,8 +374,41 @@ __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx
     const int G = t * 16 + i;
     jc[t] = G & 1;
     jok[t] = (G >> 1) < acnt;
-    ja[t] = s + min(G >> 1, acnt - 1) * astep;
+    jl[t] = min(G >> 1, acnt - 1);
+    ja[t] = s + jl[t] * astep;
     boff[t] = ((size_t)ja[t] * nk + k) * nb;
+  }
+  // A psi of this k staged once per workgroup (A.bstage; the host sets it when nb acnt
+  // complex values fit the LDS budget): read with bands fastest over the threads, so
+  // every load is a contiguous band row of one alpha, where each wave otherwise loads
+  // 2 NT scattered (alpha, band) pairs per lane and step from L2
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* Bs = reinterpret_cast<double2*>(smem);  // A psi(n, j) at Bs[n * acnt + j]
+  const bool staged = A.bstage != 0;
+  if (staged) {
+    const int nel = nb * acnt;
+    constexpr int kF = 8;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kF * blockDim.x) {
+      double2 pv[kF];
+      double mv[kF];
+#pragma unroll
+      for (int f = 0; f < kF; ++f) {
+        const int e = min(i0 + f * (int)blockDim.x, nel - 1);
+        const int j = e / nb, n = e - j * nb;
+        const size_t off = ((size_t)(s + j * astep) * nk + k) * nb + n;
+        pv[f] = A.psi[off];
+        mv[f] = A.allow[off];
+      }
+#pragma unroll
+      for (int f = 0; f < kF; ++f) {
+        const int e = i0 + f * (int)blockDim.x;
+        if (e < nel) {
+          const int j = e / nb, n = e - j * nb;
+          Bs[n * acnt + j] = make_double2(pv[f].x * mv[f], pv[f].y * mv[f]);
+        }
+      }
+    }
+    __syncthreads();
   }
   const double2* __restrict__ up = A.u + ((size_t)s * nk + k) * nb * A.ldu;
   int xr[MT];
@@ -388,9 +424,13 @@ __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx
     const int n = min(n0 + kk, nb - 1);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      double2 pv = A.psi[boff[t] + n];
-      const double m = A.allow[boff[t] + n];
-      bn[t] = make_double2(pv.x * m, pv.y * m);
+      if (staged) {
+        bn[t] = Bs[n * acnt + jl[t]];
+      } else {
+        double2 pv = A.psi[boff[t] + n];
+        const double m = A.allow[boff[t] + n];
+        bn[t] = make_double2(pv.x * m, pv.y * m);
+      }
     }
 #pragma unroll
     for (int m = 0; m < MT; ++m) an[m] = ld_stream(up + (size_t)n * A.ldu + xr[m]);
@@ -450,9 +490,10 @@ __device__ __forceinline__ void proj_dispatch(const ActArgs<T2>& A, int b) {
 // One problem: a 1-D grid of px * nk * ns slab tiles.  Workgroups are dispatched
 // round-robin over the 8 XCDs; the remap gives each XCD a contiguous range of tiles,
 // so the px tiles of one k (which stage the same A psi) share an L2.
-// The FP64 matrix-core form keeps four waves per SIMD (at most 128 VGPRs).
+// The FP64 matrix-core form at twelve alphas keeps four waves per SIMD (at most 128
+// VGPRs); at 24 alphas that cap spilled 96 bytes per lane, so it runs at three.
 template <typename T2, int NA>
-constexpr int kProjWaves = (sizeof(T2) == 16 && NA >= 6) ? 4 : 1;
+constexpr int kProjWaves = (sizeof(T2) == 16 && NA >= 6 && NA <= 12) ? 4 : 1;
 
 template <typename T2, int NA>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<T2, NA>))) void k_proj(

@@ -194,8 +194,10 @@ __global__ __launch_bounds__(kBlock) void k_hay3(double2* __restrict__ h, const 
 // Every problem gets kRedBlocks workgroups and the same partition as the
 // single-problem kernels, so a batched recursion reproduces each problem's
 // stand-alone coefficients bit for bit.  The iteration number lives on the
-// device (*itp) so that one captured iteration replays unchanged: k_hay_advance
-// moves it on after k_hay3_batch.
+// device so that one captured iteration replays unchanged: k_hay1_batch and
+// k_hay2_batch read it from itp[0]; the first workgroup of k_hay2_batch copies it to
+// itp[1], which k_hay3_batch reads while its first workgroup advances itp[0] (no
+// workgroup of k_hay3_batch reads itp[0], and the next k_hay1_batch runs after it).
 // ---------------------------------------------------------------------------
 struct HayArgs {
   double2* h;            // H psi on entry, psi_{i+1} on exit
@@ -229,12 +231,13 @@ __global__ __launch_bounds__(kBlock) void k_hay1_batch(const HayArgs* __restrict
   if (threadIdx.x == 0) { A.part1[2 * lb] = v[0]; A.part1[2 * lb + 1] = v[1]; }
 }
 
-__global__ __launch_bounds__(kBlock) void k_hay2_batch(const HayArgs* __restrict__ As, const int* __restrict__ itp) {
+__global__ __launch_bounds__(kBlock) void k_hay2_batch(const HayArgs* __restrict__ As, int* __restrict__ itp) {
   __shared__ double lds[8];
   __shared__ double s_a;
   const HayArgs A = As[blockIdx.x / kRedBlocks];
   const int lb = blockIdx.x % kRedBlocks;
-  const int it = *itp;
+  const int it = itp[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) itp[1] = it;
   if (threadIdx.x < 64) {
     const double ar = wave_sum_parts(A.part1, kRedBlocks, 2, 0, threadIdx.x);
     const double ai = wave_sum_parts(A.part1, kRedBlocks, 2, 1, threadIdx.x);
@@ -257,12 +260,12 @@ __global__ __launch_bounds__(kBlock) void k_hay2_batch(const HayArgs* __restrict
   if (threadIdx.x == 0) A.part2[lb] = v[0];
 }
 
-__global__ __launch_bounds__(kBlock) void k_hay3_batch(const HayArgs* __restrict__ As,
-                                                       const int* __restrict__ itp) {
+__global__ __launch_bounds__(kBlock) void k_hay3_batch(const HayArgs* __restrict__ As, int* __restrict__ itp) {
   __shared__ double s_inv;
   const HayArgs A = As[blockIdx.x / kRedBlocks];
   const int lb = blockIdx.x % kRedBlocks;
-  const int it = *itp;
+  const int it = itp[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) itp[0] = it + 1;
   if (threadIdx.x < 64) {
     const double b = sqrt(wave_sum_parts(A.part2, kRedBlocks, 1, 0, threadIdx.x));
     if (threadIdx.x == 0) {
@@ -279,12 +282,6 @@ __global__ __launch_bounds__(kBlock) void k_hay3_batch(const HayArgs* __restrict
   }
 }
 
-// The iteration number advances once every workgroup of the three steps has read it:
-// one thread after k_hay3_batch (an arrival counter in k_hay3_batch serialised one
-// device-scope atomic per workgroup on one address: 22 us for 1536 workgroups).
-__global__ void k_hay_advance(int* __restrict__ itp) {
-  if (threadIdx.x == 0) *itp += 1;
-}
 
 // ---------------------------------------------------------------------------
 // Continued fraction with the square-root terminator (write_core,
