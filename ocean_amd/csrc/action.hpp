@@ -128,7 +128,9 @@ __device__ __forceinline__ void proj_mfma_body(const ActArgs<float2>& A, int bx,
   // issued before the products of batch b, so each wave keeps HBM busy while its
   // matrix-core work runs.  The first batch is issued before the B staging below,
   // so the slab stream starts while psi and allow are read.
-  constexpr int kS = 8;
+  // (four at 24 alphas: 84.5 -> 66.8 us at nb 64, 111 instead of 133 VGPRs, so two
+  // 512-thread workgroups fit a CU)
+  constexpr int kS = NA > 12 ? 4 : 8;
   const int nfull = nb / (4 * kS);
   float4 qa[kS], qb[kS];
   auto load_batch = [&](float4* q, int b) {
@@ -384,7 +386,7 @@ __device__ __forceinline__ void proj_mf64_body(const ActArgs<double2>& A, int bx
   // 2 NT scattered (alpha, band) pairs per lane and step from L2
   extern __shared__ __align__(16) unsigned char smem[];
   double2* Bs = reinterpret_cast<double2*>(smem);  // A psi(n, j) at Bs[n * acnt + j]
-  const bool staged = A.bstage != 0;
+  const bool staged = NA <= 12 && A.bstage != 0;  // not built at 24 alphas (slower there)
   if (staged) {
     const int nel = nb * acnt;
     constexpr int kF = 8;
