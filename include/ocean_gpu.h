@@ -4,8 +4,10 @@
  *
  *   OCEAN_xact            OCEAN2/OCEAN_action.f90:286   -> ocean_gpu_xact, ocean_gpu_xact_vec
  *   OCEAN_haydock_do      OCEAN2/OCEAN_haydock.f90:167  -> ocean_gpu_haydock
+ *     (complex_haydock branch, :53-160, haydock_abc_1 :1250) -> ocean_gpu_haydock_abc
  *   OCEAN_gmres_do_recycle OCEAN2/OCEAN_gmres.f90:508   -> ocean_gpu_gmres
- *   write_core            OCEAN2/OCEAN_haydock.f90:1801 -> ocean_gpu_spectrum
+ *   write_core            OCEAN2/OCEAN_haydock.f90:1801 -> ocean_gpu_spectrum,
+ *                                                          ocean_gpu_spectrum_abc
  *   OCEAN_psi_* vector ops OCEAN2/OCEAN_psi.f90         -> ocean_gpu_vec_*
  *   module state loaded by lr_init / OCEAN_create_central / OCEAN_energies_load
  *                          (long_range.f90:108-260, OCEAN_multiplet.f90:248-570,
@@ -240,6 +242,18 @@ int  ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const d
                              int32_t niter, double* real_a, double* real_b, double* imag_a,
                              int32_t* niter_done);
 
+/* Non-Hermitian (bi-Lanczos) Haydock recursion: the reference's complex_haydock
+ * branch, OCEAN_haydock_nonHerm_do (OCEAN2/OCEAN_haydock.f90:53-160) with
+ * haydock_abc_1 (:1250-1433).  psi and back_psi start from psi0 under the allow
+ * mask; each iteration applies the action to psi and the backwards action (the
+ * conjugated spin-orbit and energy terms, ocean_gpu_xact with backwards = 1) to
+ * back_psi.  real_a .. imag_c have niter+1 entries (index 0..niter); b(0) = c(0) = 0.
+ * niter_done and the NaN return (-1) as ocean_gpu_haydock.  Refuses a
+ * multi-right-hand-side handle (OCEAN_GPU_ERR_UNSUPPORTED). */
+int  ocean_gpu_haydock_abc(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im,
+                           int32_t niter, double* real_a, double* imag_a, double* real_b,
+                           double* imag_b, double* real_c, double* imag_c, int32_t* niter_done);
+
 /* Recycled preconditioned GMRES over energies (OCEAN2/OCEAN_gmres.f90:508-908).
  * energies_eV[nE]; gres_eV, gprc_eV as bse.in; ffff the |g|^2 threshold; depth the
  * restart length.  out[6*nE]: per energy (E_eV, 1 - Re<b|x> kpref, -Im<b|x> kpref,
@@ -282,6 +296,15 @@ int  ocean_gpu_spectral_grad(ocean_gpu_handle* h, const double* hay_re, const do
 int  ocean_gpu_spectrum(const double* real_a, const double* real_b, const double* imag_a,
                         int32_t niter, double kpref, double gam, double el, double eh,
                         int32_t ne, int32_t literal_cmplx, double* out);
+
+/* ocean_gpu_spectrum with complex b and c, the bi-Lanczos coefficients of
+ * ocean_gpu_haydock_abc (write_core with the products cmplx(b) cmplx(c),
+ * OCEAN2/OCEAN_haydock.f90:1846-1847, :1870).  With imag_b = imag_c = 0 and
+ * real_c = real_b it equals ocean_gpu_spectrum. */
+int  ocean_gpu_spectrum_abc(const double* real_a, const double* imag_a, const double* real_b,
+                            const double* imag_b, const double* real_c, const double* imag_c,
+                            int32_t niter, double kpref, double gam, double el, double eh,
+                            int32_t ne, int32_t literal_cmplx, double* out);
 
 /* ocean_gpu_spectrum without allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight

@@ -93,6 +93,9 @@ _SIGS = {
                                   C.c_double, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "ocean_gpu_spectrum": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
+    "ocean_gpu_haydock_abc": (C.c_int, [_H, _DP, _DP, C.c_int32, _DP, _DP, _DP, _DP, _DP, _DP, _IP]),
+    "ocean_gpu_spectrum_abc": (C.c_int, [_DP, _DP, _DP, _DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_spectrum_noalloc": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_double,
                                              C.c_double, C.c_double, C.c_int32, C.c_int32, _DP]),
     "ocean_gpu_write_core": (C.c_int, [_DP, _DP, _DP, C.c_int32, C.c_double, C.c_void_p, _DP, _IP]),
@@ -174,6 +177,21 @@ def spectrum(real_a, real_b, niter, kpref, gam, el, eh, ne, imag_a=None, literal
     out = np.zeros((ne + 1, 5))
     _chk(lib.ocean_gpu_spectrum(_p(ra), _p(rb), _p(ia), niter, kpref, gam, el, eh, ne,
                                 1 if literal_cmplx else 0, out.ctypes.data_as(_DP)), "spectrum")
+    return out
+
+
+def spectrum_abc(real_a, imag_a, real_b, imag_b, real_c, imag_c, niter, kpref, gam, el, eh, ne,
+                 literal_cmplx=True):
+    """write_core's fraction with complex b and c (the bi-Lanczos coefficients of
+    OceanGPU.haydock_abc) on the GPU; returns (ne+1, 5) columns."""
+    lib = _lib()
+    arrs = [np.ascontiguousarray(v, np.float64) for v in (real_a, imag_a, real_b, imag_b, real_c, imag_c)]
+    for v in arrs:
+        if v.shape[0] < niter + 1:
+            raise ValueError("coefficient arrays need niter + 1 entries")
+    out = np.zeros((ne + 1, 5))
+    _chk(lib.ocean_gpu_spectrum_abc(*[_p(v) for v in arrs], niter, kpref, gam, el, eh, ne,
+                                    1 if literal_cmplx else 0, out.ctypes.data_as(_DP)), "spectrum_abc")
     return out
 
 
@@ -563,6 +581,20 @@ class OceanGPU:
             return a, b, ai, done.value, rc
         _chk(rc, "haydock")
         return a, b, ai
+
+    def haydock_abc(self, psi0, niter, return_status=False):
+        """Non-Hermitian (bi-Lanczos) recursion (ocean_gpu_haydock_abc).  Returns
+        (real_a, imag_a, real_b, imag_b, real_c, imag_c), niter+1 entries each; with
+        return_status also (niter_done, rc) instead of raising."""
+        pr, pi = _planes(psi0, self.shape, "psi0")
+        outs = [np.zeros(niter + 1) for _ in range(6)]
+        done = C.c_int32()
+        rc = _lib().ocean_gpu_haydock_abc(self.h, _p(pr), _p(pi), niter, *[_p(v) for v in outs],
+                                          C.byref(done))
+        if return_status:
+            return (*outs, done.value, rc)
+        _chk(rc, "haydock_abc")
+        return tuple(outs)
 
     def haydock_multi(self, psi0s, niter, return_status=False):
         """The recursions of all nrhs right-hand sides at once (ocean_gpu_haydock_multi).

@@ -1785,6 +1785,102 @@ int ocean_gpu_haydock_multi(ocean_gpu_handle* h, const double* psi0_re, const do
   return rc ? rc : rc_nan;
 }
 
+// The non-Hermitian (bi-Lanczos) recursion of OCEAN2's complex_haydock branch
+// (OCEAN2/OCEAN_haydock.f90:60-160): both vectors start from psi0 and the allow mask,
+// each iteration applies the action to psi and its Hermitian conjugate (the
+// backwards action) to bpsi, then haydock_abc_1 (:1250-1433).  Vectors stay on the
+// device; the iteration number is a kernel argument.
+int ocean_gpu_haydock_abc(ocean_gpu_handle* h, const double* psi0_re, const double* psi0_im, int32_t niter,
+                          double* real_a, double* imag_a, double* real_b, double* imag_b, double* real_c,
+                          double* imag_c, int32_t* niter_done) {
+  if (!h || !psi0_re || !psi0_im || niter <= 0 || !real_a || !imag_a || !real_b || !imag_b || !real_c ||
+      !imag_c)
+    return OCEAN_GPU_ERR_ARG;
+  if (h->nrhs != 1) return OCEAN_GPU_ERR_UNSUPPORTED;
+  const int f = flags_of(h, 0, nullptr);
+  OG_TRY(check_ready(h, f));
+  const int fb = flags_of(h, 1, nullptr);
+  const size_t nv = h->nvec();
+  const int n1 = niter + 1;
+  double2* V[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // psi, old, h; bpsi, bold, bh
+  double* S = nullptr;  // coef [6][n1], part1, part2
+  int rc = OCEAN_GPU_OK;
+  for (int i = 0; i < 6 && rc == 0; ++i) rc = dmalloc(&V[i], nv);
+  if (rc == 0) rc = dmalloc(&S, 6 * (size_t)n1 + 4 * (size_t)kRedBlocks);
+  if (rc == 0) rc = put_planes(h, psi0_re, psi0_im, V[0], nv);
+  if (rc == 0) {
+    // psi = bpsi = allow . psi0; old = bold = 0; coefficients 0 (b(0) = c(0) = 0)
+    k_mask<<<grid_for(nv), kBlock, 0, h->stream>>>(V[0], V[0], h->allow, nv);
+    og::k_copy<<<grid_for(nv), kBlock, 0, h->stream>>>(V[3], V[0], nv);
+    og::k_fill<<<grid_for(nv), kBlock, 0, h->stream>>>(V[1], make_double2(0.0, 0.0), nv);
+    og::k_fill<<<grid_for(nv), kBlock, 0, h->stream>>>(V[4], make_double2(0.0, 0.0), nv);
+    og::k_fill_d<<<grid_for(6 * (size_t)n1), kBlock, 0, h->stream>>>(S, 0.0, 6 * (size_t)n1);
+    if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  }
+  int ip = 0, io = 1, ih = 2;  // rotation of (psi, old, h); the backward set is +3
+  for (int it = 1; it <= niter && rc == 0; ++it) {
+    rc = xact_dev(h, V[ip], V[ih], f);
+    if (rc == 0) rc = xact_dev(h, V[3 + ip], V[3 + ih], fb);
+    if (rc) break;
+    og::AbcArgs A{V[ih], V[3 + ih], V[io], V[3 + io], V[ip], V[3 + ip], h->allow, S,
+                  S + 6 * (size_t)n1, S + 6 * (size_t)n1 + 2 * kRedBlocks, nv, n1};
+    og::k_abc1<<<kRedBlocks, kBlock, 0, h->stream>>>(A);
+    og::k_abc2<<<kRedBlocks, kBlock, 0, h->stream>>>(A, it);
+    og::k_abc3<<<kRedBlocks, kBlock, 0, h->stream>>>(A, it);
+    if (hipGetLastError() != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+    // old <- psi, psi <- h (the old old buffer takes the next action)
+    const int t = io;
+    io = ip;
+    ip = ih;
+    ih = t;
+  }
+  std::vector<double> hc(6 * (size_t)n1);
+  if (rc == 0 && (hipMemcpyAsync(hc.data(), S, hc.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = OCEAN_GPU_ERR_HIP;
+  for (auto& v : V) dfree(v);
+  dfree(S);
+  if (rc) return rc;
+  int done = niter;
+  for (int i = 0; i < niter; ++i)
+    if (hc[i] != hc[i]) { done = i + 1; rc = OCEAN_GPU_ERR_NAN; break; }
+  double* outs[6] = {real_a, imag_a, real_b, imag_b, real_c, imag_c};
+  for (int q = 0; q < 6; ++q)
+    for (int i = 0; i < n1; ++i) {
+      const bool keep = q < 2 ? i < done : i <= done;  // a(0:done-1), b and c (0:done)
+      outs[q][i] = keep ? hc[(size_t)q * n1 + i] : 0.0;
+    }
+  if (niter_done) *niter_done = done;
+  return rc;
+}
+
+// The continued fraction of ocean_gpu_spectrum with complex b and c (the
+// bi-Lanczos coefficients), as write_core evaluates them (OCEAN2/OCEAN_haydock.f90:
+// 1846-1847, :1870).
+int ocean_gpu_spectrum_abc(const double* real_a, const double* imag_a, const double* real_b, const double* imag_b,
+                           const double* real_c, const double* imag_c, int32_t niter, double kpref, double gam,
+                           double el, double eh, int32_t ne, int32_t literal_cmplx, double* out) {
+  if (!real_a || !imag_a || !real_b || !imag_b || !real_c || !imag_c || !out || niter <= 0 || ne <= 0)
+    return OCEAN_GPU_ERR_ARG;
+  const size_t n1 = (size_t)niter + 1;
+  double* d = nullptr;
+  OG_TRY(dmalloc(&d, 6 * n1 + 5 * ((size_t)ne + 1)));
+  const double* src[6] = {real_a, real_b, imag_a, imag_b, real_c, imag_c};  // device order ra, rb, ia, ib, rc, ic
+  int rc = OCEAN_GPU_OK;
+  for (int q = 0; q < 6 && rc == 0; ++q)
+    if (hipMemcpy(d + q * n1, src[q], n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = OCEAN_GPU_ERR_HIP;
+  if (rc == 0) {
+    og::k_cfrac<<<(ne + 1 + 127) / 128, 128>>>(d, d + n1, d + 2 * n1, niter, kpref, gam, el, eh, ne, literal_cmplx,
+                                               kHartree2eV, d + 6 * n1, d + 3 * n1, d + 4 * n1, d + 5 * n1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(out, d + 6 * n1, 5 * ((size_t)ne + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = OCEAN_GPU_ERR_HIP;
+  }
+  dfree(d);
+  return rc;
+}
+
 // ---- spectrum, OCEAN 3.2.1 write_core (nine columns / Voigt) ----
 int ocean_gpu_write_core(const double* real_a, const double* real_b, const double* imag_a, int32_t niter,
                          double kpref, const ocean_spect_desc* sd, double* out, int32_t* ncols) {

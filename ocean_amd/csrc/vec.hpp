@@ -310,34 +310,175 @@ __device__ __forceinline__ double2 c_div(double2 a, double2 b) {
 
 __device__ __forceinline__ double f32r(double v) { return (double)(float)v; }
 
+// ---------------------------------------------------------------------------
+// Non-Hermitian (bi-Lanczos) Haydock step, haydock_abc_1 (OCEAN2/OCEAN_haydock.f90:
+// 1250-1433), over the forward vectors v (psi, old, h = H psi) and the backward ones
+// w (bpsi, bold, bh = H^+ bpsi), in three passes with fixed-order reductions:
+//   1: h, bh *= allow; partials of alpha = <bpsi|h>
+//   2: alpha; h += -b(i-1) old; bh += -conj(c(i-1)) bold; h += -alpha psi;
+//      bh += -conj(alpha) bpsi; h, bh *= allow; partials of r = <bh|h>
+//   3: r; c(i) = sqrt(r) (principal root as written), b(i) = conj(r / c(i));
+//      bh /= (Re b(i) - i Im b(i)); h /= c(i)
+// Each axpy and divide keeps OCEAN_psi_axpy's and OCEAN_psi_divide's operation order.
+// coef: [ra, ia, rb, ib, rc, ic], n1 entries each.
+// ---------------------------------------------------------------------------
+struct AbcArgs {
+  double2* h;
+  double2* bh;
+  const double2* old;
+  const double2* bold;
+  const double2* psi;
+  const double2* bpsi;
+  const double* allow;
+  double* coef;
+  double* part1;  // [2 kRedBlocks]
+  double* part2;  // [2 kRedBlocks]
+  size_t n;
+  int n1;
+};
+
+__device__ __forceinline__ double2 axpy_elem(double ar, double ai, double2 x, double2 y) {
+  y.x += ar * x.x; y.y += ar * x.y;
+  y.x += -ai * x.y; y.y += ai * x.x;
+  return y;
+}
+
+// x / (r + i i) as OCEAN_psi_divide with both parts given (OCEAN_psi.f90:1797-1810)
+__device__ __forceinline__ double2 divide_elem(double r, double i, double2 v) {
+  const double den = r * r + i * i, rn = r / den, in = i / den;
+  return make_double2(rn * v.x + in * v.y, -in * v.x + rn * v.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_abc1(AbcArgs A) {
+  __shared__ double lds[8 * 2];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += (size_t)gridDim.x * blockDim.x) {
+    const double m = A.allow[i];
+    double2 a = A.h[i], b = A.bh[i];
+    a.x *= m; a.y *= m; b.x *= m; b.y *= m;
+    A.h[i] = a;
+    A.bh[i] = b;
+    const double2 w = A.bpsi[i];
+    v[0] = fma(w.x, a.x, v[0]); v[0] = fma(w.y, a.y, v[0]);
+    v[1] = fma(w.x, a.y, v[1]); v[1] = fma(-w.y, a.x, v[1]);
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { A.part1[2 * blockIdx.x] = v[0]; A.part1[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_abc2(AbcArgs A, int it) {
+  __shared__ double lds[8 * 2];
+  __shared__ double s_a[2];
+  if (threadIdx.x < 64) {
+    const double ar = wave_sum_parts(A.part1, kRedBlocks, 2, 0, threadIdx.x);
+    const double ai = wave_sum_parts(A.part1, kRedBlocks, 2, 1, threadIdx.x);
+    if (threadIdx.x == 0) {
+      s_a[0] = ar; s_a[1] = ai;
+      if (blockIdx.x == 0) { A.coef[it - 1] = ar; A.coef[A.n1 + it - 1] = ai; }
+    }
+  }
+  __syncthreads();
+  const double ar = s_a[0], ai = s_a[1];
+  const double* c = A.coef;
+  const int n1 = A.n1;
+  const double brm = -c[2 * n1 + it - 1], bim = -c[3 * n1 + it - 1];
+  const double crm = -c[4 * n1 + it - 1], cim = c[5 * n1 + it - 1];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += (size_t)gridDim.x * blockDim.x) {
+    const double m = A.allow[i];
+    double2 a = axpy_elem(brm, bim, A.old[i], A.h[i]);
+    double2 b = axpy_elem(crm, cim, A.bold[i], A.bh[i]);
+    a = axpy_elem(-ar, -ai, A.psi[i], a);
+    b = axpy_elem(-ar, ai, A.bpsi[i], b);
+    a.x *= m; a.y *= m; b.x *= m; b.y *= m;
+    A.h[i] = a;
+    A.bh[i] = b;
+    v[0] = fma(b.x, a.x, v[0]); v[0] = fma(b.y, a.y, v[0]);
+    v[1] = fma(b.x, a.y, v[1]); v[1] = fma(-b.y, a.x, v[1]);
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { A.part2[2 * blockIdx.x] = v[0]; A.part2[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_abc3(AbcArgs A, int it) {
+  __shared__ double s_c[4];
+  if (threadIdx.x < 64) {
+    const double rr = wave_sum_parts(A.part2, kRedBlocks, 2, 0, threadIdx.x);
+    const double ri = wave_sum_parts(A.part2, kRedBlocks, 2, 1, threadIdx.x);
+    if (threadIdx.x == 0) {
+      const double mag = sqrt(rr * rr + ri * ri);
+      const double cr = sqrt((mag + rr) / 2.0), ci = copysign(sqrt((mag - rr) / 2.0), ri);
+      // r / c in complex(DP) division, then b = conj
+      const double2 t = c_div(make_double2(rr, ri), make_double2(cr, ci));
+      s_c[0] = t.x; s_c[1] = -t.y; s_c[2] = cr; s_c[3] = ci;
+      if (blockIdx.x == 0) {
+        const int n1 = A.n1;
+        A.coef[2 * n1 + it] = t.x; A.coef[3 * n1 + it] = -t.y;
+        A.coef[4 * n1 + it] = cr; A.coef[5 * n1 + it] = ci;
+      }
+    }
+  }
+  __syncthreads();
+  const double br = s_c[0], bi = s_c[1], cr = s_c[2], ci = s_c[3];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += (size_t)gridDim.x * blockDim.x) {
+    A.bh[i] = divide_elem(br, -bi, A.bh[i]);
+    A.h[i] = divide_elem(cr, ci, A.h[i]);
+  }
+}
+
+// (ar + i ai)(br + i bi) in single precision with every product and sum rounded on
+// its own (Fortran complex(4) arithmetic).  The operators are written here under
+// contract(off): a fused multiply-add would round the product and the sum once.
+__device__ __forceinline__ double2 cmul32(float ar, float ai, float br, float bi) {
+#pragma clang fp contract(off)
+  const float re = ar * br - ai * bi;
+  const float im = ar * bi + ai * br;
+  return make_double2((double)re, (double)im);
+}
+
+// b(j) c(j) of the fraction: b = c real for the Hermitian recursion (ib == nullptr),
+// complex b and c for the bi-Lanczos one (write_core, OCEAN_haydock.f90:1846-1847,
+// :1870); literal rounds the kind-less cmplx() operands and products to single
+// precision as OCEAN2's build does
+__device__ __forceinline__ double2 cf_bc(const double* __restrict__ rb, const double* __restrict__ ib,
+                                         const double* __restrict__ rc, const double* __restrict__ ic, int j,
+                                         double scale, int literal) {
+  if (ib == nullptr) {
+    if (literal) {
+      const float b32 = (float)rb[j];
+      return make_double2((double)(((float)scale * b32) * b32), 0.0);
+    }
+    return make_double2(scale * rb[j] * rb[j], 0.0);
+  }
+  if (literal) {
+    const float s = (float)scale;
+    return cmul32(__fmul_rn(s, (float)rb[j]), __fmul_rn(s, (float)ib[j]), (float)rc[j], (float)ic[j]);
+  }
+  const double br = scale * rb[j], bi = scale * ib[j];
+  return make_double2(br * rc[j] - bi * ic[j], br * ic[j] + bi * rc[j]);
+}
+
 // delta(e) of the n-level fraction with its square-root terminator; the spectrum
 // is kpref |Im delta| / |delta|^2 and SPKK kpref Re delta / |delta|^2.
 __device__ double2 cf_delta(const double* __restrict__ ra, const double* __restrict__ rb,
-                            const double* __restrict__ ia, int n, double e, double gam, int literal) {
+                            const double* __restrict__ ia, int n, double e, double gam, int literal,
+                            const double* __restrict__ ib = nullptr, const double* __restrict__ rc = nullptr,
+                            const double* __restrict__ ic = nullptr) {
   double2 ct = make_double2(e - ra[n - 1], gam + ia[n - 1]);
-  double fourbc;
-  if (literal) {
-    const float b32 = (float)rb[n];
-    fourbc = (double)((4.0f * b32) * b32);
-  } else {
-    fourbc = 4.0 * rb[n] * rb[n];
-  }
-  double2 z2 = make_double2(ct.x * ct.x - ct.y * ct.y - fourbc, 2.0 * ct.x * ct.y);
+  const double2 fourbc = cf_bc(rb, ib, rc, ic, n, 4.0, literal);
+  double2 z2 = make_double2(ct.x * ct.x - ct.y * ct.y - fourbc.x, 2.0 * ct.x * ct.y - fourbc.y);
   const double2 disc = c_sqrt(z2);
   double2 delta = (disc.y > 0.0) ? make_double2(0.5 * (ct.x + disc.x), 0.5 * (ct.y + disc.y))
                                  : make_double2(0.5 * (ct.x - disc.x), 0.5 * (ct.y - disc.y));
   for (int jj = n - 1; jj >= 0; --jj) {
-    double bc;
+    const double2 bc = cf_bc(rb, ib, rc, ic, jj + 1, 1.0, literal);
     double2 z;
     if (literal) {
-      const float b32 = (float)rb[jj + 1];
-      bc = (double)(b32 * b32);
       z = make_double2(f32r(e - ra[jj]), f32r(gam + ia[jj]));
     } else {
-      bc = rb[jj + 1] * rb[jj + 1];
       z = make_double2(e - ra[jj], gam + ia[jj]);
     }
-    const double2 q = c_div(make_double2(bc, 0.0), delta);
+    const double2 q = c_div(bc, delta);
     delta = make_double2(z.x - q.x, z.y - q.y);
   }
   return delta;
@@ -345,11 +486,13 @@ __device__ double2 cf_delta(const double* __restrict__ ra, const double* __restr
 
 __global__ void k_cfrac(const double* __restrict__ ra, const double* __restrict__ rb,
                         const double* __restrict__ ia, int n, double kpref, double gam, double el,
-                        double eh, int ne, int literal, double hartree2ev, double* __restrict__ out) {
+                        double eh, int ne, int literal, double hartree2ev, double* __restrict__ out,
+                        const double* __restrict__ ib = nullptr, const double* __restrict__ rc = nullptr,
+                        const double* __restrict__ ic = nullptr) {
   const int ie = blockIdx.x * blockDim.x + threadIdx.x;
   if (ie > ne) return;
   const double e = el + (eh - el) * (double)(2 * ie) / (double)(2 * ne);
-  const double2 delta = cf_delta(ra, rb, ia, n, e, gam, literal);
+  const double2 delta = cf_delta(ra, rb, ia, n, e, gam, literal, ib, rc, ic);
   const double dr = delta.x, di = fabs(delta.y);
   const double den = dr * dr + di * di;
   const double sp = kpref * di / den, kk = kpref * dr / den;

@@ -186,6 +186,29 @@ module ocean_gpu_iface
       real(c_double), intent(out) :: out(*)
     end function
 
+    ! complex_haydock branch: OCEAN_haydock_nonHerm_do / haydock_abc_1
+    ! (OCEAN2/OCEAN_haydock.f90:53-160, :1250-1433)
+    integer(c_int) function ocean_gpu_haydock_abc(h, psi_re, psi_im, niter, real_a, imag_a, real_b, &
+        imag_b, real_c, imag_c, niter_done) bind(C, name='ocean_gpu_haydock_abc')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: psi_re(*), psi_im(*)
+      integer(c_int32_t), value :: niter
+      real(c_double), intent(out) :: real_a(*), imag_a(*), real_b(*), imag_b(*), real_c(*), imag_c(*)
+      integer(c_int32_t), intent(out) :: niter_done
+    end function
+
+    ! write_core's fraction with complex b and c (OCEAN2/OCEAN_haydock.f90:1836-1880)
+    integer(c_int) function ocean_gpu_spectrum_abc(real_a, imag_a, real_b, imag_b, real_c, imag_c, &
+        niter, kpref, gam, el, eh, ne, literal_cmplx, out) bind(C, name='ocean_gpu_spectrum_abc')
+      import :: c_int, c_double, c_int32_t
+      real(c_double), intent(in) :: real_a(*), imag_a(*), real_b(*), imag_b(*), real_c(*), imag_c(*)
+      integer(c_int32_t), value :: niter
+      real(c_double), value :: kpref, gam, el, eh
+      integer(c_int32_t), value :: ne, literal_cmplx
+      real(c_double), intent(out) :: out(*)
+    end function
+
     integer(c_int) function ocean_gpu_write_core(real_a, real_b, imag_a, niter, kpref, sd, out, ncols) &
         bind(C, name='ocean_gpu_write_core')
       import :: c_int, c_double, c_int32_t, ocean_spect_desc

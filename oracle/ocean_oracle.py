@@ -37,6 +37,11 @@ Pinning (what checks each function; "parity unpinned" where none does):
                        squared equals eps, R and the loss function from their definitions
   redtrid              TEST/Fe/Results abslanc eigenvalue block
   haydock              dense Lanczos/eigendecomposition brute force (tests/test_oracle.py)
+  haydock_abc          reduction to haydock on a Hermitian problem; the three-term relations
+                       of both sides with the dense H and backwards operator; closed forms
+                       of a(0), c(1), b(1), a(1) in the moments <v|A^k|v> (tests/test_oracle_abc.py)
+  write_core_abc       write_core for real c = b; the (0,0) resolvent element of the
+                       tridiagonal matrix when b(n) c(n) = 0
   lr_act               dense operator brute force, loop-literal lr_kernel restatement
   lr_act_obf2          -N_k lr_act with u = sum_o obf2u obf phs (the Bloch functions the
                        reference's own use_fake_obf path builds, OCEAN_obf.f90:84-96),
@@ -1142,6 +1147,72 @@ def haydock(P: Problem, psi0, niter, act=None):
     return ra, rb, ia, 0
 
 
+def psi_axpy(rval, x, y, ival=0.0):
+    """y + (rval + i ival) x in the reference's order: the real part of the scalar
+    first, then the imaginary one.  OCEAN_psi.f90:1386-1458 (OCEAN_psi_axpy)."""
+    yr = y.real + rval * x.real
+    yi = y.imag + rval * x.imag
+    yr = yr + (-ival) * x.imag
+    yi = yi + ival * x.real
+    return np.asfortranarray(yr + 1j * yi)
+
+
+def haydock_abc(P: Problem, psi0, niter, vectors=False):
+    """Non-Hermitian (bi-Lanczos) Haydock recursion.
+
+    Driver OCEAN_haydock.f90:60-160 (OCEAN_haydock_nonHerm_do): psi = back_psi = psi0,
+    old vectors zero; per iteration the allow mask on psi and back_psi, H psi and the
+    Hermitian conjugate's action on back_psi (OCEAN_xact with backwards), then
+    haydock_abc_1 (:1250-1433):
+      mask h, bh; alpha = (bpsi, h) [= <bpsi|h>];
+      h  += -b(i-1) old;          bh += -conj(c(i-1)) bold;
+      h  += -alpha psi;           bh += -conj(alpha) bpsi;
+      mask h, bh; r = <bh|h>;
+      c(i) = sqrt((|r| + Re r)/2) + i sign(sqrt((|r| - Re r)/2), Im r);
+      b(i) = conj(r / c(i));
+      bh /= conj(b(i)); h /= c(i); old <- psi, psi <- h (and the back set).
+    b(0) = c(0) = 0.  Returns (ra, ia, rb, ib, rc, ic, ierr); ierr = -1 when a is NaN
+    (:1414-1422).  With vectors, also the lists of masked psi and back_psi of
+    iterations 1 .. niter + 1 (for the three-term relations of the tests).
+    """
+    n1 = niter + 1
+    ra, ia, rb, ib, rc, ic = (np.zeros(n1) for _ in range(6))
+    psi = np.asfortranarray(np.array(psi0, np.complex128))
+    bpsi = psi.copy(order="F")
+    old = np.zeros_like(psi)
+    bold = np.zeros_like(psi)
+    V, W = [], []
+    for it in range(1, n1):
+        psi = P.allow * psi
+        bpsi = P.allow * bpsi
+        V.append(psi)
+        W.append(bpsi)
+        h = P.allow * xact(P, psi)
+        bh = P.allow * xact(P, bpsi, backwards=True)
+        ar, ai = psi_dot(bpsi, h)
+        h = psi_axpy(-rb[it - 1], old, h, -ib[it - 1])
+        bh = psi_axpy(-rc[it - 1], bold, bh, ic[it - 1])
+        h = psi_axpy(-ar, psi, h, -ai)
+        bh = psi_axpy(-ar, bpsi, bh, ai)
+        ra[it - 1], ia[it - 1] = ar, ai
+        h = P.allow * h
+        bh = P.allow * bh
+        rr, ri = psi_dot(bh, h)
+        old, bold = psi, bpsi
+        mag = math.sqrt(rr ** 2 + ri ** 2)
+        rc[it] = math.sqrt((mag + rr) / 2.0)
+        ic[it] = math.copysign(math.sqrt((mag - rr) / 2.0), ri)
+        t = complex(rr, ri) / complex(rc[it], ic[it])
+        rb[it], ib[it] = t.real, -t.imag
+        bpsi = psi_divide(bh, rb[it], -ib[it])
+        psi = psi_divide(h, rc[it], ic[it])
+        if ar != ar:
+            return (ra, ia, rb, ib, rc, ic, -1) + ((V, W) if vectors else ())
+    V.append(P.allow * psi)
+    W.append(P.allow * bpsi)
+    return (ra, ia, rb, ib, rc, ic, 0) + ((V, W) if vectors else ())
+
+
 def write_core(a, b, niter, kpref, gam0, el, eh, ne, imag_a=None, literal_cmplx=True,
                hartree2ev=HARTREE2EV, ebase=0.0):
     """Continued fraction with square-root terminator (Lorentzian, gamGauss off).
@@ -1186,6 +1257,48 @@ def write_core(a, b, niter, kpref, gam0, el, eh, ne, imag_a=None, literal_cmplx=
     ener = ebase + hartree2ev * e
     del f32
     return np.stack([ener, spct, spct, spkk, spkk], axis=1)
+
+
+def write_core_abc(ra, ia, rb, ib, rc, ic, niter, kpref, gam0, el, eh, ne, literal_cmplx=True,
+                   hartree2ev=HARTREE2EV):
+    """write_core's continued fraction with complex b and c (the bi-Lanczos
+    coefficients of haydock_abc), OCEAN_haydock.f90:1836-1880:
+      ctmp = cmplx(e - a(n-1), gam + ia(n-1), DP);
+      disc = sqrt(ctmp^2 - 4 cmplx(b(n)) cmplx(c(n))), delta = (ctmp +/- disc)/2 by the
+        sign of Im disc;
+      delta = cmplx(e - a(jj), gam + ia(jj)) - cmplx(b(jj+1)) cmplx(c(jj+1)) / delta.
+    With literal_cmplx the kind-less cmplx() operands and their products are single
+    precision (complex(4) arithmetic, every product and sum rounded on its own); the
+    division by delta and the rest are FP64.  Returns (E_eV, Spect, Spect(0), SPKK,
+    SPKK(0)) columns."""
+    ra, ia, rb, ib, rc, ic = (np.asarray(v, np.float64) for v in (ra, ia, rb, ib, rc, ic))
+    n = niter
+    e = el + (eh - el) * np.arange(0, 2 * ne + 1, 2, dtype=np.float64) / (2 * ne)
+    gam = gam0
+    f = np.float32
+
+    def bc(j, scale):
+        if literal_cmplx:
+            x, y = f(scale) * f(rb[j]), f(scale) * f(ib[j])
+            u, v = f(rc[j]), f(ic[j])
+            return complex(np.float64(f(x * u) - f(y * v)), np.float64(f(x * v) + f(y * u)))
+        return scale * complex(rb[j], ib[j]) * complex(rc[j], ic[j])
+
+    ctmp = (e - ra[n - 1]) + 1j * (gam + ia[n - 1])
+    disc = np.sqrt(ctmp ** 2 - bc(n, 4.0))
+    delta = np.where(disc.imag > 0.0, (ctmp + disc) / 2.0, (ctmp - disc) / 2.0)
+    for jj in range(n - 1, -1, -1):
+        t = bc(jj + 1, 1.0)
+        if literal_cmplx:
+            z = f(e - ra[jj]).astype(np.float64) + 1j * np.float64(f(gam + ia[jj]))
+        else:
+            z = (e - ra[jj]) + 1j * (gam + ia[jj])
+        delta = z - t / delta
+    dr = delta.real
+    di = np.abs(delta.imag)
+    spct = kpref * di / (dr ** 2 + di ** 2)
+    spkk = kpref * dr / (dr ** 2 + di ** 2)
+    return np.stack([hartree2ev * e, spct, spct, spkk, spkk], axis=1)
 
 
 # OCEAN_constants.f90:17-29 (2022 CODATA)
