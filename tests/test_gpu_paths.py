@@ -8,6 +8,7 @@ Against the numpy oracle on seeded inputs:
     at 9, 10 and 12; the 24-edge class: direct sums at 15, 18, 20 and 24) and a 16^3
     mesh whose transform strip needs more than 64 KB of LDS;
   * 400 bands at the L2,3 edge (adjoint rows and FP32 B staging past 64 KB of LDS);
+  * the 8^3 transform at every strip width (bit-identical to each other);
   * GMRES at the reference's default depth of 80 (cnbse.gmres.nloop,
     Common/oparse.json:16) and at 100, with restarts inside one energy.
 Failure returns (each ends normally on the host): NaN in the starting vector -> -1
@@ -74,6 +75,32 @@ def test_action_matches_oracle(name):
             err = relerr(g.xact(psi, backwards=back), ref)
             print("  mixed backwards", back, "relerr", err, flush=True)
             assert err < 1e-5, (name, back, err)
+
+
+@pytest.mark.parametrize("spec", [dict(nb=8, kmesh=(8, 8, 8), xmesh=(3, 3, 3), lc=0, seed=61),
+                                  dict(nb=5, kmesh=(8, 8, 8), xmesh=(5, 1, 1), lc=1, seed=62, xi_eV=2.0)])
+def test_cube8_transform_every_strip_width(spec):
+    """The 8^3 transform at strip widths 1, 2, 3, 4, 5 and 8 (FP64 caps 8 to 7 columns):
+    each column's arithmetic does not depend on the width, so the action is
+    bit-identical across widths; the last strips are ragged (108 and 60 columns)."""
+    inp = synth.make_inputs(**spec)
+    P = to_problem(inp)
+    print("cube8", spec, flush=True)
+    with core.OceanGPU.from_inputs(inp) as g:
+        psi = _rand(g.shape, 4)
+        for prec, tol in ((core.FP64, 1e-12), (core.MIXED, 1e-5)):
+            g.set_precision(prec)
+            for back in (False, True):
+                ref = orc.xact(P, psi, backwards=back)
+                outs = []
+                for G in (1, 2, 3, 4, 5, 8):
+                    g.set_tiles(256, 512, G, -1)
+                    outs.append(g.xact(psi, backwards=back))
+                err = relerr(outs[0], ref)
+                print("  precision", prec, "backwards", back, "relerr", err, flush=True)
+                assert err < tol
+                for o in outs[1:]:
+                    assert np.array_equal(o, outs[0])
 
 
 @pytest.mark.parametrize("name", ["k16", "l23_x512_nb72", "k24"])
