@@ -96,7 +96,7 @@ def cpu_baseline(inp, budget_s):
     nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
     cs = c_oracle.CSystem(inp)
     t0 = time.perf_counter()
-    cs.xact(inp.psi0, nthreads=nthr)
+    ref = cs.xact(inp.psi0, nthreads=nthr)
     one = time.perf_counter() - t0
     nrep = max(1, min(400, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
@@ -105,7 +105,7 @@ def cpu_baseline(inp, budget_s):
     return {"value": nrep / dt, "unit": "H·ψ/s", "cores": nthr, "kind": "port",
             "cpu_model": model, "cpus_visible": total, "cpus_allowed": avail,
             "sample": f"{nrep} FP64 H.psi applications of the same workload "
-                      f"(C/OpenMP restatement, OpenMP over x points, {nthr} threads, {dt:.1f} s)"}
+                      f"(C/OpenMP restatement, OpenMP over x points, {nthr} threads, {dt:.1f} s)"}, ref
 
 
 def batch_workload(n, niter, reps, log_every=8):
@@ -443,6 +443,9 @@ def main():
             "algorithmic_bytes": bytes_["fp64"][fam],
             "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()}}
 
+    # the headline product itself, for the check against the oracle's at the same size
+    hpsi_gpu = g.xact(inp.psi0)
+
     extras = {}
     if not args.no_extras:
         p = inp.params
@@ -502,9 +505,15 @@ def main():
         extras.update(batch_workload(args.batch_n, args.batch_niter, reps=5))
 
     cpu = None
+    parity = None
     if not args.no_cpu:
-        cpu = cpu_baseline(inp, args.cpu_seconds)
+        cpu, ref = cpu_baseline(inp, args.cpu_seconds)
         log("cpu baseline:", cpu)
+        err = float(np.abs(hpsi_gpu - ref).max() / np.abs(ref).max())
+        parity = {"hpsi_relerr_vs_oracle": err, "tol": 1e-12, "ok": err < 1e-12,
+                  "oracle": "C restatement, H.psi0 of the headline workload"}
+        if err >= 1e-12:
+            log("WARNING: GPU H.psi differs from the oracle's:", err)
 
     out = {
         "metric": METRIC, "value": round(hpsi_fp64, 1), "unit": "H·ψ/s", "n_gpus": 1,
@@ -513,7 +522,7 @@ def main():
         "config": {"workload": "K-edge core BSE H.psi, FP64 drop-in (nb=64, k-mesh 8^3, x-mesh 8^3, "
                                "nalpha=4, one site)", "nb": 64, "kmesh": [8, 8, 8], "xmesh": [8, 8, 8],
                    "nalpha": 4, "haydock_niter": inp.params["niter"]},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
     out["hpsi_per_s_host_planes"] = round(hpsi_host, 1)
     out.update(extras)

@@ -542,44 +542,44 @@ __global__ __launch_bounds__(kBlock) void k_multidot2(const int* __restrict__ sk
 // then a butterfly (fixed order, so every run gives the same bits).
 __global__ void k_multifinish(const int* __restrict__ skip, const double* __restrict__ part, int nblk, int nv,
                               double* __restrict__ out) {
-  if (*skip) return;
+  // the flag is read with the partials (one round trip) and tested before the store
+  const int sk = *skip;
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= nv) return;
   const double s = og::wave_sum_parts(part, nblk, nv, j, lane);
-  if (lane == 0) out[j] = s;
+  if (lane == 0 && !sk) out[j] = s;
 }
 
-// x += sum_i c_i U_i and ax += sum_i c_i AU_i together (the updates of update_gmres,
-// OCEAN_gmres.f90:336-341), eight rows of each per step with their loads in flight,
-// each sum in i order.
-__global__ void k_multiaxpy2(const int* __restrict__ skip, const double2* const* __restrict__ Ul,
-                             const double2* const* __restrict__ AUl, const double2* __restrict__ c, int m,
-                             double2* __restrict__ x, double2* __restrict__ ax, size_t n) {
-  if (*skip) return;
+// x += sum_i C_i U_i over the cycle's cumulative least-squares coefficients (the sum of
+// the per-iteration updates of update_gmres, OCEAN_gmres.f90:336-341, formed once
+// when the basis is about to be replaced or the solve has converged); terms with
+// C_i = 0 are skipped (their U_i may not have been formed).  skip may be null.
+__global__ __launch_bounds__(kBlock) void k_materialize(const int* __restrict__ skip,
+                                                        const double2* const* __restrict__ Ul,
+                                                        const double2* __restrict__ C, int m,
+                                                        double2* __restrict__ x, size_t n) {
+  if (skip && *skip) return;
   constexpr int R = 8;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double2 xv = x[e], av = ax[e];
+    double2 xv = x[e];
     for (int i0 = 0; i0 < m; i0 += R) {
-      double2 w[R], a[R];
+      double2 w[R], ci[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        w[q] = Ul[min(i0 + q, m - 1)][e];
-        a[q] = AUl[min(i0 + q, m - 1)][e];
+        ci[q] = C[min(i0 + q, m - 1)];
+        const bool use = i0 + q < m && (ci[q].x != 0.0 || ci[q].y != 0.0);
+        w[q] = use ? Ul[i0 + q][e] : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        if (i0 + q < m) {
-          const double2 ci = c[i0 + q];
-          xv.x += ci.x * w[q].x; xv.y += ci.x * w[q].y;
-          xv.x += -ci.y * w[q].y; xv.y += ci.y * w[q].x;
-          av.x += ci.x * a[q].x; av.y += ci.x * a[q].y;
-          av.x += -ci.y * a[q].y; av.y += ci.y * a[q].x;
+        if (i0 + q < m && (ci[q].x != 0.0 || ci[q].y != 0.0)) {
+          xv.x += ci[q].x * w[q].x; xv.y += ci[q].x * w[q].y;
+          xv.x += -ci[q].y * w[q].y; xv.y += ci[q].y * w[q].x;
         }
       }
     }
     x[e] = xv;
-    ax[e] = av;
   }
 }
 
@@ -595,7 +595,8 @@ __global__ void k_multiaxpy2(const int* __restrict__ skip, const double2* const*
 // one broadcast of the finished value plus one FMA per owned index; the strictly
 // lower triangle of L is staged in LDS (packed rows) when it fits, otherwise read
 // from global memory.  A zero pivot leaves its 1-based index in *info (zhptrf's
-// info) and the coefficients at 0.
+// info) and the coefficients at 0.  The coefficients are also added to the cycle's
+// cumulative ones C (x = x_b + sum_i C_i U_i, formed when the basis is replaced).
 constexpr int kLsSlots = 4;
 constexpr int kMaxGmresDepth = 64 * kLsSlots;
 constexpr int kLsPacked = 3916;  // LDS-staged rows 1 .. cur-2: (cur-2)(cur-1)/2 entries, cur <= 90
@@ -611,19 +612,41 @@ __device__ __forceinline__ double2 ls_bcast(const double2 (&v)[kLsSlots], int k)
 
 __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const double* __restrict__ fin,
                                                   double2* __restrict__ L, double* __restrict__ D,
-                                                  double2* __restrict__ coef, int cur, int depth,
-                                                  int* __restrict__ info) {
-  if (*skip) return;
+                                                  double2* __restrict__ coef, double2* __restrict__ C, int cur,
+                                                  int depth, int* __restrict__ info) {
+  // every input is loaded before the flag is tested, so the flag, the Gram column
+  // and the stored factor arrive in one round trip
+  const int sk = *skip;
   __shared__ double2 Ls[kLsPacked];
   __shared__ double Dl[kMaxGmresDepth];
   const int lane = threadIdx.x;
   const int j = cur - 1;
   const int npk = j * (j - 1) / 2;  // strictly lower rows 1 .. j - 1, packed row by row
   const bool in_lds = npk <= kLsPacked;
-  // rows 1 .. j - 1 of L (contiguous in the packed layout: four loads in flight per
-  // lane) and D(0 .. j-1); row j is formed here and kept in registers
+  // C(j, k) = conj(<au_k|au_j>) for k < j (the lower triangle of the Hermitian
+  // matrix whose upper column j holds <au_k|au_j>); C(j, j) = |au_j|^2; the
+  // right-hand side -<au_i|g>; D(0 .. j-1)
+  double2 t[kLsSlots], lj[kLsSlots], r[kLsSlots];
+  double dv[kLsSlots];
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q) {
+    const int i = lane + 64 * q;
+    t[q] = i < j ? make_double2(fin[4 * i], -fin[4 * i + 1]) : make_double2(0.0, 0.0);
+    r[q] = i < cur ? make_double2(-fin[4 * i + 2], -fin[4 * i + 3]) : make_double2(0.0, 0.0);
+    dv[q] = i < j ? D[i] : 0.0;
+    lj[q] = make_double2(0.0, 0.0);
+  }
+  // rows 1 .. j - 1 of L (contiguous in the packed layout); row j is formed here and
+  // kept in registers.  The first four loads per lane are issued with the others.
+  double2 v0[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v0[q] = L[min(lane + 64 * q, max(npk - 1, 0))];
+  if (sk) return;
   if (in_lds) {
-    for (int e0 = lane; e0 < npk; e0 += 4 * 64) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (lane + 64 * q < npk) Ls[lane + 64 * q] = v0[q];
+    for (int e0 = lane + 4 * 64; e0 < npk; e0 += 4 * 64) {
       double2 v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = L[min(e0 + 64 * q, npk - 1)];
@@ -632,20 +655,13 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
         if (e0 + 64 * q < npk) Ls[e0 + 64 * q] = v[q];
     }
   }
-  for (int k = lane; k < j; k += 64) Dl[k] = D[k];
+#pragma unroll
+  for (int q = 0; q < kLsSlots; ++q)
+    if (lane + 64 * q < j) Dl[lane + 64 * q] = dv[q];
   __syncthreads();
   auto Lat = [&](int i, int k) -> double2 {  // L(i, k), k < i < j
     return in_lds ? Ls[i * (i - 1) / 2 + k] : L[(size_t)i * (i - 1) / 2 + k];
   };
-  // C(j, k) = conj(<au_k|au_j>) for k < j (the lower triangle of the Hermitian
-  // matrix whose upper column j holds <au_k|au_j>); C(j, j) = |au_j|^2
-  double2 t[kLsSlots], lj[kLsSlots];
-#pragma unroll
-  for (int q = 0; q < kLsSlots; ++q) {
-    const int i = lane + 64 * q;
-    t[q] = i < j ? make_double2(fin[4 * i], -fin[4 * i + 1]) : make_double2(0.0, 0.0);
-    lj[q] = make_double2(0.0, 0.0);
-  }
   // row j: after step k, t(k) is final and L(j, k) = t(k) / D(k) updates every t(kk), kk > k
   double dsum = 0.0;
   for (int k = 0; k < j; ++k) {
@@ -689,12 +705,6 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
   }
   // coef = -C^-1 rvec: L y = -rvec, then D, then L^H coef = y (row j of L from the
   // registers of its owners)
-  double2 r[kLsSlots];
-#pragma unroll
-  for (int q = 0; q < kLsSlots; ++q) {
-    const int i = lane + 64 * q;
-    r[q] = i < cur ? make_double2(-fin[4 * i + 2], -fin[4 * i + 3]) : make_double2(0.0, 0.0);
-  }
   for (int k = 0; k < cur; ++k) {
     const double2 v = ls_bcast(r, k);
     const double2 ljk = ls_bcast(lj, k);
@@ -728,16 +738,13 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
 #pragma unroll
   for (int q = 0; q < kLsSlots; ++q) {
     const int i = lane + 64 * q;
-    if (i < cur) coef[i] = r[q];
+    if (i < cur) {
+      coef[i] = r[q];
+      double2 ci = C[i];
+      ci.x += r[q].x; ci.y += r[q].y;
+      C[i] = ci;
+    }
   }
-}
-
-// z = x .* y (complex)  (OCEAN_psi_3element_mult)
-__global__ void k_cmul3(const int* __restrict__ skip, double2* __restrict__ z, const double2* __restrict__ x,
-                        const double2* __restrict__ y, size_t n) {
-  if (*skip) return;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    z[e] = og::cmul(x[e], y[e]);
 }
 
 // pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2)  (OCEAN_psi_min_set_prec, :2907-2994)
@@ -771,6 +778,7 @@ struct GmCtl {
   int info;     // zhptrf-style pivot index of a failure
   int count;    // arrival counter of the residual workgroups (rests at 0)
   double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
+  double rxr, rxi, pad3, pad4;    // <rhs|x_b> of the materialised x (x = x_b + sum_i C_i U_i)
 };
 
 __global__ void k_gm_reset(GmCtl* __restrict__ c) {
@@ -793,21 +801,24 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
   }
 }
 
-// One residual step: g = ax - rhs, the next basis vector unext = g . pc when given
-// (the first product of the next iteration, OCEAN_gmres.f90:760-766), the partial
-// sums of |g|^2 and <rhs|x>, and in the last workgroup to arrive (device-scope
-// counter) the three sums in a fixed order and, with check, the reference's
-// stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and
-// (-|g|^2 / Im<rhs|x> < 0.001 or |g|^2 < ffff / 10).
-// The sums of |g|^2, <rhs|x> of one workgroup (v[3], block-summed in thread 0) into
-// its partials and, in the last workgroup to arrive (device-scope counter), the three
-// totals in a fixed order and, with check, the reference's stopping test
-// (OCEAN_gmres.f90:845-852): |g|^2 < ffff and (-|g|^2 / Im<rhs|x> < 0.001 or
-// |g|^2 < ffff / 10).  The counter returns to zero.
+// The sums of one workgroup (v[3], block-summed in thread 0) into its partials and, in
+// the last workgroup to arrive (device-scope counter), the three totals in a fixed
+// order; the counter returns to zero.  By mode:
+//   kResX   (x materialised, C = 0): v = (|g|^2, <rhs|x>); <rhs|x> becomes <rhs|x_b>;
+//   kResU   v = (|g|^2, <rhs|U_slot>) (slot < 0: no new vector); <rhs|x> is
+//           <rhs|x_b> + sum_{i<m} C_i <rhs|U_i>;
+//   kDotU   v = (0, <rhs|U_slot>): the dot only.
+// kResX and kResU store |g|^2 and <rhs|x> and, with check, apply the reference's
+// stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and (-|g|^2 / Im<rhs|x> < 0.001
+// or |g|^2 < ffff / 10).
+enum { kResX = 0, kResU = 1, kDotU = 2 };
+
 __device__ __forceinline__ void resid_finish(GmCtl* __restrict__ c, double (&v)[3], double* __restrict__ part,
-                                             double ffff, int it, int check) {
+                                             double ffff, int it, int check, int mode,
+                                             const double2* __restrict__ C, double2* __restrict__ rU, int m,
+                                             int slot) {
   __shared__ double lds[4 * 3];
-  __shared__ double sv[3];
+  __shared__ double sv[5];
   __shared__ int is_last;
   og::block_sum<3>(v, lds);
   if (threadIdx.x == 0) {
@@ -826,90 +837,176 @@ __device__ __forceinline__ void resid_finish(GmCtl* __restrict__ c, double (&v)[
       acc += __hip_atomic_load(&part[3 * b + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     acc = og::wave_sum(acc);
     if (lane == 0) sv[w] = acc;
+  } else if (mode == kResU) {
+    // sum_{i<m} C_i <rhs|U_i>, lane-strided then a butterfly (fixed order)
+    double ar = 0.0, ai = 0.0;
+    for (int i = lane; i < m; i += 64) {
+      const double2 ci = C[i], u = rU[i];
+      ar += ci.x * u.x - ci.y * u.y;
+      ai += ci.x * u.y + ci.y * u.x;
+    }
+    ar = og::wave_sum(ar);
+    ai = og::wave_sum(ai);
+    if (lane == 0) { sv[3] = ar; sv[4] = ai; }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    c->gval = sv[0];
-    c->rval = sv[1];
-    c->ival = sv[2];
-    if (check && sv[0] < ffff) {
-      const double rel = -sv[0] / sv[2];
-      if (rel < 0.001 || sv[0] < ffff / 10.0) {
-        c->done = 1;
-        c->conv_it = it;
+    if (slot >= 0) rU[slot] = make_double2(sv[1], sv[2]);
+    if (mode != kDotU) {
+      double rr, ri;
+      if (mode == kResX) {
+        rr = sv[1]; ri = sv[2];
+        c->rxr = rr; c->rxi = ri;
+      } else {
+        rr = c->rxr + sv[3]; ri = c->rxi + sv[4];
+      }
+      c->gval = sv[0];
+      c->rval = rr;
+      c->ival = ri;
+      if (check && sv[0] < ffff) {
+        const double rel = -sv[0] / ri;
+        if (rel < 0.001 || sv[0] < ffff / 10.0) {
+          c->done = 1;
+          c->conv_it = it;
+        }
       }
     }
     atomicExch(&c->count, 0);
   }
 }
 
-// One element's residual step: g = ax - rhs, the next basis vector g . pc
-// (OCEAN_gmres.f90:760-766), and its terms of |g|^2 and <rhs|x>.
-__device__ __forceinline__ void resid_elem(size_t e, double2 a, double2 xv, const double2* __restrict__ rhs,
-                                           double2* __restrict__ g, const double2* __restrict__ pc,
-                                           double2* __restrict__ unext, double (&v)[3]) {
+// g = ax - rhs for one element and its term of |g|^2
+__device__ __forceinline__ double2 resid_g(size_t e, double2 a, const double2* __restrict__ rhs,
+                                           double2* __restrict__ g, double& gg) {
   const double2 b = rhs[e];
   const double2 gv = make_double2(a.x - b.x, a.y - b.y);
   g[e] = gv;
-  if (unext) unext[e] = og::cmul(gv, pc[e]);
-  v[0] = fma(gv.x, gv.x, v[0]); v[0] = fma(gv.y, gv.y, v[0]);
-  v[1] = fma(b.x, xv.x, v[1]); v[1] = fma(b.y, xv.y, v[1]);
-  v[2] = fma(b.x, xv.y, v[2]); v[2] = fma(-b.y, xv.x, v[2]);
+  gg = fma(gv.x, gv.x, gg); gg = fma(gv.y, gv.y, gg);
+  return b;
 }
 
-// The residual step of iteration it on its own (after an action has recomputed ax).
+// <b|v> term of one element into (vr, vi)
+__device__ __forceinline__ void dot_term(double2 b, double2 xv, double& vr, double& vi) {
+  vr = fma(b.x, xv.x, vr); vr = fma(b.y, xv.y, vr);
+  vi = fma(b.x, xv.y, vi); vi = fma(-b.y, xv.x, vi);
+}
+
+// The residual step of iteration it with x materialised (after an action has
+// recomputed ax from x): g = ax - rhs, |g|^2, <rhs|x> = <rhs|x_b>.
 __global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, const double2* __restrict__ ax,
                                                        const double2* __restrict__ rhs, const double2* __restrict__ x,
-                                                       double2* __restrict__ g, const double2* __restrict__ pc,
-                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
+                                                       double2* __restrict__ g, size_t n, double* __restrict__ part,
                                                        double ffff, int it, int check) {
   if (c->done) return;
   double v[3] = {0, 0, 0};
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    resid_elem(e, ax[e], x[e], rhs, g, pc, unext, v);
-  resid_finish(c, v, part, ffff, it, check);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double2 b = resid_g(e, ax[e], rhs, g, v[0]);
+    dot_term(b, x[e], v[1], v[2]);
+  }
+  resid_finish(c, v, part, ffff, it, check, kResX, nullptr, nullptr, 0, -1);
 }
 
-// x += sum_i c_i U_i and ax += sum_i c_i AU_i (update_gmres, OCEAN_gmres.f90:336-341;
-// eight rows of each per step with their loads in flight, each sum in i order),
-// followed in the same pass by the residual step of iteration it: the updated x and
-// ax are not read back.
-__global__ __launch_bounds__(kBlock) void k_axpy_resid(GmCtl* __restrict__ c, const double2* const* __restrict__ Ul,
-                                                       const double2* const* __restrict__ AUl,
-                                                       const double2* __restrict__ coef, int m,
-                                                       double2* __restrict__ x, double2* __restrict__ ax,
-                                                       const double2* __restrict__ rhs, double2* __restrict__ g,
-                                                       const double2* __restrict__ pc, double2* __restrict__ unext,
-                                                       size_t n, double* __restrict__ part, double ffff, int it,
-                                                       int check) {
+// The first basis vector of a cycle, U = g . pc (OCEAN_gmres.f90:760-766), and
+// <rhs|U> into rU[slot].
+__global__ __launch_bounds__(kBlock) void k_pc_apply(GmCtl* __restrict__ c, double2* __restrict__ u,
+                                                     const double2* __restrict__ g, const double2* __restrict__ pc,
+                                                     const double2* __restrict__ rhs, size_t n,
+                                                     double* __restrict__ part, double2* __restrict__ rU, int slot) {
   if (c->done) return;
-  constexpr int R = 8;
   double v[3] = {0, 0, 0};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double2 xv = x[e], av = ax[e];
+    const double2 uv = og::cmul(g[e], pc[e]);
+    u[e] = uv;
+    dot_term(rhs[e], uv, v[1], v[2]);
+  }
+  resid_finish(c, v, part, 0.0, 0, 0, kDotU, nullptr, rU, 0, slot);
+}
+
+// ax += sum_i c_i AU_i (update_gmres, OCEAN_gmres.f90:336-341; eight rows per step
+// with their loads in flight, the sum in i order), then in the same pass the residual
+// step of iteration it: g = ax - rhs, |g|^2, the next basis vector unext = g . pc
+// (nullable; into rU[slot] goes <rhs|unext>), and <rhs|x> from the cumulative
+// coefficients C (already holding this update) without forming x.
+__global__ __launch_bounds__(kBlock) void k_axpy_resid(GmCtl* __restrict__ c, const double2* const* __restrict__ AUl,
+                                                       const double2* __restrict__ coef, int m,
+                                                       double2* __restrict__ ax, const double2* __restrict__ rhs,
+                                                       double2* __restrict__ g, const double2* __restrict__ pc,
+                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
+                                                       double ffff, int it, int check, const double2* __restrict__ C,
+                                                       double2* __restrict__ rU, int slot) {
+  // the flag is read with the first loads and tested before the first store
+  const int done = c->done;
+  constexpr int R = 8;
+  double v[3] = {0, 0, 0};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const double2 z = make_double2(0.0, 0.0);
+  // two elements (e0 and e0 + stride, the grid-stride order) per step, so that twice
+  // the rows are in flight
+  for (size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < n; e0 += 2 * stride) {
+    const size_t e1 = e0 + stride;
+    const bool has1 = e1 < n;
+    double2 av0 = ax[e0], av1 = has1 ? ax[e1] : z;
     for (int i0 = 0; i0 < m; i0 += R) {
-      double2 w[R], a[R];
+      double2 a0[R], a1[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        w[q] = Ul[min(i0 + q, m - 1)][e];
-        a[q] = AUl[min(i0 + q, m - 1)][e];
+        const double2* row = AUl[min(i0 + q, m - 1)];
+        a0[q] = row[e0];
+        a1[q] = has1 ? row[e1] : z;
       }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         if (i0 + q < m) {
           const double2 ci = coef[i0 + q];
-          xv.x += ci.x * w[q].x; xv.y += ci.x * w[q].y;
-          xv.x += -ci.y * w[q].y; xv.y += ci.y * w[q].x;
-          av.x += ci.x * a[q].x; av.y += ci.x * a[q].y;
-          av.x += -ci.y * a[q].y; av.y += ci.y * a[q].x;
+          av0.x += ci.x * a0[q].x; av0.y += ci.x * a0[q].y;
+          av0.x += -ci.y * a0[q].y; av0.y += ci.y * a0[q].x;
+          av1.x += ci.x * a1[q].x; av1.y += ci.x * a1[q].y;
+          av1.x += -ci.y * a1[q].y; av1.y += ci.y * a1[q].x;
         }
       }
     }
-    x[e] = xv;
-    ax[e] = av;
-    resid_elem(e, av, xv, rhs, g, pc, unext, v);
+    if (done) return;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const size_t e = h ? e1 : e0;
+      if (h && !has1) break;
+      const double2 av = h ? av1 : av0;
+      ax[e] = av;
+      const double2 b = resid_g(e, av, rhs, g, v[0]);
+      if (unext) {
+        const double2 gv = make_double2(av.x - b.x, av.y - b.y);
+        const double2 uv = og::cmul(gv, pc[e]);
+        unext[e] = uv;
+        dot_term(b, uv, v[1], v[2]);
+      }
+    }
   }
-  resid_finish(c, v, part, ffff, it, check);
+  if (done) return;
+  resid_finish(c, v, part, ffff, it, check, kResU, C, rU, m, unext ? slot : -1);
+}
+
+// After the basis coefficients have been folded into x (k_materialize): <rhs|x_b> +=
+// sum_i C_i <rhs|U_i> and C = 0.  One wave; skip may be null.
+__global__ __launch_bounds__(64) void k_fold_c(const int* __restrict__ skip, GmCtl* __restrict__ c,
+                                               double2* __restrict__ C, const double2* __restrict__ rU, int m) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int i = lane; i < m; i += 64) {
+    const double2 ci = C[i];
+    if (ci.x != 0.0 || ci.y != 0.0) {
+      const double2 u = rU[i];
+      ar += ci.x * u.x - ci.y * u.y;
+      ai += ci.x * u.y + ci.y * u.x;
+    }
+    C[i] = make_double2(0.0, 0.0);
+  }
+  ar = og::wave_sum(ar);
+  ai = og::wave_sum(ai);
+  if (lane == 0) {
+    c->rxr += ar;
+    c->rxi += ai;
+  }
 }
 
 }  // namespace
@@ -2119,16 +2216,20 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = dmalloc(&part, (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth + 3 * (size_t)kRedBlocks + 16);
   double2* dL = nullptr;
   double* dD = nullptr;
+  double2* dC = nullptr;   // cumulative least-squares coefficients of the cycle
+  double2* dRU = nullptr;  // <rhs|U_i>
   GmCtl* ctl = nullptr;
   GmCtl* hctl = nullptr;
   if (rc == 0) rc = dmalloc(&dL, (size_t)depth * (depth - 1) / 2 + 1);
   if (rc == 0) rc = dmalloc(&dD, (size_t)depth);
+  if (rc == 0) rc = dmalloc(&dC, (size_t)depth);
+  if (rc == 0) rc = dmalloc(&dRU, (size_t)depth);
   if (rc == 0) rc = dmalloc(&ctl, (size_t)1);
   if (rc == 0 && hipHostMalloc((void**)&hctl, sizeof(GmCtl), hipHostMallocDefault) != hipSuccess)
     rc = OCEAN_GPU_ERR_NOMEM;
   auto cleanup = [&]() {
     for (auto& p : V) dfree(p);
-    dfree(dlist); dfree(dcoef); dfree(part); dfree(dL); dfree(dD); dfree(ctl);
+    dfree(dlist); dfree(dcoef); dfree(part); dfree(dL); dfree(dD); dfree(dC); dfree(dRU); dfree(ctl);
     if (hctl) (void)hipHostFree(hctl);
   };
   if (rc) { cleanup(); return rc; }
@@ -2148,6 +2249,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   k_gm_reset<<<1, 64, 0, st>>>(ctl);
   og::k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double2*>(ctl), make_double2(0.0, 0.0), sizeof(GmCtl) / 16);
   og::k_fill<<<grid, kBlock, 0, st>>>(x, make_double2(1.0, 0.0), nv);
+  og::k_fill<<<1, 64, 0, st>>>(dC, make_double2(0.0, 0.0), (size_t)depth);
   rc = launched();
   if (rc == 0) rc = xact_dev(h, x, h1, f);  // h1 = H . 1
   if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
@@ -2160,29 +2262,41 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     OG_TRY(xact_dev(h, in, outv, f, nullptr, skip, &c));
     return launched();
   };
-  // g = ax - rhs, its sums and (with check) the stopping test of iteration it; unext
-  // (nullable) receives the next basis vector g . pc
-  double* rpart = part + (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth;
-  auto resid = [&](int it, int check, double2* unext) -> int {
-    k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, pc, unext, nv, rpart, ffff, it, check);
+  // x is kept as x_b + sum_i C_i U_i: each iteration's update goes into the
+  // cumulative coefficients C and into ax, and <rhs|x> for the stopping test comes
+  // from <rhs|x_b> and <rhs|U_i>; x itself is formed (materialise) only where the
+  // basis is about to be replaced, at a restart, and when the solve has converged.
+  // gated: dropped once the solve has stopped (the converged x is formed by the host's
+  // ungated call after reading the state)
+  auto materialise = [&](bool gated) -> int {
+    k_materialize<<<grid, kBlock, 0, st>>>(gated ? skip : nullptr, (const double2* const*)(dlist + depth), dC, depth,
+                                           x, nv);
+    k_fold_c<<<1, 64, 0, st>>>(gated ? skip : nullptr, ctl, dC, dRU, depth);
     return launched();
   };
-  // update_gmres(cur): Gram column cur, least-squares solve and x / ax updates
-  // (OCEAN_gmres.f90:241-344); with rit >= 0 the updates carry the residual step of
-  // iteration rit (as resid(rit, check, unext))
+  // g = ax - rhs with x materialised, its sums and (with check) the stopping test of
+  // iteration it
+  double* rpart = part + (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth;
+  auto resid = [&](int it, int check) -> int {
+    k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, nv, rpart, ffff, it, check);
+    return launched();
+  };
+  // update_gmres(cur): Gram column cur, least-squares solve and the x / ax updates
+  // (OCEAN_gmres.f90:241-344); with rit >= 0 the ax update carries the residual step
+  // of iteration rit, and unext (nullable) receives the next basis vector U[cur];
+  // without, the update ends a cycle: x is materialised (ax is recomputed from it)
   auto update = [&](int cur, int rit = -1, int check = 0, double2* unext = nullptr) -> int {
     k_multidot2<<<kRedBlocks, kBlock, 0, st>>>(skip, (const double2* const*)dlist, cur, AU[cur - 1], g, nv, part);
     double* fin = part + (size_t)kRedBlocks * 4 * depth;
     k_multifinish<<<(4 * cur + 3) / 4, 256, 0, st>>>(skip, part, kRedBlocks, 4 * cur, fin);
-    k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, cur, depth, info);
-    if (rit >= 0)
-      k_axpy_resid<<<kRedBlocks, kBlock, 0, st>>>(ctl, (const double2* const*)(dlist + depth),
-                                                  (const double2* const*)dlist, dcoef, cur, x, ax, rhs, g, pc, unext,
-                                                  nv, rpart, ffff, rit, check);
-    else
-      k_multiaxpy2<<<grid, kBlock, 0, st>>>(skip, (const double2* const*)(dlist + depth), (const double2* const*)dlist,
-                                            dcoef, cur, x, ax, nv);
-    return launched();
+    k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, dC, cur, depth, info);
+    if (rit >= 0) {
+      k_axpy_resid<<<kRedBlocks, kBlock, 0, st>>>(ctl, (const double2* const*)dlist, dcoef, cur, ax, rhs, g, pc,
+                                                  unext, nv, rpart, ffff, rit, check, dC, dRU, cur);
+      return launched();
+    }
+    OG_TRY(launched());
+    return materialise(true);
   };
   auto read_ctl = [&]() -> int {
     OG_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st));
@@ -2215,13 +2329,13 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         og::k_fill<<<grid, kBlock, 0, st>>>(x, make_double2(0.0, 0.0), nv);
         og::k_fill<<<grid, kBlock, 0, st>>>(ax, make_double2(0.0, 0.0), nv);
       }
-      if ((rc = resid(0, 0, nullptr))) break;
+      if ((rc = resid(0, 0))) break;
     } else {
       // make_recycle: au += (E - E_prev) u, then the updates over the kept basis
       const double delta = E - energies_eV[step - 1] * kEV2Hartree;
       for (int i = 0; i < prev_iter; ++i) k_axpy_gated<<<grid, kBlock, 0, st>>>(skip, delta, 0.0, U[i], AU[i], nv);
       if ((rc = axmy_xact(x, ax, E))) break;
-      if ((rc = resid(0, 0, nullptr))) break;
+      if ((rc = resid(0, 0))) break;
       for (int it = 1; it <= prev_iter && rc == 0; ++it) rc = update(it, 0, 0, nullptr);
       if (rc || (rc = read_ctl())) break;
       const double gval = hctl->gval, ival = hctl->ival;
@@ -2230,6 +2344,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         if (rel < 0.001 || gval < ffff / 10.0) converged = true;
       }
       if (!converged && std::fabs(prev_energy - E) > std::fabs(gprc) / 4.0) {
+        // a new cycle replaces the kept basis: form x from it first
+        if ((rc = materialise(false))) break;
         prev_iter = 0;
         prev_energy = E;
         if ((rc = set_prec(E))) break;
@@ -2248,7 +2364,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         for (; it <= iend && rc == 0; ++it) {
           // the first basis vector of a cycle from g and the current preconditioner;
           // later ones were formed by the previous iteration's residual step
-          if (it == cycle_start + 1) k_cmul3<<<grid, kBlock, 0, st>>>(skip, U[it - 1], g, pc, nv);
+          if (it == cycle_start + 1)
+            k_pc_apply<<<kRedBlocks, kBlock, 0, st>>>(ctl, U[it - 1], g, pc, rhs, nv, rpart, dRU, it - 1);
           if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
           if (it < depth) {
             rc = update(it, it, 1, U[it]);
@@ -2257,7 +2374,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
             if ((rc = update(it))) break;
             if ((rc = axmy_xact(x, ax, E))) break;
             if (new_prec_at_end && (rc = set_prec(E))) break;
-            rc = resid(it, 1, nullptr);
+            rc = resid(it, 1);
           }
         }
         if (rc || (rc = read_ctl())) break;
@@ -2278,6 +2395,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
       prev_iter = 0;
     }
     if (rc) break;
+    // the converged x (C is zero if it was formed at a restart)
+    if ((rc = materialise(false))) break;
     const double gval = hctl->gval, rval = hctl->rval, ival = hctl->ival;
     double* o = out + 6 * (size_t)step;
     o[0] = energies_eV[step];
