@@ -1,9 +1,9 @@
 """Non-Hermitian (bi-Lanczos) Haydock recursion (ocean_gpu_haydock_abc) and the
 continued fraction with complex b and c (ocean_gpu_spectrum_abc) against the oracle.
 
-The problems are made non-Hermitian the two ways the backwards action is the
-Hermitian conjugate for: a complex-symmetric spin-orbit matrix and complex band
-energies (a lifetime term).  Tolerances: the coefficients of 30 FP64 steps 1e-9
+The problems are made non-Hermitian the ways the backwards action is the Hermitian
+conjugate for: a complex-symmetric spin-orbit matrix (random, or the reference's
+soprep with core-hole lifetimes) and complex band energies (a lifetime term).  Tolerances: the coefficients of 30 FP64 steps 1e-9
 relative to the largest coefficient (the two recursions sum their dots in different
 orders and the bi-Lanczos recursion amplifies rounding more than the Hermitian one);
 the fraction kernel 1e-10 of peak on the same coefficients; spectra of the two
@@ -28,7 +28,12 @@ CASES = {
     # L2,3: twelve alphas (matrix-core projection and adjoint) with its own
     # spin-orbit term made complex-symmetric
     "l23": dict(nb=13, kmesh=(3, 2, 2), xmesh=(3, 3, 5), lc=1, seed=43, xi_eV=3.0),
+    # the branch's physical use: the L2,3 spin-orbit term with core-hole lifetimes
+    # (core_broaden.ipt), S = S_r + i S_i from the reference's soprep, both parts real
+    # symmetric, so the backwards action is exactly H^dagger; real band energies
+    "l23_lifetime": dict(nb=13, kmesh=(3, 2, 2), xmesh=(3, 3, 5), lc=1, seed=44, xi_eV=3.0),
 }
+LIFETIME_EV = {"l23_lifetime": (0.6, 0.9)}
 
 
 def _inputs(name, energy_lifetime=True):
@@ -36,12 +41,19 @@ def _inputs(name, energy_lifetime=True):
     inp = synth.make_inputs(**kw)
     rng = np.random.default_rng(kw["seed"] + 200)
     na = inp.energy.shape[2]
-    s = rng.standard_normal((na, na)) + 1j * rng.standard_normal((na, na))
-    s = 2e-3 * (s + s.T)
-    if inp.mult.have_spin_orbit:
-        s = s + inp.mult.somelr  # keep the real (symmetric) part of the L2,3 term
-    inp.mult.somelr = np.asfortranarray(s.real)
-    inp.mult.someli = np.asfortranarray(s.imag)
+    if name in LIFETIME_EV:
+        sr, si, _ = orc.soprep(kw["lc"], kw["xi_eV"], LIFETIME_EV[name])
+        assert np.abs(si).max() > 0 and np.array_equal(si, si.T) and np.array_equal(sr, sr.T)
+        inp.mult.somelr = np.asfortranarray(sr)
+        inp.mult.someli = np.asfortranarray(si)
+        energy_lifetime = False
+    else:
+        s = rng.standard_normal((na, na)) + 1j * rng.standard_normal((na, na))
+        s = 2e-3 * (s + s.T)
+        if inp.mult.have_spin_orbit:
+            s = s + inp.mult.somelr  # keep the real (symmetric) part of the L2,3 term
+        inp.mult.somelr = np.asfortranarray(s.real)
+        inp.mult.someli = np.asfortranarray(s.imag)
     inp.mult.have_spin_orbit = True
     if energy_lifetime:
         inp.energy = np.asfortranarray(inp.energy + 1j * 0.01 * rng.uniform(size=inp.energy.shape))
