@@ -335,6 +335,23 @@ def gmres_workload(inp, g, n=64):
     return {"gmres_energies_per_s_fp64": round(n / dt, 2), "gmres_matvecs": int(rows[-1, 5])}
 
 
+def abc_workload(inp, g, niter=300):
+    """The non-Hermitian (bi-Lanczos) recursion at the 64-band K-edge workload: niter
+    steps (each the action and the backwards action plus three vector passes) and the
+    complex-b/c continued fraction, after a one-solve warm-up."""
+    from ocean_amd import core
+
+    p = inp.params
+    E2H = core.EV2HARTREE
+    g.haydock_abc(inp.psi0, 8)
+    t0 = time.perf_counter()
+    coef = g.haydock_abc(inp.psi0, niter)
+    t1 = time.perf_counter()
+    core.spectrum_abc(*coef, niter, inp.kpref, p["gam"] * E2H, p["el"] * E2H, p["eh"] * E2H, p["ne"])
+    t2 = time.perf_counter()
+    return {"abc_steps_per_s_fp64": round(niter / (t1 - t0), 1), "abc_spectra_per_s_fp64": round(1.0 / (t2 - t0), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +378,8 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip the shared-Hamiltonian right-hand sides")
     ap.add_argument("--gmres-only", action="store_true",
                     help="only GMRES at 64 energies at the 64-band K edge (for profiling)")
+    ap.add_argument("--abc-only", action="store_true",
+                    help="only the bi-Lanczos recursion (300 steps) at the 64-band K edge (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -392,12 +411,14 @@ def main():
             out.update(multi_rhs_workload(inp, nrhs=r, reps=args.steps, niter=40))
         print(json.dumps(out), flush=True)
         return
-    if args.mixed_only or args.gmres_only:
+    if args.mixed_only or args.gmres_only or args.abc_only:
         from ocean_amd import core, synth
         inp = synth.workload("k_nb64")
         g = core.OceanGPU.from_inputs(inp, precision=core.MIXED if args.mixed_only else core.FP64)
         if args.gmres_only:
             out = gmres_workload(inp, g)
+        elif args.abc_only:
+            out = abc_workload(inp, g)
         else:
             vin, vout = g.vec_new(inp.psi0), g.vec_new()
             g.xact_repeat(vin, vout, max(args.warmup, 1))
@@ -490,6 +511,8 @@ def main():
     if not args.no_extras:
         extras.update(gmres_workload(inp, g))
         log("gmres:", {k: v for k, v in extras.items() if k.startswith("gmres")})
+        extras.update(abc_workload(inp, g))
+        log("bi-Lanczos:", {k: v for k, v in extras.items() if k.startswith("abc")})
     if not args.no_extras and not args.no_obf:
         extras.update(obf_workload(inp))
     if not args.no_extras and not args.no_l23:
