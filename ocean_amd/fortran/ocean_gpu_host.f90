@@ -4,9 +4,12 @@
 !                                OCEAN2/OCEAN_action.f90:286
 !   OCEAN_haydock_do_gpu      <- OCEAN_haydock_do / OCEAN_haydock_Herm_do
 !                                OCEAN2/OCEAN_haydock.f90:167, :190-320
+!   OCEAN_haydock_nonHerm_do_gpu <- OCEAN_haydock_nonHerm_do (complex_haydock branch)
+!                                OCEAN2/OCEAN_haydock.f90:53-160, :1250-1433
 !   OCEAN_gmres_do_recycle_gpu <- OCEAN_gmres_do_recycle, OCEAN2/OCEAN_gmres.f90:508
 !   write_core_gpu      <- write_core, OCEAN2/OCEAN_haydock.f90:1801-1894
 !   write_lanczos_gpu   <- write_lanczos, OCEAN2/OCEAN_haydock.f90:2318-2358
+!   write_lanczos_abc_gpu <- write_lanczos, complex branch, OCEAN2/OCEAN_haydock.f90:2337-2344
 !
 ! ierr follows the reference: 0 on success, nonzero aborts up the stack.
 module ocean_gpu_host
@@ -21,8 +24,8 @@ module ocean_gpu_host
   real(DP), parameter :: Hartree2eV = 27.211386245980_DP   ! OCEAN2/OCEAN_constants.f90:16
 
   public :: OCEAN_gpu_init, OCEAN_gpu_load, OCEAN_xact_gpu, OCEAN_haydock_do_gpu, &
-            OCEAN_gmres_do_recycle_gpu, write_core_gpu, write_lanczos_gpu, &
-            OCEAN_gpu_finalize, OCEAN_precision_gpu
+            OCEAN_haydock_nonHerm_do_gpu, OCEAN_gmres_do_recycle_gpu, write_core_gpu, &
+            write_lanczos_gpu, write_lanczos_abc_gpu, OCEAN_gpu_finalize, OCEAN_precision_gpu
 
 contains
 
@@ -102,6 +105,22 @@ contains
     end if
     niter_done = done
   end subroutine OCEAN_haydock_do_gpu
+
+  ! Non-Hermitian (bi-Lanczos) recursion, the reference's complex_haydock branch
+  ! (OCEAN_haydock_nonHerm_do, OCEAN_haydock.f90:53-160, with haydock_abc_1 at
+  ! :1250-1433); the six coefficient arrays are (0:niter) as in the reference.
+  subroutine OCEAN_haydock_nonHerm_do_gpu(psi_r, psi_i, niter, real_a, imag_a, real_b, imag_b, &
+                                          real_c, imag_c, niter_done, ierr)
+    real(DP), intent(in) :: psi_r(*), psi_i(*)
+    integer, intent(in) :: niter
+    real(DP), intent(out), dimension(0:niter) :: real_a, imag_a, real_b, imag_b, real_c, imag_c
+    integer, intent(out) :: niter_done
+    integer, intent(inout) :: ierr
+    integer(c_int32_t) :: done
+    ierr = ocean_gpu_haydock_abc(handle, psi_r, psi_i, int(niter, c_int32_t), real_a, imag_a, real_b, &
+                                 imag_b, real_c, imag_c, done)
+    niter_done = done
+  end subroutine OCEAN_haydock_nonHerm_do_gpu
 
   ! Recycled GMRES over energies (eV); writes the absspct rows of OCEAN_gmres.f90:864-869
   ! to unit fh when fh > 0.
@@ -213,6 +232,20 @@ contains
       end if
     end if
   end subroutine write_lanczos_gpu
+
+  ! abslanc writer of the complex branch (write_lanczos, OCEAN_haydock.f90:2337-2344):
+  ! a(0) and, for i = 1..n, a(i), b(i), c(i) as real and imaginary parts.
+  subroutine write_lanczos_abc_gpu(fh, n, kpref, real_a, imag_a, real_b, imag_b, real_c, imag_c)
+    integer, intent(in) :: fh, n
+    real(DP), intent(in) :: kpref
+    real(DP), intent(in), dimension(0:n) :: real_a, imag_a, real_b, imag_b, real_c, imag_c
+    integer :: i
+    write(fh, '(1i8,1x,1ES24.17)') n, kpref
+    write(fh, '(2(2x,ES24.17))') real_a(0), imag_a(0)
+    do i = 1, n
+      write(fh, '(2x,6ES24.17)') real_a(i), imag_a(i), real_b(i), imag_b(i), real_c(i), imag_c(i)
+    end do
+  end subroutine write_lanczos_abc_gpu
 
   subroutine OCEAN_gpu_finalize(ierr)
     integer, intent(inout) :: ierr

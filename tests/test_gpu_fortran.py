@@ -76,3 +76,50 @@ def test_fortran_early_exit(tmp_path):
     assert np.max(np.abs(a[:done] - ra[:done]) / np.abs(ra[:done])) < 1e-10
     lanc = open(tmp_path / "out.abslanc").read().split("\n")
     assert int(lanc[0].split()[0]) == done
+
+
+def test_fortran_nonhermitian_branch(tmp_path):
+    """OCEAN_haydock_nonHerm_do_gpu (the complex_haydock branch) from the Fortran host:
+    an L2,3 problem whose spin-orbit term carries core-hole lifetimes (the reference's
+    soprep with core_broaden, so the backwards action is H^dagger).  The coefficients
+    against the oracle's bi-Lanczos recursion, and the six-column abslanc of
+    write_lanczos's complex branch (OCEAN2/OCEAN_haydock.f90:2337-2344) against them."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    inp = synth.make_inputs(nb=8, kmesh=(2, 2, 2), xmesh=(3, 3, 3), lc=1, seed=15, xi_eV=4.0)
+    sr, si, _ = orc.soprep(1, 4.0, (0.6, 0.9))
+    inp.mult.somelr = np.asfortranarray(sr)
+    inp.mult.someli = np.asfortranarray(si)
+    inp.mult.have_spin_orbit = True
+    P = to_problem(inp)
+    E2H = orc.EV2HARTREE
+    niter = 30
+    fin = tmp_path / "bse_gpu.in"
+    write_driver_input(fin, inp, niter, 100, 0.5 * E2H, -5 * E2H, 40 * E2H)
+    print("fortran driver case: bi-Lanczos nb=8 k=2^3 x=3^3 lc=1 with lifetimes", flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(tmp_path / "out"), "abc"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(tmp_path / "out.bin", "rb").read()
+    ierr, done = np.frombuffer(raw[:8], dtype="<i4")
+    d = np.frombuffer(raw[8:], dtype="<f8")
+    N, n1 = inp.nb * inp.nk * inp.nalpha, niter + 1
+    co = d[2 * N:2 * N + 6 * n1].reshape(6, n1)  # ra, rb, ia, ib, rc, ic
+    got = (co[0], co[2], co[1], co[3], co[4], co[5])  # ra, ia, rb, ib, rc, ic
+    assert ierr == 0 and done == niter
+    ref = orc.haydock_abc(P, inp.psi0, niter)
+    assert ref[-1] == 0
+    scale = max(np.abs(v).max() for v in ref[:6])
+    err = max(np.abs(g - rv).max() for g, rv in zip(got, ref[:6])) / scale
+    print("bi-Lanczos coefficient err", err, flush=True)
+    assert err < 1e-9
+    lines = open(tmp_path / "out.abslanc").read().split("\n")
+    assert int(lines[0].split()[0]) == niter
+    # fixed-width fields: (2(2x,ES24.17)) and (2x,6ES24.17), the latter with no blank
+    # between fields (a minus sign takes the leading blank)
+    a0 = [float(lines[1][2:26]), float(lines[1][28:52])]
+    assert np.allclose(a0, [got[0][0], got[1][0]], rtol=1e-15, atol=0)
+    rows = np.array([[float(ln[2 + 24 * j:2 + 24 * (j + 1)]) for j in range(6)] for ln in lines[2:2 + niter]])
+    want = np.stack([g[1:niter + 1] for g in got], axis=1)
+    assert rows.shape == (niter, 6)
+    assert np.allclose(rows, want, rtol=1e-15, atol=1e-300)
