@@ -659,92 +659,101 @@ __global__ __launch_bounds__(64) void k_ls_update(int* __restrict__ skip, const 
   for (int q = 0; q < kLsSlots; ++q)
     if (lane + 64 * q < j) Dl[lane + 64 * q] = dv[q];
   __syncthreads();
-  auto Lat = [&](int i, int k) -> double2 {  // L(i, k), k < i < j
-    return in_lds ? Ls[i * (i - 1) / 2 + k] : L[(size_t)i * (i - 1) / 2 + k];
-  };
-  // row j: after step k, t(k) is final and L(j, k) = t(k) / D(k) updates every t(kk), kk > k
-  double dsum = 0.0;
-  for (int k = 0; k < j; ++k) {
-    const double2 tk = ls_bcast(t, k);
-    const double dk = Dl[k];
-    const double2 a = make_double2(tk.x / dk, tk.y / dk);
-    if (lane == 0) dsum -= (a.x * a.x + a.y * a.y) * dk;
-#pragma unroll
-    for (int q = 0; q < kLsSlots; ++q) {
-      const int kk = lane + 64 * q;
-      if (kk == k) lj[q] = a;
-      if (kk > k && kk < j) {
-        const double2 b = Lat(kk, k);
-        // a * conj(b) * d
-        t[q].x -= (a.x * b.x + a.y * b.y) * dk;
-        t[q].y -= (a.y * b.x - a.x * b.y) * dk;
+  // The sweeps, compiled once for L staged in LDS and once for L in global memory: a
+  // run-time choice between the two address spaces would make every L read a flat
+  // load that waits on the vector memory counter inside the serial sweeps.
+  auto sweeps = [&](auto in_lds_tag) {
+    constexpr bool kInLds = decltype(in_lds_tag)::value;
+    auto Lat = [&](int i, int k) -> double2 {  // L(i, k), k < i < j
+      if constexpr (kInLds) return Ls[i * (i - 1) / 2 + k];
+      else return L[(size_t)i * (i - 1) / 2 + k];
+    };
+    // row j: after step k, t(k) is final and L(j, k) = t(k) / D(k) updates every t(kk), kk > k
+    double dsum = 0.0;
+    for (int k = 0; k < j; ++k) {
+      const double2 tk = ls_bcast(t, k);
+      const double dk = Dl[k];
+      const double2 a = make_double2(tk.x / dk, tk.y / dk);
+      if (lane == 0) dsum -= (a.x * a.x + a.y * a.y) * dk;
+  #pragma unroll
+      for (int q = 0; q < kLsSlots; ++q) {
+        const int kk = lane + 64 * q;
+        if (kk == k) lj[q] = a;
+        if (kk > k && kk < j) {
+          const double2 b = Lat(kk, k);
+          // a * conj(b) * d
+          t[q].x -= (a.x * b.x + a.y * b.y) * dk;
+          t[q].y -= (a.y * b.x - a.x * b.y) * dk;
+        }
       }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < kLsSlots; ++q) {
-    const int k = lane + 64 * q;
-    if (k < j) L[(size_t)npk + k] = lj[q];  // row j at j (j - 1) / 2
-  }
-  if (lane == 0) {
-    const double dj = fin[4 * j] + dsum;
-    D[j] = dj;
-    Dl[j] = dj;
-  }
-  __syncthreads();
-  int bad = 0;
-  for (int k = 0; k <= j; ++k)
-    if (Dl[k] == 0.0) { bad = k + 1; break; }
-  if (bad) {
+  #pragma unroll
+    for (int q = 0; q < kLsSlots; ++q) {
+      const int k = lane + 64 * q;
+      if (k < j) L[(size_t)npk + k] = lj[q];  // row j at j (j - 1) / 2
+    }
     if (lane == 0) {
-      *info = bad;
-      *skip = 2;  // the solve stops here: later enqueued work is dropped
+      const double dj = fin[4 * j] + dsum;
+      D[j] = dj;
+      Dl[j] = dj;
     }
-    for (int i = lane; i < cur; i += 64) coef[i] = make_double2(0.0, 0.0);
-    return;
-  }
-  // coef = -C^-1 rvec: L y = -rvec, then D, then L^H coef = y (row j of L from the
-  // registers of its owners)
-  for (int k = 0; k < cur; ++k) {
-    const double2 v = ls_bcast(r, k);
-    const double2 ljk = ls_bcast(lj, k);
-#pragma unroll
-    for (int q = 0; q < kLsSlots; ++q) {
-      const int i = lane + 64 * q;
-      if (i > k && i < cur) {
-        const double2 l = i == j ? ljk : Lat(i, k);
-        r[q].x -= l.x * v.x - l.y * v.y;
-        r[q].y -= l.x * v.y + l.y * v.x;
+    __syncthreads();
+    int bad = 0;
+    for (int k = 0; k <= j; ++k)
+      if (Dl[k] == 0.0) { bad = k + 1; break; }
+    if (bad) {
+      if (lane == 0) {
+        *info = bad;
+        *skip = 2;  // the solve stops here: later enqueued work is dropped
+      }
+      for (int i = lane; i < cur; i += 64) coef[i] = make_double2(0.0, 0.0);
+      return;
+    }
+    // coef = -C^-1 rvec: L y = -rvec, then D, then L^H coef = y (row j of L from the
+    // registers of its owners)
+    for (int k = 0; k < cur; ++k) {
+      const double2 v = ls_bcast(r, k);
+      const double2 ljk = ls_bcast(lj, k);
+  #pragma unroll
+      for (int q = 0; q < kLsSlots; ++q) {
+        const int i = lane + 64 * q;
+        if (i > k && i < cur) {
+          const double2 l = i == j ? ljk : Lat(i, k);
+          r[q].x -= l.x * v.x - l.y * v.y;
+          r[q].y -= l.x * v.y + l.y * v.x;
+        }
       }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < kLsSlots; ++q) {
-    const int i = lane + 64 * q;
-    if (i < cur) { r[q].x /= Dl[i]; r[q].y /= Dl[i]; }
-  }
-  for (int k = cur - 1; k > 0; --k) {
-    const double2 v = ls_bcast(r, k);
-#pragma unroll
+  #pragma unroll
     for (int q = 0; q < kLsSlots; ++q) {
       const int i = lane + 64 * q;
-      if (i < k) {
-        const double2 l = k == j ? lj[q] : Lat(k, i);  // conj(l) * v
-        r[q].x -= l.x * v.x + l.y * v.y;
-        r[q].y -= l.x * v.y - l.y * v.x;
+      if (i < cur) { r[q].x /= Dl[i]; r[q].y /= Dl[i]; }
+    }
+    for (int k = cur - 1; k > 0; --k) {
+      const double2 v = ls_bcast(r, k);
+  #pragma unroll
+      for (int q = 0; q < kLsSlots; ++q) {
+        const int i = lane + 64 * q;
+        if (i < k) {
+          const double2 l = k == j ? lj[q] : Lat(k, i);  // conj(l) * v
+          r[q].x -= l.x * v.x + l.y * v.y;
+          r[q].y -= l.x * v.y - l.y * v.x;
+        }
       }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < kLsSlots; ++q) {
-    const int i = lane + 64 * q;
-    if (i < cur) {
-      coef[i] = r[q];
-      double2 ci = C[i];
-      ci.x += r[q].x; ci.y += r[q].y;
-      C[i] = ci;
+  #pragma unroll
+    for (int q = 0; q < kLsSlots; ++q) {
+      const int i = lane + 64 * q;
+      if (i < cur) {
+        coef[i] = r[q];
+        double2 ci = C[i];
+        ci.x += r[q].x; ci.y += r[q].y;
+        C[i] = ci;
+      }
     }
-  }
+  };
+  if (in_lds) sweeps(std::true_type{});
+  else sweeps(std::false_type{});
 }
 
 // pcdiv = (E - (h1_r + i(h1_i + g))) / ((E - h1)^2 + g^2)  (OCEAN_psi_min_set_prec, :2907-2994)
