@@ -785,7 +785,7 @@ struct GmCtl {
   int done;     // 0 running, 1 converged, 2 least-squares pivot failure
   int conv_it;  // the iteration (1..depth of its cycle) at which it converged
   int info;     // zhptrf-style pivot index of a failure
-  int count;    // arrival counter of the residual workgroups (rests at 0)
+  int pad1;     // (keeps the doubles 16-byte aligned)
   double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
   double rxr, rxi, pad3, pad4;    // <rhs|x_b> of the materialised x (x = x_b + sum_i C_i U_i)
 };
@@ -810,9 +810,10 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
   }
 }
 
-// The sums of one workgroup (v[3], block-summed in thread 0) into its partials and, in
-// the last workgroup to arrive (device-scope counter), the three totals in a fixed
-// order; the counter returns to zero.  By mode:
+// The residual kernels write the sums of each workgroup (v[3], block-summed in thread
+// 0) as partials; k_resid_fin, one workgroup launched after them, sums the partials
+// in a fixed order.  (A last-workgroup-to-arrive finish inside the kernels serialised
+// a device-scope atomic per workgroup.)  By mode:
 //   kResX   (x materialised, C = 0): v = (|g|^2, <rhs|x>); <rhs|x> becomes <rhs|x_b>;
 //   kResU   v = (|g|^2, <rhs|U_slot>) (slot < 0: no new vector); <rhs|x> is
 //           <rhs|x_b> + sum_{i<m} C_i <rhs|U_i>;
@@ -822,28 +823,24 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
 // or |g|^2 < ffff / 10).
 enum { kResX = 0, kResU = 1, kDotU = 2 };
 
-__device__ __forceinline__ void resid_finish(GmCtl* __restrict__ c, double (&v)[3], double* __restrict__ part,
-                                             double ffff, int it, int check, int mode,
-                                             const double2* __restrict__ C, double2* __restrict__ rU, int m,
-                                             int slot) {
+__device__ __forceinline__ void resid_part(double (&v)[3], double* __restrict__ part) {
   __shared__ double lds[4 * 3];
-  __shared__ double sv[5];
-  __shared__ int is_last;
   og::block_sum<3>(v, lds);
   if (threadIdx.x == 0) {
     part[3 * blockIdx.x] = v[0]; part[3 * blockIdx.x + 1] = v[1]; part[3 * blockIdx.x + 2] = v[2];
-    __threadfence();
-    is_last = atomicAdd(&c->count, 1) == (int)gridDim.x - 1;
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
+
+__global__ __launch_bounds__(256) void k_resid_fin(GmCtl* __restrict__ c, const double* __restrict__ part,
+                                                   int nblk, double ffff, int it, int check, int mode,
+                                                   const double2* __restrict__ C, double2* __restrict__ rU,
+                                                   int m, int slot) {
+  if (c->done) return;
+  __shared__ double sv[5];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (w < 3) {
-    // other workgroups (possibly on other XCDs) wrote the partials: device-scope loads
     double acc = 0.0;
-    for (int b = lane; b < (int)gridDim.x; b += 64)
-      acc += __hip_atomic_load(&part[3 * b + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = lane; b < nblk; b += 64) acc += part[3 * b + w];
     acc = og::wave_sum(acc);
     if (lane == 0) sv[w] = acc;
   } else if (mode == kResU) {
@@ -880,7 +877,6 @@ __device__ __forceinline__ void resid_finish(GmCtl* __restrict__ c, double (&v)[
         }
       }
     }
-    atomicExch(&c->count, 0);
   }
 }
 
@@ -900,27 +896,26 @@ __device__ __forceinline__ void dot_term(double2 b, double2 xv, double& vr, doub
   vi = fma(b.x, xv.y, vi); vi = fma(-b.y, xv.x, vi);
 }
 
-// The residual step of iteration it with x materialised (after an action has
-// recomputed ax from x): g = ax - rhs, |g|^2, <rhs|x> = <rhs|x_b>.
+// The residual step with x materialised (after an action has recomputed ax from x):
+// g = ax - rhs and the partials of |g|^2 and <rhs|x> (k_resid_fin, kResX).
 __global__ __launch_bounds__(kBlock) void k_resid_step(GmCtl* __restrict__ c, const double2* __restrict__ ax,
                                                        const double2* __restrict__ rhs, const double2* __restrict__ x,
-                                                       double2* __restrict__ g, size_t n, double* __restrict__ part,
-                                                       double ffff, int it, int check) {
+                                                       double2* __restrict__ g, size_t n, double* __restrict__ part) {
   if (c->done) return;
   double v[3] = {0, 0, 0};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     const double2 b = resid_g(e, ax[e], rhs, g, v[0]);
     dot_term(b, x[e], v[1], v[2]);
   }
-  resid_finish(c, v, part, ffff, it, check, kResX, nullptr, nullptr, 0, -1);
+  resid_part(v, part);
 }
 
-// The first basis vector of a cycle, U = g . pc (OCEAN_gmres.f90:760-766), and
-// <rhs|U> into rU[slot].
+// The first basis vector of a cycle, U = g . pc (OCEAN_gmres.f90:760-766), and the
+// partials of <rhs|U> (k_resid_fin, kDotU).
 __global__ __launch_bounds__(kBlock) void k_pc_apply(GmCtl* __restrict__ c, double2* __restrict__ u,
                                                      const double2* __restrict__ g, const double2* __restrict__ pc,
                                                      const double2* __restrict__ rhs, size_t n,
-                                                     double* __restrict__ part, double2* __restrict__ rU, int slot) {
+                                                     double* __restrict__ part) {
   if (c->done) return;
   double v[3] = {0, 0, 0};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -928,21 +923,20 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(GmCtl* __restrict__ c, doub
     u[e] = uv;
     dot_term(rhs[e], uv, v[1], v[2]);
   }
-  resid_finish(c, v, part, 0.0, 0, 0, kDotU, nullptr, rU, 0, slot);
+  resid_part(v, part);
 }
 
 // ax += sum_i c_i AU_i (update_gmres, OCEAN_gmres.f90:336-341; eight rows per step
 // with their loads in flight, the sum in i order), then in the same pass the residual
-// step of iteration it: g = ax - rhs, |g|^2, the next basis vector unext = g . pc
-// (nullable; into rU[slot] goes <rhs|unext>), and <rhs|x> from the cumulative
+// step: g = ax - rhs, |g|^2, the next basis vector unext = g . pc (nullable) and the
+// partials of <rhs|unext>; k_resid_fin (kResU) forms <rhs|x> from the cumulative
 // coefficients C (already holding this update) without forming x.
 __global__ __launch_bounds__(kBlock) void k_axpy_resid(GmCtl* __restrict__ c, const double2* const* __restrict__ AUl,
                                                        const double2* __restrict__ coef, int m,
                                                        double2* __restrict__ ax, const double2* __restrict__ rhs,
                                                        double2* __restrict__ g, const double2* __restrict__ pc,
-                                                       double2* __restrict__ unext, size_t n, double* __restrict__ part,
-                                                       double ffff, int it, int check, const double2* __restrict__ C,
-                                                       double2* __restrict__ rU, int slot) {
+                                                       double2* __restrict__ unext, size_t n,
+                                                       double* __restrict__ part) {
   // the flag is read with the first loads and tested before the first store
   const int done = c->done;
   constexpr int R = 8;
@@ -991,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy_resid(GmCtl* __restrict__ c, co
     }
   }
   if (done) return;
-  resid_finish(c, v, part, ffff, it, check, kResU, C, rU, m, unext ? slot : -1);
+  resid_part(v, part);
 }
 
 // After the basis coefficients have been folded into x (k_materialize): <rhs|x_b> +=
@@ -2287,7 +2281,8 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   // iteration it
   double* rpart = part + (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth;
   auto resid = [&](int it, int check) -> int {
-    k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, nv, rpart, ffff, it, check);
+    k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, nv, rpart);
+    k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, ffff, it, check, kResX, nullptr, nullptr, 0, -1);
     return launched();
   };
   // update_gmres(cur): Gram column cur, least-squares solve and the x / ax updates
@@ -2301,7 +2296,9 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     k_ls_update<<<1, 64, 0, st>>>(skip, fin, dL, dD, dcoef, dC, cur, depth, info);
     if (rit >= 0) {
       k_axpy_resid<<<kRedBlocks, kBlock, 0, st>>>(ctl, (const double2* const*)dlist, dcoef, cur, ax, rhs, g, pc,
-                                                  unext, nv, rpart, ffff, rit, check, dC, dRU, cur);
+                                                  unext, nv, rpart);
+      k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, ffff, rit, check, kResU, dC, dRU, cur,
+                                     unext ? cur : -1);
       return launched();
     }
     OG_TRY(launched());
@@ -2373,8 +2370,10 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
         for (; it <= iend && rc == 0; ++it) {
           // the first basis vector of a cycle from g and the current preconditioner;
           // later ones were formed by the previous iteration's residual step
-          if (it == cycle_start + 1)
-            k_pc_apply<<<kRedBlocks, kBlock, 0, st>>>(ctl, U[it - 1], g, pc, rhs, nv, rpart, dRU, it - 1);
+          if (it == cycle_start + 1) {
+            k_pc_apply<<<kRedBlocks, kBlock, 0, st>>>(ctl, U[it - 1], g, pc, rhs, nv, rpart);
+            k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, 0.0, 0, 0, kDotU, nullptr, dRU, 0, it - 1);
+          }
           if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
           if (it < depth) {
             rc = update(it, it, 1, U[it]);
