@@ -18,6 +18,89 @@ enum ActFlags : int {
   F_LR = 1, F_FG = 2, F_SO = 4, F_E0 = 8, F_BACK = 16,
 };
 
+// GMRES control state on the device: the solve of one energy enqueues several
+// iterations ahead and every kernel of them returns at once when `done` is set, so
+// the host reads the state back once per block of iterations, not per iteration.
+struct GmCtl {
+  int done;     // 0 running, 1 converged, 2 least-squares pivot failure
+  int conv_it;  // the iteration (1..depth of its cycle) at which it converged
+  int info;     // zhptrf-style pivot index of a failure
+  int pad1;     // (keeps the doubles 16-byte aligned)
+  double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
+  double rxr, rxi, pad3, pad4;    // <rhs|x_b> of the materialised x (x = x_b + sum_i C_i U_i)
+};
+
+// The finish of a GMRES residual step: the residual kernel wrote one partial triple
+// per workgroup; the finish sums them in a fixed order and, by mode:
+//   kResX   (x materialised, C = 0): v = (|g|^2, <rhs|x>); <rhs|x> becomes <rhs|x_b>;
+//   kResU   v = (|g|^2, <rhs|U_slot>) (slot < 0: no new vector); <rhs|x> is
+//           <rhs|x_b> + sum_{i<m} C_i <rhs|U_i>;
+//   kDotU   v = (0, <rhs|U_slot>): the dot only.
+// kResX and kResU store |g|^2 and <rhs|x> and, with check, apply the reference's
+// stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and (-|g|^2 / Im<rhs|x> < 0.001
+// or |g|^2 < ffff / 10).  It runs as its own one-workgroup kernel or, inside a
+// restart cycle, in workgroup 0 of the next action's projection (c == nullptr: none).
+enum { kResX = 0, kResU = 1, kDotU = 2 };
+struct GmFin {
+  GmCtl* c;
+  const double* part;
+  const double2* C;
+  double2* rU;
+  double ffff;
+  int nblk, it, check, mode, m, slot;
+};
+
+// One workgroup of any size (one wave or more).
+__device__ __noinline__ void gm_finish(const GmFin F) {
+  GmCtl* c = F.c;
+  if (c->done) return;
+  __shared__ double sv[5];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int task = w; task < 4; task += nw) {
+    if (task < 3) {
+      double acc = 0.0;
+      for (int b = lane; b < F.nblk; b += 64) acc += F.part[3 * b + task];
+      acc = wave_sum(acc);
+      if (lane == 0) sv[task] = acc;
+    } else if (F.mode == kResU) {
+      // sum_{i<m} C_i <rhs|U_i>, lane-strided then a butterfly (fixed order)
+      double ar = 0.0, ai = 0.0;
+      for (int i = lane; i < F.m; i += 64) {
+        const double2 ci = F.C[i], u = F.rU[i];
+        ar += ci.x * u.x - ci.y * u.y;
+        ai += ci.x * u.y + ci.y * u.x;
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      if (lane == 0) { sv[3] = ar; sv[4] = ai; }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (F.slot >= 0) F.rU[F.slot] = make_double2(sv[1], sv[2]);
+    if (F.mode != kDotU) {
+      double rr, ri;
+      if (F.mode == kResX) {
+        rr = sv[1]; ri = sv[2];
+        c->rxr = rr; c->rxi = ri;
+      } else {
+        rr = c->rxr + sv[3]; ri = c->rxi + sv[4];
+      }
+      c->gval = sv[0];
+      c->rval = rr;
+      c->ival = ri;
+      if (F.check && sv[0] < F.ffff) {
+        const double rel = -sv[0] / ri;
+        if (rel < 0.001 || sv[0] < F.ffff / 10.0) {
+          c->done = 1;
+          c->conv_it = F.it;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // Everything one problem's action kernels read: pointers into its device state,
 // its sizes and its launch geometry.  Passed by value to the single-problem
 // kernels; the batched kernels read an array of them, one per problem.
@@ -56,6 +139,8 @@ struct ActArgs {
                               // enqueued ahead)
   int axmy;                   // 1: the adjoint writes c psi - H psi instead of H psi
   double2 axmy_c;             // c = E + i gamma of a GMRES step (OCEAN_psi_axmy)
+  GmFin fin;                  // fin.c != nullptr: workgroup 0 of the projection first runs
+                              // the previous GMRES residual step's finish
 };
 
 // out = c psi - r in OCEAN_psi_axmy's operation order (OCEAN2/OCEAN_psi.f90:1468-1562)
@@ -501,6 +586,7 @@ template <typename T2, int NA>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kProjWaves<T2, NA>))) void k_proj(
     const ActArgs<T2> A) {
   if (skipped(A)) return;
+  if (A.fin.c != nullptr && blockIdx.x == 0) gm_finish(A.fin);
   int b = blockIdx.x;
   const int n = gridDim.x;
   if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);

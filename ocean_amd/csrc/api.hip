@@ -290,7 +290,8 @@ Tiles tiles_for(const H* h) {
 
 template <typename T2, int NA>
 int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* phi,
-           const typename og::Scalar<T2>::T* W, hipEvent_t* ev, const int* skip, const double2* axmy) {
+           const typename og::Scalar<T2>::T* W, hipEvent_t* ev, const int* skip, const double2* axmy,
+           const og::GmFin* fin) {
   og::ActArgs<T2> a;
   Geom<T2> g;
   const Tiles tl = tiles_for<T2, NA>(h);
@@ -300,6 +301,9 @@ int xact_t(H* h, const double2* in, double2* out, int flags, T2* u, T2* uT, T2* 
     a.axmy = 1;
     a.axmy_c = *axmy;
   }
+  // a pending GMRES finish rides on the projection (the caller passes one only when
+  // the long-range term, and so k_proj, runs)
+  if (fin) a.fin = *fin;
   if (ev) OG_CHECK(hipEventRecord(ev[0], h->stream));
   if (flags & og::F_LR) {
     OG_TRY(lds_opt_in(og::k_proj<T2, NA>, g.sm_proj));
@@ -420,7 +424,7 @@ int xact_obf_t(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev,
 // skip (nullable): every kernel returns at once while *skip != 0; axmy (nullable):
 // out = c in - H in with c = *axmy, formed in the adjoint's epilogue.
 int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = nullptr,
-             const int* skip = nullptr, const double2* axmy = nullptr) {
+             const int* skip = nullptr, const double2* axmy = nullptr, const og::GmFin* fin = nullptr) {
   const int acnt = (h->ns == 2) ? h->na / 2 : h->na;
   const int NA = na_tmpl(acnt);
   const bool mixed = (h->precision == OCEAN_GPU_MIXED);
@@ -432,8 +436,8 @@ int xact_dev(H* h, const double2* in, double2* out, int flags, hipEvent_t* ev = 
   }
 #define OG_DISPATCH(NAV)                                                              \
   if (NA == NAV)                                                                      \
-    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev, skip, axmy) \
-                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev, skip, axmy);
+    return mixed ? xact_t<float2, NAV>(h, in, out, flags, h->u32, h->uT32, h->phi32, h->W32, ev, skip, axmy, fin) \
+                 : xact_t<double2, NAV>(h, in, out, flags, h->u64, h->uT64, h->phi64, h->W64, ev, skip, axmy, fin);
   OG_DISPATCH(2)
   OG_DISPATCH(4)
   OG_DISPATCH(6)
@@ -778,17 +782,11 @@ __global__ void k_mask(double2* __restrict__ y, const double2* __restrict__ x, c
   }
 }
 
-// GMRES control state on the device: the solve of one energy enqueues several
-// iterations ahead and every kernel of them returns at once when `done` is set, so
-// the host reads the state back once per block of iterations, not per iteration.
-struct GmCtl {
-  int done;     // 0 running, 1 converged, 2 least-squares pivot failure
-  int conv_it;  // the iteration (1..depth of its cycle) at which it converged
-  int info;     // zhptrf-style pivot index of a failure
-  int pad1;     // (keeps the doubles 16-byte aligned)
-  double gval, rval, ival, pad2;  // |g|^2, Re / Im <rhs|x> of the last residual
-  double rxr, rxi, pad3, pad4;    // <rhs|x_b> of the materialised x (x = x_b + sum_i C_i U_i)
-};
+using og::GmCtl;
+using og::GmFin;
+using og::kResX;
+using og::kResU;
+using og::kDotU;
 
 __global__ void k_gm_reset(GmCtl* __restrict__ c) {
   if (threadIdx.x == 0) {
@@ -811,18 +809,9 @@ __global__ void k_axpy_gated(const int* __restrict__ skip, double ar, double ai,
 }
 
 // The residual kernels write the sums of each workgroup (v[3], block-summed in thread
-// 0) as partials; k_resid_fin, one workgroup launched after them, sums the partials
-// in a fixed order.  (A last-workgroup-to-arrive finish inside the kernels serialised
-// a device-scope atomic per workgroup.)  By mode:
-//   kResX   (x materialised, C = 0): v = (|g|^2, <rhs|x>); <rhs|x> becomes <rhs|x_b>;
-//   kResU   v = (|g|^2, <rhs|U_slot>) (slot < 0: no new vector); <rhs|x> is
-//           <rhs|x_b> + sum_{i<m} C_i <rhs|U_i>;
-//   kDotU   v = (0, <rhs|U_slot>): the dot only.
-// kResX and kResU store |g|^2 and <rhs|x> and, with check, apply the reference's
-// stopping test (OCEAN_gmres.f90:845-852): |g|^2 < ffff and (-|g|^2 / Im<rhs|x> < 0.001
-// or |g|^2 < ffff / 10).
-enum { kResX = 0, kResU = 1, kDotU = 2 };
-
+// 0) as partials; og::gm_finish sums them in a fixed order, in k_resid_fin or in the
+// next action's projection.  (A last-workgroup-to-arrive finish inside the kernels
+// serialised a device-scope atomic per workgroup.)
 __device__ __forceinline__ void resid_part(double (&v)[3], double* __restrict__ part) {
   __shared__ double lds[4 * 3];
   og::block_sum<3>(v, lds);
@@ -831,54 +820,7 @@ __device__ __forceinline__ void resid_part(double (&v)[3], double* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void k_resid_fin(GmCtl* __restrict__ c, const double* __restrict__ part,
-                                                   int nblk, double ffff, int it, int check, int mode,
-                                                   const double2* __restrict__ C, double2* __restrict__ rU,
-                                                   int m, int slot) {
-  if (c->done) return;
-  __shared__ double sv[5];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (w < 3) {
-    double acc = 0.0;
-    for (int b = lane; b < nblk; b += 64) acc += part[3 * b + w];
-    acc = og::wave_sum(acc);
-    if (lane == 0) sv[w] = acc;
-  } else if (mode == kResU) {
-    // sum_{i<m} C_i <rhs|U_i>, lane-strided then a butterfly (fixed order)
-    double ar = 0.0, ai = 0.0;
-    for (int i = lane; i < m; i += 64) {
-      const double2 ci = C[i], u = rU[i];
-      ar += ci.x * u.x - ci.y * u.y;
-      ai += ci.x * u.y + ci.y * u.x;
-    }
-    ar = og::wave_sum(ar);
-    ai = og::wave_sum(ai);
-    if (lane == 0) { sv[3] = ar; sv[4] = ai; }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (slot >= 0) rU[slot] = make_double2(sv[1], sv[2]);
-    if (mode != kDotU) {
-      double rr, ri;
-      if (mode == kResX) {
-        rr = sv[1]; ri = sv[2];
-        c->rxr = rr; c->rxi = ri;
-      } else {
-        rr = c->rxr + sv[3]; ri = c->rxi + sv[4];
-      }
-      c->gval = sv[0];
-      c->rval = rr;
-      c->ival = ri;
-      if (check && sv[0] < ffff) {
-        const double rel = -sv[0] / ri;
-        if (rel < 0.001 || sv[0] < ffff / 10.0) {
-          c->done = 1;
-          c->conv_it = it;
-        }
-      }
-    }
-  }
-}
+__global__ __launch_bounds__(256) void k_resid_fin(const GmFin F) { og::gm_finish(F); }
 
 // g = ax - rhs for one element and its term of |g|^2
 __device__ __forceinline__ double2 resid_g(size_t e, double2 a, const double2* __restrict__ rhs,
@@ -2258,11 +2200,18 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   if (rc == 0) rc = hipMemcpy(dlist, hlist.data(), 2 * depth * sizeof(double2*), hipMemcpyHostToDevice) == hipSuccess
                         ? OCEAN_GPU_OK : OCEAN_GPU_ERR_HIP;
 
+  // A finish of a residual step that the next action's projection runs (within a
+  // restart cycle the next kernel after the residual step is always that projection)
+  const bool ride = (f & og::F_LR) && !h->obf_on;
+  GmFin pend{};
+  bool have_pend = false;
   // outv = (E + i gres) in - H in (formed in the action's epilogue), dropped once
   // the solve has stopped
   auto axmy_xact = [&](double2* in, double2* outv, double E) -> int {
     const double2 c = make_double2(E, gres);
-    OG_TRY(xact_dev(h, in, outv, f, nullptr, skip, &c));
+    const GmFin* fp = have_pend ? &pend : nullptr;
+    have_pend = false;
+    OG_TRY(xact_dev(h, in, outv, f, nullptr, skip, &c, fp));
     return launched();
   };
   // x is kept as x_b + sum_i C_i U_i: each iteration's update goes into the
@@ -2280,9 +2229,15 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   // g = ax - rhs with x materialised, its sums and (with check) the stopping test of
   // iteration it
   double* rpart = part + (size_t)kRedBlocks * 4 * depth + 4 * (size_t)depth;
+  auto fin_of = [&](int it, int check, int mode, int m, int slot) -> GmFin {
+    GmFin F{};
+    F.c = ctl; F.part = rpart; F.C = dC; F.rU = dRU; F.ffff = ffff;
+    F.nblk = kRedBlocks; F.it = it; F.check = check; F.mode = mode; F.m = m; F.slot = slot;
+    return F;
+  };
   auto resid = [&](int it, int check) -> int {
     k_resid_step<<<kRedBlocks, kBlock, 0, st>>>(ctl, ax, rhs, x, g, nv, rpart);
-    k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, ffff, it, check, kResX, nullptr, nullptr, 0, -1);
+    k_resid_fin<<<1, 256, 0, st>>>(fin_of(it, check, kResX, 0, -1));
     return launched();
   };
   // update_gmres(cur): Gram column cur, least-squares solve and the x / ax updates
@@ -2297,8 +2252,14 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
     if (rit >= 0) {
       k_axpy_resid<<<kRedBlocks, kBlock, 0, st>>>(ctl, (const double2* const*)dlist, dcoef, cur, ax, rhs, g, pc,
                                                   unext, nv, rpart);
-      k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, ffff, rit, check, kResU, dC, dRU, cur,
-                                     unext ? cur : -1);
+      const GmFin F = fin_of(rit, check, kResU, cur, unext ? cur : -1);
+      if (ride && check && unext) {
+        // the next kernel is the next iteration's projection: the finish rides on it
+        pend = F;
+        have_pend = true;
+      } else {
+        k_resid_fin<<<1, 256, 0, st>>>(F);
+      }
       return launched();
     }
     OG_TRY(launched());
@@ -2321,6 +2282,9 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
   for (int step = 0; step < nE && rc == 0; ++step) {
     int complete_iter = 0;
     const double E = energies_eV[step] * kEV2Hartree;
+    // a finish still pending belongs to an iteration enqueued after the previous
+    // energy converged (its kernels were dropped): it must not run in this energy
+    have_pend = false;
     k_gm_reset<<<1, 64, 0, st>>>(ctl);
     if ((rc = launched())) break;
     if (prev_iter == 0) {
@@ -2372,7 +2336,7 @@ int gmres_run(H* h, const double2* rhs_dev, int nE, const double* energies_eV, d
           // later ones were formed by the previous iteration's residual step
           if (it == cycle_start + 1) {
             k_pc_apply<<<kRedBlocks, kBlock, 0, st>>>(ctl, U[it - 1], g, pc, rhs, nv, rpart);
-            k_resid_fin<<<1, 256, 0, st>>>(ctl, rpart, kRedBlocks, 0.0, 0, 0, kDotU, nullptr, dRU, 0, it - 1);
+            k_resid_fin<<<1, 256, 0, st>>>(fin_of(0, 0, kDotU, 0, it - 1));
           }
           if ((rc = axmy_xact(U[it - 1], AU[it - 1], E))) break;
           if (it < depth) {
