@@ -42,12 +42,12 @@ struct GmCtl {
 // restart cycle, in workgroup 0 of the next action's projection (c == nullptr: none).
 enum { kResX = 0, kResU = 1, kDotU = 2 };
 struct GmFin {
-  GmCtl* c;
-  const double* part;
-  const double2* C;
-  double2* rU;
-  double ffff;
-  int nblk, it, check, mode, m, slot;
+  GmCtl* c = nullptr;
+  const double* part = nullptr;
+  const double2* C = nullptr;
+  double2* rU = nullptr;
+  double ffff = 0.0;
+  int nblk = 0, it = 0, check = 0, mode = 0, m = 0, slot = -1;
 };
 
 // One workgroup of any size (one wave or more).
